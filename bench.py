@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Throughput of the wavefront timestep on one MI355X.
+
+One "step" is one full engine::run loop iteration (src/engine.rs:747-782):
+the H-phase and E-phase kernels over the whole grid (CPML fused), the source
+injection and the probe accumulation.  Default workload: the 1024^3 FP32
+random dielectric fill (200 spheres + 200 boxes, 255-material palette,
+20-cell requested / 24-cell effective CPML, 5 GHz sinusoidal Ez source).
+
+Prints one JSON line: value = Mcell-updates/s (cells x steps / time) with the
+scene resident in HBM, plus the roofline of the dominant kernel and the CPU
+oracle's rate on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-updates/s + % HBM roofline, 1024³ FP32, 1 MI355X; refl. learned vs CPML"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, steps, threads, timeout):
+    """Times the CPU oracle on `steps` steps of the same workload in a child
+    process (so a host-memory failure cannot take the GPU result with it)."""
+    code = f"""
+import sys, time, json
+sys.path.insert(0, {ROOT!r})
+from wavefront_amd import workloads
+from tests.helpers import oracle_sim_for
+w = workloads.random_1024(steps={steps}, n={n})
+sim = oracle_sim_for(w)
+sim.run(1, w.dt, threads={threads})
+t0 = time.perf_counter()
+sim.run({steps}, w.dt, first_step=1, threads={threads})
+el = time.perf_counter() - t0
+print(json.dumps({{"seconds": el, "cells": w.cells()}}))
+"""
+    try:
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                           timeout=timeout)
+        if r.returncode != 0:
+            return {"error": f"exit {r.returncode}: {r.stderr.strip()[-300:]}"}
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+        return res
+    except subprocess.TimeoutExpired:
+        return {"error": f"timeout after {timeout}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1024, help="edge length of the random-fill cube")
+    ap.add_argument("--kernel-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part "
+                                   "of this build (--gpus must be 1)"}))
+        sys.exit(2)
+
+    from wavefront_amd import workloads
+    from wavefront_amd.engine import DeviceEngine
+
+    t0 = time.perf_counter()
+    total = args.warmup + args.steps + args.kernel_steps
+    w = workloads.random_1024(steps=total, n=args.n)
+    log(f"scene {w.name}: {w.notes['materials']} materials, voxelized in "
+        f"{time.perf_counter() - t0:.1f}s")
+    eng = DeviceEngine(w.dims, w.ids, w.table, w.pml())
+    eng.set_sources(w.sources, w.dt, total)
+    eng.set_probes(w.probes, w.dt, total)
+    log("engine ready; warmup")
+    eng.step(args.warmup)
+    eng.sync()
+    log(f"timing {args.steps} steps")
+    wall0 = time.perf_counter()
+    ms = eng.time_steps(args.steps)
+    eng.sync()
+    wall = time.perf_counter() - wall0
+    h_ms, e_ms = eng.time_kernels(args.kernel_steps)
+    h_bytes, e_bytes = eng.phase_bytes()
+    eng.probe_sums()
+    eng.close()
+
+    cells = w.cells()
+    ms_per_step = ms / args.steps
+    mcells = cells * args.steps / (ms / 1e3) / 1e6
+    if h_ms >= e_ms:
+        kname, kms, kbytes = "k_update_h", h_ms, h_bytes
+    else:
+        kname, kms, kbytes = "k_update_e", e_ms, e_bytes
+    achieved = kbytes / (kms / 1e3) / 1e9
+    step_bytes = h_bytes + e_bytes
+    log(f"{ms_per_step:.3f} ms/step ({mcells:.0f} Mcell/s), H {h_ms:.3f} ms, E {e_ms:.3f} ms, "
+        f"wall {wall:.2f}s")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        log(f"cpu baseline: oracle, {args.cpu_steps} steps of {w.name}, {threads} threads")
+        res = cpu_baseline(args.n, args.cpu_steps, threads, timeout=900)
+        if "error" in res:
+            cpu = {"value": None, "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+                   "sample": f"{args.cpu_steps} steps of {w.name}", "error": res["error"]}
+        else:
+            cpu = {"value": round(res["cells"] * args.cpu_steps / res["seconds"] / 1e6, 2),
+                   "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+                   "sample": f"{args.cpu_steps} timed steps (after 1 untimed) of the full "
+                             f"{w.name} workload on the C oracle, Z-slab threads"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(mcells, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{w.name}: {args.n}^3 FP32 random fill (200 spheres + 200 boxes, "
+                        f"{w.notes['materials']} materials), CPML 20 (effective 24), 5 GHz "
+                        f"sinusoid Ez source, 1 probe",
+            "cells": cells,
+            "bytes_per_step_algorithmic": step_bytes,
+            "hbm_roofline_pct_step": round(100.0 * (step_bytes / (ms_per_step / 1e3) / 1e9)
+                                           / HBM_PEAK_GBS, 2),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kname,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,219 @@
+/*
+ * wavefront_hip.h -- C ABI of the MI355X (gfx950) timestep engine.
+ *
+ * Drop-in boundary: the body of the reference's engine::run loop
+ * (src/engine.rs:747-782), i.e. step_all_slabs (src/engine.rs:291-422) +
+ * Source::inject (src/source.rs:79-95) per source + Probe::accumulate
+ * (src/probe.rs:126-138) per probe, plus the snapshot serializer
+ * (src/engine.rs:622-640).  The reference has no FFI of its own; these are the
+ * entry points a Rust `-sys` crate binds.
+ *
+ * Conventions: extern "C" only; no exceptions cross the boundary; every
+ * function returning int returns WF_OK (0) or a negative WF_ERR_* code and
+ * leaves a message for wf_last_error() (thread-local).  Host buffers are
+ * caller-owned; device buffers are owned by the wf_ctx.  A context is not
+ * thread-safe; separate contexts may be used from separate threads.
+ *
+ * Field state on the host side always uses the reference snapshot format:
+ * block-major 8x8x8 AoSoA, per block ex[512] ey[512] ez[512] hx[512] hy[512]
+ * hz[512] little-endian f32, local index (lz*8+ly)*8+lx, blocks ordered
+ * (bz*nby+by)*nbx+bx (src/layout.rs:697-787).  On the device the fields are
+ * six x-fastest SoA arrays.
+ */
+#ifndef WAVEFRONT_HIP_H
+#define WAVEFRONT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WF_OK 0
+#define WF_ERR_INVALID (-1) /* bad argument or precondition */
+#define WF_ERR_HIP (-2)     /* HIP runtime error (incl. no device) */
+#define WF_ERR_OOM (-3)     /* device or host allocation failed */
+#define WF_ERR_STATE (-4)   /* call out of order (e.g. step past source table) */
+#define WF_ERR_IO (-5)      /* snapshot file I/O (incl. short writes) */
+
+#define WF_BLOCK_DIM 8
+
+/* Field components, src/probe.rs:30-38 (sources use 0..2 only). */
+#define WF_EX 0
+#define WF_EY 1
+#define WF_EZ 2
+#define WF_HX 3
+#define WF_HY 4
+#define WF_HZ 5
+
+/* Waveform kinds, src/source.rs:30-43. */
+#define WF_WAVE_GAUSSIAN 0 /* p0 = t0, p1 = spread */
+#define WF_WAVE_SINUSOID 1 /* p0 = freq_hz */
+#define WF_WAVE_RICKER 2   /* p0 = peak_freq_hz, p1 = t0 */
+
+typedef struct wf_ctx wf_ctx;
+
+/* A point source site.  Replaces Source's location/component
+ * (src/source.rs:65-72).  hard != 0 assigns instead of adding (the sheet
+ * source of examples/convergence_study.rs:165-173). */
+typedef struct wf_source_site {
+    uint32_t x, y, z;
+    int32_t component; /* WF_EX..WF_EZ */
+    int32_t hard;
+} wf_source_site;
+
+/* A full source description for the host-side evaluator
+ * (src/source.rs:65-72 + src/main.rs:431-460 derive t0/spread). */
+typedef struct wf_source_desc {
+    wf_source_site site;
+    float amplitude;
+    int32_t waveform; /* WF_WAVE_* */
+    float p0, p1;
+} wf_source_desc;
+
+/* A probe site (src/probe.rs:63-75 minus the accumulators). */
+typedef struct wf_probe_site {
+    uint32_t x, y, z;
+    int32_t component; /* WF_EX..WF_HZ */
+    uint32_t nfreq;
+} wf_probe_site;
+
+/* ---------------------------------------------------------------------- */
+/* Errors / info                                                           */
+/* ---------------------------------------------------------------------- */
+const char *wf_last_error(void);
+/* Number of visible HIP devices (0 without a GPU). */
+int wf_device_count(void);
+
+/* ---------------------------------------------------------------------- */
+/* Host-side table builders (same arithmetic as the reference)              */
+/* ---------------------------------------------------------------------- */
+/* FixedQ16_16::from_f32 / to_f32, src/layout.rs:114-125 */
+int32_t wf_q16_from_f32(float v);
+float wf_q16_to_f32(int32_t q);
+/* MaterialTable::coeffs_from_physical, src/layout.rs:194-208.
+ * out = {ca, cb, da, db} in Q16.16. */
+void wf_material_coeffs(float eps_r, float mu_r, float sigma, float dt, float d,
+                        int32_t out[4]);
+/* MaterialTable::vacuum_filled, src/layout.rs:175-180: table[256*4]. */
+void wf_material_table_vacuum(float dt, float d, int32_t *table);
+/* effective_pml_thickness_blocks, src/layout.rs:525-533 */
+uint32_t wf_effective_pml_thickness_blocks(uint32_t nx, uint32_t ny, uint32_t nz,
+                                           uint32_t requested_voxels);
+/* PmlProfile1D::build, src/layout.rs:471-513: n entries of b, a, inv_kappa. */
+void wf_pml_profile(uint32_t n, float grading_order, float kappa_max, float alpha_max,
+                    float target_reflection, float dt, float dx, uint32_t thickness_voxels,
+                    float *b, float *a, float *inv_kappa);
+/* Waveform::evaluate, src/source.rs:46-59 */
+float wf_waveform(int32_t kind, float p0, float p1, float t);
+/* amplitude * waveform(t) for t = (float)(first_step+k+1)*dt (src/engine.rs:769),
+ * k < nsteps, written as values[k*n + s]. */
+int wf_source_values(uint32_t n, const wf_source_desc *src, float dt, uint64_t first_step,
+                     uint64_t nsteps, float *values);
+/* Probe twiddles for Probe::accumulate (src/probe.rs:126-138): for probe p,
+ * frequency i (flattened j = sum_{q<p} nfreq_q + i) and step k:
+ * cos_tab[k*nf+j] = cos(2*pi*(double)f * (double)t), sin_tab likewise,
+ * active[k*n+p] = !(t < start[p]), with t = (float)(first_step+k+1)*dt. */
+int wf_probe_twiddles(uint32_t n, const wf_probe_site *sites, const float *freqs,
+                      const float *starts, float dt, uint64_t first_step, uint64_t nsteps,
+                      double *cos_tab, double *sin_tab, uint8_t *active);
+/* Probe::spectrum, src/probe.rs:143-167 */
+void wf_probe_spectrum(uint32_t nfreq, const double *sum_cos, const double *sum_sin,
+                       uint64_t n, double *amplitude, double *phase_rad);
+/* Scene::parse + Scene::voxelize, src/scene.rs:81-230.  ids (x-fastest,
+ * nx*ny*nz, pre-filled by the caller, normally 0) and table (256*4, normally
+ * wf_material_table_vacuum) are updated in place.  Returns the number of
+ * distinct materials (>= 0) or WF_ERR_INVALID with a message (parse error,
+ * more than 255 materials). */
+int wf_scene_voxelize(const char *text, uint32_t nx, uint32_t ny, uint32_t nz, float dt,
+                      float dx, uint8_t *ids, int32_t *table);
+/* voxelize_demo_sphere, src/main.rs:403-425 (material id 1). */
+void wf_voxelize_demo_sphere(uint32_t nx, uint32_t ny, uint32_t nz, uint8_t *ids);
+
+/* ---------------------------------------------------------------------- */
+/* Device engine                                                            */
+/* ---------------------------------------------------------------------- */
+/*
+ * Creates a device context on HIP device `device` and uploads the scene.
+ *   nx, ny, nz      voxels, each a positive multiple of 8 (GridDims::new,
+ *                   src/layout.rs:60-67).
+ *   material_ids    nx*ny*nz u8, x-fastest (MaterialGrid, src/layout.rs:327);
+ *                   NULL = all vacuum (id 0).
+ *   coeff_table     256 x {ca, cb, da, db} Q16.16 (MaterialTable); per-voxel
+ *                   coefficients are table[id], exactly CoeffGrid::build
+ *                   (src/layout.rs:230-260).
+ *   pml_shell_blocks NULL = PML disabled (engine::run with pml = None).
+ *                   Otherwise [3] per-axis shell depth in blocks: a block is
+ *                   in the PML shell iff some axis a has shell[a] > 0 and its
+ *                   block coordinate is < shell[a] or >= nb[a]-shell[a]
+ *                   (PmlAuxGrid::build, src/layout.rs:594-618).  Requires
+ *                   2*shell[a] <= nb[a].
+ *   pml_profiles    required with pml_shell_blocks: b, a, inv_kappa for x
+ *                   (3*nx floats), then y (3*ny), then z (3*nz) -- the
+ *                   PmlProfile1D arrays.  Entries outside [0, 8*shell[a]) and
+ *                   [n-8*shell[a], n) must be identity (b=1, a=0, ik=1), as
+ *                   PmlProfile1D::build makes them.
+ *   periodic_mask   bit a set = axis a wraps instead of reading the zero
+ *                   boundary (OUTER_ZERO_BLOCK, src/engine.rs:83); the
+ *                   validation studies wrap y and z.
+ */
+int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids,
+              const int32_t *coeff_table, const uint32_t *pml_shell_blocks,
+              const float *pml_profiles, uint32_t periodic_mask, int device, wf_ctx **out);
+void wf_destroy(wf_ctx *ctx);
+
+/* Uploads the source table: n sites and values[k*n + s] for the next nsteps
+ * steps (from the current step counter).  Sources are applied in list order
+ * after the E update of each step (src/engine.rs:769-772). */
+int wf_set_sources(wf_ctx *ctx, uint32_t n, const wf_source_site *sites, uint64_t nsteps,
+                   const float *values);
+/* Uploads the probe table (see wf_probe_twiddles for the layout) and zeroes
+ * the accumulators.  trace != 0 also records the sampled f32 value of every
+ * probe at every step (read with wf_read_trace). */
+int wf_set_probes(wf_ctx *ctx, uint32_t n, const wf_probe_site *sites, uint64_t nsteps,
+                  const double *cos_tab, const double *sin_tab, const uint8_t *active,
+                  int trace);
+
+/* Advances nsteps full timesteps: H, E, sources, probes per step
+ * (src/engine.rs:748-782).  Asynchronous on the context's stream. */
+int wf_step(wf_ctx *ctx, uint64_t nsteps);
+/* Waits for all work queued on the context. */
+int wf_sync(wf_ctx *ctx);
+/* Steps advanced since creation. */
+uint64_t wf_step_count(const wf_ctx *ctx);
+
+/* Field state in the reference snapshot format (nx*ny*nz*24 bytes). */
+uint64_t wf_field_bytes(const wf_ctx *ctx);
+int wf_read_fields(wf_ctx *ctx, void *dst);
+int wf_write_fields(wf_ctx *ctx, const void *src);
+/* CPML psi memory in the reference PmlAux layout (src/layout.rs:545-558):
+ * 12 x 512 f32 per shell block, shell blocks in block-major order. */
+uint64_t wf_psi_shell_blocks(const wf_ctx *ctx);
+int wf_read_psi(wf_ctx *ctx, float *dst);
+
+/* Probe accumulators: flattened over (probe, frequency) like the twiddles;
+ * n[p] = samples recorded by probe p. */
+int wf_read_probe_sums(wf_ctx *ctx, double *sum_cos, double *sum_sin, uint64_t *n);
+/* trace[k*nprobes + p] for the steps since wf_set_probes (k < steps taken). */
+int wf_read_trace(wf_ctx *ctx, float *trace, uint64_t max_steps);
+
+/* ---------------------------------------------------------------------- */
+/* Measurement                                                              */
+/* ---------------------------------------------------------------------- */
+/* Times nsteps full steps with a HIP event pair on the context's stream;
+ * writes the elapsed milliseconds. */
+int wf_time_steps(wf_ctx *ctx, uint64_t nsteps, double *total_ms);
+/* Runs nsteps full steps with a HIP event pair around every H-phase and
+ * E-phase launch (same stream); writes the mean per-launch milliseconds of
+ * each phase kernel. */
+int wf_time_kernels(wf_ctx *ctx, uint64_t nsteps, double *h_ms, double *e_ms);
+/* Algorithmic HBM bytes of one launch of each phase kernel: per cell 12 B
+ * read of the curl field, 12 B read + 12 B write of the updated field, 1 B
+ * material id; plus 8 B (read + write) per non-identity psi term. */
+int wf_phase_bytes(const wf_ctx *ctx, uint64_t *h_bytes, uint64_t *e_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

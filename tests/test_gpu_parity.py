@@ -1,0 +1,239 @@
+"""GPU engine vs the CPU oracle: bit-exact field, psi and probe state.
+
+Every comparison is bitwise (FP32 with contraction off on both sides; f64
+probe sums).  Each test prints its case before touching the GPU."""
+import csv
+import os
+
+import numpy as np
+import pytest
+
+from oracle import wfo
+from tests.helpers import assert_bitwise, oracle_pml, oracle_sim_for
+from wavefront_amd.engine import PERIODIC_X, PERIODIC_Y, PERIODIC_Z, DeviceEngine
+from wavefront_amd.layout import GridDims, MaterialTable, PmlConfig, PmlContext, PmlProfile1D
+from wavefront_amd import workloads
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def _run_workload_gpu(w, steps):
+    eng = DeviceEngine(w.dims, w.ids, w.table, w.pml())
+    eng.set_sources(w.sources, w.dt, steps)
+    eng.set_probes(w.probes, w.dt, steps)
+    eng.step(steps)
+    eng.sync()
+    return eng
+
+
+def test_plain_64_bit_exact():
+    """64^3 vacuum, 16-cell CPML, Gaussian point source, 6 probes, 200 steps:
+    fields, psi and probe accumulators identical to the CPU engine."""
+    w = workloads.plain_64()
+    _say(f"plain_64 steps={w.num_steps}")
+    eng = _run_workload_gpu(w, w.num_steps)
+    sim = oracle_sim_for(w)
+    sim.run(w.num_steps, w.dt, threads=4)
+    assert_bitwise(eng.read_fields(), sim.read_fields(), "fields")
+    assert_bitwise(eng.read_psi(), sim.read_psi(), "psi")
+    sc, ss, n = eng.probe_sums()
+    oc, os_, on = sim.probe_sums()
+    assert_bitwise(sc, oc, "sum_cos")
+    assert_bitwise(ss, os_, "sum_sin")
+    assert list(n) == list(on) == [w.num_steps] * 6
+    # the wave reached the probes
+    assert np.abs(sc).max() > 0
+    eng.close()
+
+
+def _random_case(dims, shell, periodic, steps, seed, nmat=5, pml=True):
+    rng = np.random.default_rng(seed)
+    dt, dx = workloads.DT_COURANT_099, 1e-3
+    table = MaterialTable.vacuum_filled(dt, dx)
+    otable = wfo.vacuum_table(dt, dx)
+    for m in range(1, nmat):
+        eps, sig = float(rng.uniform(1, 10)), float(rng.uniform(0, 0.05))
+        table.set_material(m, eps, 1.0, sig, dt, dx)
+        otable[m] = wfo.material_coeffs(eps, 1.0, sig, dt, dx)
+    ids = rng.integers(0, nmat, dims.voxel_count(), dtype=np.uint8)
+    fields = rng.standard_normal((dims.block_count(), 6, 512)).astype(np.float32)
+    # exercise signed zeros and exact cancellations too
+    fields[rng.random(fields.shape) < 0.05] = 0.0
+    fields[rng.random(fields.shape) < 0.02] = -0.0
+    ctx = None
+    oshell = oprof = None
+    if pml:
+        n = (dims.nx, dims.ny, dims.nz)
+        profs = [PmlProfile1D.build(n[a], PmlConfig(), dt, dx, 8 * shell[a]) for a in range(3)]
+        ctx = PmlContext(profs, shell)
+        oshell = shell
+        oprof = np.concatenate([wfo.pml_profile(n[a], 8 * shell[a], dt, dx).reshape(-1)
+                                for a in range(3)])
+    eng = DeviceEngine(dims, ids, table, ctx, periodic=periodic)
+    eng.write_fields(fields)
+    sim = wfo.OracleSim(dims.nx, dims.ny, dims.nz, ids=ids, table=otable, shell_blocks=oshell,
+                        profiles=oprof, periodic=periodic)
+    sim.write_fields(fields)
+    eng.step(steps)
+    for _ in range(steps):
+        sim.step_fields(threads=2)
+    return eng, sim
+
+
+@pytest.mark.parametrize("dims,shell,periodic", [
+    ((48, 40, 56), (1, 2, 1), 0),
+    ((64, 64, 64), (2, 2, 2), 0),
+    ((136, 24, 32), (3, 1, 2), 0),
+    ((72, 48, 40), (1, 0, 2), PERIODIC_Y),
+    ((40, 64, 48), (0, 1, 0), PERIODIC_X | PERIODIC_Z),
+])
+def test_random_fields_pml(dims, shell, periodic):
+    """Random fields, random lossy materials, CPML on uneven per-axis shells:
+    every voxel class (interior, faces, edges, corners, identity axes inside
+    shell blocks, signed zeros) after several steps."""
+    d = GridDims(*dims)
+    _say(f"random_fields_pml dims={dims} shell={shell} periodic={periodic}")
+    eng, sim = _random_case(d, shell, periodic, steps=5, seed=sum(dims))
+    assert_bitwise(eng.read_fields(), sim.read_fields(), "fields")
+    assert_bitwise(eng.read_psi(), sim.read_psi(), "psi")
+    eng.close()
+
+
+@pytest.mark.parametrize("periodic", [0, PERIODIC_X, PERIODIC_Y, PERIODIC_Z, 7])
+def test_random_fields_plain(periodic):
+    d = GridDims(80, 32, 24)
+    _say(f"random_fields_plain periodic={periodic}")
+    eng, sim = _random_case(d, None, periodic, steps=4, seed=11 + periodic, pml=False)
+    assert_bitwise(eng.read_fields(), sim.read_fields(), "fields")
+    eng.close()
+
+
+def test_zchunk_invariance(monkeypatch):
+    """The z-chunking of the launch grid must not change any value."""
+    d = GridDims(64, 32, 96)
+    outs = []
+    for zc in ("1", "7", "96"):
+        monkeypatch.setenv("WF_ZCHUNK", zc)
+        _say(f"zchunk={zc}")
+        eng, sim = _random_case(d, (1, 1, 2), 0, steps=3, seed=5)
+        outs.append(eng.read_fields())
+        eng.close()
+    assert_bitwise(outs[0], outs[1], "zchunk 1 vs 7")
+    assert_bitwise(outs[0], outs[2], "zchunk 1 vs 96")
+
+
+def test_sources_order_soft_and_hard():
+    """Several sources on one voxel: applied in list order (src/engine.rs:770)."""
+    from wavefront_amd.source import FieldComponent as SC, Source, Waveform
+    d = GridDims(16, 16, 16)
+    dt, dx = 1.5e-12, 1e-3
+    srcs = [Source(5, 6, 7, SC.Ey, 0.7, Waveform.sinusoid(3e10)),
+            Source(5, 6, 7, SC.Ey, 1.3, Waveform.ricker_wavelet(2e10, 5e-11), hard=True),
+            Source(5, 6, 7, SC.Ey, 0.2, Waveform.gaussian_pulse(2e-11, 1e-11)),
+            Source(9, 1, 2, SC.Ex, 2.0, Waveform.sinusoid(1e10))]
+    _say("sources_order")
+    eng = DeviceEngine(d, None, None, None, dt=dt, dx=dx)
+    eng.set_sources(srcs, dt, 12)
+    eng.step(12)
+    sim = wfo.OracleSim(16, 16, 16, table=wfo.vacuum_table(dt, dx))
+    for s in srcs:
+        sim.add_source(s.x, s.y, s.z, int(s.component), s.amplitude, s.waveform.kind,
+                       s.waveform.p0, s.waveform.p1, hard=s.hard)
+    sim.run(12, dt)
+    assert_bitwise(eng.read_fields(), sim.read_fields(), "fields")
+    eng.close()
+
+
+def test_scene_spheres_128():
+    """The two-sphere scene scaled to 128^3 (lossy sphere, CPML 16, Ricker)."""
+    w = workloads.spheres_512(steps=60, n=128)
+    _say("spheres_128 steps=60")
+    eng = _run_workload_gpu(w, 60)
+    sim = oracle_sim_for(w)
+    sim.run(60, w.dt, threads=4)
+    assert_bitwise(eng.read_fields(), sim.read_fields(), "fields")
+    assert_bitwise(eng.read_psi(), sim.read_psi(), "psi")
+    sc, ss, n = eng.probe_sums()
+    oc, os_, on = sim.probe_sums()
+    assert_bitwise(sc, oc, "sum_cos")
+    assert_bitwise(ss, os_, "sum_sin")
+    eng.close()
+
+
+def test_random_fill_128():
+    """The random sphere/box fill (255-entry palette) at 128^3."""
+    w = workloads.random_1024(steps=40, n=128)
+    _say("random_fill_128 steps=40")
+    eng = _run_workload_gpu(w, 40)
+    sim = oracle_sim_for(w)
+    sim.run(40, w.dt, threads=4)
+    assert_bitwise(eng.read_fields(), sim.read_fields(), "fields")
+    assert_bitwise(eng.read_psi(), sim.read_psi(), "psi")
+    eng.close()
+
+
+def test_write_read_roundtrip():
+    d = GridDims(24, 32, 40)
+    f = np.random.default_rng(3).standard_normal((d.block_count(), 6, 512)).astype(np.float32)
+    _say("write_read_roundtrip")
+    eng = DeviceEngine(d, None, None, None, dt=1e-12, dx=1e-3)
+    eng.write_fields(f)
+    assert_bitwise(eng.read_fields(), f, "roundtrip")
+    eng.close()
+
+
+def test_wave_arrives_at_c():
+    """src/engine.rs:841-932: Ricker 30 GHz at the centre of a 64^3 vacuum
+    grid, no PML; |Ez| at +20 cells first exceeds 5e-4 within 20% of 20*dx/c."""
+    from wavefront_amd.probe import FieldComponent as PC, Probe
+    from wavefront_amd.source import FieldComponent as SC, Source, Waveform
+    d = GridDims(64, 64, 64)
+    dx, dt = np.float32(1e-3), np.float32(1.5e-12)
+    src = Source(32, 32, 32, SC.Ez, 1.0, Waveform.ricker_wavelet(3e10, float(np.float32(1.0) / np.float32(3e10))))
+    _say("wave_arrives_at_c")
+    eng = DeviceEngine(d, None, None, None, dt=float(dt), dx=float(dx))
+    eng.set_sources([src], float(dt), 200)
+    eng.set_probes([Probe(52, 32, 32, PC.Ez, [], 0.0)], float(dt), 200, trace=True)
+    eng.step(200)
+    tr = eng.read_trace(200)[:, 0]
+    hits = np.flatnonzero(np.abs(tr) > 5e-4)
+    assert hits.size, "wave never reached the probe point within max_steps"
+    measured = (np.float32(hits[0]) + np.float32(1.0)) * dt
+    expected = (np.float32(20.0) * dx) / np.float32(299792458.0)
+    assert abs(measured - expected) / expected < 0.20
+    eng.close()
+
+
+def _golden(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return list(csv.reader(f))[1:]
+
+
+def test_convergence_study_on_gpu():
+    """examples/convergence_study.rs on the device: measured phase velocity
+    and error columns equal validation/convergence_data.csv bit for bit."""
+    from wavefront_amd.studies import convergence_rows
+    _say("convergence_study")
+    rows = convergence_rows()
+    gold = _golden("convergence_data.csv")
+    for (dx, ppw, v, err), g in zip(rows, gold):
+        assert np.float32(dx) == np.float32(float(g[0]))
+        assert np.float32(v) == np.float32(float(g[2])), (v, g[2])
+        assert np.float32(err) == np.float32(float(g[4])), (err, g[4])
+
+
+def test_pml_reflection_study_on_gpu():
+    """examples/pml_reflection_study.rs on the device: reflection coefficients
+    equal validation/pml_reflection_data.csv exactly (f64)."""
+    from wavefront_amd.studies import pml_reflection_rows
+    _say("pml_reflection_study")
+    got = pml_reflection_rows()
+    gold = [float(r[2]) for r in _golden("pml_reflection_data.csv")]
+    assert got == gold
+    assert all(b < a for a, b in zip(got, got[1:]))
+    assert got[-1] < 1e-2

@@ -1,0 +1,1039 @@
+// Device engine for gfx950: context management, the H/E timestep kernels,
+// source/probe point kernels and the SoA <-> reference-AoSoA converters.
+//
+// Built with -ffp-contract=off: every f32 operation in the update kernels is
+// a separately rounded IEEE operation in the same order as the reference's
+// src/fdtd.rs, so results are bit-identical to the CPU engine.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "wf_internal.h"
+#include "wf_kernels.h"
+
+using namespace wf;
+
+#define WF_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(WF_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                           \
+    } while (0)
+
+namespace {
+
+constexpr int kTX = 64;  // threads along x (one wavefront)
+constexpr int kTY = 4;   // rows per workgroup
+
+// ---------------------------------------------------------------------------
+// Per-thread geometry helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool in_shell_axis(const Geo &g, int ax, int c)
+{
+    const int b = c >> 3;
+    return g.sb[ax] > 0 && (b < g.sb[ax] || b >= g.nb[ax] - g.sb[ax]);
+}
+
+__device__ __forceinline__ bool graded(const Geo &g, int ax, int c, int n)
+{
+    return g.T[ax] > 0 && (c < g.T[ax] || c >= n - g.T[ax]);
+}
+
+__device__ __forceinline__ int slab_index(const Geo &g, int ax, int c, int n)
+{
+    return c < g.T[ax] ? c : c - (n - 2 * g.T[ax]);
+}
+
+// Neighbour coordinate c+1 / c-1 on an axis of length n: -1 = outside the
+// domain (reads the zero boundary, OUTER_ZERO_BLOCK in src/engine.rs:83).
+__device__ __forceinline__ int plus1(int c, int n, bool wrap)
+{
+    return c + 1 < n ? c + 1 : (wrap ? 0 : -1);
+}
+
+__device__ __forceinline__ int minus1(int c, int n, bool wrap)
+{
+    return c > 0 ? c - 1 : (wrap ? n - 1 : -1);
+}
+
+// ---------------------------------------------------------------------------
+// H update: update_h_field / update_h_field_pml (src/fdtd.rs:194-345).
+// One thread per (x, y) column, marching z over a chunk; E(z+1) is carried
+// in registers as the next iteration's E(z).
+// ---------------------------------------------------------------------------
+template <bool PML>
+__global__ __launch_bounds__(kTX *kTY) void k_update_h(KArgs k, int zchunk)
+{
+    __shared__ float2 s_co[256];  // (da, db)
+    const int tid = threadIdx.y * kTX + threadIdx.x;
+    for (int i = tid; i < 256; i += kTX * kTY) {
+        const float4 c = k.p.coef[i];
+        s_co[i] = make_float2(c.z, c.w);
+    }
+    __syncthreads();
+
+    const Geo &g = k.g;
+    const int x = blockIdx.x * kTX + threadIdx.x;
+    const int y = blockIdx.y * kTY + threadIdx.y;
+    if (x >= g.nx || y >= g.ny) return;
+    const int z0 = blockIdx.z * zchunk;
+    const int z1 = min(z0 + zchunk, g.nz);
+    const size_t nx = g.nx, plane = (size_t)g.nx * g.ny;
+
+    float *__restrict__ ex = k.p.f[0];
+    float *__restrict__ ey = k.p.f[1];
+    float *__restrict__ ez = k.p.f[2];
+    float *__restrict__ hx = k.p.f[3];
+    float *__restrict__ hy = k.p.f[4];
+    float *__restrict__ hz = k.p.f[5];
+
+    const int xp = plus1(x, g.nx, g.periodic & 1u);
+    const int yp = plus1(y, g.ny, g.periodic & 2u);
+
+    bool shell_xy = false, gx = false, gy = false;
+    int gxi = 0, gyi = 0;
+    float pxb = 1.f, pxa = 0.f, pxk = 1.f, pyb = 1.f, pya = 0.f, pyk = 1.f;
+    if (PML) {
+        shell_xy = in_shell_axis(g, 0, x) || in_shell_axis(g, 1, y);
+        gx = graded(g, 0, x, g.nx);
+        gy = graded(g, 1, y, g.ny);
+        gxi = slab_index(g, 0, x, g.nx);
+        gyi = slab_index(g, 1, y, g.ny);
+        pxb = k.p.prof[0][0][x];
+        pxa = k.p.prof[0][1][x];
+        pxk = k.p.prof[0][2][x];
+        pyb = k.p.prof[1][0][y];
+        pya = k.p.prof[1][1][y];
+        pyk = k.p.prof[1][2][y];
+    }
+
+    size_t o = (size_t)z0 * plane + (size_t)y * nx + x;
+    float ex_c = ex[o], ey_c = ey[o];
+    for (int z = z0; z < z1; ++z, o += plane) {
+        const float ez_c = ez[o];
+        const int zp = plus1(z, g.nz, g.periodic & 4u);
+        float ex_pz = 0.f, ey_pz = 0.f;
+        if (zp >= 0) {
+            const size_t oz = (size_t)zp * plane + (size_t)y * nx + x;
+            ex_pz = ex[oz];
+            ey_pz = ey[oz];
+        }
+        float ez_px = 0.f, ey_px = 0.f;
+        if (xp >= 0) {
+            const size_t ox = o - x + xp;
+            ez_px = ez[ox];
+            ey_px = ey[ox];
+        }
+        float ez_py = 0.f, ex_py = 0.f;
+        if (yp >= 0) {
+            const size_t oy = o + ((long)yp - y) * (long)nx;
+            ez_py = ez[oy];
+            ex_py = ex[oy];
+        }
+
+        float d_ez_dy = ez_py - ez_c, d_ex_dy = ex_py - ex_c;
+        float d_ey_dz = ey_pz - ey_c, d_ex_dz = ex_pz - ex_c;
+        float d_ez_dx = ez_px - ez_c, d_ey_dx = ey_px - ey_c;
+
+        if (PML && (shell_xy || in_shell_axis(g, 2, z))) {
+            if (gy) {
+                const size_t i = ((size_t)z * (2 * g.T[1]) + gyi) * nx + x;
+                float p0 = k.p.psi[1][0][i], p1 = k.p.psi[1][1][i];
+                d_ez_dy = cpml(d_ez_dy, pyb, pya, pyk, p0);
+                d_ex_dy = cpml(d_ex_dy, pyb, pya, pyk, p1);
+                k.p.psi[1][0][i] = p0;
+                k.p.psi[1][1][i] = p1;
+            } else {
+                d_ez_dy = d_ez_dy + 0.0f;
+                d_ex_dy = d_ex_dy + 0.0f;
+            }
+            if (graded(g, 2, z, g.nz)) {
+                const int gzi = slab_index(g, 2, z, g.nz);
+                const size_t i = ((size_t)gzi * g.ny + y) * nx + x;
+                const float b = k.p.prof[2][0][z], a = k.p.prof[2][1][z], ik = k.p.prof[2][2][z];
+                float p0 = k.p.psi[2][0][i], p1 = k.p.psi[2][1][i];
+                d_ey_dz = cpml(d_ey_dz, b, a, ik, p0);
+                d_ex_dz = cpml(d_ex_dz, b, a, ik, p1);
+                k.p.psi[2][0][i] = p0;
+                k.p.psi[2][1][i] = p1;
+            } else {
+                d_ey_dz = d_ey_dz + 0.0f;
+                d_ex_dz = d_ex_dz + 0.0f;
+            }
+            if (gx) {
+                const size_t i = ((size_t)z * g.ny + y) * (2 * g.T[0]) + gxi;
+                float p0 = k.p.psi[0][0][i], p1 = k.p.psi[0][1][i];
+                d_ez_dx = cpml(d_ez_dx, pxb, pxa, pxk, p0);
+                d_ey_dx = cpml(d_ey_dx, pxb, pxa, pxk, p1);
+                k.p.psi[0][0][i] = p0;
+                k.p.psi[0][1][i] = p1;
+            } else {
+                d_ez_dx = d_ez_dx + 0.0f;
+                d_ey_dx = d_ey_dx + 0.0f;
+            }
+        }
+
+        const float chx = d_ez_dy - d_ey_dz;
+        const float chy = d_ex_dz - d_ez_dx;
+        const float chz = d_ey_dx - d_ex_dy;
+        const float2 c = s_co[k.p.id[o]];
+        hx[o] = c.x * hx[o] - c.y * chx;
+        hy[o] = c.x * hy[o] - c.y * chy;
+        hz[o] = c.x * hz[o] - c.y * chz;
+
+        ex_c = ex_pz;
+        ey_c = ey_pz;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// E update: update_e_field / update_e_field_pml (src/fdtd.rs:369-509).
+// H(z) is carried as the next iteration's H(z-1).
+// ---------------------------------------------------------------------------
+template <bool PML>
+__global__ __launch_bounds__(kTX *kTY) void k_update_e(KArgs k, int zchunk)
+{
+    __shared__ float2 s_co[256];  // (ca, cb)
+    const int tid = threadIdx.y * kTX + threadIdx.x;
+    for (int i = tid; i < 256; i += kTX * kTY) {
+        const float4 c = k.p.coef[i];
+        s_co[i] = make_float2(c.x, c.y);
+    }
+    __syncthreads();
+
+    const Geo &g = k.g;
+    const int x = blockIdx.x * kTX + threadIdx.x;
+    const int y = blockIdx.y * kTY + threadIdx.y;
+    if (x >= g.nx || y >= g.ny) return;
+    const int z0 = blockIdx.z * zchunk;
+    const int z1 = min(z0 + zchunk, g.nz);
+    const size_t nx = g.nx, plane = (size_t)g.nx * g.ny;
+
+    float *__restrict__ ex = k.p.f[0];
+    float *__restrict__ ey = k.p.f[1];
+    float *__restrict__ ez = k.p.f[2];
+    float *__restrict__ hx = k.p.f[3];
+    float *__restrict__ hy = k.p.f[4];
+    float *__restrict__ hz = k.p.f[5];
+
+    const int xm = minus1(x, g.nx, g.periodic & 1u);
+    const int ym = minus1(y, g.ny, g.periodic & 2u);
+
+    bool shell_xy = false, gx = false, gy = false;
+    int gxi = 0, gyi = 0;
+    float pxb = 1.f, pxa = 0.f, pxk = 1.f, pyb = 1.f, pya = 0.f, pyk = 1.f;
+    if (PML) {
+        shell_xy = in_shell_axis(g, 0, x) || in_shell_axis(g, 1, y);
+        gx = graded(g, 0, x, g.nx);
+        gy = graded(g, 1, y, g.ny);
+        gxi = slab_index(g, 0, x, g.nx);
+        gyi = slab_index(g, 1, y, g.ny);
+        pxb = k.p.prof[0][0][x];
+        pxa = k.p.prof[0][1][x];
+        pxk = k.p.prof[0][2][x];
+        pyb = k.p.prof[1][0][y];
+        pya = k.p.prof[1][1][y];
+        pyk = k.p.prof[1][2][y];
+    }
+
+    float hx_mz = 0.f, hy_mz = 0.f;
+    {
+        const int zm = minus1(z0, g.nz, g.periodic & 4u);
+        if (zm >= 0) {
+            const size_t om = (size_t)zm * plane + (size_t)y * nx + x;
+            hx_mz = hx[om];
+            hy_mz = hy[om];
+        }
+    }
+    size_t o = (size_t)z0 * plane + (size_t)y * nx + x;
+    for (int z = z0; z < z1; ++z, o += plane) {
+        const float hx_c = hx[o], hy_c = hy[o], hz_c = hz[o];
+        float hz_mx = 0.f, hy_mx = 0.f;
+        if (xm >= 0) {
+            const size_t ox = o - x + xm;
+            hz_mx = hz[ox];
+            hy_mx = hy[ox];
+        }
+        float hz_my = 0.f, hx_my = 0.f;
+        if (ym >= 0) {
+            const size_t oy = o + ((long)ym - y) * (long)nx;
+            hz_my = hz[oy];
+            hx_my = hx[oy];
+        }
+
+        float d_hz_dy = hz_c - hz_my, d_hx_dy = hx_c - hx_my;
+        float d_hy_dz = hy_c - hy_mz, d_hx_dz = hx_c - hx_mz;
+        float d_hz_dx = hz_c - hz_mx, d_hy_dx = hy_c - hy_mx;
+
+        if (PML && (shell_xy || in_shell_axis(g, 2, z))) {
+            if (gy) {
+                const size_t i = ((size_t)z * (2 * g.T[1]) + gyi) * nx + x;
+                float p0 = k.p.psi[1][2][i], p1 = k.p.psi[1][3][i];
+                d_hz_dy = cpml(d_hz_dy, pyb, pya, pyk, p0);
+                d_hx_dy = cpml(d_hx_dy, pyb, pya, pyk, p1);
+                k.p.psi[1][2][i] = p0;
+                k.p.psi[1][3][i] = p1;
+            } else {
+                d_hz_dy = d_hz_dy + 0.0f;
+                d_hx_dy = d_hx_dy + 0.0f;
+            }
+            if (graded(g, 2, z, g.nz)) {
+                const int gzi = slab_index(g, 2, z, g.nz);
+                const size_t i = ((size_t)gzi * g.ny + y) * nx + x;
+                const float b = k.p.prof[2][0][z], a = k.p.prof[2][1][z], ik = k.p.prof[2][2][z];
+                float p0 = k.p.psi[2][2][i], p1 = k.p.psi[2][3][i];
+                d_hy_dz = cpml(d_hy_dz, b, a, ik, p0);
+                d_hx_dz = cpml(d_hx_dz, b, a, ik, p1);
+                k.p.psi[2][2][i] = p0;
+                k.p.psi[2][3][i] = p1;
+            } else {
+                d_hy_dz = d_hy_dz + 0.0f;
+                d_hx_dz = d_hx_dz + 0.0f;
+            }
+            if (gx) {
+                const size_t i = ((size_t)z * g.ny + y) * (2 * g.T[0]) + gxi;
+                float p0 = k.p.psi[0][2][i], p1 = k.p.psi[0][3][i];
+                d_hz_dx = cpml(d_hz_dx, pxb, pxa, pxk, p0);
+                d_hy_dx = cpml(d_hy_dx, pxb, pxa, pxk, p1);
+                k.p.psi[0][2][i] = p0;
+                k.p.psi[0][3][i] = p1;
+            } else {
+                d_hz_dx = d_hz_dx + 0.0f;
+                d_hy_dx = d_hy_dx + 0.0f;
+            }
+        }
+
+        const float cex = d_hz_dy - d_hy_dz;
+        const float cey = d_hx_dz - d_hz_dx;
+        const float cez = d_hy_dx - d_hx_dy;
+        const float2 c = s_co[k.p.id[o]];
+        ex[o] = c.x * ex[o] + c.y * cex;
+        ey[o] = c.x * ey[o] + c.y * cey;
+        ez[o] = c.x * ez[o] + c.y * cez;
+
+        hx_mz = hx_c;
+        hy_mz = hy_c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sources (src/source.rs:79-95): applied in list order by one lane.
+// ---------------------------------------------------------------------------
+__global__ void k_sources(Ptrs p, const DevSource *__restrict__ src, int n,
+                          const float *__restrict__ values, uint64_t row)
+{
+    if (threadIdx.x != 0) return;
+    const float *v = values + row * (uint64_t)n;
+    for (int s = 0; s < n; ++s) {
+        float *f = p.f[src[s].comp] + src[s].off;
+        if (src[s].hard)
+            *f = v[s];
+        else
+            *f = *f + v[s];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Probes (src/probe.rs:126-138): one lane per probe, frequencies in order.
+// ---------------------------------------------------------------------------
+__global__ void k_probes(Ptrs p, const DevProbe *__restrict__ pr, int n, int nf,
+                         const double *__restrict__ cos_tab, const double *__restrict__ sin_tab,
+                         const uint8_t *__restrict__ active, uint64_t row, double *sums,
+                         unsigned long long *counts, float *trace, uint64_t trace_row)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevProbe q = pr[i];
+    const float v32 = p.f[q.comp][q.off];
+    if (trace) trace[trace_row * (uint64_t)n + i] = v32;
+    if (!active[row * (uint64_t)n + i]) return;
+    const double v = (double)v32;
+    const double *ct = cos_tab + row * (uint64_t)nf + q.foff;
+    const double *st = sin_tab + row * (uint64_t)nf + q.foff;
+    double *sc = sums + q.foff;
+    double *ss = sums + nf + q.foff;
+    for (int j = 0; j < q.nfreq; ++j) {
+        sc[j] = sc[j] + v * ct[j];
+        ss[j] = ss[j] + v * st[j];
+    }
+    counts[i] += 1ull;
+}
+
+// ---------------------------------------------------------------------------
+// SoA <-> reference AoSoA (src/engine.rs:622-640) for block planes
+// [bz0, bz0 + nbz_chunk).  One thread per voxel of the chunk.
+// ---------------------------------------------------------------------------
+__global__ void k_pack(Ptrs p, Geo g, int bz0, uint64_t nvox, float *__restrict__ out)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nvox) return;
+    const uint64_t blk = t >> 9;
+    const int l = (int)(t & 511);
+    const int lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+    const int bx = (int)(blk % g.nb[0]);
+    const int by = (int)((blk / g.nb[0]) % g.nb[1]);
+    const int bz = bz0 + (int)(blk / ((uint64_t)g.nb[0] * g.nb[1]));
+    const size_t o = ((size_t)(bz * 8 + lz) * g.ny + (by * 8 + ly)) * (size_t)g.nx + bx * 8 + lx;
+    float *dst = out + blk * 3072 + l;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) dst[c * 512] = p.f[c][o];
+}
+
+__global__ void k_unpack(Ptrs p, Geo g, int bz0, uint64_t nvox, const float *__restrict__ in)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nvox) return;
+    const uint64_t blk = t >> 9;
+    const int l = (int)(t & 511);
+    const int lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+    const int bx = (int)(blk % g.nb[0]);
+    const int by = (int)((blk / g.nb[0]) % g.nb[1]);
+    const int bz = bz0 + (int)(blk / ((uint64_t)g.nb[0] * g.nb[1]));
+    const size_t o = ((size_t)(bz * 8 + lz) * g.ny + (by * 8 + ly)) * (size_t)g.nx + bx * 8 + lx;
+    const float *src = in + blk * 3072 + l;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) p.f[c][o] = src[c * 512];
+}
+
+// psi in the reference PmlAux layout (src/layout.rs:545-558) for the shell
+// blocks listed in `blocks` (packed bx | by<<20 | bz<<40).  Terms whose axis
+// is not graded at a voxel are identically +0 in the reference.
+__global__ void k_pack_psi(Ptrs p, Geo g, const uint64_t *__restrict__ blocks, uint64_t nblk,
+                           float *__restrict__ out)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nblk * 512) return;
+    const uint64_t b = t >> 9;
+    const int l = (int)(t & 511);
+    const uint64_t code = blocks[b];
+    const int bx = (int)(code & 0xFFFFF), by = (int)((code >> 20) & 0xFFFFF),
+              bz = (int)(code >> 40);
+    const int x = bx * 8 + (l & 7), y = by * 8 + ((l >> 3) & 7), z = bz * 8 + (l >> 6);
+    const bool gx = graded(g, 0, x, g.nx), gy = graded(g, 1, y, g.ny), gz = graded(g, 2, z, g.nz);
+    const size_t nx = g.nx;
+    const size_t ix = gx ? ((size_t)z * g.ny + y) * (2 * g.T[0]) + slab_index(g, 0, x, g.nx) : 0;
+    const size_t iy = gy ? ((size_t)z * (2 * g.T[1]) + slab_index(g, 1, y, g.ny)) * nx + x : 0;
+    const size_t iz = gz ? ((size_t)slab_index(g, 2, z, g.nz) * g.ny + y) * nx + x : 0;
+    // PmlAux order -> (axis, term)
+    const int ax[12] = {0, 0, 1, 1, 2, 2, 0, 0, 1, 1, 2, 2};
+    const int tm[12] = {0, 1, 0, 1, 0, 1, 2, 3, 3, 2, 2, 3};
+    float *dst = out + b * 6144 + l;
+    for (int k = 0; k < 12; ++k) {
+        const int a = ax[k];
+        const bool on = a == 0 ? gx : (a == 1 ? gy : gz);
+        const size_t i = a == 0 ? ix : (a == 1 ? iy : iz);
+        dst[k * 512] = on ? p.psi[a][tm[k]][i] : 0.0f;
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// Context
+// ===========================================================================
+struct wf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Geo g{};
+    Ptrs p{};
+    bool pml = false;
+    int zchunk = 16;
+
+    float *d_fields = nullptr;
+    uint8_t *d_id = nullptr;
+    float4 *d_coef = nullptr;
+    float *d_prof = nullptr;
+    float *d_psi = nullptr;
+    uint64_t nshell_blocks = 0;
+    uint64_t *d_shell_list = nullptr;
+
+    // sources
+    uint32_t nsrc = 0;
+    DevSource *d_src = nullptr;
+    float *d_srcval = nullptr;
+    uint64_t src_base = 0, src_steps = 0;
+
+    // probes
+    uint32_t nprobe = 0, nfreq = 0;
+    DevProbe *d_probe = nullptr;
+    double *d_cos = nullptr, *d_sin = nullptr, *d_sums = nullptr;
+    uint8_t *d_active = nullptr;
+    unsigned long long *d_counts = nullptr;
+    float *d_trace = nullptr;
+    uint64_t probe_base = 0, probe_steps = 0;
+
+    uint64_t step = 0;
+
+    float *d_stage = nullptr;
+    uint64_t stage_bytes = 0;
+};
+
+namespace {
+
+template <typename T>
+int dmalloc(T **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) return WF_OK;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? WF_ERR_OOM : WF_ERR_HIP,
+                    "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return WF_OK;
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void free_sources(wf_ctx *c)
+{
+    dfree(c->d_src);
+    dfree(c->d_srcval);
+    c->nsrc = 0;
+    c->src_steps = 0;
+}
+
+void free_probes(wf_ctx *c)
+{
+    dfree(c->d_probe);
+    dfree(c->d_cos);
+    dfree(c->d_sin);
+    dfree(c->d_sums);
+    dfree(c->d_active);
+    dfree(c->d_counts);
+    dfree(c->d_trace);
+    c->nprobe = 0;
+    c->nfreq = 0;
+    c->probe_steps = 0;
+}
+
+KArgs kargs(const wf_ctx *c) { return KArgs{c->g, c->p}; }
+
+// One full step: H phase, E phase, sources, probes (src/engine.rs:748-782).
+// ev (optional) = 4 events recorded around the H and E launches.
+int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
+{
+    const dim3 blk(kTX, kTY, 1);
+    const dim3 grd((c->g.nx + kTX - 1) / kTX, (c->g.ny + kTY - 1) / kTY,
+                   (c->g.nz + c->zchunk - 1) / c->zchunk);
+    const KArgs k = kargs(c);
+    if (ev) WF_HIP(hipEventRecord(ev[0], c->stream));
+    if (c->pml)
+        hipLaunchKernelGGL(k_update_h<true>, grd, blk, 0, c->stream, k, c->zchunk);
+    else
+        hipLaunchKernelGGL(k_update_h<false>, grd, blk, 0, c->stream, k, c->zchunk);
+    if (ev) WF_HIP(hipEventRecord(ev[1], c->stream));
+    if (ev) WF_HIP(hipEventRecord(ev[2], c->stream));
+    if (c->pml)
+        hipLaunchKernelGGL(k_update_e<true>, grd, blk, 0, c->stream, k, c->zchunk);
+    else
+        hipLaunchKernelGGL(k_update_e<false>, grd, blk, 0, c->stream, k, c->zchunk);
+    if (ev) WF_HIP(hipEventRecord(ev[3], c->stream));
+    if (c->nsrc) {
+        hipLaunchKernelGGL(k_sources, dim3(1), dim3(64), 0, c->stream, c->p, c->d_src,
+                           (int)c->nsrc, c->d_srcval, c->step - c->src_base);
+    }
+    if (c->nprobe) {
+        const uint64_t row = c->step - c->probe_base;
+        hipLaunchKernelGGL(k_probes, dim3((c->nprobe + 63) / 64), dim3(64), 0, c->stream,
+                           c->p, c->d_probe, (int)c->nprobe, (int)c->nfreq, c->d_cos,
+                           c->d_sin, c->d_active, row, c->d_sums, c->d_counts, c->d_trace,
+                           row);
+    }
+    WF_HIP(hipGetLastError());
+    c->step += 1;
+    return WF_OK;
+}
+
+int ensure_stage(wf_ctx *c, uint64_t bytes)
+{
+    if (c->stage_bytes >= bytes) return WF_OK;
+    dfree(c->d_stage);
+    c->stage_bytes = 0;
+    if (int rc = dmalloc(&c->d_stage, bytes)) return rc;
+    c->stage_bytes = bytes;
+    return WF_OK;
+}
+
+// Block planes per staging chunk: at most ~256 MiB.
+uint64_t planes_per_chunk(const wf_ctx *c)
+{
+    const uint64_t plane_bytes = (uint64_t)c->g.nb[0] * c->g.nb[1] * 512 * 24;
+    uint64_t n = (256ull << 20) / plane_bytes;
+    if (n < 1) n = 1;
+    if (n > (uint64_t)c->g.nb[2]) n = c->g.nb[2];
+    return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wf_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids,
+              const int32_t *coeff_table, const uint32_t *pml_shell_blocks,
+              const float *pml_profiles, uint32_t periodic_mask, int device, wf_ctx **out)
+{
+    clear_error();
+    if (!out) return fail(WF_ERR_INVALID, "wf_create: out is null");
+    *out = nullptr;
+    if (!nx || !ny || !nz || nx % 8 || ny % 8 || nz % 8)
+        return fail(WF_ERR_INVALID,
+                    "grid dimensions must be positive multiples of 8 (got %ux%ux%u)", nx, ny, nz);
+    if (nx > (1u << 20) || ny > (1u << 20) || nz > (1u << 20))
+        return fail(WF_ERR_INVALID, "grid axis longer than 2^20 voxels");
+    if (!coeff_table) return fail(WF_ERR_INVALID, "wf_create: coeff_table is null");
+    if (pml_shell_blocks && !pml_profiles)
+        return fail(WF_ERR_INVALID, "wf_create: PML shell given without profiles");
+    if (periodic_mask > 7u) return fail(WF_ERR_INVALID, "wf_create: periodic_mask > 7");
+    if (fault_injected("wf_create")) return fail(WF_ERR_HIP, "fault injected: wf_create");
+
+    const uint32_t n[3] = {nx, ny, nz};
+    Geo g{};
+    g.nx = (int)nx;
+    g.ny = (int)ny;
+    g.nz = (int)nz;
+    for (int a = 0; a < 3; ++a) g.nb[a] = (int)(n[a] / 8);
+    g.periodic = periodic_mask;
+    if (pml_shell_blocks) {
+        const float *pp = pml_profiles;
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t sb = pml_shell_blocks[a];
+            if (2ull * sb > (uint64_t)g.nb[a])
+                return fail(WF_ERR_INVALID,
+                            "PML shell of %u blocks on axis %d exceeds half the axis (%d blocks)",
+                            sb, a, g.nb[a]);
+            g.sb[a] = (int)sb;
+            g.T[a] = (int)(8 * sb);
+            for (uint32_t i = 0; i < n[a]; ++i) {
+                const bool in_layer = i < 8 * sb || i >= n[a] - 8 * sb;
+                if (!in_layer && (pp[i] != 1.0f || pp[n[a] + i] != 0.0f || pp[2 * n[a] + i] != 1.0f))
+                    return fail(WF_ERR_INVALID,
+                                "PML profile on axis %d is not identity at %u, outside the "
+                                "graded layer of %u voxels",
+                                a, i, 8 * sb);
+            }
+            pp += 3 * n[a];
+        }
+    }
+
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0)
+        return fail(WF_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= ndev)
+        return fail(WF_ERR_INVALID, "device %d out of range (%d devices)", device, ndev);
+    WF_HIP(hipSetDevice(device));
+
+    wf_ctx *c = new wf_ctx();
+    c->device = device;
+    c->g = g;
+    c->pml = pml_shell_blocks != nullptr;
+    auto bail = [&](int rc) {
+        wf_destroy(c);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(WF_ERR_HIP, "hipStreamCreate failed"));
+
+    const uint64_t N = (uint64_t)nx * ny * nz;
+    if (int rc = dmalloc(&c->d_fields, 6 * N * sizeof(float))) return bail(rc);
+    if (int rc = dmalloc(&c->d_id, N)) return bail(rc);
+    if (int rc = dmalloc(&c->d_coef, 256 * sizeof(float4))) return bail(rc);
+    for (int i = 0; i < 6; ++i) c->p.f[i] = c->d_fields + (uint64_t)i * N;
+    c->p.id = c->d_id;
+    c->p.coef = c->d_coef;
+
+    if (hipMemsetAsync(c->d_fields, 0, 6 * N * sizeof(float), c->stream) != hipSuccess)
+        return bail(fail(WF_ERR_HIP, "hipMemset(fields) failed"));
+    if (material_ids) {
+        if (hipMemcpyAsync(c->d_id, material_ids, N, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return bail(fail(WF_ERR_HIP, "upload of material ids failed"));
+    } else if (hipMemsetAsync(c->d_id, 0, N, c->stream) != hipSuccess) {
+        return bail(fail(WF_ERR_HIP, "hipMemset(ids) failed"));
+    }
+    // Dequantize once: FixedQ16_16::to_f32 (src/layout.rs:123-125).
+    std::vector<float4> coef(256);
+    for (int i = 0; i < 256; ++i)
+        coef[i] = make_float4(wf_q16_to_f32(coeff_table[4 * i + 0]), wf_q16_to_f32(coeff_table[4 * i + 1]),
+                              wf_q16_to_f32(coeff_table[4 * i + 2]), wf_q16_to_f32(coeff_table[4 * i + 3]));
+    if (hipMemcpyAsync(c->d_coef, coef.data(), 256 * sizeof(float4), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess)
+        return bail(fail(WF_ERR_HIP, "upload of coefficient table failed"));
+
+    // Profiles (identity when PML is off) and psi slabs.
+    {
+        std::vector<float> prof(3 * (nx + ny + nz));
+        if (pml_profiles) {
+            std::memcpy(prof.data(), pml_profiles, prof.size() * sizeof(float));
+        } else {
+            float *pp = prof.data();
+            for (int a = 0; a < 3; ++a) {
+                for (uint32_t i = 0; i < n[a]; ++i) {
+                    pp[i] = 1.0f;
+                    pp[n[a] + i] = 0.0f;
+                    pp[2 * n[a] + i] = 1.0f;
+                }
+                pp += 3 * n[a];
+            }
+        }
+        if (int rc = dmalloc(&c->d_prof, prof.size() * sizeof(float))) return bail(rc);
+        if (hipMemcpy(c->d_prof, prof.data(), prof.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(WF_ERR_HIP, "upload of PML profiles failed"));
+        float *dp = c->d_prof;
+        for (int a = 0; a < 3; ++a) {
+            for (int j = 0; j < 3; ++j) c->p.prof[a][j] = dp + j * n[a];
+            dp += 3 * n[a];
+        }
+    }
+    uint64_t slab[3] = {2ull * g.T[0] * ny * nz, 2ull * g.T[1] * nx * nz, 2ull * g.T[2] * nx * ny};
+    const uint64_t psi_total = 4 * (slab[0] + slab[1] + slab[2]);
+    if (psi_total) {
+        if (int rc = dmalloc(&c->d_psi, psi_total * sizeof(float))) return bail(rc);
+        if (hipMemsetAsync(c->d_psi, 0, psi_total * sizeof(float), c->stream) != hipSuccess)
+            return bail(fail(WF_ERR_HIP, "hipMemset(psi) failed"));
+    }
+    {
+        float *pp = c->d_psi;
+        for (int a = 0; a < 3; ++a)
+            for (int t = 0; t < 4; ++t) {
+                c->p.psi[a][t] = slab[a] ? pp : nullptr;
+                pp += slab[a];
+            }
+    }
+    // Shell block list for wf_read_psi.
+    if (c->pml) {
+        std::vector<uint64_t> list;
+        for (int bz = 0; bz < g.nb[2]; ++bz)
+            for (int by = 0; by < g.nb[1]; ++by)
+                for (int bx = 0; bx < g.nb[0]; ++bx) {
+                    const int bc[3] = {bx, by, bz};
+                    bool in = false;
+                    for (int a = 0; a < 3; ++a)
+                        if (g.sb[a] > 0 && (bc[a] < g.sb[a] || bc[a] >= g.nb[a] - g.sb[a])) in = true;
+                    if (in) list.push_back((uint64_t)bx | ((uint64_t)by << 20) | ((uint64_t)bz << 40));
+                }
+        c->nshell_blocks = list.size();
+        if (!list.empty()) {
+            if (int rc = dmalloc(&c->d_shell_list, list.size() * sizeof(uint64_t))) return bail(rc);
+            if (hipMemcpy(c->d_shell_list, list.data(), list.size() * sizeof(uint64_t),
+                          hipMemcpyHostToDevice) != hipSuccess)
+                return bail(fail(WF_ERR_HIP, "upload of shell list failed"));
+        }
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+        return bail(fail(WF_ERR_HIP, "initialisation failed"));
+    if (const char *zc = std::getenv("WF_ZCHUNK")) {
+        const int v = std::atoi(zc);
+        if (v > 0) c->zchunk = v;
+    }
+    *out = c;
+    return WF_OK;
+}
+
+void wf_destroy(wf_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_sources(c);
+    free_probes(c);
+    dfree(c->d_fields);
+    dfree(c->d_id);
+    dfree(c->d_coef);
+    dfree(c->d_prof);
+    dfree(c->d_psi);
+    dfree(c->d_shell_list);
+    dfree(c->d_stage);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int wf_set_sources(wf_ctx *c, uint32_t n, const wf_source_site *sites, uint64_t nsteps,
+                   const float *values)
+{
+    clear_error();
+    if (!c) return fail(WF_ERR_INVALID, "wf_set_sources: null context");
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    free_sources(c);
+    if (n == 0) return WF_OK;
+    if (!sites || (!values && nsteps)) return fail(WF_ERR_INVALID, "wf_set_sources: null pointer");
+    std::vector<DevSource> ds(n);
+    for (uint32_t s = 0; s < n; ++s) {
+        const wf_source_site &q = sites[s];
+        if (q.x >= (uint32_t)c->g.nx || q.y >= (uint32_t)c->g.ny || q.z >= (uint32_t)c->g.nz)
+            return fail(WF_ERR_INVALID, "source %u at (%u, %u, %u) is outside the domain", s, q.x, q.y, q.z);
+        if (q.component < 0 || q.component > 2)
+            return fail(WF_ERR_INVALID, "source %u: component %d is not Ex/Ey/Ez", s, q.component);
+        ds[s].off = ((uint64_t)q.z * c->g.ny + q.y) * c->g.nx + q.x;
+        ds[s].comp = q.component;
+        ds[s].hard = q.hard ? 1 : 0;
+    }
+    if (int rc = dmalloc(&c->d_src, n * sizeof(DevSource))) return rc;
+    if (int rc = dmalloc(&c->d_srcval, nsteps * n * sizeof(float))) return rc;
+    WF_HIP(hipMemcpy(c->d_src, ds.data(), n * sizeof(DevSource), hipMemcpyHostToDevice));
+    if (nsteps) WF_HIP(hipMemcpy(c->d_srcval, values, nsteps * n * sizeof(float), hipMemcpyHostToDevice));
+    c->nsrc = n;
+    c->src_base = c->step;
+    c->src_steps = nsteps;
+    return WF_OK;
+}
+
+int wf_set_probes(wf_ctx *c, uint32_t n, const wf_probe_site *sites, uint64_t nsteps,
+                  const double *cos_tab, const double *sin_tab, const uint8_t *active, int trace)
+{
+    clear_error();
+    if (!c) return fail(WF_ERR_INVALID, "wf_set_probes: null context");
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    free_probes(c);
+    if (n == 0) return WF_OK;
+    if (!sites || (nsteps && !active)) return fail(WF_ERR_INVALID, "wf_set_probes: null pointer");
+    std::vector<DevProbe> dp(n);
+    uint32_t nf = 0;
+    for (uint32_t p = 0; p < n; ++p) {
+        const wf_probe_site &q = sites[p];
+        if (q.x >= (uint32_t)c->g.nx || q.y >= (uint32_t)c->g.ny || q.z >= (uint32_t)c->g.nz)
+            return fail(WF_ERR_INVALID, "probe position (%u, %u, %u) is outside the %dx%dx%d domain",
+                        q.x, q.y, q.z, c->g.nx, c->g.ny, c->g.nz);
+        if (q.component < 0 || q.component > 5)
+            return fail(WF_ERR_INVALID, "probe %u: component %d out of range", p, q.component);
+        dp[p].off = ((uint64_t)q.z * c->g.ny + q.y) * c->g.nx + q.x;
+        dp[p].comp = q.component;
+        dp[p].nfreq = (int)q.nfreq;
+        dp[p].foff = (int)nf;
+        nf += q.nfreq;
+    }
+    if (nf && nsteps && (!cos_tab || !sin_tab))
+        return fail(WF_ERR_INVALID, "wf_set_probes: null twiddle table");
+    if (int rc = dmalloc(&c->d_probe, n * sizeof(DevProbe))) return rc;
+    if (int rc = dmalloc(&c->d_cos, nsteps * nf * sizeof(double))) return rc;
+    if (int rc = dmalloc(&c->d_sin, nsteps * nf * sizeof(double))) return rc;
+    if (int rc = dmalloc(&c->d_sums, 2ull * (nf ? nf : 1) * sizeof(double))) return rc;
+    if (int rc = dmalloc(&c->d_active, nsteps * n)) return rc;
+    if (int rc = dmalloc(&c->d_counts, n * sizeof(unsigned long long))) return rc;
+    if (trace) {
+        if (int rc = dmalloc(&c->d_trace, nsteps * n * sizeof(float))) return rc;
+    }
+    WF_HIP(hipMemcpy(c->d_probe, dp.data(), n * sizeof(DevProbe), hipMemcpyHostToDevice));
+    if (nsteps && nf) {
+        WF_HIP(hipMemcpy(c->d_cos, cos_tab, nsteps * nf * sizeof(double), hipMemcpyHostToDevice));
+        WF_HIP(hipMemcpy(c->d_sin, sin_tab, nsteps * nf * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (nsteps) WF_HIP(hipMemcpy(c->d_active, active, nsteps * n, hipMemcpyHostToDevice));
+    WF_HIP(hipMemset(c->d_sums, 0, 2ull * (nf ? nf : 1) * sizeof(double)));
+    WF_HIP(hipMemset(c->d_counts, 0, n * sizeof(unsigned long long)));
+    c->nprobe = n;
+    c->nfreq = nf;
+    c->probe_base = c->step;
+    c->probe_steps = nsteps;
+    return WF_OK;
+}
+
+int wf_step(wf_ctx *c, uint64_t nsteps)
+{
+    clear_error();
+    if (!c) return fail(WF_ERR_INVALID, "wf_step: null context");
+    if (fault_injected("wf_step")) return fail(WF_ERR_HIP, "fault injected: wf_step");
+    if (c->nsrc && c->step + nsteps > c->src_base + c->src_steps)
+        return fail(WF_ERR_STATE, "wf_step: source table covers %llu steps, %llu requested past step %llu",
+                    (unsigned long long)c->src_steps, (unsigned long long)nsteps,
+                    (unsigned long long)(c->step - c->src_base));
+    if (c->nprobe && c->step + nsteps > c->probe_base + c->probe_steps)
+        return fail(WF_ERR_STATE, "wf_step: probe table covers %llu steps", (unsigned long long)c->probe_steps);
+    WF_HIP(hipSetDevice(c->device));
+    for (uint64_t s = 0; s < nsteps; ++s)
+        if (int rc = launch_step(c)) return rc;
+    return WF_OK;
+}
+
+int wf_sync(wf_ctx *c)
+{
+    if (!c) return fail(WF_ERR_INVALID, "wf_sync: null context");
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    return WF_OK;
+}
+
+uint64_t wf_step_count(const wf_ctx *c) { return c ? c->step : 0; }
+
+uint64_t wf_field_bytes(const wf_ctx *c)
+{
+    return c ? (uint64_t)c->g.nx * c->g.ny * c->g.nz * 24ull : 0;
+}
+
+int wf_read_fields(wf_ctx *c, void *dst)
+{
+    clear_error();
+    if (!c || !dst) return fail(WF_ERR_INVALID, "wf_read_fields: null pointer");
+    if (fault_injected("wf_read_fields")) return fail(WF_ERR_HIP, "fault injected: wf_read_fields");
+    WF_HIP(hipSetDevice(c->device));
+    const uint64_t ppc = planes_per_chunk(c);
+    const uint64_t plane_vox = (uint64_t)c->g.nb[0] * c->g.nb[1] * 512;
+    if (int rc = ensure_stage(c, ppc * plane_vox * 24)) return rc;
+    for (uint64_t bz = 0; bz < (uint64_t)c->g.nb[2]; bz += ppc) {
+        const uint64_t np = std::min<uint64_t>(ppc, c->g.nb[2] - bz);
+        const uint64_t nvox = np * plane_vox;
+        hipLaunchKernelGGL(k_pack, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, c->stream,
+                           c->p, c->g, (int)bz, nvox, c->d_stage);
+        WF_HIP(hipGetLastError());
+        WF_HIP(hipMemcpyAsync(static_cast<char *>(dst) + bz * plane_vox * 24, c->d_stage, nvox * 24,
+                              hipMemcpyDeviceToHost, c->stream));
+        WF_HIP(hipStreamSynchronize(c->stream));
+    }
+    return WF_OK;
+}
+
+int wf_write_fields(wf_ctx *c, const void *src)
+{
+    clear_error();
+    if (!c || !src) return fail(WF_ERR_INVALID, "wf_write_fields: null pointer");
+    WF_HIP(hipSetDevice(c->device));
+    const uint64_t ppc = planes_per_chunk(c);
+    const uint64_t plane_vox = (uint64_t)c->g.nb[0] * c->g.nb[1] * 512;
+    if (int rc = ensure_stage(c, ppc * plane_vox * 24)) return rc;
+    for (uint64_t bz = 0; bz < (uint64_t)c->g.nb[2]; bz += ppc) {
+        const uint64_t np = std::min<uint64_t>(ppc, c->g.nb[2] - bz);
+        const uint64_t nvox = np * plane_vox;
+        WF_HIP(hipMemcpyAsync(c->d_stage, static_cast<const char *>(src) + bz * plane_vox * 24, nvox * 24,
+                              hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_unpack, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, c->stream,
+                           c->p, c->g, (int)bz, nvox, c->d_stage);
+        WF_HIP(hipGetLastError());
+        WF_HIP(hipStreamSynchronize(c->stream));
+    }
+    return WF_OK;
+}
+
+uint64_t wf_psi_shell_blocks(const wf_ctx *c) { return c ? c->nshell_blocks : 0; }
+
+int wf_read_psi(wf_ctx *c, float *dst)
+{
+    clear_error();
+    if (!c || !dst) return fail(WF_ERR_INVALID, "wf_read_psi: null pointer");
+    if (!c->nshell_blocks) return WF_OK;
+    WF_HIP(hipSetDevice(c->device));
+    const uint64_t per = 12ull * 512 * sizeof(float);
+    uint64_t chunk = (256ull << 20) / per;
+    if (int rc = ensure_stage(c, std::min<uint64_t>(chunk, c->nshell_blocks) * per)) return rc;
+    for (uint64_t b = 0; b < c->nshell_blocks; b += chunk) {
+        const uint64_t nb = std::min<uint64_t>(chunk, c->nshell_blocks - b);
+        hipLaunchKernelGGL(k_pack_psi, dim3((unsigned)((nb * 512 + 255) / 256)), dim3(256), 0,
+                           c->stream, c->p, c->g, c->d_shell_list + b, nb, c->d_stage);
+        WF_HIP(hipGetLastError());
+        WF_HIP(hipMemcpyAsync(reinterpret_cast<char *>(dst) + b * per, c->d_stage, nb * per,
+                              hipMemcpyDeviceToHost, c->stream));
+        WF_HIP(hipStreamSynchronize(c->stream));
+    }
+    return WF_OK;
+}
+
+int wf_read_probe_sums(wf_ctx *c, double *sum_cos, double *sum_sin, uint64_t *n)
+{
+    clear_error();
+    if (!c) return fail(WF_ERR_INVALID, "wf_read_probe_sums: null context");
+    if (!c->nprobe) return WF_OK;
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    if (c->nfreq) {
+        WF_HIP(hipMemcpy(sum_cos, c->d_sums, c->nfreq * sizeof(double), hipMemcpyDeviceToHost));
+        WF_HIP(hipMemcpy(sum_sin, c->d_sums + c->nfreq, c->nfreq * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    static_assert(sizeof(unsigned long long) == sizeof(uint64_t), "count width");
+    WF_HIP(hipMemcpy(n, c->d_counts, c->nprobe * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return WF_OK;
+}
+
+int wf_read_trace(wf_ctx *c, float *trace, uint64_t max_steps)
+{
+    clear_error();
+    if (!c || !trace) return fail(WF_ERR_INVALID, "wf_read_trace: null pointer");
+    if (!c->d_trace) return fail(WF_ERR_STATE, "wf_read_trace: probes were set without a trace");
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    uint64_t k = c->step - c->probe_base;
+    if (k > max_steps) k = max_steps;
+    if (k) WF_HIP(hipMemcpy(trace, c->d_trace, k * c->nprobe * sizeof(float), hipMemcpyDeviceToHost));
+    return WF_OK;
+}
+
+int wf_time_kernels(wf_ctx *c, uint64_t nsteps, double *h_ms, double *e_ms)
+{
+    clear_error();
+    if (!c || !h_ms || !e_ms) return fail(WF_ERR_INVALID, "wf_time_kernels: null pointer");
+    if (c->nsrc && c->step + nsteps > c->src_base + c->src_steps)
+        return fail(WF_ERR_STATE, "wf_time_kernels: source table too short");
+    if (c->nprobe && c->step + nsteps > c->probe_base + c->probe_steps)
+        return fail(WF_ERR_STATE, "wf_time_kernels: probe table too short");
+    WF_HIP(hipSetDevice(c->device));
+    hipEvent_t ev[4];
+    for (auto &e : ev) WF_HIP(hipEventCreate(&e));
+    double hs = 0.0, es = 0.0;
+    int rc = WF_OK;
+    for (uint64_t s = 0; s < nsteps && rc == WF_OK; ++s) {
+        rc = launch_step(c, ev);
+        if (rc) break;
+        WF_HIP(hipEventSynchronize(ev[3]));
+        float a = 0.f, b = 0.f;
+        WF_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        WF_HIP(hipEventElapsedTime(&b, ev[2], ev[3]));
+        hs += a;
+        es += b;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    *h_ms = nsteps ? hs / (double)nsteps : 0.0;
+    *e_ms = nsteps ? es / (double)nsteps : 0.0;
+    return rc;
+}
+
+int wf_phase_bytes(const wf_ctx *c, uint64_t *h_bytes, uint64_t *e_bytes)
+{
+    if (!c || !h_bytes || !e_bytes) return fail(WF_ERR_INVALID, "wf_phase_bytes: null pointer");
+    const uint64_t nx = c->g.nx, ny = c->g.ny, nz = c->g.nz, N = nx * ny * nz;
+    // non-identity psi terms per phase: 2 per graded axis per voxel
+    const uint64_t terms = 2ull * (2ull * c->g.T[0] * ny * nz + 2ull * c->g.T[1] * nx * nz +
+                                   2ull * c->g.T[2] * nx * ny);
+    *h_bytes = 37ull * N + 8ull * terms;
+    *e_bytes = 37ull * N + 8ull * terms;
+    return WF_OK;
+}
+
+int wf_time_steps(wf_ctx *c, uint64_t nsteps, double *total_ms)
+{
+    clear_error();
+    if (!c || !total_ms) return fail(WF_ERR_INVALID, "wf_time_steps: null pointer");
+    WF_HIP(hipSetDevice(c->device));
+    hipEvent_t e0, e1;
+    WF_HIP(hipEventCreate(&e0));
+    WF_HIP(hipEventCreate(&e1));
+    WF_HIP(hipEventRecord(e0, c->stream));
+    int rc = wf_step(c, nsteps);
+    WF_HIP(hipEventRecord(e1, c->stream));
+    WF_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    WF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *total_ms = ms;
+    return rc;
+}
+
+}  // extern "C"
