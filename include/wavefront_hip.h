@@ -208,10 +208,17 @@ int wf_time_steps(wf_ctx *ctx, uint64_t nsteps, double *total_ms);
  * E-phase launch (same stream); writes the mean per-launch milliseconds of
  * each phase kernel. */
 int wf_time_kernels(wf_ctx *ctx, uint64_t nsteps, double *h_ms, double *e_ms);
-/* Algorithmic HBM bytes of one launch of each phase kernel: per cell 12 B
- * read of the curl field, 12 B read + 12 B write of the updated field, 1 B
- * material id; plus 8 B (read + write) per non-identity psi term. */
+/* Algorithmic HBM bytes of one launch of each phase kernel.  Split variant
+ * (separate H and E kernels, in place): per cell 12 B read of the curl
+ * field, 12 B read + 12 B write of the updated field, 1 B material id, plus
+ * 8 B (read + write) per non-identity psi term of that phase.  Fused variant
+ * (one kernel per step, reported in h_bytes, e_bytes = 0): 24 B read + 24 B
+ * write + 1 B id per cell plus 8 B per psi term of both phases. */
 int wf_phase_bytes(const wf_ctx *ctx, uint64_t *h_bytes, uint64_t *e_bytes);
+/* "fused" (default: one single-pass kernel per step, ping-pong state) or
+ * "split" (H kernel then E kernel, in place); chosen at wf_create from the
+ * WF_KERNEL environment variable. */
+const char *wf_kernel_variant(const wf_ctx *ctx);
 
 #ifdef __cplusplus
 }

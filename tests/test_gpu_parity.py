@@ -18,8 +18,16 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
+@pytest.fixture(autouse=True, params=["fused", "split"])
+def kernel_variant(request, monkeypatch):
+    """Every case runs on both step kernels: the single-pass fused kernel and
+    the separate in-place H and E kernels."""
+    monkeypatch.setenv("WF_KERNEL", request.param)
+    return request.param
+
+
 def _say(msg):
-    print(f"[gpu-case] {msg}", flush=True)
+    print(f"[gpu-case] {msg} kernel={os.environ.get('WF_KERNEL')}", flush=True)
 
 
 def _run_workload_gpu(w, steps):

@@ -85,6 +85,7 @@ _SIGNATURES = {
     "wf_time_steps": (c_int, [c_void_p, c_uint64, POINTER(c_double)]),
     "wf_time_kernels": (c_int, [c_void_p, c_uint64, POINTER(c_double), POINTER(c_double)]),
     "wf_phase_bytes": (c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64)]),
+    "wf_kernel_variant": (c_char_p, [c_void_p]),
 }
 
 _lib = None

@@ -322,6 +322,274 @@ __global__ __launch_bounds__(kTX *kTY) void k_update_e(KArgs k, int zchunk)
 }
 
 // ---------------------------------------------------------------------------
+// Fused single-pass step: H(n+1/2) and E(n+1) in one sweep.
+//
+// Reads state A (E^n, H^{n-1/2}, psi) and writes state B; the engine swaps
+// A and B after every step.  Per cell that is 24 B read + 24 B written + 1 B
+// id instead of the two-pass 74 B, with identical arithmetic: every H value
+// is computed exactly as update_h_field[_pml] computes it, then every E
+// value exactly as update_e_field[_pml] does from those H values.
+//
+// Workgroup = 16 waves x 64 lanes.  Lane l of wave w holds the column
+// x = bx*63 - 1 + l, y = by*15 - 1 + w: lane 0 and wave 0 are a one-cell
+// halo whose H(n+1/2) the E update of the owned 63 x 15 cells needs at x-1
+// and y-1.  Halo cells recompute H from state A (bit-identical to the owner's
+// value) but never store.  The sweep marches z over a chunk, starting one
+// plane early for H(z0-1); E(z+1) and H(z-1) ride in registers, H at x-1
+// comes from the neighbouring lane and H at y-1 through LDS.
+// ---------------------------------------------------------------------------
+constexpr int kFW = 64;
+constexpr int kFY = 16;
+constexpr int kFOX = kFW - 1;
+constexpr int kFOY = kFY - 1;
+
+struct FArgs {
+    Geo g;
+    Ptrs a;  // state n (read)
+    Ptrs b;  // state n+1 (written)
+};
+
+template <bool PML>
+__global__ __launch_bounds__(kFW *kFY) void k_fused(FArgs k, int zchunk)
+{
+    __shared__ float4 s_co[256];             // (ca, cb, da, db)
+    __shared__ float2 s_h[2][kFY][kFW];      // (hz, hx) of H(n+1/2), double-buffered by plane
+    const int lane = threadIdx.x, w = threadIdx.y;
+    for (int i = w * kFW + lane; i < 256; i += kFW * kFY) s_co[i] = k.a.coef[i];
+    __syncthreads();
+
+    const Geo &g = k.g;
+    const size_t nx = g.nx, plane = (size_t)g.nx * g.ny;
+    const int xr = blockIdx.x * kFOX - 1 + lane;
+    const int yr = blockIdx.y * kFOY - 1 + w;
+    // Halo coordinates map through the boundary: -1 wraps (periodic) or
+    // stands for the zero field outside the domain.
+    int x = xr, y = yr;
+    bool live = true;
+    if (xr < 0) {
+        if (g.periodic & 1u) x = g.nx - 1; else live = false;
+    }
+    if (yr < 0) {
+        if (g.periodic & 2u) y = g.ny - 1; else live = false;
+    }
+    if (xr >= g.nx || yr >= g.ny) live = false;
+    const bool own = live && lane >= 1 && w >= 1;
+    if (!live) {
+        x = 0;
+        y = 0;
+    }
+    const int xp = plus1(x, g.nx, g.periodic & 1u);
+    const int yp = plus1(y, g.ny, g.periodic & 2u);
+
+    bool shell_xy = false, gx = false, gy = false;
+    int gxi = 0, gyi = 0;
+    float pxb = 1.f, pxa = 0.f, pxk = 1.f, pyb = 1.f, pya = 0.f, pyk = 1.f;
+    if (PML) {
+        shell_xy = in_shell_axis(g, 0, x) || in_shell_axis(g, 1, y);
+        gx = graded(g, 0, x, g.nx);
+        gy = graded(g, 1, y, g.ny);
+        gxi = slab_index(g, 0, x, g.nx);
+        gyi = slab_index(g, 1, y, g.ny);
+        pxb = k.a.prof[0][0][x];
+        pxa = k.a.prof[0][1][x];
+        pxk = k.a.prof[0][2][x];
+        pyb = k.a.prof[1][0][y];
+        pya = k.a.prof[1][1][y];
+        pyk = k.a.prof[1][2][y];
+    }
+
+    const float *__restrict__ aex = k.a.f[0];
+    const float *__restrict__ aey = k.a.f[1];
+    const float *__restrict__ aez = k.a.f[2];
+    const float *__restrict__ ahx = k.a.f[3];
+    const float *__restrict__ ahy = k.a.f[4];
+    const float *__restrict__ ahz = k.a.f[5];
+    const size_t row = (size_t)y * nx + x;
+    const size_t rowp_x = (size_t)y * nx + (xp >= 0 ? xp : 0);
+    const size_t rowp_y = (size_t)(yp >= 0 ? yp : 0) * nx + x;
+
+    const int z0 = blockIdx.z * zchunk;
+    const int z1 = min(z0 + zchunk, g.nz);
+    const int zpre = minus1(z0, g.nz, g.periodic & 4u);
+
+    // E^n at the current plane (own column) and H^{n+1/2} one plane below.
+    float ex_c = 0.f, ey_c = 0.f, ez_c = 0.f;
+    float hx_mz = 0.f, hy_mz = 0.f;
+    {
+        const int zc = zpre >= 0 ? zpre : z0;
+        if (live) {
+            const size_t o = (size_t)zc * plane + row;
+            ex_c = aex[o];
+            ey_c = aey[o];
+            ez_c = aez[o];
+        }
+    }
+
+    for (int it = (zpre >= 0 ? -1 : 0); it < z1 - z0; ++it) {
+        const int z = it < 0 ? zpre : z0 + it;
+        const int zp = plus1(z, g.nz, g.periodic & 4u);
+        const size_t zo = (size_t)z * plane;
+        const bool store = own && it >= 0;
+
+        // ---- loads (state A) ------------------------------------------------
+        float ex_n = 0.f, ey_n = 0.f, ez_n = 0.f;
+        float ez_px = 0.f, ey_px = 0.f, ez_py = 0.f, ex_py = 0.f;
+        float hx = 0.f, hy = 0.f, hz = 0.f;
+        int mid = 0;
+        if (live) {
+            if (zp >= 0) {
+                const size_t on = (size_t)zp * plane + row;
+                ex_n = aex[on];
+                ey_n = aey[on];
+                ez_n = aez[on];
+            }
+            if (xp >= 0) {
+                ez_px = aez[zo + rowp_x];
+                ey_px = aey[zo + rowp_x];
+            }
+            if (yp >= 0) {
+                ez_py = aez[zo + rowp_y];
+                ex_py = aex[zo + rowp_y];
+            }
+            hx = ahx[zo + row];
+            hy = ahy[zo + row];
+            hz = ahz[zo + row];
+            mid = k.a.id[zo + row];
+        }
+        const float4 co = s_co[mid];
+
+        // ---- H(n+1/2): update_h_field[_pml] --------------------------------
+        float d_ez_dy = ez_py - ez_c, d_ex_dy = ex_py - ex_c;
+        float d_ey_dz = ey_n - ey_c, d_ex_dz = ex_n - ex_c;
+        float d_ez_dx = ez_px - ez_c, d_ey_dx = ey_px - ey_c;
+        const bool shell = PML && (shell_xy || in_shell_axis(g, 2, z));
+        const bool gz = PML && graded(g, 2, z, g.nz);
+        size_t ipy = 0, ipz = 0, ipx = 0;
+        float pzb = 1.f, pza = 0.f, pzk = 1.f;
+        if (PML && shell && live) {
+            ipy = ((size_t)z * (2 * g.T[1]) + gyi) * nx + x;
+            ipx = ((size_t)z * g.ny + y) * (2 * g.T[0]) + gxi;
+            if (gz) {
+                ipz = ((size_t)slab_index(g, 2, z, g.nz) * g.ny + y) * nx + x;
+                pzb = k.a.prof[2][0][z];
+                pza = k.a.prof[2][1][z];
+                pzk = k.a.prof[2][2][z];
+            }
+            if (gy) {
+                float p0 = k.a.psi[1][0][ipy], p1 = k.a.psi[1][1][ipy];
+                d_ez_dy = cpml(d_ez_dy, pyb, pya, pyk, p0);
+                d_ex_dy = cpml(d_ex_dy, pyb, pya, pyk, p1);
+                if (store) {
+                    k.b.psi[1][0][ipy] = p0;
+                    k.b.psi[1][1][ipy] = p1;
+                }
+            } else {
+                d_ez_dy = d_ez_dy + 0.0f;
+                d_ex_dy = d_ex_dy + 0.0f;
+            }
+            if (gz) {
+                float p0 = k.a.psi[2][0][ipz], p1 = k.a.psi[2][1][ipz];
+                d_ey_dz = cpml(d_ey_dz, pzb, pza, pzk, p0);
+                d_ex_dz = cpml(d_ex_dz, pzb, pza, pzk, p1);
+                if (store) {
+                    k.b.psi[2][0][ipz] = p0;
+                    k.b.psi[2][1][ipz] = p1;
+                }
+            } else {
+                d_ey_dz = d_ey_dz + 0.0f;
+                d_ex_dz = d_ex_dz + 0.0f;
+            }
+            if (gx) {
+                float p0 = k.a.psi[0][0][ipx], p1 = k.a.psi[0][1][ipx];
+                d_ez_dx = cpml(d_ez_dx, pxb, pxa, pxk, p0);
+                d_ey_dx = cpml(d_ey_dx, pxb, pxa, pxk, p1);
+                if (store) {
+                    k.b.psi[0][0][ipx] = p0;
+                    k.b.psi[0][1][ipx] = p1;
+                }
+            } else {
+                d_ez_dx = d_ez_dx + 0.0f;
+                d_ey_dx = d_ey_dx + 0.0f;
+            }
+        }
+        float hxn = co.z * hx - co.w * (d_ez_dy - d_ey_dz);
+        float hyn = co.z * hy - co.w * (d_ex_dz - d_ez_dx);
+        float hzn = co.z * hz - co.w * (d_ey_dx - d_ex_dy);
+        if (!live) {
+            hxn = 0.f;
+            hyn = 0.f;
+            hzn = 0.f;
+        }
+
+        if (it >= 0) {
+            // ---- neighbours of H(n+1/2) at x-1 (lane-1) and y-1 (LDS) -------
+            const float hz_mx = __shfl_up(hzn, 1, kFW);
+            const float hy_mx = __shfl_up(hyn, 1, kFW);
+            const int buf = it & 1;
+            s_h[buf][w][lane] = make_float2(hzn, hxn);
+            __syncthreads();
+            float hz_my = 0.f, hx_my = 0.f;
+            if (w > 0) {
+                const float2 m = s_h[buf][w - 1][lane];
+                hz_my = m.x;
+                hx_my = m.y;
+            }
+
+            if (store) {
+                // ---- E(n+1): update_e_field[_pml] ---------------------------
+                float d_hz_dy = hzn - hz_my, d_hx_dy = hxn - hx_my;
+                float d_hy_dz = hyn - hy_mz, d_hx_dz = hxn - hx_mz;
+                float d_hz_dx = hzn - hz_mx, d_hy_dx = hyn - hy_mx;
+                if (PML && shell) {
+                    if (gy) {
+                        float p0 = k.a.psi[1][2][ipy], p1 = k.a.psi[1][3][ipy];
+                        d_hz_dy = cpml(d_hz_dy, pyb, pya, pyk, p0);
+                        d_hx_dy = cpml(d_hx_dy, pyb, pya, pyk, p1);
+                        k.b.psi[1][2][ipy] = p0;
+                        k.b.psi[1][3][ipy] = p1;
+                    } else {
+                        d_hz_dy = d_hz_dy + 0.0f;
+                        d_hx_dy = d_hx_dy + 0.0f;
+                    }
+                    if (gz) {
+                        float p0 = k.a.psi[2][2][ipz], p1 = k.a.psi[2][3][ipz];
+                        d_hy_dz = cpml(d_hy_dz, pzb, pza, pzk, p0);
+                        d_hx_dz = cpml(d_hx_dz, pzb, pza, pzk, p1);
+                        k.b.psi[2][2][ipz] = p0;
+                        k.b.psi[2][3][ipz] = p1;
+                    } else {
+                        d_hy_dz = d_hy_dz + 0.0f;
+                        d_hx_dz = d_hx_dz + 0.0f;
+                    }
+                    if (gx) {
+                        float p0 = k.a.psi[0][2][ipx], p1 = k.a.psi[0][3][ipx];
+                        d_hz_dx = cpml(d_hz_dx, pxb, pxa, pxk, p0);
+                        d_hy_dx = cpml(d_hy_dx, pxb, pxa, pxk, p1);
+                        k.b.psi[0][2][ipx] = p0;
+                        k.b.psi[0][3][ipx] = p1;
+                    } else {
+                        d_hz_dx = d_hz_dx + 0.0f;
+                        d_hy_dx = d_hy_dx + 0.0f;
+                    }
+                }
+                const size_t o = zo + row;
+                k.b.f[0][o] = co.x * ex_c + co.y * (d_hz_dy - d_hy_dz);
+                k.b.f[1][o] = co.x * ey_c + co.y * (d_hx_dz - d_hz_dx);
+                k.b.f[2][o] = co.x * ez_c + co.y * (d_hy_dx - d_hx_dy);
+                k.b.f[3][o] = hxn;
+                k.b.f[4][o] = hyn;
+                k.b.f[5][o] = hzn;
+            }
+        }
+        hx_mz = hxn;
+        hy_mz = hyn;
+        ex_c = ex_n;
+        ey_c = ey_n;
+        ez_c = ez_n;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Sources (src/source.rs:79-95): applied in list order by one lane.
 // ---------------------------------------------------------------------------
 __global__ void k_sources(Ptrs p, const DevSource *__restrict__ src, int n,
@@ -440,11 +708,16 @@ struct wf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     Geo g{};
-    Ptrs p{};
+    Ptrs p{};     // current state (fields + psi); sources, probes, I/O use it
+    Ptrs alt{};   // second state for the fused kernel (written, then swapped)
     bool pml = false;
+    bool fused = true;
     int zchunk = 16;
+    int fzchunk = 64;
 
     float *d_fields = nullptr;
+    float *d_fields2 = nullptr;
+    float *d_psi2 = nullptr;
     uint8_t *d_id = nullptr;
     float4 *d_coef = nullptr;
     float *d_prof = nullptr;
@@ -528,18 +801,34 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
     const dim3 grd((c->g.nx + kTX - 1) / kTX, (c->g.ny + kTY - 1) / kTY,
                    (c->g.nz + c->zchunk - 1) / c->zchunk);
     const KArgs k = kargs(c);
-    if (ev) WF_HIP(hipEventRecord(ev[0], c->stream));
-    if (c->pml)
-        hipLaunchKernelGGL(k_update_h<true>, grd, blk, 0, c->stream, k, c->zchunk);
-    else
-        hipLaunchKernelGGL(k_update_h<false>, grd, blk, 0, c->stream, k, c->zchunk);
-    if (ev) WF_HIP(hipEventRecord(ev[1], c->stream));
-    if (ev) WF_HIP(hipEventRecord(ev[2], c->stream));
-    if (c->pml)
-        hipLaunchKernelGGL(k_update_e<true>, grd, blk, 0, c->stream, k, c->zchunk);
-    else
-        hipLaunchKernelGGL(k_update_e<false>, grd, blk, 0, c->stream, k, c->zchunk);
-    if (ev) WF_HIP(hipEventRecord(ev[3], c->stream));
+    if (c->fused) {
+        const FArgs f{c->g, c->p, c->alt};
+        const dim3 fblk(kFW, kFY, 1);
+        const dim3 fgrd((c->g.nx + kFOX - 1) / kFOX, (c->g.ny + kFOY - 1) / kFOY,
+                        (c->g.nz + c->fzchunk - 1) / c->fzchunk);
+        if (ev) WF_HIP(hipEventRecord(ev[0], c->stream));
+        if (c->pml)
+            hipLaunchKernelGGL(k_fused<true>, fgrd, fblk, 0, c->stream, f, c->fzchunk);
+        else
+            hipLaunchKernelGGL(k_fused<false>, fgrd, fblk, 0, c->stream, f, c->fzchunk);
+        if (ev) WF_HIP(hipEventRecord(ev[1], c->stream));
+        if (ev) WF_HIP(hipEventRecord(ev[2], c->stream));
+        if (ev) WF_HIP(hipEventRecord(ev[3], c->stream));
+        std::swap(c->p, c->alt);
+    } else {
+        if (ev) WF_HIP(hipEventRecord(ev[0], c->stream));
+        if (c->pml)
+            hipLaunchKernelGGL(k_update_h<true>, grd, blk, 0, c->stream, k, c->zchunk);
+        else
+            hipLaunchKernelGGL(k_update_h<false>, grd, blk, 0, c->stream, k, c->zchunk);
+        if (ev) WF_HIP(hipEventRecord(ev[1], c->stream));
+        if (ev) WF_HIP(hipEventRecord(ev[2], c->stream));
+        if (c->pml)
+            hipLaunchKernelGGL(k_update_e<true>, grd, blk, 0, c->stream, k, c->zchunk);
+        else
+            hipLaunchKernelGGL(k_update_e<false>, grd, blk, 0, c->stream, k, c->zchunk);
+        if (ev) WF_HIP(hipEventRecord(ev[3], c->stream));
+    }
     if (c->nsrc) {
         hipLaunchKernelGGL(k_sources, dim3(1), dim3(64), 0, c->stream, c->p, c->d_src,
                            (int)c->nsrc, c->d_srcval, c->step - c->src_base);
@@ -653,13 +942,24 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(WF_ERR_HIP, "hipStreamCreate failed"));
 
+    if (const char *kv = std::getenv("WF_KERNEL")) {
+        if (std::strcmp(kv, "split") == 0) c->fused = false;
+        else if (std::strcmp(kv, "fused") == 0) c->fused = true;
+        else return bail(fail(WF_ERR_INVALID, "WF_KERNEL must be 'fused' or 'split' (got '%s')", kv));
+    }
     const uint64_t N = (uint64_t)nx * ny * nz;
     if (int rc = dmalloc(&c->d_fields, 6 * N * sizeof(float))) return bail(rc);
+    if (c->fused) {
+        if (int rc = dmalloc(&c->d_fields2, 6 * N * sizeof(float))) return bail(rc);
+        for (int i = 0; i < 6; ++i) c->alt.f[i] = c->d_fields2 + (uint64_t)i * N;
+        if (hipMemsetAsync(c->d_fields2, 0, 6 * N * sizeof(float), c->stream) != hipSuccess)
+            return bail(fail(WF_ERR_HIP, "hipMemset(fields) failed"));
+    }
     if (int rc = dmalloc(&c->d_id, N)) return bail(rc);
     if (int rc = dmalloc(&c->d_coef, 256 * sizeof(float4))) return bail(rc);
     for (int i = 0; i < 6; ++i) c->p.f[i] = c->d_fields + (uint64_t)i * N;
-    c->p.id = c->d_id;
-    c->p.coef = c->d_coef;
+    c->p.id = c->alt.id = c->d_id;
+    c->p.coef = c->alt.coef = c->d_coef;
 
     if (hipMemsetAsync(c->d_fields, 0, 6 * N * sizeof(float), c->stream) != hipSuccess)
         return bail(fail(WF_ERR_HIP, "hipMemset(fields) failed"));
@@ -699,7 +999,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
             return bail(fail(WF_ERR_HIP, "upload of PML profiles failed"));
         float *dp = c->d_prof;
         for (int a = 0; a < 3; ++a) {
-            for (int j = 0; j < 3; ++j) c->p.prof[a][j] = dp + j * n[a];
+            for (int j = 0; j < 3; ++j) c->p.prof[a][j] = c->alt.prof[a][j] = dp + j * n[a];
             dp += 3 * n[a];
         }
     }
@@ -709,13 +1009,20 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
         if (int rc = dmalloc(&c->d_psi, psi_total * sizeof(float))) return bail(rc);
         if (hipMemsetAsync(c->d_psi, 0, psi_total * sizeof(float), c->stream) != hipSuccess)
             return bail(fail(WF_ERR_HIP, "hipMemset(psi) failed"));
+        if (c->fused) {
+            if (int rc = dmalloc(&c->d_psi2, psi_total * sizeof(float))) return bail(rc);
+            if (hipMemsetAsync(c->d_psi2, 0, psi_total * sizeof(float), c->stream) != hipSuccess)
+                return bail(fail(WF_ERR_HIP, "hipMemset(psi) failed"));
+        }
     }
     {
-        float *pp = c->d_psi;
+        float *pp = c->d_psi, *pp2 = c->d_psi2;
         for (int a = 0; a < 3; ++a)
             for (int t = 0; t < 4; ++t) {
                 c->p.psi[a][t] = slab[a] ? pp : nullptr;
+                c->alt.psi[a][t] = (slab[a] && pp2) ? pp2 : nullptr;
                 pp += slab[a];
+                if (pp2) pp2 += slab[a];
             }
     }
     // Shell block list for wf_read_psi.
@@ -742,7 +1049,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
         return bail(fail(WF_ERR_HIP, "initialisation failed"));
     if (const char *zc = std::getenv("WF_ZCHUNK")) {
         const int v = std::atoi(zc);
-        if (v > 0) c->zchunk = v;
+        if (v > 0) c->zchunk = c->fzchunk = v;
     }
     *out = c;
     return WF_OK;
@@ -756,6 +1063,8 @@ void wf_destroy(wf_ctx *c)
     free_sources(c);
     free_probes(c);
     dfree(c->d_fields);
+    dfree(c->d_fields2);
+    dfree(c->d_psi2);
     dfree(c->d_id);
     dfree(c->d_coef);
     dfree(c->d_prof);
@@ -1004,6 +1313,11 @@ int wf_time_kernels(wf_ctx *c, uint64_t nsteps, double *h_ms, double *e_ms)
     return rc;
 }
 
+const char *wf_kernel_variant(const wf_ctx *c)
+{
+    return !c ? "" : (c->fused ? "fused" : "split");
+}
+
 int wf_phase_bytes(const wf_ctx *c, uint64_t *h_bytes, uint64_t *e_bytes)
 {
     if (!c || !h_bytes || !e_bytes) return fail(WF_ERR_INVALID, "wf_phase_bytes: null pointer");
@@ -1011,8 +1325,14 @@ int wf_phase_bytes(const wf_ctx *c, uint64_t *h_bytes, uint64_t *e_bytes)
     // non-identity psi terms per phase: 2 per graded axis per voxel
     const uint64_t terms = 2ull * (2ull * c->g.T[0] * ny * nz + 2ull * c->g.T[1] * nx * nz +
                                    2ull * c->g.T[2] * nx * ny);
-    *h_bytes = 37ull * N + 8ull * terms;
-    *e_bytes = 37ull * N + 8ull * terms;
+    if (c->fused) {
+        // one launch: 24 B read + 24 B write of E and H, 1 B id per cell
+        *h_bytes = 49ull * N + 16ull * terms;
+        *e_bytes = 0;
+    } else {
+        *h_bytes = 37ull * N + 8ull * terms;
+        *e_bytes = 37ull * N + 8ull * terms;
+    }
     return WF_OK;
 }
 

@@ -136,6 +136,9 @@ class DeviceEngine:
         check(lib().wf_time_kernels(self._h, nsteps, byref(h), byref(e)))
         return h.value, e.value
 
+    def kernel_variant(self) -> str:
+        return lib().wf_kernel_variant(self._h).decode()
+
     def phase_bytes(self):
         h, e = c_uint64(), c_uint64()
         check(lib().wf_phase_bytes(self._h, byref(h), byref(e)))
