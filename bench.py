@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-gib", type=float, default=2.0,
+                    help="size of the plain copy that measures the HBM ceiling (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -74,7 +76,12 @@ def main():
         sys.exit(2)
 
     from wavefront_amd import workloads
-    from wavefront_amd.engine import DeviceEngine
+    from wavefront_amd.engine import DeviceEngine, hbm_copy_gbs
+
+    copy_gbs = None
+    if args.copy_gib > 0:
+        copy_gbs = hbm_copy_gbs(int(args.copy_gib * (1 << 30)))
+        log(f"plain copy kernel: {copy_gbs:.0f} GB/s (read + write)")
 
     t0 = time.perf_counter()
     total = args.warmup + args.steps + args.kernel_steps
@@ -157,6 +164,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
+            "copy_kernel_gbs": None if copy_gbs is None else round(copy_gbs, 1),
         },
         "cpu_baseline": cpu,
     }

@@ -219,6 +219,11 @@ int wf_phase_bytes(const wf_ctx *ctx, uint64_t *h_bytes, uint64_t *e_bytes);
  * "split" (H kernel then E kernel, in place); chosen at wf_create from the
  * WF_KERNEL environment variable. */
 const char *wf_kernel_variant(const wf_ctx *ctx);
+/* Copies `bytes` (read + write = 2*bytes of HBM traffic) iters times with a
+ * plain 16-byte-per-lane copy kernel on `device`; writes the mean ms per
+ * copy.  No counterpart in the reference: the measured HBM ceiling and the
+ * scale for the FETCH_SIZE / WRITE_SIZE counters. */
+int wf_hbm_copy(uint64_t bytes, int device, int iters, double *ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,8 @@ import os
 from ctypes import (POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_uint8,
                     c_uint32, c_uint64, c_void_p)
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libwavefront_hip.so")
+LIB_PATH = os.environ.get("WF_LIB_PATH") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "libwavefront_hip.so")
 
 WF_OK = 0
 WF_ERR_INVALID = -1
@@ -86,6 +87,7 @@ _SIGNATURES = {
     "wf_time_kernels": (c_int, [c_void_p, c_uint64, POINTER(c_double), POINTER(c_double)]),
     "wf_phase_bytes": (c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64)]),
     "wf_kernel_variant": (c_char_p, [c_void_p]),
+    "wf_hbm_copy": (c_int, [c_uint64, c_int, c_int, POINTER(c_double)]),
 }
 
 _lib = None

@@ -27,566 +27,16 @@ using namespace wf;
 
 namespace {
 
-constexpr int kTX = 64;  // threads along x (one wavefront)
-constexpr int kTY = 4;   // rows per workgroup
-
 // ---------------------------------------------------------------------------
-// Per-thread geometry helpers
+// Plain device copy of known size: the practical HBM ceiling and the
+// reference point that scales the FETCH_SIZE / WRITE_SIZE counters.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool in_shell_axis(const Geo &g, int ax, int c)
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src,
+                                              float4 *__restrict__ dst, uint64_t n4)
 {
-    const int b = c >> 3;
-    return g.sb[ax] > 0 && (b < g.sb[ax] || b >= g.nb[ax] - g.sb[ax]);
-}
-
-__device__ __forceinline__ bool graded(const Geo &g, int ax, int c, int n)
-{
-    return g.T[ax] > 0 && (c < g.T[ax] || c >= n - g.T[ax]);
-}
-
-__device__ __forceinline__ int slab_index(const Geo &g, int ax, int c, int n)
-{
-    return c < g.T[ax] ? c : c - (n - 2 * g.T[ax]);
-}
-
-// Neighbour coordinate c+1 / c-1 on an axis of length n: -1 = outside the
-// domain (reads the zero boundary, OUTER_ZERO_BLOCK in src/engine.rs:83).
-__device__ __forceinline__ int plus1(int c, int n, bool wrap)
-{
-    return c + 1 < n ? c + 1 : (wrap ? 0 : -1);
-}
-
-__device__ __forceinline__ int minus1(int c, int n, bool wrap)
-{
-    return c > 0 ? c - 1 : (wrap ? n - 1 : -1);
-}
-
-// ---------------------------------------------------------------------------
-// H update: update_h_field / update_h_field_pml (src/fdtd.rs:194-345).
-// One thread per (x, y) column, marching z over a chunk; E(z+1) is carried
-// in registers as the next iteration's E(z).
-// ---------------------------------------------------------------------------
-template <bool PML>
-__global__ __launch_bounds__(kTX *kTY) void k_update_h(KArgs k, int zchunk)
-{
-    __shared__ float2 s_co[256];  // (da, db)
-    const int tid = threadIdx.y * kTX + threadIdx.x;
-    for (int i = tid; i < 256; i += kTX * kTY) {
-        const float4 c = k.p.coef[i];
-        s_co[i] = make_float2(c.z, c.w);
-    }
-    __syncthreads();
-
-    const Geo &g = k.g;
-    const int x = blockIdx.x * kTX + threadIdx.x;
-    const int y = blockIdx.y * kTY + threadIdx.y;
-    if (x >= g.nx || y >= g.ny) return;
-    const int z0 = blockIdx.z * zchunk;
-    const int z1 = min(z0 + zchunk, g.nz);
-    const size_t nx = g.nx, plane = (size_t)g.nx * g.ny;
-
-    float *__restrict__ ex = k.p.f[0];
-    float *__restrict__ ey = k.p.f[1];
-    float *__restrict__ ez = k.p.f[2];
-    float *__restrict__ hx = k.p.f[3];
-    float *__restrict__ hy = k.p.f[4];
-    float *__restrict__ hz = k.p.f[5];
-
-    const int xp = plus1(x, g.nx, g.periodic & 1u);
-    const int yp = plus1(y, g.ny, g.periodic & 2u);
-
-    bool shell_xy = false, gx = false, gy = false;
-    int gxi = 0, gyi = 0;
-    float pxb = 1.f, pxa = 0.f, pxk = 1.f, pyb = 1.f, pya = 0.f, pyk = 1.f;
-    if (PML) {
-        shell_xy = in_shell_axis(g, 0, x) || in_shell_axis(g, 1, y);
-        gx = graded(g, 0, x, g.nx);
-        gy = graded(g, 1, y, g.ny);
-        gxi = slab_index(g, 0, x, g.nx);
-        gyi = slab_index(g, 1, y, g.ny);
-        pxb = k.p.prof[0][0][x];
-        pxa = k.p.prof[0][1][x];
-        pxk = k.p.prof[0][2][x];
-        pyb = k.p.prof[1][0][y];
-        pya = k.p.prof[1][1][y];
-        pyk = k.p.prof[1][2][y];
-    }
-
-    size_t o = (size_t)z0 * plane + (size_t)y * nx + x;
-    float ex_c = ex[o], ey_c = ey[o];
-    for (int z = z0; z < z1; ++z, o += plane) {
-        const float ez_c = ez[o];
-        const int zp = plus1(z, g.nz, g.periodic & 4u);
-        float ex_pz = 0.f, ey_pz = 0.f;
-        if (zp >= 0) {
-            const size_t oz = (size_t)zp * plane + (size_t)y * nx + x;
-            ex_pz = ex[oz];
-            ey_pz = ey[oz];
-        }
-        float ez_px = 0.f, ey_px = 0.f;
-        if (xp >= 0) {
-            const size_t ox = o - x + xp;
-            ez_px = ez[ox];
-            ey_px = ey[ox];
-        }
-        float ez_py = 0.f, ex_py = 0.f;
-        if (yp >= 0) {
-            const size_t oy = o + ((long)yp - y) * (long)nx;
-            ez_py = ez[oy];
-            ex_py = ex[oy];
-        }
-
-        float d_ez_dy = ez_py - ez_c, d_ex_dy = ex_py - ex_c;
-        float d_ey_dz = ey_pz - ey_c, d_ex_dz = ex_pz - ex_c;
-        float d_ez_dx = ez_px - ez_c, d_ey_dx = ey_px - ey_c;
-
-        if (PML && (shell_xy || in_shell_axis(g, 2, z))) {
-            if (gy) {
-                const size_t i = ((size_t)z * (2 * g.T[1]) + gyi) * nx + x;
-                float p0 = k.p.psi[1][0][i], p1 = k.p.psi[1][1][i];
-                d_ez_dy = cpml(d_ez_dy, pyb, pya, pyk, p0);
-                d_ex_dy = cpml(d_ex_dy, pyb, pya, pyk, p1);
-                k.p.psi[1][0][i] = p0;
-                k.p.psi[1][1][i] = p1;
-            } else {
-                d_ez_dy = d_ez_dy + 0.0f;
-                d_ex_dy = d_ex_dy + 0.0f;
-            }
-            if (graded(g, 2, z, g.nz)) {
-                const int gzi = slab_index(g, 2, z, g.nz);
-                const size_t i = ((size_t)gzi * g.ny + y) * nx + x;
-                const float b = k.p.prof[2][0][z], a = k.p.prof[2][1][z], ik = k.p.prof[2][2][z];
-                float p0 = k.p.psi[2][0][i], p1 = k.p.psi[2][1][i];
-                d_ey_dz = cpml(d_ey_dz, b, a, ik, p0);
-                d_ex_dz = cpml(d_ex_dz, b, a, ik, p1);
-                k.p.psi[2][0][i] = p0;
-                k.p.psi[2][1][i] = p1;
-            } else {
-                d_ey_dz = d_ey_dz + 0.0f;
-                d_ex_dz = d_ex_dz + 0.0f;
-            }
-            if (gx) {
-                const size_t i = ((size_t)z * g.ny + y) * (2 * g.T[0]) + gxi;
-                float p0 = k.p.psi[0][0][i], p1 = k.p.psi[0][1][i];
-                d_ez_dx = cpml(d_ez_dx, pxb, pxa, pxk, p0);
-                d_ey_dx = cpml(d_ey_dx, pxb, pxa, pxk, p1);
-                k.p.psi[0][0][i] = p0;
-                k.p.psi[0][1][i] = p1;
-            } else {
-                d_ez_dx = d_ez_dx + 0.0f;
-                d_ey_dx = d_ey_dx + 0.0f;
-            }
-        }
-
-        const float chx = d_ez_dy - d_ey_dz;
-        const float chy = d_ex_dz - d_ez_dx;
-        const float chz = d_ey_dx - d_ex_dy;
-        const float2 c = s_co[k.p.id[o]];
-        hx[o] = c.x * hx[o] - c.y * chx;
-        hy[o] = c.x * hy[o] - c.y * chy;
-        hz[o] = c.x * hz[o] - c.y * chz;
-
-        ex_c = ex_pz;
-        ey_c = ey_pz;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// E update: update_e_field / update_e_field_pml (src/fdtd.rs:369-509).
-// H(z) is carried as the next iteration's H(z-1).
-// ---------------------------------------------------------------------------
-template <bool PML>
-__global__ __launch_bounds__(kTX *kTY) void k_update_e(KArgs k, int zchunk)
-{
-    __shared__ float2 s_co[256];  // (ca, cb)
-    const int tid = threadIdx.y * kTX + threadIdx.x;
-    for (int i = tid; i < 256; i += kTX * kTY) {
-        const float4 c = k.p.coef[i];
-        s_co[i] = make_float2(c.x, c.y);
-    }
-    __syncthreads();
-
-    const Geo &g = k.g;
-    const int x = blockIdx.x * kTX + threadIdx.x;
-    const int y = blockIdx.y * kTY + threadIdx.y;
-    if (x >= g.nx || y >= g.ny) return;
-    const int z0 = blockIdx.z * zchunk;
-    const int z1 = min(z0 + zchunk, g.nz);
-    const size_t nx = g.nx, plane = (size_t)g.nx * g.ny;
-
-    float *__restrict__ ex = k.p.f[0];
-    float *__restrict__ ey = k.p.f[1];
-    float *__restrict__ ez = k.p.f[2];
-    float *__restrict__ hx = k.p.f[3];
-    float *__restrict__ hy = k.p.f[4];
-    float *__restrict__ hz = k.p.f[5];
-
-    const int xm = minus1(x, g.nx, g.periodic & 1u);
-    const int ym = minus1(y, g.ny, g.periodic & 2u);
-
-    bool shell_xy = false, gx = false, gy = false;
-    int gxi = 0, gyi = 0;
-    float pxb = 1.f, pxa = 0.f, pxk = 1.f, pyb = 1.f, pya = 0.f, pyk = 1.f;
-    if (PML) {
-        shell_xy = in_shell_axis(g, 0, x) || in_shell_axis(g, 1, y);
-        gx = graded(g, 0, x, g.nx);
-        gy = graded(g, 1, y, g.ny);
-        gxi = slab_index(g, 0, x, g.nx);
-        gyi = slab_index(g, 1, y, g.ny);
-        pxb = k.p.prof[0][0][x];
-        pxa = k.p.prof[0][1][x];
-        pxk = k.p.prof[0][2][x];
-        pyb = k.p.prof[1][0][y];
-        pya = k.p.prof[1][1][y];
-        pyk = k.p.prof[1][2][y];
-    }
-
-    float hx_mz = 0.f, hy_mz = 0.f;
-    {
-        const int zm = minus1(z0, g.nz, g.periodic & 4u);
-        if (zm >= 0) {
-            const size_t om = (size_t)zm * plane + (size_t)y * nx + x;
-            hx_mz = hx[om];
-            hy_mz = hy[om];
-        }
-    }
-    size_t o = (size_t)z0 * plane + (size_t)y * nx + x;
-    for (int z = z0; z < z1; ++z, o += plane) {
-        const float hx_c = hx[o], hy_c = hy[o], hz_c = hz[o];
-        float hz_mx = 0.f, hy_mx = 0.f;
-        if (xm >= 0) {
-            const size_t ox = o - x + xm;
-            hz_mx = hz[ox];
-            hy_mx = hy[ox];
-        }
-        float hz_my = 0.f, hx_my = 0.f;
-        if (ym >= 0) {
-            const size_t oy = o + ((long)ym - y) * (long)nx;
-            hz_my = hz[oy];
-            hx_my = hx[oy];
-        }
-
-        float d_hz_dy = hz_c - hz_my, d_hx_dy = hx_c - hx_my;
-        float d_hy_dz = hy_c - hy_mz, d_hx_dz = hx_c - hx_mz;
-        float d_hz_dx = hz_c - hz_mx, d_hy_dx = hy_c - hy_mx;
-
-        if (PML && (shell_xy || in_shell_axis(g, 2, z))) {
-            if (gy) {
-                const size_t i = ((size_t)z * (2 * g.T[1]) + gyi) * nx + x;
-                float p0 = k.p.psi[1][2][i], p1 = k.p.psi[1][3][i];
-                d_hz_dy = cpml(d_hz_dy, pyb, pya, pyk, p0);
-                d_hx_dy = cpml(d_hx_dy, pyb, pya, pyk, p1);
-                k.p.psi[1][2][i] = p0;
-                k.p.psi[1][3][i] = p1;
-            } else {
-                d_hz_dy = d_hz_dy + 0.0f;
-                d_hx_dy = d_hx_dy + 0.0f;
-            }
-            if (graded(g, 2, z, g.nz)) {
-                const int gzi = slab_index(g, 2, z, g.nz);
-                const size_t i = ((size_t)gzi * g.ny + y) * nx + x;
-                const float b = k.p.prof[2][0][z], a = k.p.prof[2][1][z], ik = k.p.prof[2][2][z];
-                float p0 = k.p.psi[2][2][i], p1 = k.p.psi[2][3][i];
-                d_hy_dz = cpml(d_hy_dz, b, a, ik, p0);
-                d_hx_dz = cpml(d_hx_dz, b, a, ik, p1);
-                k.p.psi[2][2][i] = p0;
-                k.p.psi[2][3][i] = p1;
-            } else {
-                d_hy_dz = d_hy_dz + 0.0f;
-                d_hx_dz = d_hx_dz + 0.0f;
-            }
-            if (gx) {
-                const size_t i = ((size_t)z * g.ny + y) * (2 * g.T[0]) + gxi;
-                float p0 = k.p.psi[0][2][i], p1 = k.p.psi[0][3][i];
-                d_hz_dx = cpml(d_hz_dx, pxb, pxa, pxk, p0);
-                d_hy_dx = cpml(d_hy_dx, pxb, pxa, pxk, p1);
-                k.p.psi[0][2][i] = p0;
-                k.p.psi[0][3][i] = p1;
-            } else {
-                d_hz_dx = d_hz_dx + 0.0f;
-                d_hy_dx = d_hy_dx + 0.0f;
-            }
-        }
-
-        const float cex = d_hz_dy - d_hy_dz;
-        const float cey = d_hx_dz - d_hz_dx;
-        const float cez = d_hy_dx - d_hx_dy;
-        const float2 c = s_co[k.p.id[o]];
-        ex[o] = c.x * ex[o] + c.y * cex;
-        ey[o] = c.x * ey[o] + c.y * cey;
-        ez[o] = c.x * ez[o] + c.y * cez;
-
-        hx_mz = hx_c;
-        hy_mz = hy_c;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Fused single-pass step: H(n+1/2) and E(n+1) in one sweep.
-//
-// Reads state A (E^n, H^{n-1/2}, psi) and writes state B; the engine swaps
-// A and B after every step.  Per cell that is 24 B read + 24 B written + 1 B
-// id instead of the two-pass 74 B, with identical arithmetic: every H value
-// is computed exactly as update_h_field[_pml] computes it, then every E
-// value exactly as update_e_field[_pml] does from those H values.
-//
-// Workgroup = 16 waves x 64 lanes.  Lane l of wave w holds the column
-// x = bx*63 - 1 + l, y = by*15 - 1 + w: lane 0 and wave 0 are a one-cell
-// halo whose H(n+1/2) the E update of the owned 63 x 15 cells needs at x-1
-// and y-1.  Halo cells recompute H from state A (bit-identical to the owner's
-// value) but never store.  The sweep marches z over a chunk, starting one
-// plane early for H(z0-1); E(z+1) and H(z-1) ride in registers, H at x-1
-// comes from the neighbouring lane and H at y-1 through LDS.
-// ---------------------------------------------------------------------------
-constexpr int kFW = 64;
-constexpr int kFY = 16;
-constexpr int kFOX = kFW - 1;
-constexpr int kFOY = kFY - 1;
-
-struct FArgs {
-    Geo g;
-    Ptrs a;  // state n (read)
-    Ptrs b;  // state n+1 (written)
-};
-
-template <bool PML>
-__global__ __launch_bounds__(kFW *kFY) void k_fused(FArgs k, int zchunk)
-{
-    __shared__ float4 s_co[256];             // (ca, cb, da, db)
-    __shared__ float2 s_h[2][kFY][kFW];      // (hz, hx) of H(n+1/2), double-buffered by plane
-    const int lane = threadIdx.x, w = threadIdx.y;
-    for (int i = w * kFW + lane; i < 256; i += kFW * kFY) s_co[i] = k.a.coef[i];
-    __syncthreads();
-
-    const Geo &g = k.g;
-    const size_t nx = g.nx, plane = (size_t)g.nx * g.ny;
-    const int xr = blockIdx.x * kFOX - 1 + lane;
-    const int yr = blockIdx.y * kFOY - 1 + w;
-    // Halo coordinates map through the boundary: -1 wraps (periodic) or
-    // stands for the zero field outside the domain.
-    int x = xr, y = yr;
-    bool live = true;
-    if (xr < 0) {
-        if (g.periodic & 1u) x = g.nx - 1; else live = false;
-    }
-    if (yr < 0) {
-        if (g.periodic & 2u) y = g.ny - 1; else live = false;
-    }
-    if (xr >= g.nx || yr >= g.ny) live = false;
-    const bool own = live && lane >= 1 && w >= 1;
-    if (!live) {
-        x = 0;
-        y = 0;
-    }
-    const int xp = plus1(x, g.nx, g.periodic & 1u);
-    const int yp = plus1(y, g.ny, g.periodic & 2u);
-
-    bool shell_xy = false, gx = false, gy = false;
-    int gxi = 0, gyi = 0;
-    float pxb = 1.f, pxa = 0.f, pxk = 1.f, pyb = 1.f, pya = 0.f, pyk = 1.f;
-    if (PML) {
-        shell_xy = in_shell_axis(g, 0, x) || in_shell_axis(g, 1, y);
-        gx = graded(g, 0, x, g.nx);
-        gy = graded(g, 1, y, g.ny);
-        gxi = slab_index(g, 0, x, g.nx);
-        gyi = slab_index(g, 1, y, g.ny);
-        pxb = k.a.prof[0][0][x];
-        pxa = k.a.prof[0][1][x];
-        pxk = k.a.prof[0][2][x];
-        pyb = k.a.prof[1][0][y];
-        pya = k.a.prof[1][1][y];
-        pyk = k.a.prof[1][2][y];
-    }
-
-    const float *__restrict__ aex = k.a.f[0];
-    const float *__restrict__ aey = k.a.f[1];
-    const float *__restrict__ aez = k.a.f[2];
-    const float *__restrict__ ahx = k.a.f[3];
-    const float *__restrict__ ahy = k.a.f[4];
-    const float *__restrict__ ahz = k.a.f[5];
-    const size_t row = (size_t)y * nx + x;
-    const size_t rowp_x = (size_t)y * nx + (xp >= 0 ? xp : 0);
-    const size_t rowp_y = (size_t)(yp >= 0 ? yp : 0) * nx + x;
-
-    const int z0 = blockIdx.z * zchunk;
-    const int z1 = min(z0 + zchunk, g.nz);
-    const int zpre = minus1(z0, g.nz, g.periodic & 4u);
-
-    // E^n at the current plane (own column) and H^{n+1/2} one plane below.
-    float ex_c = 0.f, ey_c = 0.f, ez_c = 0.f;
-    float hx_mz = 0.f, hy_mz = 0.f;
-    {
-        const int zc = zpre >= 0 ? zpre : z0;
-        if (live) {
-            const size_t o = (size_t)zc * plane + row;
-            ex_c = aex[o];
-            ey_c = aey[o];
-            ez_c = aez[o];
-        }
-    }
-
-    for (int it = (zpre >= 0 ? -1 : 0); it < z1 - z0; ++it) {
-        const int z = it < 0 ? zpre : z0 + it;
-        const int zp = plus1(z, g.nz, g.periodic & 4u);
-        const size_t zo = (size_t)z * plane;
-        const bool store = own && it >= 0;
-
-        // ---- loads (state A) ------------------------------------------------
-        float ex_n = 0.f, ey_n = 0.f, ez_n = 0.f;
-        float ez_px = 0.f, ey_px = 0.f, ez_py = 0.f, ex_py = 0.f;
-        float hx = 0.f, hy = 0.f, hz = 0.f;
-        int mid = 0;
-        if (live) {
-            if (zp >= 0) {
-                const size_t on = (size_t)zp * plane + row;
-                ex_n = aex[on];
-                ey_n = aey[on];
-                ez_n = aez[on];
-            }
-            if (xp >= 0) {
-                ez_px = aez[zo + rowp_x];
-                ey_px = aey[zo + rowp_x];
-            }
-            if (yp >= 0) {
-                ez_py = aez[zo + rowp_y];
-                ex_py = aex[zo + rowp_y];
-            }
-            hx = ahx[zo + row];
-            hy = ahy[zo + row];
-            hz = ahz[zo + row];
-            mid = k.a.id[zo + row];
-        }
-        const float4 co = s_co[mid];
-
-        // ---- H(n+1/2): update_h_field[_pml] --------------------------------
-        float d_ez_dy = ez_py - ez_c, d_ex_dy = ex_py - ex_c;
-        float d_ey_dz = ey_n - ey_c, d_ex_dz = ex_n - ex_c;
-        float d_ez_dx = ez_px - ez_c, d_ey_dx = ey_px - ey_c;
-        const bool shell = PML && (shell_xy || in_shell_axis(g, 2, z));
-        const bool gz = PML && graded(g, 2, z, g.nz);
-        size_t ipy = 0, ipz = 0, ipx = 0;
-        float pzb = 1.f, pza = 0.f, pzk = 1.f;
-        if (PML && shell && live) {
-            ipy = ((size_t)z * (2 * g.T[1]) + gyi) * nx + x;
-            ipx = ((size_t)z * g.ny + y) * (2 * g.T[0]) + gxi;
-            if (gz) {
-                ipz = ((size_t)slab_index(g, 2, z, g.nz) * g.ny + y) * nx + x;
-                pzb = k.a.prof[2][0][z];
-                pza = k.a.prof[2][1][z];
-                pzk = k.a.prof[2][2][z];
-            }
-            if (gy) {
-                float p0 = k.a.psi[1][0][ipy], p1 = k.a.psi[1][1][ipy];
-                d_ez_dy = cpml(d_ez_dy, pyb, pya, pyk, p0);
-                d_ex_dy = cpml(d_ex_dy, pyb, pya, pyk, p1);
-                if (store) {
-                    k.b.psi[1][0][ipy] = p0;
-                    k.b.psi[1][1][ipy] = p1;
-                }
-            } else {
-                d_ez_dy = d_ez_dy + 0.0f;
-                d_ex_dy = d_ex_dy + 0.0f;
-            }
-            if (gz) {
-                float p0 = k.a.psi[2][0][ipz], p1 = k.a.psi[2][1][ipz];
-                d_ey_dz = cpml(d_ey_dz, pzb, pza, pzk, p0);
-                d_ex_dz = cpml(d_ex_dz, pzb, pza, pzk, p1);
-                if (store) {
-                    k.b.psi[2][0][ipz] = p0;
-                    k.b.psi[2][1][ipz] = p1;
-                }
-            } else {
-                d_ey_dz = d_ey_dz + 0.0f;
-                d_ex_dz = d_ex_dz + 0.0f;
-            }
-            if (gx) {
-                float p0 = k.a.psi[0][0][ipx], p1 = k.a.psi[0][1][ipx];
-                d_ez_dx = cpml(d_ez_dx, pxb, pxa, pxk, p0);
-                d_ey_dx = cpml(d_ey_dx, pxb, pxa, pxk, p1);
-                if (store) {
-                    k.b.psi[0][0][ipx] = p0;
-                    k.b.psi[0][1][ipx] = p1;
-                }
-            } else {
-                d_ez_dx = d_ez_dx + 0.0f;
-                d_ey_dx = d_ey_dx + 0.0f;
-            }
-        }
-        float hxn = co.z * hx - co.w * (d_ez_dy - d_ey_dz);
-        float hyn = co.z * hy - co.w * (d_ex_dz - d_ez_dx);
-        float hzn = co.z * hz - co.w * (d_ey_dx - d_ex_dy);
-        if (!live) {
-            hxn = 0.f;
-            hyn = 0.f;
-            hzn = 0.f;
-        }
-
-        if (it >= 0) {
-            // ---- neighbours of H(n+1/2) at x-1 (lane-1) and y-1 (LDS) -------
-            const float hz_mx = __shfl_up(hzn, 1, kFW);
-            const float hy_mx = __shfl_up(hyn, 1, kFW);
-            const int buf = it & 1;
-            s_h[buf][w][lane] = make_float2(hzn, hxn);
-            __syncthreads();
-            float hz_my = 0.f, hx_my = 0.f;
-            if (w > 0) {
-                const float2 m = s_h[buf][w - 1][lane];
-                hz_my = m.x;
-                hx_my = m.y;
-            }
-
-            if (store) {
-                // ---- E(n+1): update_e_field[_pml] ---------------------------
-                float d_hz_dy = hzn - hz_my, d_hx_dy = hxn - hx_my;
-                float d_hy_dz = hyn - hy_mz, d_hx_dz = hxn - hx_mz;
-                float d_hz_dx = hzn - hz_mx, d_hy_dx = hyn - hy_mx;
-                if (PML && shell) {
-                    if (gy) {
-                        float p0 = k.a.psi[1][2][ipy], p1 = k.a.psi[1][3][ipy];
-                        d_hz_dy = cpml(d_hz_dy, pyb, pya, pyk, p0);
-                        d_hx_dy = cpml(d_hx_dy, pyb, pya, pyk, p1);
-                        k.b.psi[1][2][ipy] = p0;
-                        k.b.psi[1][3][ipy] = p1;
-                    } else {
-                        d_hz_dy = d_hz_dy + 0.0f;
-                        d_hx_dy = d_hx_dy + 0.0f;
-                    }
-                    if (gz) {
-                        float p0 = k.a.psi[2][2][ipz], p1 = k.a.psi[2][3][ipz];
-                        d_hy_dz = cpml(d_hy_dz, pzb, pza, pzk, p0);
-                        d_hx_dz = cpml(d_hx_dz, pzb, pza, pzk, p1);
-                        k.b.psi[2][2][ipz] = p0;
-                        k.b.psi[2][3][ipz] = p1;
-                    } else {
-                        d_hy_dz = d_hy_dz + 0.0f;
-                        d_hx_dz = d_hx_dz + 0.0f;
-                    }
-                    if (gx) {
-                        float p0 = k.a.psi[0][2][ipx], p1 = k.a.psi[0][3][ipx];
-                        d_hz_dx = cpml(d_hz_dx, pxb, pxa, pxk, p0);
-                        d_hy_dx = cpml(d_hy_dx, pxb, pxa, pxk, p1);
-                        k.b.psi[0][2][ipx] = p0;
-                        k.b.psi[0][3][ipx] = p1;
-                    } else {
-                        d_hz_dx = d_hz_dx + 0.0f;
-                        d_hy_dx = d_hy_dx + 0.0f;
-                    }
-                }
-                const size_t o = zo + row;
-                k.b.f[0][o] = co.x * ex_c + co.y * (d_hz_dy - d_hy_dz);
-                k.b.f[1][o] = co.x * ey_c + co.y * (d_hx_dz - d_hz_dx);
-                k.b.f[2][o] = co.x * ez_c + co.y * (d_hy_dx - d_hx_dy);
-                k.b.f[3][o] = hxn;
-                k.b.f[4][o] = hyn;
-                k.b.f[5][o] = hzn;
-            }
-        }
-        hx_mz = hxn;
-        hy_mz = hyn;
-        ex_c = ex_n;
-        ey_c = ey_n;
-        ez_c = ez_n;
-    }
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+        dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -791,43 +241,16 @@ void free_probes(wf_ctx *c)
     c->probe_steps = 0;
 }
 
-KArgs kargs(const wf_ctx *c) { return KArgs{c->g, c->p}; }
-
 // One full step: H phase, E phase, sources, probes (src/engine.rs:748-782).
 // ev (optional) = 4 events recorded around the H and E launches.
 int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
 {
-    const dim3 blk(kTX, kTY, 1);
-    const dim3 grd((c->g.nx + kTX - 1) / kTX, (c->g.ny + kTY - 1) / kTY,
-                   (c->g.nz + c->zchunk - 1) / c->zchunk);
-    const KArgs k = kargs(c);
+    StepLaunch s{c->g, c->p, c->alt, c->pml, c->fused ? c->fzchunk : c->zchunk, c->stream};
     if (c->fused) {
-        const FArgs f{c->g, c->p, c->alt};
-        const dim3 fblk(kFW, kFY, 1);
-        const dim3 fgrd((c->g.nx + kFOX - 1) / kFOX, (c->g.ny + kFOY - 1) / kFOY,
-                        (c->g.nz + c->fzchunk - 1) / c->fzchunk);
-        if (ev) WF_HIP(hipEventRecord(ev[0], c->stream));
-        if (c->pml)
-            hipLaunchKernelGGL(k_fused<true>, fgrd, fblk, 0, c->stream, f, c->fzchunk);
-        else
-            hipLaunchKernelGGL(k_fused<false>, fgrd, fblk, 0, c->stream, f, c->fzchunk);
-        if (ev) WF_HIP(hipEventRecord(ev[1], c->stream));
-        if (ev) WF_HIP(hipEventRecord(ev[2], c->stream));
-        if (ev) WF_HIP(hipEventRecord(ev[3], c->stream));
+        WF_HIP(launch_fused(s, ev));
         std::swap(c->p, c->alt);
     } else {
-        if (ev) WF_HIP(hipEventRecord(ev[0], c->stream));
-        if (c->pml)
-            hipLaunchKernelGGL(k_update_h<true>, grd, blk, 0, c->stream, k, c->zchunk);
-        else
-            hipLaunchKernelGGL(k_update_h<false>, grd, blk, 0, c->stream, k, c->zchunk);
-        if (ev) WF_HIP(hipEventRecord(ev[1], c->stream));
-        if (ev) WF_HIP(hipEventRecord(ev[2], c->stream));
-        if (c->pml)
-            hipLaunchKernelGGL(k_update_e<true>, grd, blk, 0, c->stream, k, c->zchunk);
-        else
-            hipLaunchKernelGGL(k_update_e<false>, grd, blk, 0, c->stream, k, c->zchunk);
-        if (ev) WF_HIP(hipEventRecord(ev[3], c->stream));
+        WF_HIP(launch_split(s, ev));
     }
     if (c->nsrc) {
         hipLaunchKernelGGL(k_sources, dim3(1), dim3(64), 0, c->stream, c->p, c->d_src,
@@ -1310,6 +733,53 @@ int wf_time_kernels(wf_ctx *c, uint64_t nsteps, double *h_ms, double *e_ms)
     for (auto &e : ev) (void)hipEventDestroy(e);
     *h_ms = nsteps ? hs / (double)nsteps : 0.0;
     *e_ms = nsteps ? es / (double)nsteps : 0.0;
+    return rc;
+}
+
+int wf_hbm_copy(uint64_t bytes, int device, int iters, double *ms_per_copy)
+{
+    clear_error();
+    if (!ms_per_copy || bytes < 16 || iters < 1)
+        return fail(WF_ERR_INVALID, "wf_hbm_copy: bad arguments");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(WF_ERR_HIP, "wf_hbm_copy: no device %d", device);
+    WF_HIP(hipSetDevice(device));
+    const uint64_t n4 = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    if (int rc = dmalloc(&a, n4 * 16)) return rc;
+    if (int rc = dmalloc(&b, n4 * 16)) {
+        dfree(a);
+        return rc;
+    }
+    hipStream_t s;
+    hipEvent_t e0, e1;
+    int rc = WF_OK;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        dfree(a);
+        dfree(b);
+        return fail(WF_ERR_HIP, "wf_hbm_copy: stream");
+    }
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipMemsetAsync(a, 0, n4 * 16, s);
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) cus = prop.multiProcessorCount;
+    const dim3 grid((unsigned)std::min<uint64_t>((n4 + 255) / 256, (uint64_t)cus * 16));
+    hipLaunchKernelGGL(k_copy, grid, dim3(256), 0, s, a, b, n4);  // warm
+    (void)hipEventRecord(e0, s);
+    for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy, grid, dim3(256), 0, s, a, b, n4);
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(WF_ERR_HIP, "wf_hbm_copy: sync failed");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *ms_per_copy = (double)ms / iters;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    dfree(a);
+    dfree(b);
     return rc;
 }
 

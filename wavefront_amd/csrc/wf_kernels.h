@@ -1,5 +1,5 @@
-// Device-side data structures and the per-voxel Yee/CPML update shared by
-// every kernel variant.  Included by HIP translation units only.
+// Device-side data structures and helpers shared by the HIP translation
+// units (step kernels, point kernels, converters).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,9 +10,9 @@ namespace wf {
 // Grid geometry and boundary description, passed by value to kernels.
 struct Geo {
     int nx, ny, nz;
-    int nb[3];        // blocks per axis (n/8)
-    int T[3];         // graded CPML depth per axis, voxels (0 = none)
-    int sb[3];        // PML shell depth per axis, blocks (0 = none)
+    int nb[3];          // blocks per axis (n/8)
+    int T[3];           // graded CPML depth per axis, voxels (0 = none)
+    int sb[3];          // PML shell depth per axis, blocks (0 = none)
     unsigned periodic;  // bit a: axis a wraps
 };
 
@@ -29,14 +29,9 @@ struct Geo {
 struct Ptrs {
     float *f[6];  // ex ey ez hx hy hz
     const uint8_t *id;
-    const float4 *coef;  // 256 x (ca, cb, da, db), dequantized Q16.16
+    const float4 *coef;       // 256 x (ca, cb, da, db), dequantized Q16.16
     const float *prof[3][3];  // [axis][b, a, inv_kappa], length n_axis
     float *psi[3][4];
-};
-
-struct KArgs {
-    Geo g;
-    Ptrs p;
 };
 
 // Source / probe tables.
@@ -54,6 +49,9 @@ struct DevProbe {
     int pad;
 };
 
+// Four consecutive x-cells: the unit a lane loads and stores (16 B).
+typedef float f4 __attribute__((ext_vector_type(4)));
+
 // One-tap recursive convolution, src/fdtd.rs:136-163:
 //   psi = b*psi + a*raw;  d = raw*inv_kappa + psi
 __device__ __forceinline__ float cpml(float raw, float b, float a, float ik, float &psi)
@@ -62,5 +60,77 @@ __device__ __forceinline__ float cpml(float raw, float b, float a, float ik, flo
     psi = pn;
     return raw * ik + pn;
 }
+
+__device__ __forceinline__ f4 cpml4(f4 raw, f4 b, f4 a, f4 ik, f4 &psi)
+{
+    const f4 pn = b * psi + a * raw;
+    psi = pn;
+    return raw * ik + pn;
+}
+
+// A block (8 cells) on axis ax is in the PML shell (PmlAuxGrid::build,
+// src/layout.rs:602-608).
+__device__ __forceinline__ bool in_shell_axis(const Geo &g, int ax, int c)
+{
+    const int b = c >> 3;
+    return g.sb[ax] > 0 && (b < g.sb[ax] || b >= g.nb[ax] - g.sb[ax]);
+}
+
+// Cell coordinate c lies in the graded layer of axis ax (non-identity profile).
+__device__ __forceinline__ bool graded(const Geo &g, int ax, int c, int n)
+{
+    return g.T[ax] > 0 && (c < g.T[ax] || c >= n - g.T[ax]);
+}
+
+__device__ __forceinline__ int slab_index(const Geo &g, int ax, int c, int n)
+{
+    return c < g.T[ax] ? c : c - (n - 2 * g.T[ax]);
+}
+
+// Neighbour coordinate c+1 / c-1 on an axis of length n: -1 = outside the
+// domain (reads the zero boundary, OUTER_ZERO_BLOCK in src/engine.rs:83).
+__device__ __forceinline__ int plus1(int c, int n, bool wrap)
+{
+    return c + 1 < n ? c + 1 : (wrap ? 0 : -1);
+}
+
+__device__ __forceinline__ int minus1(int c, int n, bool wrap)
+{
+    return c > 0 ? c - 1 : (wrap ? n - 1 : -1);
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin; renumber them so that
+// each XCD sweeps a contiguous run of tiles (x fastest, then y, then z-chunk)
+// and the halo lines shared by neighbouring tiles stay in one XCD's L2.
+__device__ __forceinline__ int xcd_tile(int i, int G)
+{
+    const int per = G >> 3;
+    return i < (per << 3) ? (i & 7) * per + (i >> 3) : i;
+}
+
+// Workgroup barrier that orders LDS only: outstanding global loads (a
+// prefetched plane) and stores are not waited for.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Launch parameters for one timestep.
+struct StepLaunch {
+    Geo g;
+    Ptrs cur;  // state at step n (updated in place by the split kernels)
+    Ptrs alt;  // state n+1 written by the fused kernel
+    bool pml;
+    int zchunk;
+    hipStream_t stream;
+};
+
+// H kernel then E kernel, in place on cur.  ev (optional, 4 events) brackets
+// the two launches.
+hipError_t launch_split(const StepLaunch &s, hipEvent_t *ev);
+// One single-pass kernel cur -> alt.  ev (optional) brackets it (ev[0..1]).
+hipError_t launch_fused(const StepLaunch &s, hipEvent_t *ev);
 
 }  // namespace wf

@@ -28,6 +28,13 @@ def device_count() -> int:
     return int(lib().wf_device_count())
 
 
+def hbm_copy_gbs(nbytes: int, device: int = 0, iters: int = 10) -> float:
+    """Read+write GB/s of a plain device copy of nbytes (the HBM ceiling)."""
+    ms = c_double()
+    check(lib().wf_hbm_copy(nbytes, device, iters, byref(ms)))
+    return 2.0 * nbytes / (ms.value / 1e3) / 1e9
+
+
 class DeviceEngine:
     """A device-resident simulation: fields, material ids, coefficient table,
     CPML profiles and psi memory on one GPU."""
