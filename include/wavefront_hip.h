@@ -204,20 +204,24 @@ int wf_read_trace(wf_ctx *ctx, float *trace, uint64_t max_steps);
 /* Times nsteps full steps with a HIP event pair on the context's stream;
  * writes the elapsed milliseconds. */
 int wf_time_steps(wf_ctx *ctx, uint64_t nsteps, double *total_ms);
-/* Runs nsteps full steps with a HIP event pair around every H-phase and
- * E-phase launch (same stream); writes the mean per-launch milliseconds of
- * each phase kernel. */
-int wf_time_kernels(wf_ctx *ctx, uint64_t nsteps, double *h_ms, double *e_ms);
-/* Algorithmic HBM bytes of one launch of each phase kernel.  Split variant
- * (separate H and E kernels, in place): per cell 12 B read of the curl
- * field, 12 B read + 12 B write of the updated field, 1 B material id, plus
- * 8 B (read + write) per non-identity psi term of that phase.  Fused variant
- * (one kernel per step, reported in h_bytes, e_bytes = 0): 24 B read + 24 B
- * write + 1 B id per cell plus 8 B per psi term of both phases. */
-int wf_phase_bytes(const wf_ctx *ctx, uint64_t *h_bytes, uint64_t *e_bytes);
-/* "fused" (default: one single-pass kernel per step, ping-pong state) or
- * "split" (H kernel then E kernel, in place); chosen at wf_create from the
- * WF_KERNEL environment variable. */
+/* Runs nsteps full steps with a HIP event pair around every step-kernel
+ * launch (same stream); writes the mean per-launch milliseconds of the fused
+ * single-pass kernel (0 in the split variant), the H-phase kernel and the
+ * E-phase kernel (in the fused variant those two cover only the frame of
+ * cells around the fused kernel's interior box, which holds the CPML). */
+int wf_time_kernels(wf_ctx *ctx, uint64_t nsteps, double *fused_ms, double *h_ms,
+                    double *e_ms);
+/* Algorithmic HBM bytes of one launch of each step kernel.  Fused kernel:
+ * 24 B read + 24 B written + 1 B material id per interior cell.  H or E
+ * kernel: per cell it covers 12 B read of the curl field, 12 B read + 12 B
+ * write of the updated field and 1 B id, plus 8 B (read + write) per
+ * non-identity psi term of that phase. */
+int wf_kernel_bytes(const wf_ctx *ctx, uint64_t *fused_bytes, uint64_t *h_bytes,
+                    uint64_t *e_bytes);
+/* "fused" (default: single-pass kernel over the interior, split H/E kernels
+ * over the CPML frame, ping-pong state) or "split" (H kernel then E kernel
+ * over the whole domain, in place); chosen at wf_create from the WF_KERNEL
+ * environment variable (WF_ZCHUNK sets the z planes per workgroup). */
 const char *wf_kernel_variant(const wf_ctx *ctx);
 /* Copies `bytes` (read + write = 2*bytes of HBM traffic) iters times with a
  * plain 16-byte-per-lane copy kernel on `device`; writes the mean ms per

@@ -99,6 +99,8 @@ def _random_case(dims, shell, periodic, steps, seed, nmat=5, pml=True):
     ((136, 24, 32), (3, 1, 2), 0),
     ((72, 48, 40), (1, 0, 2), PERIODIC_Y),
     ((40, 64, 48), (0, 1, 0), PERIODIC_X | PERIODIC_Z),
+    ((16, 16, 24), (1, 1, 1), 0),
+    ((296, 16, 16), (1, 0, 0), PERIODIC_Y | PERIODIC_Z),
 ])
 def test_random_fields_pml(dims, shell, periodic):
     """Random fields, random lossy materials, CPML on uneven per-axis shells:

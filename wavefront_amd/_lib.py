@@ -84,8 +84,10 @@ _SIGNATURES = {
     "wf_read_probe_sums": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_uint64)]),
     "wf_read_trace": (c_int, [c_void_p, POINTER(c_float), c_uint64]),
     "wf_time_steps": (c_int, [c_void_p, c_uint64, POINTER(c_double)]),
-    "wf_time_kernels": (c_int, [c_void_p, c_uint64, POINTER(c_double), POINTER(c_double)]),
-    "wf_phase_bytes": (c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64)]),
+    "wf_time_kernels": (c_int, [c_void_p, c_uint64, POINTER(c_double), POINTER(c_double),
+                                POINTER(c_double)]),
+    "wf_kernel_bytes": (c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64),
+                                POINTER(c_uint64)]),
     "wf_kernel_variant": (c_char_p, [c_void_p]),
     "wf_hbm_copy": (c_int, [c_uint64, c_int, c_int, POINTER(c_double)]),
 }

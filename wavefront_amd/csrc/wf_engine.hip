@@ -162,8 +162,12 @@ struct wf_ctx {
     Ptrs alt{};   // second state for the fused kernel (written, then swapped)
     bool pml = false;
     bool fused = true;
-    int zchunk = 16;
-    int fzchunk = 64;
+    int zchunk = 32;
+    Box interior{};     // fused variant: cells swept by the single-pass kernel
+    BoxList frame{};    // fused variant: the rest (holds the whole CPML shell)
+    BoxList whole{};    // split variant: the whole domain
+    hipStream_t side = nullptr;            // frame H kernel, overlapped
+    hipEvent_t fork = nullptr, join = nullptr;
 
     float *d_fields = nullptr;
     float *d_fields2 = nullptr;
@@ -241,16 +245,130 @@ void free_probes(wf_ctx *c)
     c->probe_steps = 0;
 }
 
+// Tiles a box for the split kernels: lanes per row = the smallest power of
+// two covering the box's quads (at most 64), kSY waves of 64/lpr rows each.
+Box split_box(int x0, int x1, int y0, int y1, int z0, int z1, int zchunk)
+{
+    Box b{x0, x1, y0, y1, z0, z1, 64, 0, 0, 0, 0};
+    const int quads = (x1 - x0 + 3) / 4;
+    int lpr = 1;
+    while (lpr < quads && lpr < 64) lpr *= 2;
+    b.lpr = lpr;
+    const int rows = split_rows_per_wg() * (64 / lpr);
+    b.gx = (quads + lpr - 1) / lpr;
+    b.gy = (y1 - y0 + rows - 1) / rows;
+    b.gz = (z1 - z0 + zchunk - 1) / zchunk;
+    return b;
+}
+
+void add_box(BoxList &bl, const Box &b)
+{
+    if (b.x1 <= b.x0 || b.y1 <= b.y0 || b.z1 <= b.z0) return;
+    bl.b[bl.n] = b;
+    bl.b[bl.n].first = bl.total;
+    bl.total += b.gx * b.gy * b.gz;
+    bl.n += 1;
+}
+
+// Interior box of the fused kernel: on every axis with a CPML layer of T
+// cells, the cells from T+1 (T+4 on x, to keep quads aligned) to n-T, so
+// that neither its cells nor its one-cell halos (x-1, y-1, z-1) are in the
+// shell; the rest of the domain becomes up to six frame boxes.
+void plan_boxes(wf_ctx *c)
+{
+    const Geo &g = c->g;
+    const int n[3] = {g.nx, g.ny, g.nz};
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        if (g.T[a] == 0) {
+            lo[a] = 0;
+            hi[a] = n[a];
+        } else if (a == 0) {
+            // halo x-1 outside the layer; rows start on a 128-byte line
+            lo[a] = (g.T[a] + 1 + 31) / 32 * 32;
+            hi[a] = n[a] - g.T[a];
+        } else {
+            lo[a] = g.T[a] + 1;
+            hi[a] = n[a] - g.T[a];
+        }
+        if (hi[a] < lo[a]) hi[a] = lo[a];
+    }
+    const int zc = c->zchunk;
+    const int fr = fused_rows_per_tile();
+    Box in{lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], 64, 0, 0, 0, 0};
+    const bool empty = hi[0] <= lo[0] || hi[1] <= lo[1] || hi[2] <= lo[2];
+    if (empty) {
+        in = Box{0, 0, 0, 0, 0, 0, 64, 0, 0, 0, 0};
+        lo[0] = hi[0] = 0;
+        lo[1] = hi[1] = 0;
+        lo[2] = hi[2] = 0;
+    } else {
+        in.gx = (hi[0] - lo[0] + 255) / 256;
+        in.gy = (hi[1] - lo[1] + fr - 1) / fr;
+        in.gz = (hi[2] - lo[2] + zc - 1) / zc;
+    }
+    c->interior = in;
+    BoxList f{};
+    if (empty) {
+        add_box(f, split_box(0, n[0], 0, n[1], 0, n[2], zc));
+    } else {
+        add_box(f, split_box(0, n[0], 0, n[1], 0, lo[2], zc));
+        add_box(f, split_box(0, n[0], 0, n[1], hi[2], n[2], zc));
+        add_box(f, split_box(0, n[0], 0, lo[1], lo[2], hi[2], zc));
+        add_box(f, split_box(0, n[0], hi[1], n[1], lo[2], hi[2], zc));
+        add_box(f, split_box(0, lo[0], lo[1], hi[1], lo[2], hi[2], zc));
+        add_box(f, split_box(hi[0], n[0], lo[1], hi[1], lo[2], hi[2], zc));
+    }
+    c->frame = f;
+    BoxList w{};
+    add_box(w, split_box(0, n[0], 0, n[1], 0, n[2], zc));
+    c->whole = w;
+}
+
 // One full step: H phase, E phase, sources, probes (src/engine.rs:748-782).
-// ev (optional) = 4 events recorded around the H and E launches.
+//
+// Fused variant: the frame's H kernel runs on a side stream alongside the
+// interior kernel (both read state A and write disjoint cells of state B),
+// then the frame's E kernel runs once both are done.  With ev (6 timing
+// events) everything runs on the main stream, one kernel at a time, with an
+// event pair around the fused (0, 1), H (2, 3) and E (4, 5) launches.
 int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
 {
-    StepLaunch s{c->g, c->p, c->alt, c->pml, c->fused ? c->fzchunk : c->zchunk, c->stream};
+    hipStream_t st = c->stream;
     if (c->fused) {
-        WF_HIP(launch_fused(s, ev));
+        const StepLaunch s{c->g, c->p, c->alt, c->pml, c->zchunk, st};
+        const bool overlap = !ev && c->frame.total > 0 && c->interior.gx > 0 && c->side;
+        if (overlap) {
+            WF_HIP(hipEventRecord(c->fork, st));
+            WF_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
+            WF_HIP(launch_h(s, c->frame, c->side));
+            WF_HIP(hipEventRecord(c->join, c->side));
+            WF_HIP(launch_fused(s, c->interior, st));
+            WF_HIP(hipStreamWaitEvent(st, c->join, 0));
+        } else {
+            if (ev) WF_HIP(hipEventRecord(ev[0], st));
+            WF_HIP(launch_fused(s, c->interior, st));
+            if (ev) WF_HIP(hipEventRecord(ev[1], st));
+            if (ev) WF_HIP(hipEventRecord(ev[2], st));
+            WF_HIP(launch_h(s, c->frame, st));
+            if (ev) WF_HIP(hipEventRecord(ev[3], st));
+        }
+        if (ev) WF_HIP(hipEventRecord(ev[4], st));
+        WF_HIP(launch_e(s, c->frame, st));
+        if (ev) WF_HIP(hipEventRecord(ev[5], st));
         std::swap(c->p, c->alt);
     } else {
-        WF_HIP(launch_split(s, ev));
+        const StepLaunch s{c->g, c->p, c->p, c->pml, c->zchunk, st};
+        if (ev) {
+            WF_HIP(hipEventRecord(ev[0], st));
+            WF_HIP(hipEventRecord(ev[1], st));
+            WF_HIP(hipEventRecord(ev[2], st));
+        }
+        WF_HIP(launch_h(s, c->whole, st));
+        if (ev) WF_HIP(hipEventRecord(ev[3], st));
+        if (ev) WF_HIP(hipEventRecord(ev[4], st));
+        WF_HIP(launch_e(s, c->whole, st));
+        if (ev) WF_HIP(hipEventRecord(ev[5], st));
     }
     if (c->nsrc) {
         hipLaunchKernelGGL(k_sources, dim3(1), dim3(64), 0, c->stream, c->p, c->d_src,
@@ -362,8 +480,11 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
         wf_destroy(c);
         return rc;
     };
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(WF_ERR_HIP, "hipStreamCreate failed"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(WF_ERR_HIP, "stream / event creation failed"));
 
     if (const char *kv = std::getenv("WF_KERNEL")) {
         if (std::strcmp(kv, "split") == 0) c->fused = false;
@@ -472,8 +593,9 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
         return bail(fail(WF_ERR_HIP, "initialisation failed"));
     if (const char *zc = std::getenv("WF_ZCHUNK")) {
         const int v = std::atoi(zc);
-        if (v > 0) c->zchunk = c->fzchunk = v;
+        if (v > 0) c->zchunk = v;
     }
+    plan_boxes(c);
     *out = c;
     return WF_OK;
 }
@@ -494,6 +616,10 @@ void wf_destroy(wf_ctx *c)
     dfree(c->d_psi);
     dfree(c->d_shell_list);
     dfree(c->d_stage);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -707,32 +833,35 @@ int wf_read_trace(wf_ctx *c, float *trace, uint64_t max_steps)
     return WF_OK;
 }
 
-int wf_time_kernels(wf_ctx *c, uint64_t nsteps, double *h_ms, double *e_ms)
+int wf_time_kernels(wf_ctx *c, uint64_t nsteps, double *fused_ms, double *h_ms, double *e_ms)
 {
     clear_error();
-    if (!c || !h_ms || !e_ms) return fail(WF_ERR_INVALID, "wf_time_kernels: null pointer");
+    if (!c || !fused_ms || !h_ms || !e_ms)
+        return fail(WF_ERR_INVALID, "wf_time_kernels: null pointer");
     if (c->nsrc && c->step + nsteps > c->src_base + c->src_steps)
         return fail(WF_ERR_STATE, "wf_time_kernels: source table too short");
     if (c->nprobe && c->step + nsteps > c->probe_base + c->probe_steps)
         return fail(WF_ERR_STATE, "wf_time_kernels: probe table too short");
     WF_HIP(hipSetDevice(c->device));
-    hipEvent_t ev[4];
+    hipEvent_t ev[6];
     for (auto &e : ev) WF_HIP(hipEventCreate(&e));
-    double hs = 0.0, es = 0.0;
+    double acc[3] = {0.0, 0.0, 0.0};
     int rc = WF_OK;
     for (uint64_t s = 0; s < nsteps && rc == WF_OK; ++s) {
         rc = launch_step(c, ev);
         if (rc) break;
-        WF_HIP(hipEventSynchronize(ev[3]));
-        float a = 0.f, b = 0.f;
-        WF_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
-        WF_HIP(hipEventElapsedTime(&b, ev[2], ev[3]));
-        hs += a;
-        es += b;
+        WF_HIP(hipEventSynchronize(ev[5]));
+        for (int k = 0; k < 3; ++k) {
+            float ms = 0.f;
+            WF_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+            acc[k] += ms;
+        }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
-    *h_ms = nsteps ? hs / (double)nsteps : 0.0;
-    *e_ms = nsteps ? es / (double)nsteps : 0.0;
+    const double d = nsteps ? (double)nsteps : 1.0;
+    *fused_ms = acc[0] / d;
+    *h_ms = acc[1] / d;
+    *e_ms = acc[2] / d;
     return rc;
 }
 
@@ -788,21 +917,26 @@ const char *wf_kernel_variant(const wf_ctx *c)
     return !c ? "" : (c->fused ? "fused" : "split");
 }
 
-int wf_phase_bytes(const wf_ctx *c, uint64_t *h_bytes, uint64_t *e_bytes)
+int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
+                    uint64_t *e_bytes)
 {
-    if (!c || !h_bytes || !e_bytes) return fail(WF_ERR_INVALID, "wf_phase_bytes: null pointer");
-    const uint64_t nx = c->g.nx, ny = c->g.ny, nz = c->g.nz, N = nx * ny * nz;
-    // non-identity psi terms per phase: 2 per graded axis per voxel
+    if (!c || !fused_bytes || !h_bytes || !e_bytes)
+        return fail(WF_ERR_INVALID, "wf_kernel_bytes: null pointer");
+    const uint64_t nx = c->g.nx, ny = c->g.ny, nz = c->g.nz;
+    // non-identity psi terms per phase: 2 per graded axis per cell
     const uint64_t terms = 2ull * (2ull * c->g.T[0] * ny * nz + 2ull * c->g.T[1] * nx * nz +
                                    2ull * c->g.T[2] * nx * ny);
-    if (c->fused) {
-        // one launch: 24 B read + 24 B write of E and H, 1 B id per cell
-        *h_bytes = 49ull * N + 16ull * terms;
-        *e_bytes = 0;
-    } else {
-        *h_bytes = 37ull * N + 8ull * terms;
-        *e_bytes = 37ull * N + 8ull * terms;
+    const BoxList &bl = c->fused ? c->frame : c->whole;
+    uint64_t cells = 0;
+    for (int k = 0; k < bl.n; ++k) {
+        const Box &b = bl.b[k];
+        cells += (uint64_t)(b.x1 - b.x0) * (b.y1 - b.y0) * (b.z1 - b.z0);
     }
+    const Box &in = c->interior;
+    *fused_bytes = c->fused ? 49ull * (uint64_t)(in.x1 - in.x0) * (in.y1 - in.y0) * (in.z1 - in.z0)
+                            : 0;
+    *h_bytes = 37ull * cells + 8ull * terms;
+    *e_bytes = 37ull * cells + 8ull * terms;
     return WF_OK;
 }
 

@@ -117,20 +117,44 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// A box of cells [x0,x1) x [y0,y1) x [z0,z1) and its tiling: lpr lanes per
+// row (split kernels), gx*gy*gz tiles, the first of which has workgroup
+// index `first` in its launch.
+struct Box {
+    int x0, x1, y0, y1, z0, z1;
+    int lpr;
+    int gx, gy, gz;
+    int first;
+};
+
+constexpr int kMaxBoxes = 6;
+struct BoxList {
+    Box b[kMaxBoxes];
+    int n;
+    int total;  // workgroups over all boxes
+};
+
 // Launch parameters for one timestep.
 struct StepLaunch {
     Geo g;
-    Ptrs cur;  // state at step n (updated in place by the split kernels)
-    Ptrs alt;  // state n+1 written by the fused kernel
+    Ptrs cur;  // state at step n
+    Ptrs alt;  // state n+1 (== cur for the in-place split variant)
     bool pml;
     int zchunk;
     hipStream_t stream;
 };
 
-// H kernel then E kernel, in place on cur.  ev (optional, 4 events) brackets
-// the two launches.
-hipError_t launch_split(const StepLaunch &s, hipEvent_t *ev);
-// One single-pass kernel cur -> alt.  ev (optional) brackets it (ev[0..1]).
-hipError_t launch_fused(const StepLaunch &s, hipEvent_t *ev);
+// H kernel over the boxes: reads E^n, H^{n-1/2}, psi from cur; writes
+// H(n+1/2) and the H-phase psi to alt.
+hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st);
+// E kernel over the boxes: reads H(n+1/2) from alt and E^n / psi from cur;
+// writes E(n+1) and the E-phase psi to alt.
+hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
+// Single-pass kernel over the interior box, cur -> alt (no CPML inside).
+hipError_t launch_fused(const StepLaunch &s, const Box &interior, hipStream_t st);
+// Tile shapes of the two kernels (owned rows per fused tile, rows per split
+// workgroup of lpr = 64).
+int fused_rows_per_tile();
+int split_rows_per_wg();
 
 }  // namespace wf

@@ -136,20 +136,22 @@ class DeviceEngine:
         check(lib().wf_time_steps(self._h, nsteps, byref(ms)))
         return ms.value
 
-    def time_kernels(self, nsteps: int):
-        """Mean per-launch ms of the H-phase and E-phase kernels over nsteps
-        full steps (HIP events on the engine's stream)."""
-        h, e = c_double(), c_double()
-        check(lib().wf_time_kernels(self._h, nsteps, byref(h), byref(e)))
-        return h.value, e.value
+    def time_kernels(self, nsteps: int) -> dict:
+        """Mean per-launch ms of each step kernel over nsteps full steps (HIP
+        events on the engine's stream): fused interior kernel, H and E
+        kernels."""
+        f, h, e = c_double(), c_double(), c_double()
+        check(lib().wf_time_kernels(self._h, nsteps, byref(f), byref(h), byref(e)))
+        return {"k_fused4": f.value, "k_h4": h.value, "k_e4": e.value}
+
+    def kernel_bytes(self) -> dict:
+        """Algorithmic HBM bytes per launch of each step kernel."""
+        f, h, e = c_uint64(), c_uint64(), c_uint64()
+        check(lib().wf_kernel_bytes(self._h, byref(f), byref(h), byref(e)))
+        return {"k_fused4": f.value, "k_h4": h.value, "k_e4": e.value}
 
     def kernel_variant(self) -> str:
         return lib().wf_kernel_variant(self._h).decode()
-
-    def phase_bytes(self):
-        h, e = c_uint64(), c_uint64()
-        check(lib().wf_phase_bytes(self._h, byref(h), byref(e)))
-        return h.value, e.value
 
     # -- state transfer --------------------------------------------------------
     def read_fields(self) -> np.ndarray:

@@ -125,7 +125,7 @@ def random_fill_text(n: int, seed: int = 3, nspheres: int = 200, nboxes: int = 2
     return "\n".join(lines) + "\n"
 
 
-def random_1024(steps: int = 1000, n: int = 1024, seed: int = 3) -> Workload:
+def random_1024(steps: int = 1000, n: int = 1024, seed: int = 3, pml: int = 20) -> Workload:
     dims = GridDims(n, n, n)
     dt, dx = DT_COURANT_099, DX
     text = random_fill_text(n, seed)
@@ -133,5 +133,5 @@ def random_1024(steps: int = 1000, n: int = 1024, seed: int = 3) -> Workload:
     c = n // 2
     src = Source(c, c, c, SC.Ez, 1.0, Waveform.sinusoid(5.0e9))
     probes = [Probe(c + 20, c, c, PC.Ez, [5.0e9], 0.0)]
-    return Workload(f"random_{n}", dims, dx, dt, ids, table, PmlConfig(thickness=20), [src],
+    return Workload(f"random_{n}", dims, dx, dt, ids, table, PmlConfig(thickness=pml), [src],
                     probes, steps, scene_text=text, notes={"materials": nmat})
