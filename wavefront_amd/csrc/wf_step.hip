@@ -416,13 +416,71 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
     const int it0 = zpre >= 0 ? -1 : 0;
     const int nit = z1 - z0;
 
+    // Everything one plane z contributes from state A: E at z+1 and at y+1,
+    // H at z, the material ids, the edge lane's x+1 cells, and the x-halo
+    // cell's inputs.  The next plane's copy is loaded before the current
+    // plane is computed, so its latency overlaps the arithmetic, the LDS
+    // exchange and the stores.
+    struct Plane {
+        f4 ex_n, ey_n, ez_n, ez_py, ex_py, hx_o, hy_o, hz_o;
+        uint32_t ids;
+        float sez, sey;                           // edge lane: E at x+4
+        float hexn, heyn, hezn, hexpy, hhy, hhz;  // x-halo cell
+        int hid;
+    };
+    auto plane_z = [&](int it) { return it < 0 ? zpre : z0 + it; };
+    auto load = [&](Plane &P, int it) {
+        const int z = plane_z(it);
+        const size_t zo = (size_t)z * plane;
+        const int zp = plus1(z, g.nz, g.periodic & 4u);
+        P.ex_n = P.ey_n = P.ez_n = P.ez_py = P.ex_py = P.hx_o = P.hy_o = P.hz_o = splat(0.f);
+        P.ids = 0;
+        P.sez = P.sey = 0.f;
+        P.hexn = P.heyn = P.hezn = P.hexpy = P.hhy = P.hhz = 0.f;
+        P.hid = 0;
+        if (live) {
+            if (zp >= 0) {
+                const size_t on = (size_t)zp * plane + row;
+                P.ex_n = ld4(aex + on);
+                P.ey_n = ld4(aey + on);
+                P.ez_n = ld4(aez + on);
+            }
+            if (yp >= 0) {
+                P.ez_py = ld4(aez + zo + rowp_y);
+                P.ex_py = ld4(aex + zo + rowp_y);
+            }
+            P.hx_o = ld4(ahx + zo + row);
+            P.hy_o = ld4(ahy + zo + row);
+            P.hz_o = ld4(ahz + zo + row);
+            P.ids = *reinterpret_cast<const uint32_t *>(a.id + zo + row);
+            if (edge && xq >= 0) {
+                P.sez = aez[zo + rowq];
+                P.sey = aey[zo + rowq];
+            }
+        }
+        if (hlive) {
+            if (zp >= 0) {
+                const size_t on = (size_t)zp * plane + rowh;
+                P.hexn = aex[on];
+                P.heyn = aey[on];
+                P.hezn = aez[on];
+            }
+            if (it >= 0) {
+                P.hexpy = yp >= 0 ? aex[zo + rowh_py] : 0.f;
+                P.hid = a.id[zo + rowh];
+                P.hhy = ahy[zo + rowh];
+                P.hhz = ahz[zo + rowh];
+            }
+        }
+    };
+
     // E^n of the own quad (and of the halo cell) at the current plane;
     // H(n+1/2) of the own quad one plane below.
     f4 ex_c = splat(0.f), ey_c = splat(0.f), ez_c = splat(0.f);
     f4 hx_mz = splat(0.f), hy_mz = splat(0.f);
     float hex = 0.f, hey = 0.f, hez = 0.f;
     {
-        const size_t o = (size_t)(it0 < 0 ? zpre : z0) * plane;
+        const size_t o = (size_t)plane_z(it0) * plane;
         if (live) {
             ex_c = ld4(aex + o + row);
             ey_c = ld4(aey + o + row);
@@ -434,76 +492,42 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
             hez = aez[o + rowh];
         }
     }
+    Plane cur;
+    load(cur, it0);
 
     for (int it = it0; it < nit; ++it) {
-        const int z = it < 0 ? zpre : z0 + it;
-        const size_t zo = (size_t)z * plane;
-        const int zp = plus1(z, g.nz, g.periodic & 4u);
+        const size_t zo = (size_t)plane_z(it) * plane;
+        Plane nxt;
+        if (it + 1 < nit) load(nxt, it + 1);
 
-        // ---- loads (state A) ------------------------------------------------
-        f4 ex_n = splat(0.f), ey_n = splat(0.f), ez_n = splat(0.f);
-        f4 ez_py = splat(0.f), ex_py = splat(0.f);
-        f4 hx_o = splat(0.f), hy_o = splat(0.f), hz_o = splat(0.f);
-        uint32_t ids = 0;
-        if (live) {
-            if (zp >= 0) {
-                const size_t on = (size_t)zp * plane + row;
-                ex_n = ld4(aex + on);
-                ey_n = ld4(aey + on);
-                ez_n = ld4(aez + on);
-            }
-            if (yp >= 0) {
-                ez_py = ld4(aez + zo + rowp_y);
-                ex_py = ld4(aex + zo + rowp_y);
-            }
-            hx_o = ld4(ahx + zo + row);
-            hy_o = ld4(ahy + zo + row);
-            hz_o = ld4(ahz + zo + row);
-            ids = *reinterpret_cast<const uint32_t *>(a.id + zo + row);
-        }
         float sez = __shfl_down(ez_c.x, 1, 64), sey = __shfl_down(ey_c.x, 1, 64);
         if (edge) {
-            sez = 0.f;
-            sey = 0.f;
-            if (live && xq >= 0) {
-                sez = aez[zo + rowq];
-                sey = aey[zo + rowq];
-            }
+            sez = cur.sez;
+            sey = cur.sey;
         }
 
         // ---- H(n+1/2) of the own quad: update_h_field -----------------------
         const f4 ez_px = f4{ez_c.y, ez_c.z, ez_c.w, sez};
         const f4 ey_px = f4{ey_c.y, ey_c.z, ey_c.w, sey};
         f4 da, db;
-        coef4(s_ch, ids, da, db);
-        f4 hxn = da * hx_o - db * ((ez_py - ez_c) - (ey_n - ey_c));
-        f4 hyn = da * hy_o - db * ((ex_n - ex_c) - (ez_px - ez_c));
-        f4 hzn = da * hz_o - db * ((ey_px - ey_c) - (ex_py - ex_c));
+        coef4(s_ch, cur.ids, da, db);
+        f4 hxn = da * cur.hx_o - db * ((cur.ez_py - ez_c) - (cur.ey_n - ey_c));
+        f4 hyn = da * cur.hy_o - db * ((cur.ex_n - ex_c) - (ez_px - ez_c));
+        f4 hzn = da * cur.hz_o - db * ((ey_px - ey_c) - (cur.ex_py - ex_c));
         if (!live) {
             hxn = splat(0.f);
             hyn = splat(0.f);
             hzn = splat(0.f);
         }
 
-        // ---- H(n+1/2) of the x-halo cell (hy, hz only) ----------------------
-        float hy_h = 0.f, hz_h = 0.f;
-        float hexn = 0.f, heyn = 0.f, hezn = 0.f;
-        if (hlive) {
-            if (zp >= 0) {
-                const size_t on = (size_t)zp * plane + rowh;
-                hexn = aex[on];
-                heyn = aey[on];
-                hezn = aez[on];
-            }
-            if (it >= 0) {
-                const float hex_py = yp >= 0 ? aex[zo + rowh_py] : 0.f;
-                const float2 c = s_ch[a.id[zo + rowh]];
-                hy_h = c.x * ahy[zo + rowh] - c.y * ((hexn - hex) - (ez_c.x - hez));
-                hz_h = c.x * ahz[zo + rowh] - c.y * ((ey_c.x - hey) - (hex_py - hex));
-            }
-        }
-
         if (it >= 0) {
+            // ---- H(n+1/2) of the x-halo cell (hy, hz only) ------------------
+            float hy_h = 0.f, hz_h = 0.f;
+            if (hlive) {
+                const float2 c = s_ch[cur.hid];
+                hy_h = c.x * cur.hhy - c.y * ((cur.hexn - hex) - (ez_c.x - hez));
+                hz_h = c.x * cur.hhz - c.y * ((ey_c.x - hey) - (cur.hexpy - hex));
+            }
             // ---- neighbours of H(n+1/2) at x-1 (lane-1) and y-1 (LDS) -------
             float hz_l = __shfl_up(hzn.w, 1, 64), hy_l = __shfl_up(hyn.w, 1, 64);
             if (lane == 0) {
@@ -521,7 +545,7 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
                 const f4 hy_mx = f4{hy_l, hyn.x, hyn.y, hyn.z};
                 // ---- E(n+1): update_e_field ---------------------------------
                 f4 ca, cb;
-                coef4(s_ce, ids, ca, cb);
+                coef4(s_ce, cur.ids, ca, cb);
                 const size_t o = zo + row;
                 st4(b.f[0] + o, ca * ex_c + cb * ((hzn - hz_my) - (hyn - hy_mz)));
                 st4(b.f[1] + o, ca * ey_c + cb * ((hxn - hx_mz) - (hzn - hz_mx)));
@@ -533,12 +557,13 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
         }
         hx_mz = hxn;
         hy_mz = hyn;
-        ex_c = ex_n;
-        ey_c = ey_n;
-        ez_c = ez_n;
-        hex = hexn;
-        hey = heyn;
-        hez = hezn;
+        ex_c = cur.ex_n;
+        ey_c = cur.ey_n;
+        ez_c = cur.ez_n;
+        hex = cur.hexn;
+        hey = cur.heyn;
+        hez = cur.hezn;
+        cur = nxt;
     }
 }
 
