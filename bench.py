@@ -59,8 +59,8 @@ print(json.dumps({{"seconds": el, "cells": w.cells()}}))
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--n", type=int, default=1024, help="edge length of the random-fill cube")
     ap.add_argument("--kernel-steps", type=int, default=10)
     ap.add_argument("--pml", type=int, default=20,
@@ -112,6 +112,16 @@ def main():
     mcells = cells * args.steps / (ms / 1e3) / 1e6
     kname = max(kms, key=lambda k: kms[k])
     achieved = kbytes[kname] / (kms[kname] / 1e3) / 1e9
+    # HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    # WRITE_SIZE, scaled by a plain copy kernel), committed under profiles/.
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{w.name}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        k = t.get("kernels", {}).get(kname)
+        if k and k.get("algorithmic_bytes") == kbytes[kname]:
+            traffic = k["traffic_bytes"]
     step_bytes = sum(kbytes.values())
     # The two-pass accounting (74 B per cell + 8 B per psi term per step)
     # against which the 70%-of-roofline target is set.
@@ -170,7 +180,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "copy_kernel_gbs": None if copy_gbs is None else round(copy_gbs, 1),
         },
         "cpu_baseline": cpu,

@@ -34,9 +34,8 @@ namespace {
 __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src,
                                               float4 *__restrict__ dst, uint64_t n4)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
-        dst[i] = src[i];
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n4) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -168,6 +167,7 @@ struct wf_ctx {
     BoxList whole{};    // split variant: the whole domain
     hipStream_t side = nullptr;            // frame H kernel, overlapped
     hipEvent_t fork = nullptr, join = nullptr;
+    bool overlap = false;                  // WF_OVERLAP=1
 
     float *d_fields = nullptr;
     float *d_fields2 = nullptr;
@@ -337,7 +337,8 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
     hipStream_t st = c->stream;
     if (c->fused) {
         const StepLaunch s{c->g, c->p, c->alt, c->pml, c->zchunk, st};
-        const bool overlap = !ev && c->frame.total > 0 && c->interior.gx > 0 && c->side;
+        const bool overlap =
+            c->overlap && !ev && c->frame.total > 0 && c->interior.gx > 0 && c->side;
         if (overlap) {
             WF_HIP(hipEventRecord(c->fork, st));
             WF_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
@@ -595,6 +596,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
         const int v = std::atoi(zc);
         if (v > 0) c->zchunk = v;
     }
+    if (const char *ov = std::getenv("WF_OVERLAP")) c->overlap = std::atoi(ov) != 0;
     plan_boxes(c);
     *out = c;
     return WF_OK;
@@ -892,10 +894,15 @@ int wf_hbm_copy(uint64_t bytes, int device, int iters, double *ms_per_copy)
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipMemsetAsync(a, 0, n4 * 16, s);
-    int cus = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) cus = prop.multiProcessorCount;
-    const dim3 grid((unsigned)std::min<uint64_t>((n4 + 255) / 256, (uint64_t)cus * 16));
+    if ((n4 + 255) / 256 > 0x7fffffffull) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+        dfree(a);
+        dfree(b);
+        return fail(WF_ERR_INVALID, "wf_hbm_copy: copy too large");
+    }
+    const dim3 grid((unsigned)((n4 + 255) / 256));
     hipLaunchKernelGGL(k_copy, grid, dim3(256), 0, s, a, b, n4);  // warm
     (void)hipEventRecord(e0, s);
     for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy, grid, dim3(256), 0, s, a, b, n4);
