@@ -187,6 +187,38 @@ def test_random_fill_128():
     eng.close()
 
 
+def test_graph_replay_matches_eager(monkeypatch):
+    """The captured step graph (8 steps per replay, device step counter) gives
+    the eager launches' bits after every replay, with eager work, odd step
+    counts and host reads between replays."""
+    w = workloads.plain_64(steps=120)
+    engines = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("WF_GRAPH", mode)
+        _say(f"graph_replay WF_GRAPH={mode}")
+        eng = DeviceEngine(w.dims, w.ids, w.table, w.pml())
+        eng.set_sources(w.sources, w.dt, w.num_steps)
+        eng.set_probes(w.probes, w.dt, w.num_steps, trace=True)
+        engines[mode] = eng
+    done = 0
+    for chunk in (8, 16, 3, 8, 9, 1, 24, 51):
+        for eng in engines.values():
+            eng.step(chunk)
+        done += chunk
+        g, e = engines["1"], engines["0"]
+        assert g.step_count() == e.step_count() == done
+        assert_bitwise(g.read_fields(), e.read_fields(), f"fields after {done} steps")
+        sg, se = g.probe_sums(), e.probe_sums()
+        assert_bitwise(sg[0], se[0], "sum_cos")
+        assert list(sg[2]) == list(se[2]) == [done] * 6
+    assert_bitwise(engines["1"].read_trace(done), engines["0"].read_trace(done), "trace")
+    sim = oracle_sim_for(w)
+    sim.run(done, w.dt, threads=4)
+    assert_bitwise(engines["1"].read_fields(), sim.read_fields(), "graph vs oracle")
+    for eng in engines.values():
+        eng.close()
+
+
 def test_write_read_roundtrip():
     d = GridDims(24, 32, 40)
     f = np.random.default_rng(3).standard_normal((d.block_count(), 6, 512)).astype(np.float32)

@@ -41,10 +41,14 @@ __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src,
 // ---------------------------------------------------------------------------
 // Sources (src/source.rs:79-95): applied in list order by one lane.
 // ---------------------------------------------------------------------------
+// The table row is the device step counter minus the step at which the table
+// was set, so a captured graph of these launches stays valid for any step.
 __global__ void k_sources(Ptrs p, const DevSource *__restrict__ src, int n,
-                          const float *__restrict__ values, uint64_t row)
+                          const float *__restrict__ values, const uint64_t *__restrict__ step,
+                          uint64_t base)
 {
     if (threadIdx.x != 0) return;
+    const uint64_t row = *step - base;
     const float *v = values + row * (uint64_t)n;
     for (int s = 0; s < n; ++s) {
         float *f = p.f[src[s].comp] + src[s].off;
@@ -60,11 +64,12 @@ __global__ void k_sources(Ptrs p, const DevSource *__restrict__ src, int n,
 // ---------------------------------------------------------------------------
 __global__ void k_probes(Ptrs p, const DevProbe *__restrict__ pr, int n, int nf,
                          const double *__restrict__ cos_tab, const double *__restrict__ sin_tab,
-                         const uint8_t *__restrict__ active, uint64_t row, double *sums,
-                         unsigned long long *counts, float *trace, uint64_t trace_row)
+                         const uint8_t *__restrict__ active, const uint64_t *__restrict__ step,
+                         uint64_t base, double *sums, unsigned long long *counts, float *trace)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const uint64_t row = *step - base, trace_row = row;
     const DevProbe q = pr[i];
     const float v32 = p.f[q.comp][q.off];
     if (trace) trace[trace_row * (uint64_t)n + i] = v32;
@@ -79,6 +84,12 @@ __global__ void k_probes(Ptrs p, const DevProbe *__restrict__ pr, int n, int nf,
         ss[j] = ss[j] + v * st[j];
     }
     counts[i] += 1ull;
+}
+
+// Advances the device step counter at the end of every step.
+__global__ void k_tick(uint64_t *step)
+{
+    if (threadIdx.x == 0) *step += 1ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -195,6 +206,10 @@ struct wf_ctx {
     uint64_t probe_base = 0, probe_steps = 0;
 
     uint64_t step = 0;
+    uint64_t *d_step = nullptr;          // device copy of `step`, read by point kernels
+    hipGraphExec_t gexec = nullptr;      // kGraphSteps captured steps
+    const float *gstate = nullptr;       // p.f[0] when the graph was captured
+    bool use_graph = true;               // WF_GRAPH=0 disables
 
     float *d_stage = nullptr;
     uint64_t stage_bytes = 0;
@@ -223,8 +238,16 @@ void dfree(T *&p)
     p = nullptr;
 }
 
+// A captured step graph holds the table pointers: drop it with the tables.
+void drop_graph_exec(wf_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+}
+
 void free_sources(wf_ctx *c)
 {
+    drop_graph_exec(c);
     dfree(c->d_src);
     dfree(c->d_srcval);
     c->nsrc = 0;
@@ -233,6 +256,7 @@ void free_sources(wf_ctx *c)
 
 void free_probes(wf_ctx *c)
 {
+    drop_graph_exec(c);
     dfree(c->d_probe);
     dfree(c->d_cos);
     dfree(c->d_sin);
@@ -372,18 +396,53 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
         if (ev) WF_HIP(hipEventRecord(ev[5], st));
     }
     if (c->nsrc) {
-        hipLaunchKernelGGL(k_sources, dim3(1), dim3(64), 0, c->stream, c->p, c->d_src,
-                           (int)c->nsrc, c->d_srcval, c->step - c->src_base);
+        hipLaunchKernelGGL(k_sources, dim3(1), dim3(64), 0, st, c->p, c->d_src, (int)c->nsrc,
+                           c->d_srcval, c->d_step, c->src_base);
     }
     if (c->nprobe) {
-        const uint64_t row = c->step - c->probe_base;
-        hipLaunchKernelGGL(k_probes, dim3((c->nprobe + 63) / 64), dim3(64), 0, c->stream,
-                           c->p, c->d_probe, (int)c->nprobe, (int)c->nfreq, c->d_cos,
-                           c->d_sin, c->d_active, row, c->d_sums, c->d_counts, c->d_trace,
-                           row);
+        hipLaunchKernelGGL(k_probes, dim3((c->nprobe + 63) / 64), dim3(64), 0, st, c->p,
+                           c->d_probe, (int)c->nprobe, (int)c->nfreq, c->d_cos, c->d_sin,
+                           c->d_active, c->d_step, c->probe_base, c->d_sums, c->d_counts,
+                           c->d_trace);
     }
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, c->d_step);
     WF_HIP(hipGetLastError());
     c->step += 1;
+    return WF_OK;
+}
+
+constexpr int kGraphSteps = 8;
+
+// Captures kGraphSteps consecutive steps (an even number, so the fused
+// variant's A/B state returns to the same buffers) into one executable graph.
+// Step kernels read the step from the device counter, so the graph replays
+// for any later step until the source or probe tables change.
+int capture_graph(wf_ctx *c)
+{
+    drop_graph_exec(c);
+    const uint64_t step0 = c->step;
+    const Ptrs p0 = c->p, a0 = c->alt;
+    hipGraph_t graph = nullptr;
+    WF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = WF_OK;
+    for (int k = 0; k < kGraphSteps && rc == WF_OK; ++k) rc = launch_step(c);
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    // capture only records: restore the host view of the state
+    c->step = step0;
+    c->p = p0;
+    c->alt = a0;
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(WF_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    const hipError_t ie = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(WF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+    }
+    c->gstate = c->p.f[0];
     return WF_OK;
 }
 
@@ -502,6 +561,9 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     }
     if (int rc = dmalloc(&c->d_id, N)) return bail(rc);
     if (int rc = dmalloc(&c->d_coef, 256 * sizeof(float4))) return bail(rc);
+    if (int rc = dmalloc(&c->d_step, sizeof(uint64_t))) return bail(rc);
+    if (hipMemsetAsync(c->d_step, 0, sizeof(uint64_t), c->stream) != hipSuccess)
+        return bail(fail(WF_ERR_HIP, "hipMemset(step) failed"));
     for (int i = 0; i < 6; ++i) c->p.f[i] = c->d_fields + (uint64_t)i * N;
     c->p.id = c->alt.id = c->d_id;
     c->p.coef = c->alt.coef = c->d_coef;
@@ -592,11 +654,21 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess)
         return bail(fail(WF_ERR_HIP, "initialisation failed"));
+    // Planes per workgroup: long z-marches reuse E(z+1) / H(z-1) from
+    // registers, but small grids need short ones to keep >= 4096 workgroups
+    // (16 per CU) in flight.  WF_ZCHUNK overrides.
+    {
+        const long tiles_xy = (long)((nx + 255) / 256) * (long)((ny + 6) / 7);
+        int z = 32;
+        while (z > 2 && tiles_xy * (long)((nz + z - 1) / z) < 4096) z /= 2;
+        c->zchunk = z;
+    }
     if (const char *zc = std::getenv("WF_ZCHUNK")) {
         const int v = std::atoi(zc);
         if (v > 0) c->zchunk = v;
     }
     if (const char *ov = std::getenv("WF_OVERLAP")) c->overlap = std::atoi(ov) != 0;
+    if (const char *gr = std::getenv("WF_GRAPH")) c->use_graph = std::atoi(gr) != 0;
     plan_boxes(c);
     *out = c;
     return WF_OK;
@@ -612,6 +684,7 @@ void wf_destroy(wf_ctx *c)
     dfree(c->d_fields);
     dfree(c->d_fields2);
     dfree(c->d_psi2);
+    dfree(c->d_step);
     dfree(c->d_id);
     dfree(c->d_coef);
     dfree(c->d_prof);
@@ -720,7 +793,22 @@ int wf_step(wf_ctx *c, uint64_t nsteps)
     if (c->nprobe && c->step + nsteps > c->probe_base + c->probe_steps)
         return fail(WF_ERR_STATE, "wf_step: probe table covers %llu steps", (unsigned long long)c->probe_steps);
     WF_HIP(hipSetDevice(c->device));
-    for (uint64_t s = 0; s < nsteps; ++s)
+    uint64_t left = nsteps;
+    if (c->use_graph && left >= (uint64_t)kGraphSteps) {
+        // the graph was captured with a given A/B buffer assignment
+        if (c->gexec && c->p.f[0] != c->gstate) {
+            if (int rc = launch_step(c)) return rc;
+            --left;
+        }
+        if (!c->gexec && left >= (uint64_t)kGraphSteps)
+            if (int rc = capture_graph(c)) return rc;
+        while (c->gexec && left >= (uint64_t)kGraphSteps) {
+            WF_HIP(hipGraphLaunch(c->gexec, c->stream));
+            c->step += kGraphSteps;
+            left -= kGraphSteps;
+        }
+    }
+    for (uint64_t s = 0; s < left; ++s)
         if (int rc = launch_step(c)) return rc;
     return WF_OK;
 }
