@@ -187,6 +187,21 @@ uint64_t wf_step_count(const wf_ctx *ctx);
 uint64_t wf_field_bytes(const wf_ctx *ctx);
 int wf_read_fields(wf_ctx *ctx, void *dst);
 int wf_write_fields(wf_ctx *ctx, const void *src);
+
+/* The whole timestep loop with snapshot egress; replaces engine::run
+ * (src/engine.rs:653-819).  Runs num_steps steps; after the step with loop
+ * index k (0-based, counted from this call) it appends a snapshot
+ * (wf_field_bytes bytes, the serialize_snapshot layout) to output_path when
+ * k % snapshot_every == 0.  The file is created or truncated and written with
+ * O_DIRECT where the file system allows it (ordinary writes otherwise), in
+ * requests of at most 1 GiB; a failed or short write returns WF_ERR_IO.
+ * Snapshots are packed into one of two device buffers and copied out through
+ * pinned host chunks by a writer thread while the loop continues; the loop
+ * only waits when both buffers are still being written.  *seconds (if not
+ * NULL) receives the wall time including the final drain.  Returns after
+ * every snapshot is on file.  Source and probe tables must cover the steps. */
+int wf_run(wf_ctx *ctx, uint64_t num_steps, uint64_t snapshot_every, const char *output_path,
+           double *seconds);
 /* CPML psi memory in the reference PmlAux layout (src/layout.rs:545-558):
  * 12 x 512 f32 per shell block, shell blocks in block-major order. */
 uint64_t wf_psi_shell_blocks(const wf_ctx *ctx);

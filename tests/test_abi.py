@@ -75,6 +75,15 @@ def test_create_argument_errors():
     assert "not identity" in _lib.last_error()
 
 
+def test_run_argument_errors():
+    from ctypes import byref, c_double
+    lib = _lib.lib()
+    secs = c_double()
+    assert lib.wf_run(None, 10, 1, b"/tmp/x.bin", byref(secs)) == _lib.WF_ERR_INVALID
+    assert "null" in _lib.last_error()
+    assert lib.wf_step(None, 1) == _lib.WF_ERR_INVALID
+
+
 def test_fault_injection_hook(monkeypatch):
     from ctypes import byref, c_void_p, c_int32
     lib = _lib.lib()

@@ -79,6 +79,7 @@ _SIGNATURES = {
     "wf_field_bytes": (c_uint64, [c_void_p]),
     "wf_read_fields": (c_int, [c_void_p, c_void_p]),
     "wf_write_fields": (c_int, [c_void_p, c_void_p]),
+    "wf_run": (c_int, [c_void_p, c_uint64, c_uint64, c_char_p, POINTER(c_double)]),
     "wf_psi_shell_blocks": (c_uint64, [c_void_p]),
     "wf_read_psi": (c_int, [c_void_p, POINTER(c_float)]),
     "wf_read_probe_sums": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_uint64)]),

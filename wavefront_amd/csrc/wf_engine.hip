@@ -7,13 +7,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "wf_internal.h"
 #include "wf_kernels.h"
+#include "wf_writer.h"
 
 using namespace wf;
 
@@ -466,6 +469,23 @@ uint64_t planes_per_chunk(const wf_ctx *c)
     return n;
 }
 
+// Packs the whole current state into `out` (the reference's snapshot byte
+// order, src/engine.rs:622-640) on the main stream, without waiting.
+int pack_snapshot(wf_ctx *c, float *out)
+{
+    const uint64_t plane_vox = (uint64_t)c->g.nb[0] * c->g.nb[1] * 512;
+    uint64_t ppc = (1ull << 30) / plane_vox;  // keeps one launch under 2^22 workgroups
+    if (ppc < 1) ppc = 1;
+    for (uint64_t bz = 0; bz < (uint64_t)c->g.nb[2]; bz += ppc) {
+        const uint64_t np = std::min<uint64_t>(ppc, c->g.nb[2] - bz);
+        const uint64_t nvox = np * plane_vox;
+        hipLaunchKernelGGL(k_pack, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, c->stream,
+                           c->p, c->g, (int)bz, nvox, out + bz * plane_vox * 6);
+        WF_HIP(hipGetLastError());
+    }
+    return WF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -847,6 +867,67 @@ int wf_read_fields(wf_ctx *c, void *dst)
                               hipMemcpyDeviceToHost, c->stream));
         WF_HIP(hipStreamSynchronize(c->stream));
     }
+    return WF_OK;
+}
+
+int wf_run(wf_ctx *c, uint64_t num_steps, uint64_t snapshot_every, const char *output_path,
+           double *seconds)
+{
+    clear_error();
+    if (!c || !output_path) return fail(WF_ERR_INVALID, "wf_run: null pointer");
+    if (snapshot_every == 0) return fail(WF_ERR_INVALID, "wf_run: snapshot_every must be at least 1");
+    if (c->nsrc && c->step + num_steps > c->src_base + c->src_steps)
+        return fail(WF_ERR_STATE, "wf_run: source table covers %llu steps, %llu requested",
+                    (unsigned long long)c->src_steps, (unsigned long long)num_steps);
+    if (c->nprobe && c->step + num_steps > c->probe_base + c->probe_steps)
+        return fail(WF_ERR_STATE, "wf_run: probe table covers %llu steps", (unsigned long long)c->probe_steps);
+    WF_HIP(hipSetDevice(c->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint64_t bytes = wf_field_bytes(c);
+    SnapshotWriter writer;
+    int rc = writer.open(output_path, bytes, c->device);
+    float *snap[2] = {nullptr, nullptr};
+    hipEvent_t packed[2] = {nullptr, nullptr};
+    if (!rc) rc = dmalloc(&snap[0], bytes);
+    if (!rc) rc = dmalloc(&snap[1], bytes);
+    for (int i = 0; i < 2 && !rc; ++i)
+        if (hipEventCreateWithFlags(&packed[i], hipEventDisableTiming) != hipSuccess)
+            rc = fail(WF_ERR_HIP, "wf_run: event creation failed");
+    // Same schedule as the reference loop (src/engine.rs:747-799): after the
+    // step with loop index k, a snapshot when k % snapshot_every == 0.  Steps
+    // between snapshots go to wf_step in one call (graph replays).
+    uint64_t k = 0;
+    int slot = 0;
+    while (!rc && k < num_steps) {
+        const uint64_t next = (k + snapshot_every - 1) / snapshot_every * snapshot_every;
+        const uint64_t upto = std::min(next, num_steps - 1);
+        rc = wf_step(c, upto - k + 1);
+        if (rc) break;
+        k = upto + 1;
+        if (upto % snapshot_every) continue;
+        // the only point where the loop waits on storage: the slot's
+        // previous snapshot has not been written yet
+        rc = writer.wait_free(slot);
+        if (!rc) rc = pack_snapshot(c, snap[slot]);
+        if (!rc && hipEventRecord(packed[slot], c->stream) != hipSuccess)
+            rc = fail(WF_ERR_HIP, "wf_run: event record failed");
+        if (!rc) rc = writer.submit(slot, snap[slot], packed[slot]);
+        slot ^= 1;
+    }
+    std::string msg = rc ? wf_last_error() : "";
+    const int wrc = writer.finish();  // drains every queued snapshot
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc && !wrc) {
+        rc = WF_ERR_HIP;
+        msg = "wf_run: stream synchronize failed";
+    }
+    for (auto &e : packed)
+        if (e) (void)hipEventDestroy(e);
+    dfree(snap[0]);
+    dfree(snap[1]);
+    if (rc) return fail(rc, "%s", msg.c_str());
+    if (wrc) return wrc;
+    if (seconds)
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return WF_OK;
 }
 

@@ -13,7 +13,8 @@ int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
 
 // Env hook for error-path tests: WF_FAULT_INJECT=<name> makes the named
-// entry point return WF_ERR_HIP before touching the device.
+// entry point return WF_ERR_HIP before touching the device ("short_write"
+// makes the snapshot writer report a short write instead of writing).
 bool fault_injected(const char *name);
 
 // Reference constants (src/layout.rs:29-31, std::f32::consts::PI).

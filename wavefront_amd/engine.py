@@ -9,7 +9,7 @@ snapshot cadence and file format.
 from __future__ import annotations
 
 import os
-import time
+import sys
 from ctypes import c_double, c_float, c_int32, c_uint8, c_uint32, c_uint64, c_void_p, byref
 from dataclasses import dataclass
 
@@ -128,6 +128,15 @@ class DeviceEngine:
     def sync(self) -> None:
         check(lib().wf_sync(self._h))
 
+    def run(self, num_steps: int, snapshot_every: int, output_path: str) -> float:
+        """engine::run's loop with snapshot egress (wf_run): num_steps steps,
+        a snapshot appended to output_path after every step whose loop index
+        k satisfies k % snapshot_every == 0.  Returns wall seconds."""
+        secs = c_double()
+        check(lib().wf_run(self._h, num_steps, snapshot_every, os.fsencode(output_path),
+                           byref(secs)))
+        return secs.value
+
     def step_count(self) -> int:
         return int(lib().wf_step_count(self._h))
 
@@ -218,27 +227,16 @@ def run(field_grid: np.ndarray | None, coeff_grid: CoeffGrid, pml: PmlContext | 
     config.output_path (reference byte format, no header), updates the probes'
     accumulators and returns the final field state."""
     dims = coeff_grid.dims
-    every = max(int(config.snapshot_every), 1)
+    if config.snapshot_every < 1:
+        raise ValueError("snapshot_every must be at least 1")
     with DeviceEngine(dims, coeff_grid.ids, coeff_grid.table, pml, device=device) as eng:
         if field_grid is not None:
             eng.write_fields(field_grid)
         eng.set_sources(sources, config.dt, config.num_steps)
         eng.set_probes(probes, config.dt, config.num_steps)
-        started = time.perf_counter()
-        with open(config.output_path, "wb") as out:
-            step = 0
-            while step < config.num_steps:
-                # advance to the next snapshot step (inclusive), or the end
-                nxt = ((step + every - 1) // every) * every
-                target = min(nxt, config.num_steps - 1)
-                eng.step(target - step + 1)
-                step = target + 1
-                if target % every == 0:
-                    out.write(eng.read_fields().tobytes())
-        eng.sync()
-        elapsed = time.perf_counter() - started
+        elapsed = eng.run(config.num_steps, config.snapshot_every, str(config.output_path))
         eng.probe_sums()
         final = eng.read_fields()
     print(f"wavefront: {config.num_steps} steps over 1 device in {elapsed:.3f}s "
-          f"({config.num_steps / max(elapsed, 1e-9):.1f} steps/s)", file=__import__("sys").stderr)
+          f"({config.num_steps / max(elapsed, 1e-9):.1f} steps/s)", file=sys.stderr)
     return final
