@@ -56,6 +56,67 @@ print(json.dumps({{"seconds": el, "cells": w.cells()}}))
         return {"error": f"timeout after {timeout}s"}
 
 
+def run_spheres_512(args):
+    """The 512^3 two-sphere run with snapshot egress: `steps` steps from a
+    zeroed state, a snapshot after every step k with k % every == 0 written
+    to --output through wf_run (device pack, pinned chunks, O_DIRECT writer),
+    timed end to end including the final drain; then the same steps without
+    snapshots, for the egress overhead."""
+    from wavefront_amd import workloads
+    from wavefront_amd.engine import DeviceEngine
+
+    steps, every = args.c2_steps, args.c2_every
+    w = workloads.spheres_512(steps=steps)
+    out_path = args.output or os.path.join("/tmp", "wave_trajectory_spheres_512.bin")
+    nsnap = len(range(0, steps, every))
+    snap_bytes = w.cells() * 24
+    log(f"{w.name}: {steps} steps, snapshot every {every} ({nsnap} x {snap_bytes / 1e9:.2f} GB) "
+        f"-> {out_path}")
+    eng = DeviceEngine(w.dims, w.ids, w.table, w.pml())
+    eng.set_sources(w.sources, w.dt, steps)
+    eng.set_probes(w.probes, w.dt, steps)
+    secs = eng.run(steps, every, out_path)
+    written = os.path.getsize(out_path)
+    eng.close()
+    if not args.keep_output:
+        os.unlink(out_path)
+    eng = DeviceEngine(w.dims, w.ids, w.table, w.pml())
+    eng.set_sources(w.sources, w.dt, steps)
+    eng.set_probes(w.probes, w.dt, steps)
+    ms = eng.time_steps(steps)
+    eng.close()
+    mcells = w.cells() * steps / secs / 1e6
+    log(f"with egress {secs:.2f}s ({mcells:.0f} Mcell/s), steps alone {ms / 1e3:.2f}s, "
+        f"{written / 1e9:.1f} GB written")
+    out = {
+        "metric": "Mcell-updates/s incl. snapshot egress, 512^3 two spheres, FP32, 1 MI355X",
+        "value": round(mcells, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": round(secs * 1e3 / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{w.name}: 512^3, sphere eps 4.0 r 80 + sphere eps 2.25 sigma 0.01 r 60, "
+                        f"CPML 16, 10 GHz Ricker Ez source, 32 probes, snapshot every {every}",
+            "cells": w.cells(),
+            "snapshots": nsnap,
+            "snapshot_bytes": snap_bytes,
+            "bytes_written": written,
+            "seconds_with_egress": round(secs, 3),
+            "seconds_steps_only": round(ms / 1e3, 3),
+            "egress_gbs_effective": round(written / secs / 1e9, 2),
+            "output": out_path,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,12 +131,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-gib", type=float, default=2.0,
                     help="size of the plain copy that measures the HBM ceiling (0 = skip)")
+    ap.add_argument("--workload", choices=["random_1024", "spheres_512"], default="random_1024",
+                    help="spheres_512 = the 512^3 two-sphere run with snapshot egress")
+    ap.add_argument("--c2-steps", type=int, default=2000)
+    ap.add_argument("--c2-every", type=int, default=100)
+    ap.add_argument("--output", default="", help="snapshot file of the spheres_512 run")
+    ap.add_argument("--keep-output", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part "
                                    "of this build (--gpus must be 1)"}))
         sys.exit(2)
+
+    if args.workload == "spheres_512":
+        run_spheres_512(args)
+        return
 
     from wavefront_amd import workloads
     from wavefront_amd.engine import DeviceEngine, hbm_copy_gbs
