@@ -202,6 +202,36 @@ int wf_write_fields(wf_ctx *ctx, const void *src);
  * every snapshot is on file.  Source and probe tables must cover the steps. */
 int wf_run(wf_ctx *ctx, uint64_t num_steps, uint64_t snapshot_every, const char *output_path,
            double *seconds);
+
+/* ---------------------------------------------------------------------- */
+/* Batches of independent scenes                                            */
+/* ---------------------------------------------------------------------- */
+/* Replaces calling the loop body of engine::run (src/engine.rs:747-782) for
+ * each of many small scenes in turn: the scenes (contexts created with
+ * wf_create on one device, sources and probes set as usual) advance together,
+ * one H launch and one E launch per step covering every scene, then one
+ * launch each for all sources and all probes.  Each scene's results are
+ * bit-identical to stepping it alone.  The batch updates each context's
+ * current state in place and advances its step count; contexts stay owned by
+ * the caller and must outlive the batch.  Work queued on a context's own
+ * stream is waited for at each wf_batch_step; wf_batch_sync before reading a
+ * scene back. */
+typedef struct wf_batch wf_batch;
+int wf_batch_create(uint32_t n, wf_ctx *const *scenes, wf_batch **out);
+void wf_batch_destroy(wf_batch *batch);
+int wf_batch_step(wf_batch *batch, uint64_t nsteps);
+int wf_batch_sync(wf_batch *batch);
+/* nsteps steps between two events on the batch stream; *total_ms = their time. */
+int wf_batch_time_steps(wf_batch *batch, uint64_t nsteps, double *total_ms);
+/* Autotuner: runs steps_per_candidate real steps (the first untimed) under
+ * each candidate tiling (z-chunk, lanes-per-row cap), keeps the fastest and
+ * reports it; *steps_taken = steps advanced (results do not depend on the
+ * tiling).  WF_BATCH_ZCHUNK sets the initial z-chunk (default 8). */
+int wf_batch_tune(wf_batch *batch, uint32_t steps_per_candidate, int *zchunk,
+                  int *lanes_per_row, uint64_t *steps_taken);
+int wf_batch_set_tiling(wf_batch *batch, int zchunk, int lanes_per_row);
+/* Sum of the scenes' cells. */
+uint64_t wf_batch_cells(const wf_batch *batch);
 /* CPML psi memory in the reference PmlAux layout (src/layout.rs:545-558):
  * 12 x 512 f32 per shell block, shell blocks in block-major order. */
 uint64_t wf_psi_shell_blocks(const wf_ctx *ctx);

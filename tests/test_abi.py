@@ -84,6 +84,19 @@ def test_run_argument_errors():
     assert lib.wf_step(None, 1) == _lib.WF_ERR_INVALID
 
 
+def test_batch_argument_errors():
+    from ctypes import byref, c_void_p
+    lib = _lib.lib()
+    h = c_void_p()
+    assert lib.wf_batch_create(0, None, byref(h)) == _lib.WF_ERR_INVALID
+    arr = (c_void_p * 2)(None, None)
+    assert lib.wf_batch_create(2, arr, byref(h)) == _lib.WF_ERR_INVALID
+    assert "null" in _lib.last_error()
+    assert lib.wf_batch_step(None, 1) == _lib.WF_ERR_INVALID
+    assert lib.wf_batch_set_tiling(None, 8, 64) == _lib.WF_ERR_INVALID
+    assert lib.wf_batch_cells(None) == 0
+
+
 def test_fault_injection_hook(monkeypatch):
     from ctypes import byref, c_void_p, c_int32
     lib = _lib.lib()

@@ -80,6 +80,15 @@ _SIGNATURES = {
     "wf_read_fields": (c_int, [c_void_p, c_void_p]),
     "wf_write_fields": (c_int, [c_void_p, c_void_p]),
     "wf_run": (c_int, [c_void_p, c_uint64, c_uint64, c_char_p, POINTER(c_double)]),
+    "wf_batch_create": (c_int, [c_uint32, POINTER(c_void_p), POINTER(c_void_p)]),
+    "wf_batch_destroy": (None, [c_void_p]),
+    "wf_batch_step": (c_int, [c_void_p, c_uint64]),
+    "wf_batch_sync": (c_int, [c_void_p]),
+    "wf_batch_time_steps": (c_int, [c_void_p, c_uint64, POINTER(c_double)]),
+    "wf_batch_tune": (c_int, [c_void_p, c_uint32, POINTER(c_int), POINTER(c_int),
+                              POINTER(c_uint64)]),
+    "wf_batch_set_tiling": (c_int, [c_void_p, c_int, c_int]),
+    "wf_batch_cells": (c_uint64, [c_void_p]),
     "wf_psi_shell_blocks": (c_uint64, [c_void_p]),
     "wf_read_psi": (c_int, [c_void_p, POINTER(c_float)]),
     "wf_read_probe_sums": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_uint64)]),

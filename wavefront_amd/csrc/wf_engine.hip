@@ -44,14 +44,12 @@ __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src,
 // ---------------------------------------------------------------------------
 // Sources (src/source.rs:79-95): applied in list order by one lane.
 // ---------------------------------------------------------------------------
-// The table row is the device step counter minus the step at which the table
-// was set, so a captured graph of these launches stays valid for any step.
-__global__ void k_sources(Ptrs p, const DevSource *__restrict__ src, int n,
-                          const float *__restrict__ values, const uint64_t *__restrict__ step,
-                          uint64_t base)
+// `row` is the device step counter minus the step at which the table was
+// set, so a captured graph of these launches stays valid for any step.
+__device__ __forceinline__ void apply_sources(const Ptrs &p, const DevSource *__restrict__ src,
+                                              int n, const float *__restrict__ values,
+                                              uint64_t row)
 {
-    if (threadIdx.x != 0) return;
-    const uint64_t row = *step - base;
     const float *v = values + row * (uint64_t)n;
     for (int s = 0; s < n; ++s) {
         float *f = p.f[src[s].comp] + src[s].off;
@@ -62,20 +60,26 @@ __global__ void k_sources(Ptrs p, const DevSource *__restrict__ src, int n,
     }
 }
 
+__global__ void k_sources(Ptrs p, const DevSource *__restrict__ src, int n,
+                          const float *__restrict__ values, const uint64_t *__restrict__ step,
+                          uint64_t base)
+{
+    if (threadIdx.x != 0) return;
+    apply_sources(p, src, n, values, *step - base);
+}
+
 // ---------------------------------------------------------------------------
 // Probes (src/probe.rs:126-138): one lane per probe, frequencies in order.
 // ---------------------------------------------------------------------------
-__global__ void k_probes(Ptrs p, const DevProbe *__restrict__ pr, int n, int nf,
-                         const double *__restrict__ cos_tab, const double *__restrict__ sin_tab,
-                         const uint8_t *__restrict__ active, const uint64_t *__restrict__ step,
-                         uint64_t base, double *sums, unsigned long long *counts, float *trace)
+__device__ __forceinline__ void accumulate_probe(const Ptrs &p, const DevProbe &q, int i, int n,
+                                                 int nf, const double *__restrict__ cos_tab,
+                                                 const double *__restrict__ sin_tab,
+                                                 const uint8_t *__restrict__ active,
+                                                 uint64_t row, double *sums,
+                                                 unsigned long long *counts, float *trace)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t row = *step - base, trace_row = row;
-    const DevProbe q = pr[i];
     const float v32 = p.f[q.comp][q.off];
-    if (trace) trace[trace_row * (uint64_t)n + i] = v32;
+    if (trace) trace[row * (uint64_t)n + i] = v32;
     if (!active[row * (uint64_t)n + i]) return;
     const double v = (double)v32;
     const double *ct = cos_tab + row * (uint64_t)nf + q.foff;
@@ -87,6 +91,37 @@ __global__ void k_probes(Ptrs p, const DevProbe *__restrict__ pr, int n, int nf,
         ss[j] = ss[j] + v * st[j];
     }
     counts[i] += 1ull;
+}
+
+__global__ void k_probes(Ptrs p, const DevProbe *__restrict__ pr, int n, int nf,
+                         const double *__restrict__ cos_tab, const double *__restrict__ sin_tab,
+                         const uint8_t *__restrict__ active, const uint64_t *__restrict__ step,
+                         uint64_t base, double *sums, unsigned long long *counts, float *trace)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    accumulate_probe(p, pr[i], i, n, nf, cos_tab, sin_tab, active, *step - base, sums, counts,
+                     trace);
+}
+
+// Batched point work, one workgroup per scene: sources, then (next launch)
+// probes and the scene's step counter.
+__global__ void k_sources_batch(const BatchScene *__restrict__ sc)
+{
+    const BatchScene &d = sc[blockIdx.x];
+    if (threadIdx.x != 0 || d.nsrc == 0) return;
+    apply_sources(d.p, d.src, d.nsrc, d.srcval, *d.step - d.src_base);
+}
+
+__global__ void k_probes_batch(const BatchScene *__restrict__ sc)
+{
+    const BatchScene &d = sc[blockIdx.x];
+    const uint64_t step = *d.step;
+    for (int i = threadIdx.x; i < d.nprobe; i += blockDim.x)
+        accumulate_probe(d.p, d.probe[i], i, d.nprobe, d.nfreq, d.cos_tab, d.sin_tab, d.active,
+                         step - d.probe_base, d.sums, d.counts, d.trace);
+    __syncthreads();  // every lane has read the counter
+    if (threadIdx.x == 0) *d.step = step + 1ull;
 }
 
 // Advances the device step counter at the end of every step.
@@ -273,13 +308,14 @@ void free_probes(wf_ctx *c)
 }
 
 // Tiles a box for the split kernels: lanes per row = the smallest power of
-// two covering the box's quads (at most 64), kSY waves of 64/lpr rows each.
-Box split_box(int x0, int x1, int y0, int y1, int z0, int z1, int zchunk)
+// two covering the box's quads (at most lpr_cap <= 64), kSY waves of 64/lpr
+// rows each.
+Box split_box(int x0, int x1, int y0, int y1, int z0, int z1, int zchunk, int lpr_cap = 64)
 {
     Box b{x0, x1, y0, y1, z0, z1, 64, 0, 0, 0, 0};
     const int quads = (x1 - x0 + 3) / 4;
     int lpr = 1;
-    while (lpr < quads && lpr < 64) lpr *= 2;
+    while (lpr < quads && lpr < lpr_cap) lpr *= 2;
     b.lpr = lpr;
     const int rows = split_rows_per_wg() * (64 / lpr);
     b.gx = (quads + lpr - 1) / lpr;
@@ -1134,6 +1170,282 @@ int wf_time_steps(wf_ctx *c, uint64_t nsteps, double *total_ms)
     (void)hipEventDestroy(e1);
     *total_ms = ms;
     return rc;
+}
+
+// ===========================================================================
+// Batches: many scenes advanced together, one launch per phase
+// ===========================================================================
+}  // extern "C"
+
+struct wf_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;  // end of the last batched steps
+    std::vector<wf_ctx *> scenes;
+    BatchScene *d_scenes = nullptr;
+    uint32_t *d_wg = nullptr;
+    uint64_t wg_cap = 0;
+    int total = 0;
+    int zchunk = 8;
+    int lpr_cap = 64;
+    int tiled_zchunk = -1, tiled_lpr = -1;  // tiling the wg table was built for
+    std::vector<Box> boxes;
+};
+
+namespace {
+
+// Scene descriptors from the contexts' current state and tables; the
+// workgroup -> scene table is rebuilt when the tiling changes.
+int batch_upload(wf_batch *b)
+{
+    const size_t n = b->scenes.size();
+    if (b->tiled_zchunk != b->zchunk || b->tiled_lpr != b->lpr_cap) {
+        b->boxes.resize(n);
+        uint64_t total = 0;
+        for (size_t i = 0; i < n; ++i) {
+            const Geo &g = b->scenes[i]->g;
+            Box bx = split_box(0, g.nx, 0, g.ny, 0, g.nz, b->zchunk, b->lpr_cap);
+            bx.first = (int)total;
+            total += (uint64_t)bx.gx * bx.gy * bx.gz;
+            b->boxes[i] = bx;
+        }
+        if (total > 0x7fffffffull) return fail(WF_ERR_INVALID, "batch: too many workgroups");
+        std::vector<uint32_t> wg(total);
+        for (size_t i = 0; i < n; ++i) {
+            const Box &bx = b->boxes[i];
+            const uint64_t cnt = (uint64_t)bx.gx * bx.gy * bx.gz;
+            for (uint64_t k = 0; k < cnt; ++k) wg[bx.first + k] = (uint32_t)i;
+        }
+        if (total > b->wg_cap) {
+            dfree(b->d_wg);
+            b->wg_cap = 0;
+            if (int rc = dmalloc(&b->d_wg, total * sizeof(uint32_t))) return rc;
+            b->wg_cap = total;
+        }
+        if (total)
+            WF_HIP(hipMemcpy(b->d_wg, wg.data(), total * sizeof(uint32_t), hipMemcpyHostToDevice));
+        b->total = (int)total;
+        b->tiled_zchunk = b->zchunk;
+        b->tiled_lpr = b->lpr_cap;
+    }
+    std::vector<BatchScene> h(n);
+    for (size_t i = 0; i < n; ++i) {
+        const wf_ctx *c = b->scenes[i];
+        BatchScene &d = h[i];
+        std::memset(&d, 0, sizeof d);
+        d.g = c->g;
+        d.p = c->p;
+        d.box = b->boxes[i];
+        d.src = c->d_src;
+        d.srcval = c->d_srcval;
+        d.src_base = c->src_base;
+        d.nsrc = (int)c->nsrc;
+        d.probe = c->d_probe;
+        d.cos_tab = c->d_cos;
+        d.sin_tab = c->d_sin;
+        d.active = c->d_active;
+        d.sums = c->d_sums;
+        d.counts = c->d_counts;
+        d.trace = c->d_trace;
+        d.probe_base = c->probe_base;
+        d.nprobe = (int)c->nprobe;
+        d.nfreq = (int)c->nfreq;
+        d.step = c->d_step;
+    }
+    WF_HIP(hipMemcpy(b->d_scenes, h.data(), n * sizeof(BatchScene), hipMemcpyHostToDevice));
+    return WF_OK;
+}
+
+int batch_launch_steps(wf_batch *b, uint64_t nsteps)
+{
+    const unsigned n = (unsigned)b->scenes.size();
+    for (uint64_t s = 0; s < nsteps; ++s) {
+        WF_HIP(launch_batch_h(b->d_scenes, b->d_wg, b->total, b->zchunk, b->stream));
+        WF_HIP(launch_batch_e(b->d_scenes, b->d_wg, b->total, b->zchunk, b->stream));
+        hipLaunchKernelGGL(k_sources_batch, dim3(n), dim3(64), 0, b->stream, b->d_scenes);
+        hipLaunchKernelGGL(k_probes_batch, dim3(n), dim3(64), 0, b->stream, b->d_scenes);
+        WF_HIP(hipGetLastError());
+    }
+    // later work queued on a scene's own stream runs after these steps
+    WF_HIP(hipEventRecord(b->done, b->stream));
+    for (wf_ctx *c : b->scenes) {
+        WF_HIP(hipStreamWaitEvent(c->stream, b->done, 0));
+        c->step += nsteps;
+    }
+    return WF_OK;
+}
+
+// Checks the tables, waits for each scene's own queued work and uploads the
+// descriptors.
+int batch_prepare(wf_batch *b, uint64_t nsteps)
+{
+    for (size_t i = 0; i < b->scenes.size(); ++i) {
+        const wf_ctx *c = b->scenes[i];
+        if (c->nsrc && c->step + nsteps > c->src_base + c->src_steps)
+            return fail(WF_ERR_STATE, "batch scene %zu: source table covers %llu steps", i,
+                        (unsigned long long)c->src_steps);
+        if (c->nprobe && c->step + nsteps > c->probe_base + c->probe_steps)
+            return fail(WF_ERR_STATE, "batch scene %zu: probe table covers %llu steps", i,
+                        (unsigned long long)c->probe_steps);
+    }
+    WF_HIP(hipSetDevice(b->device));
+    for (wf_ctx *c : b->scenes) WF_HIP(hipStreamSynchronize(c->stream));
+    // the descriptor and workgroup tables are rewritten by synchronous copies,
+    // which do not wait for the (non-blocking) batch stream: earlier batched
+    // steps must be done reading them
+    WF_HIP(hipStreamSynchronize(b->stream));
+    return batch_upload(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+int wf_batch_create(uint32_t n, wf_ctx *const *scenes, wf_batch **out)
+{
+    clear_error();
+    if (!out || (n && !scenes)) return fail(WF_ERR_INVALID, "wf_batch_create: null pointer");
+    *out = nullptr;
+    if (n == 0) return fail(WF_ERR_INVALID, "wf_batch_create: empty batch");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!scenes[i]) return fail(WF_ERR_INVALID, "wf_batch_create: scene %u is null", i);
+        if (scenes[i]->device != scenes[0]->device)
+            return fail(WF_ERR_INVALID, "wf_batch_create: scenes on different devices");
+        for (uint32_t j = 0; j < i; ++j)
+            if (scenes[j] == scenes[i])
+                return fail(WF_ERR_INVALID, "wf_batch_create: scene %u listed twice", i);
+    }
+    wf_batch *b = new wf_batch;
+    b->device = scenes[0]->device;
+    b->scenes.assign(scenes, scenes + n);
+    if (const char *z = std::getenv("WF_BATCH_ZCHUNK")) b->zchunk = std::max(1, std::atoi(z));
+    int rc = WF_OK;
+    if (hipSetDevice(b->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess)
+        rc = fail(WF_ERR_HIP, "wf_batch_create: stream creation failed");
+    if (!rc) rc = dmalloc(&b->d_scenes, n * sizeof(BatchScene));
+    if (rc) {
+        wf_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return WF_OK;
+}
+
+void wf_batch_destroy(wf_batch *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    dfree(b->d_scenes);
+    dfree(b->d_wg);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int wf_batch_step(wf_batch *b, uint64_t nsteps)
+{
+    clear_error();
+    if (!b) return fail(WF_ERR_INVALID, "wf_batch_step: null batch");
+    if (int rc = batch_prepare(b, nsteps)) return rc;
+    return batch_launch_steps(b, nsteps);
+}
+
+int wf_batch_sync(wf_batch *b)
+{
+    if (!b) return fail(WF_ERR_INVALID, "wf_batch_sync: null batch");
+    WF_HIP(hipSetDevice(b->device));
+    WF_HIP(hipStreamSynchronize(b->stream));
+    return WF_OK;
+}
+
+int wf_batch_time_steps(wf_batch *b, uint64_t nsteps, double *total_ms)
+{
+    clear_error();
+    if (!b || !total_ms) return fail(WF_ERR_INVALID, "wf_batch_time_steps: null pointer");
+    if (int rc = batch_prepare(b, nsteps)) return rc;
+    hipEvent_t e0, e1;
+    WF_HIP(hipEventCreate(&e0));
+    WF_HIP(hipEventCreate(&e1));
+    WF_HIP(hipEventRecord(e0, b->stream));
+    int rc = batch_launch_steps(b, nsteps);
+    WF_HIP(hipEventRecord(e1, b->stream));
+    WF_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    WF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *total_ms = ms;
+    return rc;
+}
+
+int wf_batch_tune(wf_batch *b, uint32_t steps_per_candidate, int *zchunk, int *lanes_per_row,
+                  uint64_t *steps_taken)
+{
+    clear_error();
+    if (!b) return fail(WF_ERR_INVALID, "wf_batch_tune: null batch");
+    if (steps_per_candidate < 2)
+        return fail(WF_ERR_INVALID, "wf_batch_tune: need at least 2 steps per candidate");
+    // (z-chunk, lanes-per-row cap): short z-marches fill the chip with small
+    // scenes; a 16-lane cap avoids idle lanes on rows of 33-48 quads
+    static const int cand[][2] = {{4, 64}, {8, 64}, {16, 64}, {32, 64}, {8, 16}, {16, 16}};
+    const int ncand = (int)(sizeof cand / sizeof cand[0]);
+    if (int rc = batch_prepare(b, (uint64_t)ncand * steps_per_candidate)) return rc;
+    hipEvent_t e0, e1;
+    WF_HIP(hipEventCreate(&e0));
+    WF_HIP(hipEventCreate(&e1));
+    double best = 0.0;
+    int bz = b->zchunk, bl = b->lpr_cap, rc = WF_OK;
+    uint64_t taken = 0;
+    for (int k = 0; k < ncand && !rc; ++k) {
+        // every trial step is a real step: results do not depend on tiling
+        b->zchunk = cand[k][0];
+        b->lpr_cap = cand[k][1];
+        rc = batch_upload(b);
+        if (!rc) rc = batch_launch_steps(b, 1);
+        if (!rc && hipEventRecord(e0, b->stream) != hipSuccess) rc = fail(WF_ERR_HIP, "event");
+        if (!rc) rc = batch_launch_steps(b, steps_per_candidate - 1);
+        if (!rc && hipEventRecord(e1, b->stream) != hipSuccess) rc = fail(WF_ERR_HIP, "event");
+        float ms = 0.f;
+        if (!rc && (hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = fail(WF_ERR_HIP, "wf_batch_tune: event timing failed");
+        taken += steps_per_candidate;
+        if (!rc && (k == 0 || ms < best)) {
+            best = ms;
+            bz = cand[k][0];
+            bl = cand[k][1];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->zchunk = bz;
+    b->lpr_cap = bl;
+    if (zchunk) *zchunk = bz;
+    if (lanes_per_row) *lanes_per_row = bl;
+    if (steps_taken) *steps_taken = taken;
+    return rc;
+}
+
+int wf_batch_set_tiling(wf_batch *b, int zchunk, int lanes_per_row)
+{
+    if (!b) return fail(WF_ERR_INVALID, "wf_batch_set_tiling: null batch");
+    if (zchunk < 1 || lanes_per_row < 1 || lanes_per_row > 64 ||
+        (lanes_per_row & (lanes_per_row - 1)))
+        return fail(WF_ERR_INVALID, "wf_batch_set_tiling: zchunk >= 1, lanes per row a power of two <= 64");
+    b->zchunk = zchunk;
+    b->lpr_cap = lanes_per_row;
+    return WF_OK;
+}
+
+uint64_t wf_batch_cells(const wf_batch *b)
+{
+    if (!b) return 0;
+    uint64_t n = 0;
+    for (const wf_ctx *c : b->scenes) n += (uint64_t)c->g.nx * c->g.ny * c->g.nz;
+    return n;
 }
 
 }  // extern "C"

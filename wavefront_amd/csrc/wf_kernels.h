@@ -157,4 +157,32 @@ hipError_t launch_fused(const StepLaunch &s, const Box &interior, hipStream_t st
 int fused_rows_per_tile();
 int split_rows_per_wg();
 
+// One scene of a batch: its geometry and state (updated in place by the
+// split kernels over `box`, the whole scene), and the tables of its point
+// work, all read by the batched kernels from device memory.
+struct BatchScene {
+    Geo g;
+    Ptrs p;
+    Box box;  // box.first = the scene's first workgroup in the batched launch
+    const DevSource *src;
+    const float *srcval;
+    uint64_t src_base;
+    const DevProbe *probe;
+    const double *cos_tab, *sin_tab;
+    const uint8_t *active;
+    double *sums;
+    unsigned long long *counts;
+    float *trace;
+    uint64_t probe_base;
+    uint64_t *step;  // the scene's device step counter
+    int nsrc, nprobe, nfreq, pad;
+};
+
+// H / E kernels over every scene of a batch in one launch: workgroup t
+// (after XCD renumbering) belongs to scene wg_scene[t].
+hipError_t launch_batch_h(const BatchScene *scenes, const uint32_t *wg_scene, int total,
+                          int zchunk, hipStream_t st);
+hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, int total,
+                          int zchunk, hipStream_t st);
+
 }  // namespace wf

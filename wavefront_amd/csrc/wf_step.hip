@@ -82,11 +82,12 @@ __device__ __forceinline__ SplitGeo split_geo(const Box &b, int local, int zchun
     return s;
 }
 
+// One workgroup of the H kernel: tile `local` of box b.  s_co: LDS for the
+// (da, db) table.
 template <bool PML>
-__global__ __launch_bounds__(64 * kSY) void k_h4(Geo g, Ptrs src, Ptrs dst, BoxList bl,
-                                                 int zchunk)
+__device__ __forceinline__ void h4_tile(const Geo &g, const Ptrs &src, const Ptrs &dst,
+                                        const Box &b, int local, int zchunk, float2 *s_co)
 {
-    __shared__ float2 s_co[256];  // (da, db)
     const int lane = threadIdx.x, w = threadIdx.y;
     for (int i = w * 64 + lane; i < 256; i += 64 * kSY) {
         const float4 c = src.coef[i];
@@ -94,8 +95,6 @@ __global__ __launch_bounds__(64 * kSY) void k_h4(Geo g, Ptrs src, Ptrs dst, BoxL
     }
     __syncthreads();
 
-    int local;
-    const Box &b = find_box(bl, xcd_tile(blockIdx.x, bl.total), local);
     const int lpr = b.lpr;
     const SplitGeo s = split_geo(b, local, zchunk, true);
     const bool live = s.live;
@@ -213,13 +212,39 @@ __global__ __launch_bounds__(64 * kSY) void k_h4(Geo g, Ptrs src, Ptrs dst, BoxL
     }
 }
 
-// hsrc: where H(n+1/2) lives; esrc: E^n and the psi memory of step n;
-// dst: E(n+1) and psi of step n+1.
 template <bool PML>
-__global__ __launch_bounds__(64 * kSY) void k_e4(Geo g, Ptrs hsrc, Ptrs esrc, Ptrs dst,
-                                                 BoxList bl, int zchunk)
+__global__ __launch_bounds__(64 * kSY) void k_h4(Geo g, Ptrs src, Ptrs dst, BoxList bl,
+                                                 int zchunk)
 {
-    __shared__ float2 s_co[256];  // (ca, cb)
+    __shared__ float2 s_co[256];  // (da, db)
+    int local;
+    const Box &b = find_box(bl, xcd_tile(blockIdx.x, bl.total), local);
+    h4_tile<PML>(g, src, dst, b, local, zchunk, s_co);
+}
+
+// Batched H kernel: every scene in place, one launch.
+template <bool PML>
+__global__ __launch_bounds__(64 * kSY) void k_h4_batch(const BatchScene *__restrict__ sc,
+                                                       const uint32_t *__restrict__ wg_scene,
+                                                       int total, int zchunk)
+{
+    __shared__ float2 s_co[256];  // (da, db)
+    const int t = xcd_tile(blockIdx.x, total);
+    const BatchScene *d = sc + wg_scene[t];
+    const Geo g = d->g;
+    const Ptrs p = d->p;
+    const Box b = d->box;
+    if (t - b.first >= b.gx * b.gy * b.gz) return;  // tables and grid always agree
+    h4_tile<PML>(g, p, p, b, t - b.first, zchunk, s_co);
+}
+
+// One workgroup of the E kernel.  hsrc: where H(n+1/2) lives; esrc: E^n and
+// the psi memory of step n; dst: E(n+1) and psi of step n+1.
+template <bool PML>
+__device__ __forceinline__ void e4_tile(const Geo &g, const Ptrs &hsrc, const Ptrs &esrc,
+                                        const Ptrs &dst, const Box &b, int local, int zchunk,
+                                        float2 *s_co)
+{
     const int lane = threadIdx.x, w = threadIdx.y;
     for (int i = w * 64 + lane; i < 256; i += 64 * kSY) {
         const float4 c = esrc.coef[i];
@@ -227,8 +252,6 @@ __global__ __launch_bounds__(64 * kSY) void k_e4(Geo g, Ptrs hsrc, Ptrs esrc, Pt
     }
     __syncthreads();
 
-    int local;
-    const Box &b = find_box(bl, xcd_tile(blockIdx.x, bl.total), local);
     const int lpr = b.lpr;
     const SplitGeo s = split_geo(b, local, zchunk, false);
     const bool live = s.live;
@@ -344,6 +367,31 @@ __global__ __launch_bounds__(64 * kSY) void k_e4(Geo g, Ptrs hsrc, Ptrs esrc, Pt
         hx_mz = hx_c;
         hy_mz = hy_c;
     }
+}
+
+template <bool PML>
+__global__ __launch_bounds__(64 * kSY) void k_e4(Geo g, Ptrs hsrc, Ptrs esrc, Ptrs dst,
+                                                 BoxList bl, int zchunk)
+{
+    __shared__ float2 s_co[256];  // (ca, cb)
+    int local;
+    const Box &b = find_box(bl, xcd_tile(blockIdx.x, bl.total), local);
+    e4_tile<PML>(g, hsrc, esrc, dst, b, local, zchunk, s_co);
+}
+
+template <bool PML>
+__global__ __launch_bounds__(64 * kSY) void k_e4_batch(const BatchScene *__restrict__ sc,
+                                                       const uint32_t *__restrict__ wg_scene,
+                                                       int total, int zchunk)
+{
+    __shared__ float2 s_co[256];  // (ca, cb)
+    const int t = xcd_tile(blockIdx.x, total);
+    const BatchScene *d = sc + wg_scene[t];
+    const Geo g = d->g;
+    const Ptrs p = d->p;
+    const Box b = d->box;
+    if (t - b.first >= b.gx * b.gy * b.gz) return;  // tables and grid always agree
+    e4_tile<PML>(g, p, p, p, b, t - b.first, zchunk, s_co);
 }
 
 // ===========================================================================
@@ -592,6 +640,24 @@ hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st)
     else
         hipLaunchKernelGGL(k_e4<false>, grid, block, 0, st, s.g, s.alt, s.cur, s.alt, bl,
                            s.zchunk);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_h(const BatchScene *scenes, const uint32_t *wg_scene, int total,
+                          int zchunk, hipStream_t st)
+{
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_h4_batch<true>, dim3((unsigned)total), dim3(64, kSY), 0, st, scenes,
+                       wg_scene, total, zchunk);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, int total,
+                          int zchunk, hipStream_t st)
+{
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_e4_batch<true>, dim3((unsigned)total), dim3(64, kSY), 0, st, scenes,
+                       wg_scene, total, zchunk);
     return hipGetLastError();
 }
 

@@ -10,7 +10,8 @@ from __future__ import annotations
 
 import os
 import sys
-from ctypes import c_double, c_float, c_int32, c_uint8, c_uint32, c_uint64, c_void_p, byref
+from ctypes import (byref, c_double, c_float, c_int, c_int32, c_uint8, c_uint32, c_uint64,
+                    c_void_p)
 from dataclasses import dataclass
 
 import numpy as np
@@ -205,6 +206,60 @@ class DeviceEngine:
         out = np.zeros((max_steps, max(len(self.probes), 1)), np.float32)
         check(lib().wf_read_trace(self._h, ptr(out, c_float), max_steps))
         return out[:, :len(self.probes)]
+
+
+class DeviceBatch:
+    """Many independent scenes (DeviceEngines on one GPU) advanced together:
+    one H and one E launch per step over every scene, then the sources and
+    probes of all scenes.  Each scene's results equal stepping it alone."""
+
+    def __init__(self, engines: list[DeviceEngine]):
+        self.engines = list(engines)
+        arr = (c_void_p * len(self.engines))(*[e.handle.value for e in self.engines])
+        handle = c_void_p()
+        check(lib().wf_batch_create(len(self.engines), arr, byref(handle)))
+        self._h = handle
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().wf_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, nsteps: int = 1) -> None:
+        check(lib().wf_batch_step(self._h, nsteps))
+
+    def sync(self) -> None:
+        check(lib().wf_batch_sync(self._h))
+
+    def time_steps(self, nsteps: int) -> float:
+        ms = c_double()
+        check(lib().wf_batch_time_steps(self._h, nsteps, byref(ms)))
+        return ms.value
+
+    def tune(self, steps_per_candidate: int = 4):
+        """Autotunes the tiling on real steps; returns (zchunk, lanes_per_row,
+        steps_taken)."""
+        z, l, n = c_int(), c_int(), c_uint64()
+        check(lib().wf_batch_tune(self._h, steps_per_candidate, byref(z), byref(l), byref(n)))
+        return z.value, l.value, n.value
+
+    def set_tiling(self, zchunk: int, lanes_per_row: int) -> None:
+        check(lib().wf_batch_set_tiling(self._h, zchunk, lanes_per_row))
+
+    def cells(self) -> int:
+        return int(lib().wf_batch_cells(self._h))
 
 
 @dataclass

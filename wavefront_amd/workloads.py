@@ -135,3 +135,38 @@ def random_1024(steps: int = 1000, n: int = 1024, seed: int = 3, pml: int = 20) 
     probes = [Probe(c + 20, c, c, PC.Ez, [5.0e9], 0.0)]
     return Workload(f"random_{n}", dims, dx, dt, ids, table, PmlConfig(thickness=pml), [src],
                     probes, steps, scene_text=text, notes={"materials": nmat})
+
+
+def batch_scenes(nscenes: int = 96, steps: int = 500, seed: int = 4, edges=None,
+                 max_edge: int = 192) -> list[Workload]:
+    """Independent small scenes for the batched engine: each edge drawn from
+    {32, 40, ..., max_edge} (multiples of 8, non-cubic), 1-4 spheres with
+    r ~ U[4, min_edge/4], eps_r ~ U[1, 10], sigma ~ U[0, 0.05], mu_r = 1,
+    requested CPML ~ U{8..16} (effective 8 or 16 voxels, at most half the
+    smallest axis), one Gaussian Ez source at the centre and one Ez probe.
+    `edges` overrides the drawn edge list (for small test batches)."""
+    rng = np.random.default_rng(seed)
+    dt, dx = DT_COURANT_099, DX
+    f0 = default_freq(dt)
+    choices = np.arange(32, max_edge + 1, 8)
+    out = []
+    for k in range(nscenes):
+        nx, ny, nz = (int(v) for v in (edges[k] if edges is not None
+                                        else rng.choice(choices, 3)))
+        dims = GridDims(nx, ny, nz)
+        m = min(nx, ny, nz)
+        lines = []
+        for _ in range(int(rng.integers(1, 5))):
+            r = float(rng.uniform(4.0, m / 4.0))
+            c = [float(rng.uniform(r, n - r)) for n in (nx, ny, nz)]
+            eps, sig = float(rng.uniform(1.0, 10.0)), float(rng.uniform(0.0, 0.05))
+            lines.append(f"sphere {_f32s(eps)} 1.0 {_f32s(sig)} "
+                         + " ".join(_f32s(v) for v in c) + f" {_f32s(r)}")
+        text = "\n".join(lines) + "\n"
+        ids, table, _ = _scene(dims, text, dt, dx)
+        pml = int(rng.integers(8, 17))
+        src = Source(nx // 2, ny // 2, nz // 2, SC.Ez, 1.0, gaussian_from_freq(f0))
+        probes = [Probe(nx // 4, ny // 2, nz // 2, PC.Ez, [float(np.float32(f0))], 0.0)]
+        out.append(Workload(f"scene{k}_{nx}x{ny}x{nz}", dims, dx, dt, ids, table,
+                            PmlConfig(thickness=pml), [src], probes, steps, scene_text=text))
+    return out
