@@ -117,6 +117,76 @@ def run_spheres_512(args):
     print(json.dumps(out), flush=True)
 
 
+def run_batch_96(args):
+    """96 independent scenes (edges 32-192, 1-4 spheres, CPML 8 or 16, one
+    Gaussian source each) for 500 steps: batched (one launch per phase over
+    all scenes, tiling autotuned on the first steps) against running the
+    scenes one after another on their own engines.  value = sum of cells x
+    steps / time of the batched run, tuning steps included."""
+    from wavefront_amd import workloads
+    from wavefront_amd.engine import DeviceBatch, DeviceEngine
+
+    steps = args.c4_steps
+    ws = workloads.batch_scenes(steps=steps)
+    cells = sum(w.cells() for w in ws)
+    log(f"batch: {len(ws)} scenes, {cells} cells, {steps} steps")
+
+    def engines():
+        out = []
+        for w in ws:
+            e = DeviceEngine(w.dims, w.ids, w.table, w.pml())
+            e.set_sources(w.sources, w.dt, steps)
+            e.set_probes(w.probes, w.dt, steps)
+            out.append(e)
+        return out
+
+    engs = engines()
+    with DeviceBatch(engs) as b:
+        t0 = time.perf_counter()
+        z, lpr, taken = b.tune(4)
+        ms_rest = b.time_steps(steps - taken)
+        b.sync()
+        wall = time.perf_counter() - t0
+    for e in engs:
+        e.close()
+    log(f"batched: tuned zchunk={z} lanes/row={lpr} on {taken} steps; {wall:.3f}s total, "
+        f"{ms_rest / (steps - taken):.3f} ms/step after tuning")
+    # the same scenes one at a time on their own engines (graph-replayed steps)
+    engs = engines()
+    solo_ms = 0.0
+    for e in engs:
+        solo_ms += e.time_steps(steps)
+        e.close()
+    log(f"solo, one scene after another: {solo_ms / 1e3:.3f}s")
+    mcells = cells * steps / wall / 1e6
+    out = {
+        "metric": "Mcell-updates/s, 96 batched scenes, FP32, 1 MI355X",
+        "value": round(mcells, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": round(wall * 1e3 / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "batch_96: 96 scenes, edges U{32..192 step 8}, 1-4 spheres, CPML 8 or "
+                        "16, Gaussian Ez source + 1 probe each, seed 4",
+            "cells": cells,
+            "tuned_zchunk": z,
+            "tuned_lanes_per_row": lpr,
+            "tuning_steps": taken,
+            "batched_ms_per_step_after_tuning": round(ms_rest / (steps - taken), 4),
+            "solo_sequential_seconds": round(solo_ms / 1e3, 3),
+            "solo_sequential_mcells": round(cells * steps / (solo_ms / 1e3) / 1e6, 1),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,8 +201,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-gib", type=float, default=2.0,
                     help="size of the plain copy that measures the HBM ceiling (0 = skip)")
-    ap.add_argument("--workload", choices=["random_1024", "spheres_512"], default="random_1024",
-                    help="spheres_512 = the 512^3 two-sphere run with snapshot egress")
+    ap.add_argument("--workload", choices=["random_1024", "spheres_512", "batch_96"],
+                    default="random_1024",
+                    help="spheres_512 = the 512^3 two-sphere run with snapshot egress; "
+                         "batch_96 = 96 small scenes batched")
+    ap.add_argument("--c4-steps", type=int, default=500)
     ap.add_argument("--c2-steps", type=int, default=2000)
     ap.add_argument("--c2-every", type=int, default=100)
     ap.add_argument("--output", default="", help="snapshot file of the spheres_512 run")
@@ -146,6 +219,9 @@ def main():
 
     if args.workload == "spheres_512":
         run_spheres_512(args)
+        return
+    if args.workload == "batch_96":
+        run_batch_96(args)
         return
 
     from wavefront_amd import workloads
