@@ -232,6 +232,76 @@ int wf_batch_tune(wf_batch *batch, uint32_t steps_per_candidate, int *zchunk,
 int wf_batch_set_tiling(wf_batch *batch, int zchunk, int lanes_per_row);
 /* Sum of the scenes' cells. */
 uint64_t wf_batch_cells(const wf_batch *batch);
+
+/* ---------------------------------------------------------------------- */
+/* Learned thin absorbing boundary                                          */
+/* ---------------------------------------------------------------------- */
+/* No counterpart in the reference, which terminates every face with a
+ * graded CPML backed by the zero field (OUTER_ZERO_BLOCK, src/engine.rs:83;
+ * PmlProfile1D, src/layout.rs:471-513).  Here a face can instead end in a
+ * thin layer whose outside "ghost" fields (the zero the boundary cell's curl
+ * update reads) are predicted each half step by a small per-face MLP:
+ * 16 inputs (4-cell-deep E/H stencil, transverse Laplacian, CPML psi, short
+ * history; scaled by their largest magnitude) -> 64 -> 64 -> 1, tanh, BF16 weights and
+ * activations on MFMA with FP32 accumulation, plus an FP32 linear skip.
+ *
+ * Parameters of one network: wf_mlp_param_count() floats, laid out
+ * W1[16][64] b1[64] W2[64][64] b2[64] W3[64] b3 wlin[16] (W[in][out]).
+ * Faces: 0 x_lo, 1 x_hi, 2 y_lo, 3 y_hi, 4 z_lo, 5 z_hi. */
+#define WF_MLP_INPUTS 16
+#define WF_SAMPLE_STRIDE 20 /* recorded sample: 16 inputs, target, 3 pad */
+uint32_t wf_mlp_param_count(void);
+
+/* Training on the GPU: full-batch Adam on the mean squared error of
+ * n samples (inputs[n][16], targets[n], both already scaled). */
+typedef struct wf_mlp_trainer wf_mlp_trainer;
+int wf_mlp_trainer_create(int device, uint64_t n, const float *inputs, const float *targets,
+                          wf_mlp_trainer **out);
+void wf_mlp_trainer_destroy(wf_mlp_trainer *t);
+/* Sets the FP32 master parameters (and resets the Adam moments). */
+int wf_mlp_set_params(wf_mlp_trainer *t, const float *params);
+int wf_mlp_get_params(wf_mlp_trainer *t, float *params);
+/* Gradient of the loss at the current parameters (the network as it runs:
+ * BF16 weights and activations) and the loss itself. */
+int wf_mlp_gradient(wf_mlp_trainer *t, float *grad, double *loss);
+/* iters Adam steps at learning rate lr (lr_skip for the linear skip wlin;
+ * 0 keeps it fixed); odd != 0 leaves the biases unchanged (zero biases make
+ * the network odd in its inputs); *loss = loss after the last. */
+int wf_mlp_train(wf_mlp_trainer *t, uint32_t iters, float lr, float lr_skip, int odd,
+                 double *loss);
+/* Network output for n scaled input rows. */
+int wf_mlp_predict(wf_mlp_trainer *t, uint64_t n, const float *inputs, float *outputs);
+
+/* Predicted faces: the faces in face_mask get ghost planes predicted by the
+ * networks params[f * wf_mlp_param_count() ...] each half step (high faces
+ * before the H phase, low faces before the E phase); other faces keep their
+ * boundary.  A face's axis must not be periodic.  Clears the faces' history.
+ * face_mask 0 removes nothing; wf_clear_faces removes every hook. */
+int wf_set_learned_faces(wf_ctx *ctx, uint32_t face_mask, const float *params);
+int wf_clear_faces(wf_ctx *ctx);
+
+/* Recording faces (training data from a thick-CPML run): a virtual face of
+ * the thin domain placed inside this domain, b0 = normal coordinate of its
+ * boundary cell, the ghost cell one further out; fast/slow ranges on the
+ * lower-/higher-numbered tangential axis.  From the next step on, every
+ * `every` steps (up to `rows` times) each position and polarization writes
+ * a sample (the same 16 inputs a predicted face computes, and the true ghost
+ * value / s); the history uses the true ghost values. */
+typedef struct wf_face_region {
+    int32_t b0, fast0, nfast, slow0, nslow;
+} wf_face_region;
+int wf_record_face_samples(wf_ctx *ctx, uint32_t face_mask, const wf_face_region *regions,
+                           uint64_t every, uint64_t rows);
+/* Replayed faces: ghost planes taken step by step from samples recorded
+ * by a thick-CPML run (samples[f]: rows * 2 * positions * WF_SAMPLE_STRIDE
+ * floats as wf_read_face_samples returns them, recorded every step over the
+ * matching region) -- the perfect predictor, for checks and bounds. */
+int wf_replay_faces(wf_ctx *ctx, uint32_t face_mask, uint64_t rows, const float *const *samples);
+/* Samples a recording face holds (rows * 2 * positions), and a copy of them
+ * (count * WF_SAMPLE_STRIDE floats: 16 scaled inputs, target / s, target, s,
+ * padding; unwritten rows are zero). */
+uint64_t wf_face_sample_count(const wf_ctx *ctx, uint32_t face);
+int wf_read_face_samples(wf_ctx *ctx, uint32_t face, float *dst);
 /* CPML psi memory in the reference PmlAux layout (src/layout.rs:545-558):
  * 12 x 512 f32 per shell block, shell blocks in block-major order. */
 uint64_t wf_psi_shell_blocks(const wf_ctx *ctx);

@@ -1,0 +1,281 @@
+"""Learned thin boundary on the GPU: the BF16-MFMA network against the CPU
+FP64 reference (forward and gradient), training, the face hook's recorded
+inputs against the fields, the ghost plumbing (a perfect predictor makes the
+thin run equal the thick-CPML run), and the reflection of the trained layer
+measured by the reference's two-run method against 32-cell CPML, the method
+without the network, and an FP64 CPU run."""
+import numpy as np
+import pytest
+
+from oracle import fdtd1d, mlp_ref
+from tests.test_learned_cpu import _params
+from wavefront_amd import learned as L
+from wavefront_amd.engine import DeviceEngine
+from wavefront_amd.layout import GridDims, MaterialTable, aosoa_to_xyz
+from wavefront_amd.source import FieldComponent as SC
+from wavefront_amd.source import Source, Waveform
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def test_mfma_forward_matches_cpu():
+    """Pins the MFMA operand layouts: the GPU network equals the CPU
+    emulation of its rounding points (tolerance 2e-3 of the output scale,
+    from f32 accumulation order and tanh), and the FP64 network within the
+    bf16 activation error (3e-2)."""
+    _say("mlp forward vs cpu")
+    rng = np.random.default_rng(3)
+    p = _params(rng).astype(np.float32)
+    u = rng.normal(0, 1, (2000, 16)).astype(np.float32)
+    tr = L.Trainer(u, np.zeros(len(u), np.float32))
+    tr.set_params(p)
+    y = tr.predict(u)
+    tr.close()
+    ye = mlp_ref.forward_bf16(p, u)
+    y64, _ = mlp_ref.forward_f64(p, u)
+    scale = np.max(np.abs(y64))
+    assert np.max(np.abs(y - ye)) < 2e-3 * scale
+    assert np.max(np.abs(y - y64)) < 3e-2 * scale
+
+
+def test_mlp_gradient_vs_cpu_fp64():
+    """GPU gradient (bf16 activations and back-propagated errors on MFMA,
+    f32 accumulation) against the exact FP64 gradient of the same network:
+    relative L2 error < 3e-2 overall and < 6e-2 per parameter group, cosine
+    similarity > 0.999."""
+    _say("mlp gradient vs cpu fp64")
+    rng = np.random.default_rng(4)
+    p = _params(rng).astype(np.float32)
+    u = rng.normal(0, 1, (3000, 16)).astype(np.float32)
+    t = rng.normal(0, 1, 3000).astype(np.float32)
+    tr = L.Trainer(u, t)
+    tr.set_params(p)
+    g, loss = tr.gradient()
+    tr.close()
+    g64 = mlp_ref.grad_f64(p, u, t)
+    assert abs(loss - mlp_ref.loss_f64(p, u, t)) < 2e-2 * mlp_ref.loss_f64(p, u, t)
+    rel = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+    cos = float(g @ g64) / (np.linalg.norm(g) * np.linalg.norm(g64))
+    print(f"gradient rel L2 {rel:.3e}, cosine {cos:.6f}")
+    assert rel < 3e-2 and cos > 0.999
+    o = 0
+    for name, n in mlp_ref.SIZES:
+        a, b = g[o:o + n], g64[o:o + n]
+        o += n
+        assert np.linalg.norm(a - b) <= 6e-2 * np.linalg.norm(b) + 1e-7, name
+
+
+def test_training_reduces_loss():
+    """Regression onto a random teacher network: Adam on the GPU lowers the
+    loss at least 20-fold."""
+    _say("mlp training")
+    rng = np.random.default_rng(5)
+    teacher = _params(rng, scale3=0.3).astype(np.float32)
+    u = rng.normal(0, 1, (20000, 16)).astype(np.float32)
+    t, _ = mlp_ref.forward_f64(teacher, u)
+    tr = L.Trainer(u, t.astype(np.float32))
+    tr.set_params(L.init_params(np.random.default_rng(6)))
+    l0 = tr.gradient()[1]
+    l1 = tr.train(600, 3e-3)
+    # odd mode leaves the biases untouched
+    p0 = tr.params()
+    tr.train(5, 1e-2, odd=True)
+    p1 = tr.params()
+    for lo, n in ((1024, 64), (5184, 64), (5312, 1)):
+        assert np.array_equal(p0[lo:lo + n], p1[lo:lo + n])
+    assert not np.array_equal(p0[:1024], p1[:1024])
+    tr.close()
+    print(f"loss {l0:.3e} -> {l1:.3e}")
+    assert l1 < l0 / 20
+
+
+def _point_sources(c):
+    g = Waveform.gaussian_pulse(*L._gauss(4.0e10))
+    r = Waveform.ricker_wavelet(3.0e10, float(np.float32(1.0) / np.float32(3.0e10)))
+    return [Source(c[0], c[1], c[2], SC.Ez, 1.0, g),
+            Source(c[0] - 3, c[1] + 2, c[2] - 1, SC.Ex, 0.7, r)]
+
+
+def test_recorded_inputs_match_fields():
+    """The hook's inputs (fields at depths 0..3, transverse Laplacian,
+    history) and true ghost values at the last step of a thick run, against
+    the same quantities taken from the fields read back."""
+    _say("face hook record vs fields")
+    thin = GridDims(24, 16, 32)
+    dx, dt, _ = L.study_grid()
+    big = L.thick_dims(thin)
+    K = 44
+    srcs = [Source(s.x + L.OFF, s.y + L.OFF, s.z + L.OFF, s.component, s.amplitude, s.waveform)
+            for s in _point_sources((18, 10, 26))]
+    regions = (L.FaceRegion * 6)()
+    nb = (big.nx, big.ny, big.nz)
+    nt = (thin.nx, thin.ny, thin.nz)
+    for f in range(6):
+        a, hi = f // 2, f % 2
+        t1, t2 = (a + 1) % 3, (a + 2) % 3
+        tf, ts = min(t1, t2), max(t1, t2)
+        regions[f] = L.FaceRegion(nb[a] - 1 - L.OFF if hi else L.OFF, L.OFF, nt[tf], L.OFF, nt[ts])
+    with DeviceEngine(big, None, MaterialTable.vacuum_filled(dt, dx),
+                      L.thick_context(big, dt, dx)) as eng:
+        eng.set_sources(srcs, dt, K)
+        L.check(L.lib().wf_record_face_samples(eng.handle, 63, regions, 1, K))
+        eng.step(K - 1)
+        F1 = aosoa_to_xyz(eng.read_fields(), big)  # (6, z, y, x)
+        eng.step(1)
+        F2 = aosoa_to_xyz(eng.read_fields(), big)
+        recs = {}
+        for f in range(6):
+            n = int(L.lib().wf_face_sample_count(eng.handle, f))
+            buf = np.zeros((n, L.STRIDE), np.float32)
+            L.check(L.lib().wf_read_face_samples(eng.handle, f, L.ptr(buf, L.c_float)))
+            recs[f] = buf.reshape(K, 2, -1, L.STRIDE)
+
+    def at(F, comp, a, n, fc, sc):
+        c = [0, 0, 0]
+        t1, t2 = (a + 1) % 3, (a + 2) % 3
+        tf, ts = min(t1, t2), max(t1, t2)
+        c[a], c[tf], c[ts] = n, fc, sc
+        return F[comp, c[2], c[1], c[0]]
+
+    rng = np.random.default_rng(7)
+    checked = 0
+    for f in range(6):
+        a, hi = f // 2, f % 2
+        t1, t2 = (a + 1) % 3, (a + 2) % 3
+        tf, ts = min(t1, t2), max(t1, t2)
+        r = regions[f]
+        d = -1 if hi else 1
+        rec = recs[f]
+        for pol in (0, 1):
+            if hi:
+                P, Q = (t1 if pol == 0 else t2), 3 + (t2 if pol == 0 else t1)
+                FP, FQ = F1, F1
+            else:
+                P, Q = 3 + (t1 if pol == 0 else t2), (t2 if pol == 0 else t1)
+                FP, FQ = F2, F1
+            for pos in rng.choice(r.nfast * r.nslow, 40, replace=False):
+                fi, si = pos % r.nfast, pos // r.nfast
+                fc, sc = r.fast0 + fi, r.slow0 + si
+                x = np.zeros(16)
+                for j in range(4):
+                    x[j] = at(FP, P, a, r.b0 + d * j, fc, sc)
+                    x[4 + j] = at(FQ, Q, a, r.b0 + d * j, fc, sc)
+                f0, f1, s0, s1 = r.fast0, r.fast0 + r.nfast, r.slow0, r.slow0 + r.nslow
+                for j in range(2):
+                    n = r.b0 + d * j
+                    # neighbours outside the face region read as zero
+                    v = lambda ff, ss: at(FP, P, a, n, ff, ss) if f0 <= ff < f1 and s0 <= ss < s1 else 0.0
+                    x[8 + j] = np.float32(np.float32(v(fc + 1, sc) + v(fc - 1, sc))
+                                          + np.float32(v(fc, sc + 1) + v(fc, sc - 1))) \
+                        - np.float32(4.0) * np.float32(at(FP, P, a, n, fc, sc))
+                row = rec[K - 1, pol, pos]
+                prev1, prev2 = rec[K - 2, pol, pos], rec[K - 3, pol, pos]
+                x[12], x[13] = prev1[17], prev2[17]
+                x[14], x[15] = prev1[0] * prev1[18], prev2[0] * prev2[18]
+                s = row[18]
+                got = row[:16].astype(np.float64) * s
+                for i in list(range(10)) + [12, 13, 14, 15]:
+                    assert abs(got[i] - x[i]) <= 1e-5 * max(abs(x[i]), 1e-3 * s) + 1e-30, (f, pol, pos, i)
+                ghost = at(FP, P, a, r.b0 - d, fc, sc)
+                assert row[17] == np.float32(ghost)
+                checked += 1
+    assert checked == 6 * 2 * 40
+
+
+@pytest.mark.parametrize("kernel", ["fused", "split"])
+def test_replayed_ghosts_reproduce_thick_run(monkeypatch, kernel):
+    """A thin run fed, step by step, the ghost values the thick-CPML run
+    recorded is bit-identical to the thick run inside the thin domain: the
+    ghost planes reach the right cells of all six faces at the right half
+    step, and the thin layer is exactly the thick layer's first cells."""
+    monkeypatch.setenv("WF_KERNEL", kernel)
+    _say(f"replay thin vs thick kernel={kernel}")
+    thin = GridDims(32, 24, 40)
+    dx, dt, _ = L.study_grid()
+    K = 70
+    srcs = _point_sources((12, 14, 30))
+    rec = L.record_run(thin, srcs, K, dt, dx)
+    big = L.thick_dims(thin)
+    table = MaterialTable.vacuum_filled(dt, dx)
+    with DeviceEngine(big, None, table, L.thick_context(big, dt, dx)) as eng:
+        eng.set_sources([Source(s.x + L.OFF, s.y + L.OFF, s.z + L.OFF, s.component, s.amplitude,
+                                s.waveform) for s in srcs], dt, K)
+        eng.step(K)
+        fb = aosoa_to_xyz(eng.read_fields(), big)
+    with L.thin_engine(thin, None, dt, dx, mode="replay", replay=rec) as eng:
+        eng.set_sources(srcs, dt, K)
+        eng.step(K)
+        ft = aosoa_to_xyz(eng.read_fields(), thin)
+    o = L.OFF
+    ref = fb[:, o:o + thin.nz, o:o + thin.ny, o:o + thin.nx]
+    assert np.abs(ref).max() > 0
+    assert np.array_equal(ft, ref)
+
+
+def _train_x_faces(log):
+    """Training data: 32 plane-wave runs along x (periodic y, z) with soft
+    Gaussian / Ricker sheet sources at random positions and frequencies
+    10-45 GHz on both polarizations; one transverse position per step (the
+    field is uniform across the sheet)."""
+    from wavefront_amd.engine import PERIODIC_Y, PERIODIC_Z
+    dx, dt, _ = L.study_grid()
+    per = PERIODIC_Y | PERIODIC_Z
+    thin = GridDims(160, 8, 8)
+    rng = np.random.default_rng(11)
+    data = {"x_lo": [], "x_hi": []}
+    for r in range(32):
+        srcs = []
+        for pol in (0, 1):
+            kind = ("gaussian", "ricker")[(r + pol) % 2]
+            freq = float(np.float32(rng.uniform(1.0e10, 4.5e10)))
+            pos = int(rng.integers(8, 152))
+            srcs += L.plane_wave_sources(0, 160, pos, kind, freq, pol, float(rng.uniform(0.3, 1.0)))
+        rec = L.record_run(thin, srcs, 1100, dt, dx, periodic=per)
+        for face in data:
+            data[face].append(L.training_rows(rec[face][:, :, :1]))
+    nets = np.zeros((6, L.param_count()), np.float32)
+    lin = np.zeros((6, L.param_count()), np.float32)
+    for f, face in ((0, "x_lo"), (1, "x_hi")):
+        u = np.concatenate([d[0] for d in data[face]])
+        t = np.concatenate([d[1] for d in data[face]])
+        log(f"{face}: {len(t)} samples")
+        p, hist = L.train_face(u, t, seed=f, log=log)
+        nets[f] = p
+        lin[f] = p
+        lin[f][L.param_count() - 16 - 1 - 64:L.param_count() - 16 - 1] = 0.0  # W3 = 0
+    return L.FaceNets(nets, dt, dx), L.FaceNets(lin, dt, dx)
+
+
+def test_learned_layer_reflection():
+    """The trained 4-cell layer against the reference's two-run reflection
+    measurement (30 GHz, 20 points per wavelength, Courant 0.4): far below
+    the method without the network (the same thin layer ending in the zero
+    boundary); reported beside 32-cell CPML, the linear skip alone, and the
+    same measurement against an FP64 CPU run of the incident wave."""
+    _say("learned layer: data, training, reflection")
+    nets, lin = _train_x_faces(print)
+    r_learned, test, ref = L.reflection("learned", nets)
+    r_lin, _, _ = L.reflection("learned", lin, reference=ref)
+    r_zero, _, _ = L.reflection("zero", reference=ref)
+    r_cpml32, _, _ = L.reflection("cpml", cpml_voxels=32, reference=ref)
+    dx, dt, freq = L.study_grid()
+    nx, xs, xp, steps, skip, nxr, xsr, xpr = L.reflection_geometry(L.LAYER)
+    ref64 = fdtd1d.plane_wave_trace_f64(nxr, MaterialTable.vacuum_filled(dt, dx).q16[0], dt, freq,
+                                        xsr, xpr, steps)
+    r_fp64 = (L.quadrature_amplitude(test - ref64, dt, freq, skip) /
+              L.quadrature_amplitude(ref64, dt, freq, skip))
+    floor = (L.quadrature_amplitude(ref - ref64, dt, freq, skip) /
+             L.quadrature_amplitude(ref64, dt, freq, skip))
+    np.save("/tmp/learned_x_faces.npy", nets.params)
+    print(f"R learned {r_learned:.3e} | linear skip only {r_lin:.3e} | no network (zero ghost) "
+          f"{r_zero:.3e} | CPML-32 {r_cpml32:.3e} | learned vs FP64 incident {r_fp64:.3e} "
+          f"(FP32 vs FP64 incident {floor:.3e})")
+    assert abs(r_cpml32 - 5.3755640258903854e-05) < 1e-12  # the golden CSV value
+    assert r_zero > 0.1
+    assert r_learned < r_zero / 1000
+    assert r_learned < 1e-4  # the thin layer's target: at least as good as 32-cell CPML's 5.4e-5 class
+    assert abs(r_fp64 - r_learned) < 0.1 * r_learned + 2 * floor

@@ -1,0 +1,78 @@
+"""CPU checks of the learned-boundary test infrastructure: the FP64 network
+reference's gradient against finite differences, its bf16 emulation against
+its f64 form, the FP64 1D plane-wave oracle against the C oracle's 3D run,
+and the thin-layer profile construction."""
+import numpy as np
+
+from oracle import fdtd1d, mlp_ref, wfo
+from wavefront_amd import learned as L
+from wavefront_amd.layout import MaterialTable
+
+
+def _params(rng, scale3=0.5):
+    p = np.zeros(mlp_ref.P)
+    d = mlp_ref.split(p)
+    d = {k: v.copy() for k, v in d.items()}
+    d["W1"] = rng.normal(0, 0.25, (16, 64))
+    d["b1"] = rng.normal(0, 0.1, 64)
+    d["W2"] = rng.normal(0, 0.125, (64, 64))
+    d["b2"] = rng.normal(0, 0.1, 64)
+    d["W3"] = rng.normal(0, scale3, 64)
+    d["b3"] = np.array([0.05])
+    d["wlin"] = rng.normal(0, 0.3, 16)
+    return mlp_ref.join(d)
+
+
+def test_gradient_formulas_match_finite_differences():
+    rng = np.random.default_rng(1)
+    p = _params(rng)
+    u = rng.normal(0, 1, (40, 16))
+    t = rng.normal(0, 1, 40)
+    g = mlp_ref.grad_exact(p, u, t)
+    idx = np.concatenate([rng.choice(mlp_ref.P, 60, replace=False), [mlp_ref.P - 17]])
+    fd = mlp_ref.grad_fd(p, u, t, idx, h=1e-6)
+    np.testing.assert_allclose(g[idx], fd, rtol=2e-5, atol=1e-9)
+
+
+def test_bf16_rounding_and_emulation():
+    x = np.array([1.0, 1.00390625, 1.005859375, -3.14159, 0.0], np.float32)
+    # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: rounds to even (1.0)
+    assert mlp_ref.bf16(x)[0] == 1.0 and mlp_ref.bf16(x)[1] == 1.0
+    assert mlp_ref.bf16(x)[2] == np.float32(1.0078125)
+    rng = np.random.default_rng(2)
+    p = _params(rng).astype(np.float32)
+    u = rng.normal(0, 1, (200, 16)).astype(np.float32)
+    y64, _ = mlp_ref.forward_f64(p, u)
+    yb = mlp_ref.forward_bf16(p, u)
+    assert np.max(np.abs(yb - y64)) < 0.05 * np.max(np.abs(y64))
+
+
+def test_fp64_plane_wave_oracle_matches_c_oracle():
+    """The 1D FP64 reduction against the C oracle's 3D periodic run (f32)."""
+    dx, dt, freq = L.study_grid()
+    nx, xs, xp, steps = 64, 20, 40, 150
+    table = MaterialTable.vacuum_filled(dt, dx)
+    sim = wfo.OracleSim(nx, 8, 8, table=wfo.vacuum_table(dt, dx), periodic=6)
+    for ly in range(8):
+        for lz in range(8):
+            sim.add_source(xs, ly, lz, 2, 1.0, 1, freq, 0.0, hard=True)
+    sim.add_probe(xp, 0, 0, 2, [], 0.0)
+    tr32 = sim.run(steps, dt, trace=True)[:, 0]
+    tr64 = fdtd1d.plane_wave_trace_f64(nx, table.q16[0], dt, freq, xs, xp, steps)
+    assert np.max(np.abs(tr32 - tr64)) < 1e-5 * np.max(np.abs(tr64))
+    assert np.max(np.abs(tr64)) > 0.1
+
+
+def test_thin_profile_is_the_thick_profile_cut():
+    dx, dt, _ = L.study_grid()
+    n = 152
+    thin = L.thin_profile(n, dt, dx)
+    thick = L.thick_profile(n + 2 * L.OFF, dt, dx)
+    for p in range(L.LAYER):
+        assert thin.get(p) == thick.get(p + L.OFF)
+        assert thin.get(n - 1 - p) == thick.get(n + 2 * L.OFF - 1 - (p + L.OFF))
+    assert thin.get(L.LAYER) == (1.0, 0.0, 1.0)
+    assert all(thin.get(i) == (1.0, 0.0, 1.0) for i in range(L.LAYER, n - L.LAYER))
+    # every layer cell is graded (non-identity), at both ends
+    for p in list(range(L.LAYER)) + list(range(n - L.LAYER, n)):
+        assert thin.get(p) != (1.0, 0.0, 1.0)

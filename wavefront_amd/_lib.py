@@ -40,6 +40,11 @@ class SourceDesc(ctypes.Structure):
                 ("p0", c_float), ("p1", c_float)]
 
 
+class FaceRegion(ctypes.Structure):
+    _fields_ = [("b0", ctypes.c_int32), ("fast0", ctypes.c_int32), ("nfast", ctypes.c_int32),
+                ("slow0", ctypes.c_int32), ("nslow", ctypes.c_int32)]
+
+
 class ProbeSite(ctypes.Structure):
     _fields_ = [("x", c_uint32), ("y", c_uint32), ("z", c_uint32),
                 ("component", c_int32), ("nfreq", c_uint32)]
@@ -89,6 +94,22 @@ _SIGNATURES = {
                               POINTER(c_uint64)]),
     "wf_batch_set_tiling": (c_int, [c_void_p, c_int, c_int]),
     "wf_batch_cells": (c_uint64, [c_void_p]),
+    "wf_mlp_param_count": (c_uint32, []),
+    "wf_mlp_trainer_create": (c_int, [c_int, c_uint64, POINTER(c_float), POINTER(c_float),
+                                      POINTER(c_void_p)]),
+    "wf_mlp_trainer_destroy": (None, [c_void_p]),
+    "wf_mlp_set_params": (c_int, [c_void_p, POINTER(c_float)]),
+    "wf_mlp_get_params": (c_int, [c_void_p, POINTER(c_float)]),
+    "wf_mlp_gradient": (c_int, [c_void_p, POINTER(c_float), POINTER(c_double)]),
+    "wf_mlp_train": (c_int, [c_void_p, c_uint32, c_float, c_float, c_int, POINTER(c_double)]),
+    "wf_mlp_predict": (c_int, [c_void_p, c_uint64, POINTER(c_float), POINTER(c_float)]),
+    "wf_set_learned_faces": (c_int, [c_void_p, c_uint32, POINTER(c_float)]),
+    "wf_clear_faces": (c_int, [c_void_p]),
+    "wf_record_face_samples": (c_int, [c_void_p, c_uint32, POINTER(FaceRegion), c_uint64,
+                                       c_uint64]),
+    "wf_replay_faces": (c_int, [c_void_p, c_uint32, c_uint64, POINTER(POINTER(c_float))]),
+    "wf_face_sample_count": (c_uint64, [c_void_p, c_uint32]),
+    "wf_read_face_samples": (c_int, [c_void_p, c_uint32, POINTER(c_float)]),
     "wf_psi_shell_blocks": (c_uint64, [c_void_p]),
     "wf_read_psi": (c_int, [c_void_p, POINTER(c_float)]),
     "wf_read_probe_sums": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_uint64)]),

@@ -16,6 +16,7 @@
 
 #include "wf_internal.h"
 #include "wf_kernels.h"
+#include "wf_learned.h"
 #include "wf_writer.h"
 
 using namespace wf;
@@ -251,6 +252,16 @@ struct wf_ctx {
 
     float *d_stage = nullptr;
     uint64_t stage_bytes = 0;
+
+    // learned boundary: face hooks (predict or record) and their buffers,
+    // per face x_lo, x_hi, y_lo, y_hi, z_lo, z_hi
+    bool hook_on[6] = {false, false, false, false, false, false};
+    FaceHook hook[6] = {};
+    float *d_ghost[6] = {};     // [2][nslow][nfast]
+    float *d_mem[6] = {};       // [2][npos][4]
+    float *d_samples[6] = {};   // [rows][2][npos][kSampleStride]
+    MlpPacked *d_net[6] = {};
+    bool ghost_reads = false;   // some predicted face feeds the step kernels
 };
 
 namespace {
@@ -395,11 +406,23 @@ void plan_boxes(wf_ctx *c)
 // then the frame's E kernel runs once both are done.  With ev (6 timing
 // events) everything runs on the main stream, one kernel at a time, with an
 // event pair around the fused (0, 1), H (2, 3) and E (4, 5) launches.
+// Face hooks of one side: high faces run before the H phase (E^n and
+// H^{n-1/2} in `es`), low faces between the phases (H^{n+1/2} in `hs`, E^n
+// and the E-phase psi in `es`).
+int run_hooks(wf_ctx *c, bool high, const Ptrs &es, const Ptrs &hs, hipStream_t st)
+{
+    for (int f = high ? 1 : 0; f < 6; f += 2)
+        if (c->hook_on[f]) WF_HIP(launch_face_hook(c->g, es, hs, c->hook[f], c->d_step, st));
+    return WF_OK;
+}
+
 int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
 {
     hipStream_t st = c->stream;
     if (c->fused) {
-        const StepLaunch s{c->g, c->p, c->alt, c->pml, c->zchunk, st};
+        StepLaunch s{c->g, c->p, c->alt, c->pml, c->zchunk, st};
+        s.ghost = c->ghost_reads;
+        if (int rc = run_hooks(c, true, c->p, c->p, st)) return rc;
         const bool overlap =
             c->overlap && !ev && c->frame.total > 0 && c->interior.gx > 0 && c->side;
         if (overlap) {
@@ -417,12 +440,15 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
             WF_HIP(launch_h(s, c->frame, st));
             if (ev) WF_HIP(hipEventRecord(ev[3], st));
         }
+        if (int rc = run_hooks(c, false, c->p, c->alt, st)) return rc;
         if (ev) WF_HIP(hipEventRecord(ev[4], st));
         WF_HIP(launch_e(s, c->frame, st));
         if (ev) WF_HIP(hipEventRecord(ev[5], st));
         std::swap(c->p, c->alt);
     } else {
-        const StepLaunch s{c->g, c->p, c->p, c->pml, c->zchunk, st};
+        StepLaunch s{c->g, c->p, c->p, c->pml, c->zchunk, st};
+        s.ghost = c->ghost_reads;
+        if (int rc = run_hooks(c, true, c->p, c->p, st)) return rc;
         if (ev) {
             WF_HIP(hipEventRecord(ev[0], st));
             WF_HIP(hipEventRecord(ev[1], st));
@@ -430,6 +456,7 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
         }
         WF_HIP(launch_h(s, c->whole, st));
         if (ev) WF_HIP(hipEventRecord(ev[3], st));
+        if (int rc = run_hooks(c, false, c->p, c->p, st)) return rc;
         if (ev) WF_HIP(hipEventRecord(ev[4], st));
         WF_HIP(launch_e(s, c->whole, st));
         if (ev) WF_HIP(hipEventRecord(ev[5], st));
@@ -747,6 +774,12 @@ void wf_destroy(wf_ctx *c)
     dfree(c->d_psi);
     dfree(c->d_shell_list);
     dfree(c->d_stage);
+    for (int f = 0; f < 6; ++f) {
+        dfree(c->d_ghost[f]);
+        dfree(c->d_mem[f]);
+        dfree(c->d_samples[f]);
+        dfree(c->d_net[f]);
+    }
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
@@ -1173,6 +1206,259 @@ int wf_time_steps(wf_ctx *c, uint64_t nsteps, double *total_ms)
 }
 
 // ===========================================================================
+// Learned boundary faces
+// ===========================================================================
+}  // extern "C"
+
+namespace {
+
+struct FaceGeom {
+    int a, hi, na, nf, ns;
+};
+
+FaceGeom face_geom(const wf_ctx *c, int f)
+{
+    const int n[3] = {c->g.nx, c->g.ny, c->g.nz};
+    FaceGeom fg;
+    fg.a = f / 2;
+    fg.hi = f % 2;
+    const int t1 = (fg.a + 1) % 3, t2 = (fg.a + 2) % 3;
+    fg.na = n[fg.a];
+    fg.nf = n[std::min(t1, t2)];
+    fg.ns = n[std::max(t1, t2)];
+    return fg;
+}
+
+void clear_face(wf_ctx *c, int f)
+{
+    dfree(c->d_ghost[f]);
+    dfree(c->d_mem[f]);
+    dfree(c->d_samples[f]);
+    dfree(c->d_net[f]);
+    c->hook_on[f] = false;
+    c->hook[f] = FaceHook{};
+    for (int q = 0; q < 2; ++q) c->p.ghost[f][q] = c->alt.ghost[f][q] = nullptr;
+}
+
+void update_ghost_reads(wf_ctx *c)
+{
+    c->ghost_reads = false;
+    for (int f = 0; f < 6; ++f)
+        if (c->hook_on[f] && c->hook[f].mode != 1) c->ghost_reads = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wf_clear_faces(wf_ctx *c)
+{
+    clear_error();
+    if (!c) return fail(WF_ERR_INVALID, "wf_clear_faces: null context");
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    drop_graph_exec(c);
+    for (int f = 0; f < 6; ++f) clear_face(c, f);
+    update_ghost_reads(c);
+    return WF_OK;
+}
+
+int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params)
+{
+    clear_error();
+    if (!c) return fail(WF_ERR_INVALID, "wf_set_learned_faces: null context");
+    if (face_mask >= 64u) return fail(WF_ERR_INVALID, "wf_set_learned_faces: face mask has bits above 5");
+    if (face_mask && !params) return fail(WF_ERR_INVALID, "wf_set_learned_faces: null parameters");
+    for (int f = 0; f < 6; ++f) {
+        if (!(face_mask >> f & 1u)) continue;
+        const FaceGeom fg = face_geom(c, f);
+        if (c->g.periodic >> fg.a & 1u)
+            return fail(WF_ERR_INVALID, "face %d lies on a periodic axis", f);
+        if (fg.na < 8) return fail(WF_ERR_INVALID, "face %d: axis shorter than 8 cells", f);
+    }
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    drop_graph_exec(c);
+    float *d_par = nullptr;
+    if (face_mask)
+        if (int rc = dmalloc(&d_par, (size_t)kMlpParams * sizeof(float))) return rc;
+    int rc = WF_OK;
+    for (int f = 0; f < 6 && !rc; ++f) {
+        if (!(face_mask >> f & 1u)) continue;
+        clear_face(c, f);
+        const FaceGeom fg = face_geom(c, f);
+        const uint64_t npos = (uint64_t)fg.nf * fg.ns;
+        rc = dmalloc(&c->d_ghost[f], 2 * npos * sizeof(float));
+        if (!rc) rc = dmalloc(&c->d_mem[f], 2 * npos * 4 * sizeof(float));
+        if (!rc) rc = dmalloc(&c->d_net[f], sizeof(MlpPacked));
+        if (rc) break;
+        if (hipMemset(c->d_ghost[f], 0, 2 * npos * sizeof(float)) != hipSuccess ||
+            hipMemset(c->d_mem[f], 0, 2 * npos * 4 * sizeof(float)) != hipSuccess ||
+            hipMemcpy(d_par, params + (size_t)f * kMlpParams, kMlpParams * sizeof(float),
+                      hipMemcpyHostToDevice) != hipSuccess ||
+            pack_mlp(d_par, c->d_net[f], c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            rc = fail(WF_ERR_HIP, "wf_set_learned_faces: upload failed");
+            break;
+        }
+        FaceHook &h = c->hook[f];
+        h.axis = fg.a;
+        h.hi = fg.hi;
+        h.b0 = fg.hi ? fg.na - 1 : 0;
+        h.fast0 = 0;
+        h.nfast = fg.nf;
+        h.slow0 = 0;
+        h.nslow = fg.ns;
+        h.na = fg.na;
+        h.nf = fg.nf;
+        h.ns = fg.ns;
+        h.Ta = c->g.T[fg.a];
+        h.mode = 0;
+        h.ghost[0] = c->d_ghost[f];
+        h.ghost[1] = c->d_ghost[f] + npos;
+        h.mem = c->d_mem[f];
+        h.net = c->d_net[f];
+        for (int q = 0; q < 2; ++q) c->p.ghost[f][q] = c->alt.ghost[f][q] = h.ghost[q];
+        c->hook_on[f] = true;
+    }
+    dfree(d_par);
+    update_ghost_reads(c);
+    return rc;
+}
+
+int wf_record_face_samples(wf_ctx *c, uint32_t face_mask, const wf_face_region *regions,
+                           uint64_t every, uint64_t rows)
+{
+    clear_error();
+    if (!c || (face_mask && !regions)) return fail(WF_ERR_INVALID, "wf_record_face_samples: null pointer");
+    if (face_mask >= 64u) return fail(WF_ERR_INVALID, "wf_record_face_samples: face mask has bits above 5");
+    if (every == 0 || rows == 0) return fail(WF_ERR_INVALID, "wf_record_face_samples: every and rows must be >= 1");
+    for (int f = 0; f < 6; ++f) {
+        if (!(face_mask >> f & 1u)) continue;
+        const FaceGeom fg = face_geom(c, f);
+        const wf_face_region &r = regions[f];
+        const int dir = fg.hi ? -1 : 1;
+        const int deep = r.b0 + 3 * dir, ghost = r.b0 - dir;
+        const bool ok = r.b0 >= 0 && r.b0 < fg.na && deep >= 0 && deep < fg.na && ghost >= 0 &&
+                        ghost < fg.na && r.nfast > 0 && r.nslow > 0 && r.fast0 >= 0 &&
+                        r.slow0 >= 0 && r.fast0 + r.nfast <= fg.nf && r.slow0 + r.nslow <= fg.ns;
+        if (!ok) return fail(WF_ERR_INVALID, "face %d: region outside the domain", f);
+    }
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    drop_graph_exec(c);
+    int rc = WF_OK;
+    for (int f = 0; f < 6 && !rc; ++f) {
+        if (!(face_mask >> f & 1u)) continue;
+        clear_face(c, f);
+        const FaceGeom fg = face_geom(c, f);
+        const wf_face_region &r = regions[f];
+        const uint64_t npos = (uint64_t)r.nfast * r.nslow;
+        const uint64_t nsamp = rows * 2 * npos * kSampleStride;
+        rc = dmalloc(&c->d_mem[f], 2 * npos * 4 * sizeof(float));
+        if (!rc) rc = dmalloc(&c->d_samples[f], nsamp * sizeof(float));
+        if (rc) break;
+        if (hipMemset(c->d_mem[f], 0, 2 * npos * 4 * sizeof(float)) != hipSuccess ||
+            hipMemset(c->d_samples[f], 0, nsamp * sizeof(float)) != hipSuccess) {
+            rc = fail(WF_ERR_HIP, "wf_record_face_samples: clear failed");
+            break;
+        }
+        FaceHook &h = c->hook[f];
+        h.axis = fg.a;
+        h.hi = fg.hi;
+        h.b0 = r.b0;
+        h.fast0 = r.fast0;
+        h.nfast = r.nfast;
+        h.slow0 = r.slow0;
+        h.nslow = r.nslow;
+        h.na = fg.na;
+        h.nf = fg.nf;
+        h.ns = fg.ns;
+        h.Ta = c->g.T[fg.a];
+        h.mode = 1;
+        h.mem = c->d_mem[f];
+        h.samples = c->d_samples[f];
+        h.emit_base = c->step;
+        h.emit_every = every;
+        h.emit_rows = rows;
+        c->hook_on[f] = true;
+    }
+    update_ghost_reads(c);
+    return rc;
+}
+
+int wf_replay_faces(wf_ctx *c, uint32_t face_mask, uint64_t rows, const float *const *samples)
+{
+    clear_error();
+    if (!c || (face_mask && !samples)) return fail(WF_ERR_INVALID, "wf_replay_faces: null pointer");
+    if (face_mask >= 64u) return fail(WF_ERR_INVALID, "wf_replay_faces: face mask has bits above 5");
+    if (rows == 0) return fail(WF_ERR_INVALID, "wf_replay_faces: rows must be >= 1");
+    for (int f = 0; f < 6; ++f) {
+        if (!(face_mask >> f & 1u)) continue;
+        if (!samples[f]) return fail(WF_ERR_INVALID, "wf_replay_faces: face %d has no samples", f);
+        if (c->g.periodic >> (f / 2) & 1u) return fail(WF_ERR_INVALID, "face %d lies on a periodic axis", f);
+    }
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    drop_graph_exec(c);
+    int rc = WF_OK;
+    for (int f = 0; f < 6 && !rc; ++f) {
+        if (!(face_mask >> f & 1u)) continue;
+        clear_face(c, f);
+        const FaceGeom fg = face_geom(c, f);
+        const uint64_t npos = (uint64_t)fg.nf * fg.ns;
+        const uint64_t nsamp = rows * 2 * npos * kSampleStride;
+        rc = dmalloc(&c->d_ghost[f], 2 * npos * sizeof(float));
+        if (!rc) rc = dmalloc(&c->d_samples[f], nsamp * sizeof(float));
+        if (rc) break;
+        if (hipMemset(c->d_ghost[f], 0, 2 * npos * sizeof(float)) != hipSuccess ||
+            hipMemcpy(c->d_samples[f], samples[f], nsamp * sizeof(float), hipMemcpyHostToDevice) !=
+                hipSuccess) {
+            rc = fail(WF_ERR_HIP, "wf_replay_faces: upload failed");
+            break;
+        }
+        FaceHook &h = c->hook[f];
+        h.axis = fg.a;
+        h.hi = fg.hi;
+        h.b0 = fg.hi ? fg.na - 1 : 0;
+        h.nfast = h.nf = fg.nf;
+        h.nslow = h.ns = fg.ns;
+        h.na = fg.na;
+        h.Ta = c->g.T[fg.a];
+        h.mode = 2;
+        h.ghost[0] = c->d_ghost[f];
+        h.ghost[1] = c->d_ghost[f] + npos;
+        h.samples = c->d_samples[f];
+        h.emit_base = c->step;
+        h.emit_every = 1;
+        h.emit_rows = rows;
+        for (int q = 0; q < 2; ++q) c->p.ghost[f][q] = c->alt.ghost[f][q] = h.ghost[q];
+        c->hook_on[f] = true;
+    }
+    update_ghost_reads(c);
+    return rc;
+}
+
+uint64_t wf_face_sample_count(const wf_ctx *c, uint32_t face)
+{
+    if (!c || face >= 6 || !c->hook_on[face] || c->hook[face].mode != 1) return 0;
+    const FaceHook &h = c->hook[face];
+    return h.emit_rows * 2ull * (uint64_t)h.nfast * h.nslow;
+}
+
+int wf_read_face_samples(wf_ctx *c, uint32_t face, float *dst)
+{
+    clear_error();
+    if (!c || !dst) return fail(WF_ERR_INVALID, "wf_read_face_samples: null pointer");
+    const uint64_t n = wf_face_sample_count(c, face);
+    if (n == 0) return fail(WF_ERR_STATE, "face %u is not recording samples", face);
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    WF_HIP(hipMemcpy(dst, c->d_samples[face], n * kSampleStride * sizeof(float), hipMemcpyDeviceToHost));
+    return WF_OK;
+}
+
+// ===========================================================================
 // Batches: many scenes advanced together, one launch per phase
 // ===========================================================================
 }  // extern "C"
@@ -1281,6 +1567,9 @@ int batch_prepare(wf_batch *b, uint64_t nsteps)
 {
     for (size_t i = 0; i < b->scenes.size(); ++i) {
         const wf_ctx *c = b->scenes[i];
+        for (int f = 0; f < 6; ++f)
+            if (c->hook_on[f])
+                return fail(WF_ERR_STATE, "batch scene %zu has learned-boundary faces (step it alone)", i);
         if (c->nsrc && c->step + nsteps > c->src_base + c->src_steps)
             return fail(WF_ERR_STATE, "batch scene %zu: source table covers %llu steps", i,
                         (unsigned long long)c->src_steps);

@@ -32,6 +32,12 @@ struct Ptrs {
     const float4 *coef;       // 256 x (ca, cb, da, db), dequantized Q16.16
     const float *prof[3][3];  // [axis][b, a, inv_kappa], length n_axis
     float *psi[3][4];
+    // Learned-boundary ghost planes (null = the zero boundary), per face
+    // x_lo, x_hi, y_lo, y_hi, z_lo, z_hi and polarization: high faces hold
+    // E just outside (x: Ey, Ez; y: Ez, Ex; z: Ex, Ey), low faces H (x: Hy, Hz;
+    // y: Hz, Hx; z: Hx, Hy).  Planes are indexed x: z*ny + y, y: z*nx + x,
+    // z: y*nx + x.
+    const float *ghost[6][2];
 };
 
 // Source / probe tables.
@@ -142,6 +148,7 @@ struct StepLaunch {
     bool pml;
     int zchunk;
     hipStream_t stream;
+    bool ghost = false;  // some face reads a learned ghost plane
 };
 
 // H kernel over the boxes: reads E^n, H^{n-1/2}, psi from cur; writes

@@ -1,0 +1,80 @@
+// Learned thin absorbing boundary: device structures shared by the engine
+// and the MLP kernels.
+//
+// A face of the domain is terminated by a few-cell layer (the first cells
+// of a thick CPML profile) plus a learned outer boundary condition: the
+// tangential fields just outside the domain (the "ghost" plane the boundary
+// cell's curl update reads, zero in the reference's PEC-backed boundary,
+// src/engine.rs:83) are predicted each half step by a small per-face MLP
+// from the 4-cell-deep E/H/psi stencil at the face and a short history.
+//
+// MLP: 16 inputs -> 64 -> 64 -> 1 with tanh, BF16 weights and activations on
+// MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation, plus an FP32
+// linear skip from the inputs.  Inputs are scaled by their largest
+// magnitude s and the output is scaled back by s, so the predictor is
+// homogeneous of degree 1 like the field equations it stands in for.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "wf_kernels.h"
+
+namespace wf {
+
+constexpr int kMlpIn = 16;
+constexpr int kMlpHid = 64;
+// FP32 parameter vector: W1 [16][64] | b1 [64] | W2 [64][64] | b2 [64] |
+// W3 [64] | b3 | wlin [16]   (W[in][out], row-major)
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kMlpIn * kMlpHid;
+constexpr int kOffW2 = kOffB1 + kMlpHid;
+constexpr int kOffB2 = kOffW2 + kMlpHid * kMlpHid;
+constexpr int kOffW3 = kOffB2 + kMlpHid;
+constexpr int kOffB3 = kOffW3 + kMlpHid;
+constexpr int kOffWL = kOffB3 + 1;
+constexpr int kMlpParams = kOffWL + kMlpIn;  // 5329
+// A recorded sample: 16 scaled inputs, target / s, target, s, 1 float padding.
+constexpr int kSampleStride = 20;
+
+typedef short bfv4 __attribute__((ext_vector_type(4)));  // 4 x bf16 (raw bits)
+typedef float fv4 __attribute__((ext_vector_type(4)));
+
+// Weights pre-arranged as per-lane MFMA operands (bf16), plus FP32 biases.
+struct MlpPacked {
+    bfv4 a1[4][64];      // layer 1, A = W1^T block o: lane holds W1[4(l/16)+j][16o + l%16]
+    bfv4 a2[4][4][64];   // layer 2, A = W2^T block (o, k)
+    bfv4 a3[4][64];      // layer 3, A = W3^T (row 0 only) block k
+    bfv4 a2b[4][4][64];  // backward, A = W2 block (i, o): W2[16i + l%16][16o + 4(l/16)+j]
+    float b1[kMlpHid], b2[kMlpHid];
+    float w3r[kMlpHid];  // W3 rounded to bf16 (the values the forward uses)
+    float wlin[kMlpIn];
+    float b3;
+    float pad[3];
+};
+
+// One face hook: a face of the domain whose ghost plane is predicted
+// (mode 0), or a virtual face inside a thick-CPML run whose stencils and true
+// ghost values are recorded as training samples (mode 1).
+struct FaceHook {
+    int axis, hi;       // normal axis, 1 = high face
+    int b0;             // normal coordinate of the depth-0 (boundary) cell
+    int fast0, nfast;   // range on the lower-numbered tangential axis
+    int slow0, nslow;   // range on the higher-numbered tangential axis
+    int na, nf, ns;     // axis lengths: normal, fast, slow
+    int Ta;             // graded CPML depth on the normal axis
+    int mode;           // 0 predict, 1 record, 2 replay recorded ghosts
+    float *ghost[2];    // mode 0: ghost planes [nslow][nfast], polarization 0 / 1
+    float *mem;         // [2][npos][4]: g(n-1), g(n-2), P0(n-1), P0(n-2)
+    float *samples;     // modes 1, 2: [rows][2][npos][kSampleStride]
+    uint64_t emit_base, emit_every, emit_rows;
+    const MlpPacked *net;
+};
+
+hipError_t launch_face_hook(const Geo &g, const Ptrs &esrc, const Ptrs &hsrc, const FaceHook &h,
+                            const uint64_t *step, hipStream_t st);
+// FP32 parameter vector (device) -> packed network (device).
+hipError_t pack_mlp(const float *params, MlpPacked *net, hipStream_t st);
+
+}  // namespace wf

@@ -1,0 +1,797 @@
+// Learned thin absorbing boundary: the per-face MLP (BF16 weights on MFMA,
+// FP32 accumulate), its training kernels (forward, backward, Adam), and the
+// face hook that gathers the boundary stencil each half step and either
+// predicts the ghost plane or records training samples.
+//
+// MFMA layout of v_mfma_f32_16x16x16_bf16 (D = A.B + C, 16x16x16): lane l
+// holds A[l%16][4(l/16)+j], B[4(l/16)+j][l%16] and D[4(l/16)+j][l%16],
+// j = 0..3.  Layers are evaluated transposed (features x samples): a wave
+// carries 16 samples as the 16 columns, so a layer's output D is already in
+// the B layout the next layer consumes.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "wf_internal.h"
+#include "wf_learned.h"
+
+namespace wf {
+namespace {
+
+__device__ __forceinline__ unsigned short bf16_rne(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ bfv4 to_bf4(fv4 v)
+{
+    bfv4 r;
+    r.x = (short)bf16_rne(v.x);
+    r.y = (short)bf16_rne(v.y);
+    r.z = (short)bf16_rne(v.z);
+    r.w = (short)bf16_rne(v.w);
+    return r;
+}
+
+__device__ __forceinline__ fv4 mfma(bfv4 a, bfv4 b, fv4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// tanh(x) = sign(x) (1 - 2 / (exp(2|x|) + 1))
+__device__ __forceinline__ float tanh_f(float x)
+{
+    const float e = __expf(2.0f * fabsf(x));
+    const float t = 1.0f - __fdividef(2.0f, e + 1.0f);
+    return copysignf(t, x);
+}
+
+__device__ __forceinline__ fv4 tanh4(fv4 z) { return fv4{tanh_f(z.x), tanh_f(z.y), tanh_f(z.z), tanh_f(z.w)}; }
+
+// The network in registers: A operands and the per-lane biases.
+struct MlpRegs {
+    bfv4 A1[4], A2[4][4], A3[4];
+    fv4 b1[4], b2[4];
+    fv4 wl;
+    float b3;
+};
+
+__device__ __forceinline__ void load_regs(const MlpPacked *__restrict__ w, int lane, MlpRegs &r)
+{
+    const int rb = 4 * (lane >> 4);
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        r.A1[o] = w->a1[o][lane];
+        r.A3[o] = w->a3[o][lane];
+        r.b1[o] = fv4{w->b1[16 * o + rb], w->b1[16 * o + rb + 1], w->b1[16 * o + rb + 2],
+                      w->b1[16 * o + rb + 3]};
+        r.b2[o] = fv4{w->b2[16 * o + rb], w->b2[16 * o + rb + 1], w->b2[16 * o + rb + 2],
+                      w->b2[16 * o + rb + 3]};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r.A2[o][k] = w->a2[o][k][lane];
+    }
+    r.wl = fv4{w->wlin[rb], w->wlin[rb + 1], w->wlin[rb + 2], w->wlin[rb + 3]};
+    r.b3 = w->b3;
+}
+
+struct Acts {
+    fv4 a1[4], a2[4];
+    bfv4 a1b[4], a2b[4];
+};
+
+// Forward pass for the wave's 16 samples.  ub/uf: this lane's 4 inputs
+// (features 4(l/16)..+3 of sample l%16).  Returns the prediction, valid on
+// lanes 0..15 (sample = lane).
+__device__ __forceinline__ float mlp_forward(const MlpRegs &r, bfv4 ub, fv4 uf, Acts &t)
+{
+    const fv4 zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        const fv4 z = mfma(r.A1[o], ub, zero) + r.b1[o];
+        t.a1[o] = tanh4(z);
+        t.a1b[o] = to_bf4(t.a1[o]);
+    }
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        fv4 z = zero;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) z = mfma(r.A2[o][k], t.a1b[k], z);
+        z = z + r.b2[o];
+        t.a2[o] = tanh4(z);
+        t.a2b[o] = to_bf4(t.a2[o]);
+    }
+    fv4 z3 = zero;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) z3 = mfma(r.A3[k], t.a2b[k], z3);
+    // linear skip, summed over the four lanes that hold one sample
+    float p = r.wl.x * uf.x + r.wl.y * uf.y + r.wl.z * uf.z + r.wl.w * uf.w;
+    p = p + __shfl_xor(p, 16);
+    p = p + __shfl_xor(p, 32);
+    return (z3.x + r.b3) + p;
+}
+
+// FP32 parameters -> per-lane MFMA operands.  One wave.
+__global__ void k_pack_mlp(const float *__restrict__ p, MlpPacked *__restrict__ w)
+{
+    const int l = threadIdx.x, c = l & 15, rb = 4 * (l >> 4);
+    auto bf = [](float v) { return (short)bf16_rne(v); };
+    for (int o = 0; o < 4; ++o) {
+        bfv4 v;
+        v.x = bf(p[kOffW1 + (rb + 0) * kMlpHid + 16 * o + c]);
+        v.y = bf(p[kOffW1 + (rb + 1) * kMlpHid + 16 * o + c]);
+        v.z = bf(p[kOffW1 + (rb + 2) * kMlpHid + 16 * o + c]);
+        v.w = bf(p[kOffW1 + (rb + 3) * kMlpHid + 16 * o + c]);
+        w->a1[o][l] = v;
+        for (int k = 0; k < 4; ++k) {
+            bfv4 u;
+            u.x = bf(p[kOffW2 + (16 * k + rb + 0) * kMlpHid + 16 * o + c]);
+            u.y = bf(p[kOffW2 + (16 * k + rb + 1) * kMlpHid + 16 * o + c]);
+            u.z = bf(p[kOffW2 + (16 * k + rb + 2) * kMlpHid + 16 * o + c]);
+            u.w = bf(p[kOffW2 + (16 * k + rb + 3) * kMlpHid + 16 * o + c]);
+            w->a2[o][k][l] = u;
+            // backward operand: block (i = k, o), A[i][k'] = W2[16k + c][16o + rb + j]
+            bfv4 b;
+            b.x = bf(p[kOffW2 + (16 * k + c) * kMlpHid + 16 * o + rb + 0]);
+            b.y = bf(p[kOffW2 + (16 * k + c) * kMlpHid + 16 * o + rb + 1]);
+            b.z = bf(p[kOffW2 + (16 * k + c) * kMlpHid + 16 * o + rb + 2]);
+            b.w = bf(p[kOffW2 + (16 * k + c) * kMlpHid + 16 * o + rb + 3]);
+            w->a2b[k][o][l] = b;
+        }
+        bfv4 a3 = {0, 0, 0, 0};
+        if (c == 0) {
+            a3.x = bf(p[kOffW3 + 16 * o + rb + 0]);
+            a3.y = bf(p[kOffW3 + 16 * o + rb + 1]);
+            a3.z = bf(p[kOffW3 + 16 * o + rb + 2]);
+            a3.w = bf(p[kOffW3 + 16 * o + rb + 3]);
+        }
+        w->a3[o][l] = a3;
+    }
+    w->b1[l] = p[kOffB1 + l];
+    w->b2[l] = p[kOffB2 + l];
+    w->w3r[l] = bf16_f(bf16_rne(p[kOffW3 + l]));
+    if (l < kMlpIn) w->wlin[l] = p[kOffWL + l];
+    if (l == 0) w->b3 = p[kOffB3];
+}
+
+// Prediction for a matrix of already-scaled inputs (tests and evaluation).
+__global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ u, int n,
+                                                     const MlpPacked *__restrict__ w,
+                                                     float *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int ngroups = (n + 15) / 16;
+    MlpRegs r;
+    load_regs(w, lane, r);
+    const int sl = lane & 15, rb = 4 * (lane >> 4);
+    for (int grp = wave; grp < ngroups; grp += nwaves) {
+        const int s = grp * 16 + sl;
+        fv4 uf = {0.f, 0.f, 0.f, 0.f};
+        if (s < n) uf = *reinterpret_cast<const fv4 *>(u + (size_t)s * kMlpIn + rb);
+        Acts t;
+        const float y = mlp_forward(r, to_bf4(uf), uf, t);
+        if (lane < 16 && s < n) out[s] = y;
+    }
+}
+
+// 16x16 bf16 tile transpose through LDS: writes X (D/B layout: lane holds
+// X[4(l/16)+j][l%16]) and returns X^T's operand: lane gets X[l%16][4(l/16)+j].
+__device__ __forceinline__ bfv4 transpose16(unsigned short *tile, bfv4 x, int lane)
+{
+    const int c = lane & 15, rb = 4 * (lane >> 4);
+    tile[(rb + 0) * 16 + c] = (unsigned short)x.x;
+    tile[(rb + 1) * 16 + c] = (unsigned short)x.y;
+    tile[(rb + 2) * 16 + c] = (unsigned short)x.z;
+    tile[(rb + 3) * 16 + c] = (unsigned short)x.w;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    const bfv4 r = *reinterpret_cast<const bfv4 *>(tile + c * 16 + rb);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    return r;
+}
+
+__device__ __forceinline__ float sum16(float v)
+{
+    v = v + __shfl_xor(v, 1);
+    v = v + __shfl_xor(v, 2);
+    v = v + __shfl_xor(v, 4);
+    v = v + __shfl_xor(v, 8);
+    return v;
+}
+
+__device__ __forceinline__ fv4 sum16v(fv4 v) { return fv4{sum16(v.x), sum16(v.y), sum16(v.z), sum16(v.w)}; }
+
+// Mean-squared-error gradient of the whole sample set: every wave loops
+// over groups of 16 samples and writes its partial gradient (and partial
+// squared error) as one row of `partials` ([nwaves][kMlpParams + 1]).
+__global__ __launch_bounds__(256) void k_mlp_train(const float *__restrict__ u,
+                                                   const float *__restrict__ tgt, int n,
+                                                   float inv_n, const MlpPacked *__restrict__ w,
+                                                   float *__restrict__ partials)
+{
+    __shared__ unsigned short tiles[4][256];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int ngroups = (n + 15) / 16;
+    const int sl = lane & 15, rb = 4 * (lane >> 4);
+    unsigned short *tile = tiles[wv];
+
+    MlpRegs r;
+    load_regs(w, lane, r);
+    bfv4 A2B[4][4];
+    fv4 w3r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int o = 0; o < 4; ++o) A2B[i][o] = w->a2b[i][o][lane];
+        w3r[i] = fv4{w->w3r[16 * i + rb], w->w3r[16 * i + rb + 1], w->w3r[16 * i + rb + 2],
+                     w->w3r[16 * i + rb + 3]};
+    }
+    const fv4 zero = {0.f, 0.f, 0.f, 0.f};
+    fv4 gW1[4], gW2[4][4], gb1[4], gb2[4], gw3[4], gwl = zero;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        gW1[i] = gb1[i] = gb2[i] = gw3[i] = zero;
+#pragma unroll
+        for (int o = 0; o < 4; ++o) gW2[i][o] = zero;
+    }
+    float gb3 = 0.f, loss = 0.f;
+
+    for (int grp = wave; grp < ngroups; grp += nwaves) {
+        const int s = grp * 16 + sl;
+        const bool live = s < n;
+        fv4 uf = zero;
+        float t = 0.f;
+        if (live) {
+            uf = *reinterpret_cast<const fv4 *>(u + (size_t)s * kMlpIn + rb);
+            t = tgt[s];
+        }
+        const bfv4 ub = to_bf4(uf);
+        Acts a;
+        const float y = mlp_forward(r, ub, uf, a);
+        const float res = (lane < 16 && live) ? y - t : 0.f;
+        loss = loss + res * res;
+        const float g = 2.0f * res * inv_n;  // dL/dy on lanes 0..15
+        const float gs = __shfl(g, sl);      // broadcast to the sample's four lanes
+        if (lane < 16) gb3 = gb3 + g;
+        gwl = gwl + uf * gs;
+        // layer 3 (rank one): dW3 = a2 g, da2 = W3 g
+        fv4 dz2[4];
+        bfv4 dz2b[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            gw3[k] = gw3[k] + a.a2[k] * gs;
+            const fv4 da2 = w3r[k] * gs;
+            dz2[k] = da2 * (1.0f - a.a2[k] * a.a2[k]);
+            gb2[k] = gb2[k] + dz2[k];
+            dz2b[k] = to_bf4(dz2[k]);
+        }
+        // dW2 = a1 dz2^T: K = the 16 samples, operands transposed via LDS
+        bfv4 a1T[4], dz2T[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            a1T[k] = transpose16(tile, a.a1b[k], lane);
+            dz2T[k] = transpose16(tile, dz2b[k], lane);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int o = 0; o < 4; ++o) gW2[i][o] = mfma(a1T[i], dz2T[o], gW2[i][o]);
+        // da1 = W2 dz2 on MFMA; dz1 = da1 (1 - a1^2)
+        bfv4 dz1b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            fv4 d = zero;
+#pragma unroll
+            for (int o = 0; o < 4; ++o) d = mfma(A2B[i][o], dz2b[o], d);
+            const fv4 dz1 = d * (1.0f - a.a1[i] * a.a1[i]);
+            gb1[i] = gb1[i] + dz1;
+            dz1b[i] = to_bf4(dz1);
+        }
+        // dW1 = u dz1^T
+        const bfv4 uT = transpose16(tile, ub, lane);
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const bfv4 dz1T = transpose16(tile, dz1b[o], lane);
+            gW1[o] = mfma(uT, dz1T, gW1[o]);
+        }
+    }
+
+    // sum the per-sample partials over the 16 lanes that share features
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        gb1[i] = sum16v(gb1[i]);
+        gb2[i] = sum16v(gb2[i]);
+        gw3[i] = sum16v(gw3[i]);
+    }
+    gwl = sum16v(gwl);
+    gb3 = sum16(gb3);
+    loss = sum16(loss);
+
+    float *out = partials + (size_t)wave * (kMlpParams + 1);
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            out[kOffW1 + (rb + j) * kMlpHid + 16 * o + sl] = gW1[o][j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                out[kOffW2 + (16 * i + rb + j) * kMlpHid + 16 * o + sl] = gW2[i][o][j];
+        }
+    }
+    if (sl == 0) {
+#pragma unroll
+        for (int o = 0; o < 4; ++o)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                out[kOffB1 + 16 * o + rb + j] = gb1[o][j];
+                out[kOffB2 + 16 * o + rb + j] = gb2[o][j];
+                out[kOffW3 + 16 * o + rb + j] = gw3[o][j];
+            }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[kOffWL + rb + j] = gwl[j];
+    }
+    if (lane == 0) {
+        out[kOffB3] = gb3;
+        out[kMlpParams] = loss;
+    }
+}
+
+// Sums the wave rows in a fixed order: grad[p] for p < kMlpParams, the
+// squared error at grad[kMlpParams].
+__global__ void k_mlp_reduce(const float *__restrict__ partials, int rows, float *__restrict__ grad)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > kMlpParams) return;
+    float s = 0.f;
+    for (int r = 0; r < rows; ++r) s = s + partials[(size_t)r * (kMlpParams + 1) + p];
+    grad[p] = s;
+}
+
+// Adam; the linear skip (wlin) moves at lr_skip instead of lr, and with
+// `odd` the biases stay where they are (zero: the network is then an odd
+// function of its inputs, like the linear field update it stands in for).
+__global__ void k_mlp_adam(float *__restrict__ prm, float *__restrict__ m, float *__restrict__ v,
+                           const float *__restrict__ grad, float lr, float lr_skip, int odd,
+                           float beta1, float beta2, float eps, float bc1, float bc2)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    if (p >= kOffWL) lr = lr_skip;
+    const bool bias = (p >= kOffB1 && p < kOffW2) || (p >= kOffB2 && p < kOffW3) || p == kOffB3;
+    if (odd && bias) return;
+    const float g = grad[p];
+    const float mn = beta1 * m[p] + (1.0f - beta1) * g;
+    const float vn = beta2 * v[p] + (1.0f - beta2) * g * g;
+    m[p] = mn;
+    v[p] = vn;
+    prm[p] = prm[p] - lr * (mn / bc1) / (sqrtf(vn / bc2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Face hook
+// ---------------------------------------------------------------------------
+// Face coordinates (normal n, fast f, slow s) -> linear cell index.  The
+// fast axis is the lower-numbered tangential axis: a = 0 (y, z), 1 (x, z),
+// 2 (x, y).
+__device__ __forceinline__ size_t face_cell(const Geo &g, int a, int n, int f, int s)
+{
+    const int x = a == 0 ? n : f;
+    const int y = a == 0 ? f : (a == 1 ? n : s);
+    const int z = a == 2 ? n : s;
+    return ((size_t)z * g.ny + y) * g.nx + x;
+}
+
+// Index into axis a's psi slab (layout in wf_kernels.h) for face
+// coordinates; T = graded depth of axis a, na = its length.
+__device__ __forceinline__ size_t face_psi(const Geo &g, int a, int T, int na, int n, int f, int s)
+{
+    const int gi = n < T ? n : n - (na - 2 * T);
+    if (a == 0) return ((size_t)s * g.ny + f) * (2 * T) + gi;   // (z, y)
+    if (a == 1) return ((size_t)s * (2 * T) + gi) * g.nx + f;  // (z, x)
+    return ((size_t)gi * g.ny + s) * g.nx + f;                  // (y, x)
+}
+
+// Pointer selection by a run-time index without indexing the by-value
+// kernel argument (which would spill it to scratch).
+__device__ __forceinline__ const float *field_ptr(const Ptrs &p, int c)
+{
+    switch (c) {
+    case 0: return p.f[0];
+    case 1: return p.f[1];
+    case 2: return p.f[2];
+    case 3: return p.f[3];
+    case 4: return p.f[4];
+    default: return p.f[5];
+    }
+}
+
+__device__ __forceinline__ const float *psi_ptr(const Ptrs &p, int a, int t)
+{
+    const int k = 4 * a + t;
+    switch (k) {
+    case 0: return p.psi[0][0];
+    case 1: return p.psi[0][1];
+    case 2: return p.psi[0][2];
+    case 3: return p.psi[0][3];
+    case 4: return p.psi[1][0];
+    case 5: return p.psi[1][1];
+    case 6: return p.psi[1][2];
+    case 7: return p.psi[1][3];
+    case 8: return p.psi[2][0];
+    case 9: return p.psi[2][1];
+    case 10: return p.psi[2][2];
+    default: return p.psi[2][3];
+    }
+}
+
+// Inputs per sample (face position, polarization): P = the component whose
+// ghost value the boundary cell's update reads, Q = its partner in the
+// curl, d = depth from the boundary cell inward:
+//   0..3   P(d)                     4..7  Q(d)
+//   8, 9   transverse Laplacian of P at d = 0, 1 (within the face region)
+//   10, 11 CPML psi of P's normal derivative at d = 0, 1
+//   12..15 g(n-1), g(n-2), P(0)(n-1), P(0)(n-2)   (history)
+// High faces (before the H phase): P = E_t1 / E_t2, Q = H_t2 / H_t1, ghost
+// E at b0+1.  Low faces (before the E phase): P = H_t1 / H_t2, Q = E_t2 /
+// E_t1, ghost H at b0-1.  t1 = (a+1)%3, t2 = (a+2)%3.
+__global__ __launch_bounds__(256) void k_face_hook(Geo g, Ptrs es, Ptrs hs, FaceHook h,
+                                                   const uint64_t *__restrict__ step)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int npos = h.nfast * h.nslow;
+    const int gpp = (npos + 15) / 16;  // groups per polarization
+    const int a = h.axis, t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+    const int q4 = lane >> 4, sl = lane & 15;
+    const int dir = h.hi ? -1 : 1;
+    const int na = h.na, Ta = h.Ta;
+    const bool pf = (g.periodic >> (t1 < t2 ? t1 : t2)) & 1u;  // fast axis wraps
+    const bool ps = (g.periodic >> (t1 < t2 ? t2 : t1)) & 1u;  // slow axis wraps
+
+    MlpRegs r;
+    if (h.mode == 0) load_regs(h.net, lane, r);
+    long row = -1;
+    if (h.mode == 1) {
+        const uint64_t k = *step - h.emit_base;
+        if (k % h.emit_every == 0 && k / h.emit_every < h.emit_rows) row = (long)(k / h.emit_every);
+    }
+
+    for (int grp = wave; grp < 2 * gpp; grp += nwaves) {
+        const int pol = grp / gpp;
+        const int pos = (grp - pol * gpp) * 16 + sl;
+        const bool live = pos < npos;
+        if (h.mode == 2) {
+            // replay: ghost values recorded by a thick-CPML run, step by step
+            if (lane < 16 && live) {
+                const uint64_t k = *step - h.emit_base;
+                const float v = k < h.emit_rows
+                                    ? h.samples[((k * 2 + pol) * npos + pos) * kSampleStride + 17]
+                                    : 0.f;
+                (pol == 0 ? h.ghost[0] : h.ghost[1])[pos] = v;
+            }
+            continue;
+        }
+        const int f = h.fast0 + (live ? pos % h.nfast : 0);
+        const int sc = h.slow0 + (live ? pos / h.nfast : 0);
+        int P, Q, term;
+        if (h.hi) {
+            P = pol == 0 ? t1 : t2;
+            Q = 3 + (pol == 0 ? t2 : t1);
+            term = (a == 1) ? pol : 1 - pol;  // H-phase psi of dP/da
+        } else {
+            P = 3 + (pol == 0 ? t1 : t2);
+            Q = pol == 0 ? t2 : t1;
+            term = (a == 1) ? 2 + pol : 3 - pol;  // E-phase psi of dP/da
+        }
+        const float *Pf = h.hi ? field_ptr(es, P) : field_ptr(hs, P);
+        const float *Qf = h.hi ? field_ptr(hs, Q) : field_ptr(es, Q);
+        const float *Ps = psi_ptr(es, a, term);
+        float *mem = h.mem + ((size_t)pol * npos + (live ? pos : 0)) * 4;
+
+        fv4 x = {0.f, 0.f, 0.f, 0.f};
+        if (live) {
+            if (q4 == 0 || q4 == 1) {
+                const float *F = q4 == 0 ? Pf : Qf;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) x[j] = F[face_cell(g, a, h.b0 + dir * j, f, sc)];
+            } else if (q4 == 2) {
+                // transverse neighbours within the face region: zero beyond
+                // it (the thin domain ends there), wrapped on periodic axes
+                // (whose region is the whole extent)
+                const int f1 = h.fast0 + h.nfast - 1, s1 = h.slow0 + h.nslow - 1;
+                const int fpi = f < f1 ? f + 1 : (pf ? h.fast0 : -1);
+                const int fmi = f > h.fast0 ? f - 1 : (pf ? f1 : -1);
+                const int spi = sc < s1 ? sc + 1 : (ps ? h.slow0 : -1);
+                const int smi = sc > h.slow0 ? sc - 1 : (ps ? s1 : -1);
+#pragma unroll
+                for (int d = 0; d < 2; ++d) {
+                    const int n = h.b0 + dir * d;
+                    const float p0 = Pf[face_cell(g, a, n, f, sc)];
+                    const float fp = fpi >= 0 ? Pf[face_cell(g, a, n, fpi, sc)] : 0.f;
+                    const float fm = fmi >= 0 ? Pf[face_cell(g, a, n, fmi, sc)] : 0.f;
+                    const float sp = spi >= 0 ? Pf[face_cell(g, a, n, f, spi)] : 0.f;
+                    const float sm = smi >= 0 ? Pf[face_cell(g, a, n, f, smi)] : 0.f;
+                    x[d] = ((fp + fm) + (sp + sm)) - 4.0f * p0;
+                    const bool gr = Ta > 0 && (n < Ta || n >= na - Ta);
+                    x[2 + d] = gr ? Ps[face_psi(g, a, Ta, na, n, f, sc)] : 0.f;
+                }
+            } else {
+                x = *reinterpret_cast<const fv4 *>(mem);
+            }
+        }
+        // scale: the largest magnitude of the 16 inputs (no squares, so no
+        // underflow for the tiny fields ahead of a wave)
+        float s = fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w)));
+        s = fmaxf(s, __shfl_xor(s, 16));
+        s = fmaxf(s, __shfl_xor(s, 32));
+        const fv4 u = s > 0.f ? x / s : fv4{0.f, 0.f, 0.f, 0.f};
+        // history on lanes 0..15: their own P(0) and the memory lanes' values
+        const float g1 = __shfl(x.x, sl + 48), p1 = __shfl(x.z, sl + 48);
+        float gnew;
+        if (h.mode == 0) {
+            Acts t;
+            const float y = mlp_forward(r, to_bf4(u), u, t);
+            gnew = s > 0.f ? s * y : 0.f;
+            if (lane < 16 && live) (pol == 0 ? h.ghost[0] : h.ghost[1])[pos] = gnew;
+        } else {
+            gnew = live ? Pf[face_cell(g, a, h.b0 - dir, f, sc)] : 0.f;
+            if (row >= 0 && live) {
+                float *o = h.samples + (((size_t)row * 2 + pol) * npos + pos) * kSampleStride;
+                *reinterpret_cast<fv4 *>(o + 4 * q4) = u;  // 80-byte records: 16-byte aligned
+                if (lane < 16) {
+                    o[16] = s > 0.f ? gnew / s : 0.f;
+                    o[17] = gnew;  // raw ghost value (replay)
+                    o[18] = s;
+                }
+            }
+        }
+        if (lane < 16 && live) *reinterpret_cast<fv4 *>(mem) = fv4{gnew, g1, x.x, p1};
+    }
+}
+
+}  // namespace
+
+hipError_t pack_mlp(const float *params, MlpPacked *net, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, st, params, net);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_hook(const Geo &g, const Ptrs &esrc, const Ptrs &hsrc, const FaceHook &h,
+                            const uint64_t *step, hipStream_t st)
+{
+    const int npos = h.nfast * h.nslow;
+    const int groups = 2 * ((npos + 15) / 16);
+    int blocks = (groups + 3) / 4;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks < 1) return hipSuccess;
+    hipLaunchKernelGGL(k_face_hook, dim3(blocks), dim3(256), 0, st, g, esrc, hsrc, h, step);
+    return hipGetLastError();
+}
+
+}  // namespace wf
+
+using namespace wf;
+
+#define WF_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(WF_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                           \
+    } while (0)
+
+// ===========================================================================
+// Trainer
+// ===========================================================================
+struct wf_mlp_trainer {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t n = 0;
+    float *d_u = nullptr, *d_t = nullptr;
+    float *d_p = nullptr, *d_m = nullptr, *d_v = nullptr, *d_grad = nullptr, *d_part = nullptr;
+    MlpPacked *d_w = nullptr;
+    int blocks = 0;
+    uint64_t adam_t = 0;
+};
+
+namespace {
+
+constexpr int kTrainBlocks = 256;  // 1024 waves
+
+template <typename T>
+int dalloc(T **p, size_t bytes)
+{
+    *p = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(p), bytes) != hipSuccess) {
+        *p = nullptr;
+        return fail(WF_ERR_OOM, "hipMalloc(%zu bytes) failed", bytes);
+    }
+    return WF_OK;
+}
+
+int trainer_grad(wf_mlp_trainer *t)
+{
+    hipLaunchKernelGGL(k_mlp_train, dim3(t->blocks), dim3(256), 0, t->stream, t->d_u, t->d_t,
+                       (int)t->n, 1.0f / (float)t->n, t->d_w, t->d_part);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, t->stream,
+                       t->d_part, t->blocks * 4, t->d_grad);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t wf_mlp_param_count(void) { return (uint32_t)kMlpParams; }
+
+void wf_mlp_trainer_destroy(wf_mlp_trainer *t)
+{
+    if (!t) return;
+    (void)hipSetDevice(t->device);
+    if (t->stream) (void)hipStreamSynchronize(t->stream);
+    for (float *p : {t->d_u, t->d_t, t->d_p, t->d_m, t->d_v, t->d_grad, t->d_part})
+        if (p) (void)hipFree(p);
+    if (t->d_w) (void)hipFree(t->d_w);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+    delete t;
+}
+
+int wf_mlp_trainer_create(int device, uint64_t n, const float *inputs, const float *targets,
+                          wf_mlp_trainer **out)
+{
+    clear_error();
+    if (!out || !inputs || !targets || n == 0) return fail(WF_ERR_INVALID, "wf_mlp_trainer_create: bad arguments");
+    if (n > 0x7fffffffull) return fail(WF_ERR_INVALID, "wf_mlp_trainer_create: too many samples");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(WF_ERR_HIP, "no HIP device %d available", device);
+    wf_mlp_trainer *t = new wf_mlp_trainer;
+    t->device = device;
+    t->n = n;
+    t->blocks = kTrainBlocks;
+    int rc = WF_OK;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(WF_ERR_HIP, "wf_mlp_trainer_create: stream");
+    if (!rc) rc = dalloc(&t->d_u, n * kMlpIn * sizeof(float));
+    if (!rc) rc = dalloc(&t->d_t, n * sizeof(float));
+    if (!rc) rc = dalloc(&t->d_p, kMlpParams * sizeof(float));
+    if (!rc) rc = dalloc(&t->d_m, kMlpParams * sizeof(float));
+    if (!rc) rc = dalloc(&t->d_v, kMlpParams * sizeof(float));
+    if (!rc) rc = dalloc(&t->d_grad, (kMlpParams + 1) * sizeof(float));
+    if (!rc) rc = dalloc(&t->d_part, (size_t)t->blocks * 4 * (kMlpParams + 1) * sizeof(float));
+    if (!rc) rc = dalloc(&t->d_w, sizeof(MlpPacked));
+    if (!rc && (hipMemcpy(t->d_u, inputs, n * kMlpIn * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(t->d_t, targets, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemset(t->d_p, 0, kMlpParams * sizeof(float)) != hipSuccess ||
+                hipMemset(t->d_m, 0, kMlpParams * sizeof(float)) != hipSuccess ||
+                hipMemset(t->d_v, 0, kMlpParams * sizeof(float)) != hipSuccess))
+        rc = fail(WF_ERR_HIP, "wf_mlp_trainer_create: upload failed");
+    if (!rc) {
+        hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, t->stream, t->d_p, t->d_w);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(t->stream) != hipSuccess)
+            rc = fail(WF_ERR_HIP, "wf_mlp_trainer_create: pack failed");
+    }
+    if (rc) {
+        wf_mlp_trainer_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return WF_OK;
+}
+
+int wf_mlp_set_params(wf_mlp_trainer *t, const float *params)
+{
+    clear_error();
+    if (!t || !params) return fail(WF_ERR_INVALID, "wf_mlp_set_params: null pointer");
+    WF_HIP(hipSetDevice(t->device));
+    WF_HIP(hipMemcpy(t->d_p, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    WF_HIP(hipMemset(t->d_m, 0, kMlpParams * sizeof(float)));
+    WF_HIP(hipMemset(t->d_v, 0, kMlpParams * sizeof(float)));
+    t->adam_t = 0;
+    hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, t->stream, t->d_p, t->d_w);
+    WF_HIP(hipGetLastError());
+    WF_HIP(hipStreamSynchronize(t->stream));
+    return WF_OK;
+}
+
+int wf_mlp_get_params(wf_mlp_trainer *t, float *params)
+{
+    clear_error();
+    if (!t || !params) return fail(WF_ERR_INVALID, "wf_mlp_get_params: null pointer");
+    WF_HIP(hipSetDevice(t->device));
+    WF_HIP(hipStreamSynchronize(t->stream));
+    WF_HIP(hipMemcpy(params, t->d_p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    return WF_OK;
+}
+
+int wf_mlp_gradient(wf_mlp_trainer *t, float *grad, double *loss)
+{
+    clear_error();
+    if (!t || !grad) return fail(WF_ERR_INVALID, "wf_mlp_gradient: null pointer");
+    WF_HIP(hipSetDevice(t->device));
+    if (int rc = trainer_grad(t)) return rc;
+    std::vector<float> g(kMlpParams + 1);
+    WF_HIP(hipMemcpyAsync(g.data(), t->d_grad, (kMlpParams + 1) * sizeof(float), hipMemcpyDeviceToHost,
+                          t->stream));
+    WF_HIP(hipStreamSynchronize(t->stream));
+    std::memcpy(grad, g.data(), kMlpParams * sizeof(float));
+    if (loss) *loss = (double)g[kMlpParams] / (double)t->n;
+    return WF_OK;
+}
+
+int wf_mlp_train(wf_mlp_trainer *t, uint32_t iters, float lr, float lr_skip, int odd, double *loss)
+{
+    clear_error();
+    if (!t) return fail(WF_ERR_INVALID, "wf_mlp_train: null trainer");
+    WF_HIP(hipSetDevice(t->device));
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (uint32_t i = 0; i < iters; ++i) {
+        if (int rc = trainer_grad(t)) return rc;
+        t->adam_t += 1;
+        const float bc1 = 1.0f - std::pow(b1, (float)t->adam_t);
+        const float bc2 = 1.0f - std::pow(b2, (float)t->adam_t);
+        hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, t->stream, t->d_p,
+                           t->d_m, t->d_v, t->d_grad, lr, lr_skip, odd, b1, b2, eps, bc1, bc2);
+        hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, t->stream, t->d_p, t->d_w);
+        WF_HIP(hipGetLastError());
+    }
+    if (loss) {
+        // loss at the final parameters
+        if (int rc = trainer_grad(t)) return rc;
+        float l = 0.f;
+        WF_HIP(hipMemcpyAsync(&l, t->d_grad + kMlpParams, sizeof(float), hipMemcpyDeviceToHost, t->stream));
+        WF_HIP(hipStreamSynchronize(t->stream));
+        *loss = (double)l / (double)t->n;
+    }
+    WF_HIP(hipStreamSynchronize(t->stream));
+    return WF_OK;
+}
+
+int wf_mlp_predict(wf_mlp_trainer *t, uint64_t n, const float *inputs, float *outputs)
+{
+    clear_error();
+    if (!t || !inputs || !outputs) return fail(WF_ERR_INVALID, "wf_mlp_predict: null pointer");
+    if (n == 0) return WF_OK;
+    if (n > 0x7fffffffull) return fail(WF_ERR_INVALID, "wf_mlp_predict: too many samples");
+    WF_HIP(hipSetDevice(t->device));
+    float *du = nullptr, *dy = nullptr;
+    if (int rc = dalloc(&du, n * kMlpIn * sizeof(float))) return rc;
+    if (int rc = dalloc(&dy, n * sizeof(float))) {
+        (void)hipFree(du);
+        return rc;
+    }
+    int rc = WF_OK;
+    if (hipMemcpy(du, inputs, n * kMlpIn * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(WF_ERR_HIP, "wf_mlp_predict: upload");
+    if (!rc) {
+        const uint64_t groups = (n + 15) / 16;
+        unsigned blocks = (unsigned)std::min<uint64_t>((groups + 3) / 4, 1024);
+        hipLaunchKernelGGL(k_mlp_predict, dim3(blocks), dim3(256), 0, t->stream, du, (int)n, t->d_w, dy);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(t->stream) != hipSuccess ||
+            hipMemcpy(outputs, dy, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(WF_ERR_HIP, "wf_mlp_predict: kernel failed");
+    }
+    (void)hipFree(du);
+    (void)hipFree(dy);
+    return rc;
+}
+
+}  // extern "C"

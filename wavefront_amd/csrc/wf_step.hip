@@ -84,7 +84,7 @@ __device__ __forceinline__ SplitGeo split_geo(const Box &b, int local, int zchun
 
 // One workgroup of the H kernel: tile `local` of box b.  s_co: LDS for the
 // (da, db) table.
-template <bool PML>
+template <bool PML, bool GHOST = false>
 __device__ __forceinline__ void h4_tile(const Geo &g, const Ptrs &src, const Ptrs &dst,
                                         const Box &b, int local, int zchunk, float2 *s_co)
 {
@@ -135,10 +135,16 @@ __device__ __forceinline__ void h4_tile(const Geo &g, const Ptrs &src, const Ptr
             if (zp >= 0) {
                 ex_n = ld4(ex + (size_t)zp * plane + row);
                 ey_n = ld4(ey + (size_t)zp * plane + row);
+            } else if (GHOST && src.ghost[5][0]) {
+                ex_n = ld4(src.ghost[5][0] + (size_t)y * nx + x);
+                ey_n = ld4(src.ghost[5][1] + (size_t)y * nx + x);
             }
             if (yp >= 0) {
                 ez_py = ld4(ez + zo + rowp_y);
                 ex_py = ld4(ex + zo + rowp_y);
+            } else if (GHOST && src.ghost[3][0]) {
+                ez_py = ld4(src.ghost[3][0] + (size_t)z * nx + x);
+                ex_py = ld4(src.ghost[3][1] + (size_t)z * nx + x);
             }
         }
         float sez = __shfl_down(ez_c.x, 1, lpr), sey = __shfl_down(ey_c.x, 1, lpr);
@@ -148,6 +154,9 @@ __device__ __forceinline__ void h4_tile(const Geo &g, const Ptrs &src, const Ptr
             if (live && xq >= 0) {
                 sez = ez[zo + rowq];
                 sey = ey[zo + rowq];
+            } else if (GHOST && live && src.ghost[1][0]) {
+                sey = src.ghost[1][0][(size_t)z * g.ny + y];
+                sez = src.ghost[1][1][(size_t)z * g.ny + y];
             }
         }
         const f4 ez_px = f4{ez_c.y, ez_c.z, ez_c.w, sez};
@@ -212,14 +221,14 @@ __device__ __forceinline__ void h4_tile(const Geo &g, const Ptrs &src, const Ptr
     }
 }
 
-template <bool PML>
+template <bool PML, bool GHOST>
 __global__ __launch_bounds__(64 * kSY) void k_h4(Geo g, Ptrs src, Ptrs dst, BoxList bl,
                                                  int zchunk)
 {
     __shared__ float2 s_co[256];  // (da, db)
     int local;
     const Box &b = find_box(bl, xcd_tile(blockIdx.x, bl.total), local);
-    h4_tile<PML>(g, src, dst, b, local, zchunk, s_co);
+    h4_tile<PML, GHOST>(g, src, dst, b, local, zchunk, s_co);
 }
 
 // Batched H kernel: every scene in place, one launch.
@@ -240,7 +249,7 @@ __global__ __launch_bounds__(64 * kSY) void k_h4_batch(const BatchScene *__restr
 
 // One workgroup of the E kernel.  hsrc: where H(n+1/2) lives; esrc: E^n and
 // the psi memory of step n; dst: E(n+1) and psi of step n+1.
-template <bool PML>
+template <bool PML, bool GHOST = false>
 __device__ __forceinline__ void e4_tile(const Geo &g, const Ptrs &hsrc, const Ptrs &esrc,
                                         const Ptrs &dst, const Box &b, int local, int zchunk,
                                         float2 *s_co)
@@ -283,6 +292,9 @@ __device__ __forceinline__ void e4_tile(const Geo &g, const Ptrs &hsrc, const Pt
         if (live && zm >= 0) {
             hx_mz = ld4(hx + (size_t)zm * plane + row);
             hy_mz = ld4(hy + (size_t)zm * plane + row);
+        } else if (GHOST && live && esrc.ghost[4][0]) {
+            hx_mz = ld4(esrc.ghost[4][0] + (size_t)y * nx + x);
+            hy_mz = ld4(esrc.ghost[4][1] + (size_t)y * nx + x);
         }
     }
     for (int z = s.z0; z < s.z1; ++z) {
@@ -296,6 +308,9 @@ __device__ __forceinline__ void e4_tile(const Geo &g, const Ptrs &hsrc, const Pt
             if (ym >= 0) {
                 hz_my = ld4(hz + zo + rowm_y);
                 hx_my = ld4(hx + zo + rowm_y);
+            } else if (GHOST && esrc.ghost[2][0]) {
+                hz_my = ld4(esrc.ghost[2][0] + (size_t)z * nx + x);
+                hx_my = ld4(esrc.ghost[2][1] + (size_t)z * nx + x);
             }
         }
         float shz = __shfl_up(hz_c.w, 1, lpr), shy = __shfl_up(hy_c.w, 1, lpr);
@@ -305,6 +320,9 @@ __device__ __forceinline__ void e4_tile(const Geo &g, const Ptrs &hsrc, const Pt
             if (live && xq >= 0) {
                 shz = hz[zo + rowq];
                 shy = hy[zo + rowq];
+            } else if (GHOST && live && esrc.ghost[0][0]) {
+                shy = esrc.ghost[0][0][(size_t)z * g.ny + y];
+                shz = esrc.ghost[0][1][(size_t)z * g.ny + y];
             }
         }
         const f4 hz_mx = f4{shz, hz_c.x, hz_c.y, hz_c.z};
@@ -369,14 +387,14 @@ __device__ __forceinline__ void e4_tile(const Geo &g, const Ptrs &hsrc, const Pt
     }
 }
 
-template <bool PML>
+template <bool PML, bool GHOST>
 __global__ __launch_bounds__(64 * kSY) void k_e4(Geo g, Ptrs hsrc, Ptrs esrc, Ptrs dst,
                                                  BoxList bl, int zchunk)
 {
     __shared__ float2 s_co[256];  // (ca, cb)
     int local;
     const Box &b = find_box(bl, xcd_tile(blockIdx.x, bl.total), local);
-    e4_tile<PML>(g, hsrc, esrc, dst, b, local, zchunk, s_co);
+    e4_tile<PML, GHOST>(g, hsrc, esrc, dst, b, local, zchunk, s_co);
 }
 
 template <bool PML>
@@ -624,10 +642,12 @@ hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st)
 {
     if (bl.total == 0) return hipSuccess;
     const dim3 grid((unsigned)bl.total), block(64, kSY);
-    if (s.pml)
-        hipLaunchKernelGGL(k_h4<true>, grid, block, 0, st, s.g, s.cur, s.alt, bl, s.zchunk);
+    if (s.ghost)
+        hipLaunchKernelGGL((k_h4<true, true>), grid, block, 0, st, s.g, s.cur, s.alt, bl, s.zchunk);
+    else if (s.pml)
+        hipLaunchKernelGGL((k_h4<true, false>), grid, block, 0, st, s.g, s.cur, s.alt, bl, s.zchunk);
     else
-        hipLaunchKernelGGL(k_h4<false>, grid, block, 0, st, s.g, s.cur, s.alt, bl, s.zchunk);
+        hipLaunchKernelGGL((k_h4<false, false>), grid, block, 0, st, s.g, s.cur, s.alt, bl, s.zchunk);
     return hipGetLastError();
 }
 
@@ -635,10 +655,14 @@ hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st)
 {
     if (bl.total == 0) return hipSuccess;
     const dim3 grid((unsigned)bl.total), block(64, kSY);
-    if (s.pml)
-        hipLaunchKernelGGL(k_e4<true>, grid, block, 0, st, s.g, s.alt, s.cur, s.alt, bl, s.zchunk);
+    if (s.ghost)
+        hipLaunchKernelGGL((k_e4<true, true>), grid, block, 0, st, s.g, s.alt, s.cur, s.alt, bl,
+                           s.zchunk);
+    else if (s.pml)
+        hipLaunchKernelGGL((k_e4<true, false>), grid, block, 0, st, s.g, s.alt, s.cur, s.alt, bl,
+                           s.zchunk);
     else
-        hipLaunchKernelGGL(k_e4<false>, grid, block, 0, st, s.g, s.alt, s.cur, s.alt, bl,
+        hipLaunchKernelGGL((k_e4<false, false>), grid, block, 0, st, s.g, s.alt, s.cur, s.alt, bl,
                            s.zchunk);
     return hipGetLastError();
 }

@@ -1,0 +1,421 @@
+"""Learned thin absorbing boundary.
+
+The reference ends every face with a graded CPML backed by the zero field
+(src/layout.rs:471-513, OUTER_ZERO_BLOCK in src/engine.rs:83).  Here a face
+can instead end after LAYER cells: the layer is the first LAYER cells
+(counted from the interior) of a THICK-cell CPML profile, and the fields just
+outside it -- the "ghost" values the boundary cell's curl update reads,
+zero in the reference -- are predicted each half step by a small per-face
+network from the 4-cell-deep E/H/psi stencil at the face and a two-step
+history (the C ABI's wf_set_learned_faces).
+
+Training data come from the engine itself: a run of the same physics with
+the full THICK-cell CPML, whose first LAYER cells coincide with the thin
+layer, records at the virtual thin-domain faces the network inputs and the
+true ghost values (wf_record_face_samples).  A predictor that reproduced the
+ghosts exactly would make the thin run equal the thick run inside the thin
+domain (wf_replay_faces checks exactly that).  The network is trained on the
+GPU by the library's BF16-MFMA forward/backward kernels (wf_mlp_train).
+"""
+from __future__ import annotations
+
+import math
+from ctypes import POINTER, byref, c_double, c_float, c_void_p
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import FaceRegion, check, lib, ptr
+from .engine import DeviceEngine
+from .layout import BLOCK_DIM, GridDims, MaterialTable, PmlConfig, PmlContext, PmlProfile1D
+from .source import FieldComponent as SC
+from .source import Source, Waveform
+
+THICK = 64           # the CPML the thin layer is cut from (voxels)
+LAYER = 4            # thin layer depth (voxels)
+OFF = THICK - LAYER  # thin-domain origin inside the thick run, per learned axis
+FACES = ("x_lo", "x_hi", "y_lo", "y_hi", "z_lo", "z_hi")
+STRIDE = 20          # floats per recorded sample
+f32 = np.float32
+
+
+def param_count() -> int:
+    return int(lib().wf_mlp_param_count())
+
+
+# ---------------------------------------------------------------------------
+# Layer profiles
+# ---------------------------------------------------------------------------
+def thick_profile(n: int, dt: float, dx: float, cfg: PmlConfig | None = None) -> PmlProfile1D:
+    """PmlProfile1D::build with a THICK-voxel graded layer (src/layout.rs:471-513)."""
+    cfg = cfg or PmlConfig(thickness=THICK)
+    return PmlProfile1D.build(n, cfg, dt, dx, THICK)
+
+
+def thin_profile(n: int, dt: float, dx: float, cfg: PmlConfig | None = None) -> PmlProfile1D:
+    """The first LAYER cells (from the interior) of the thick profile of the
+    corresponding thick domain (n + 2*OFF cells), mirrored at both ends;
+    identity elsewhere."""
+    nb = n + 2 * OFF
+    big = thick_profile(nb, dt, dx, cfg)
+    b = np.ones(n, f32)
+    a = np.zeros(n, f32)
+    k = np.ones(n, f32)
+    for p in range(LAYER):
+        b[p], a[p], k[p] = big.b[p + OFF], big.a[p + OFF], big.inv_kappa[p + OFF]
+        q = nb - 1 - (p + OFF)
+        b[n - 1 - p], a[n - 1 - p], k[n - 1 - p] = big.b[q], big.a[q], big.inv_kappa[q]
+    return PmlProfile1D(b, a, k)
+
+
+def _axes_mask(periodic: int):
+    return [not (periodic >> a & 1) for a in range(3)]
+
+
+def thin_context(dims: GridDims, dt: float, dx: float, periodic: int = 0) -> PmlContext:
+    """Shell of one block and the thin profile on every non-periodic axis."""
+    n = (dims.nx, dims.ny, dims.nz)
+    learned = _axes_mask(periodic)
+    profs = [thin_profile(n[a], dt, dx) if learned[a] else PmlProfile1D.identity(n[a])
+             for a in range(3)]
+    return PmlContext(profs, tuple(1 if learned[a] else 0 for a in range(3)))
+
+
+def thick_dims(dims: GridDims, periodic: int = 0) -> GridDims:
+    learned = _axes_mask(periodic)
+    n = (dims.nx, dims.ny, dims.nz)
+    return GridDims(*[n[a] + (2 * OFF if learned[a] else 0) for a in range(3)])
+
+
+def thick_context(dims: GridDims, dt: float, dx: float, periodic: int = 0) -> PmlContext:
+    """THICK-voxel CPML on every non-periodic axis of the thick domain."""
+    n = (dims.nx, dims.ny, dims.nz)
+    learned = _axes_mask(periodic)
+    profs = [thick_profile(n[a], dt, dx) if learned[a] else PmlProfile1D.identity(n[a])
+             for a in range(3)]
+    return PmlContext(profs, tuple(THICK // BLOCK_DIM if learned[a] else 0 for a in range(3)))
+
+
+def face_mask(periodic: int = 0) -> int:
+    m = 0
+    for a, on in enumerate(_axes_mask(periodic)):
+        if on:
+            m |= 3 << (2 * a)
+    return m
+
+
+# ---------------------------------------------------------------------------
+# Training data
+# ---------------------------------------------------------------------------
+def record_run(thin: GridDims, sources: list[Source], steps: int, dt: float, dx: float,
+               periodic: int = 0, every: int = 1, device: int = 0) -> dict:
+    """Runs the thick-CPML twin of `thin` (sources in thin coordinates) and
+    returns, per learned face, the recorded samples [rows, 2, npos, STRIDE]."""
+    big = thick_dims(thin, periodic)
+    learned = _axes_mask(periodic)
+    shift = [OFF if learned[a] else 0 for a in range(3)]
+    nb = (big.nx, big.ny, big.nz)
+    nt = (thin.nx, thin.ny, thin.nz)
+    mask = face_mask(periodic)
+    regions = (FaceRegion * 6)()
+    for f in range(6):
+        a, hi = f // 2, f % 2
+        if not (mask >> f & 1):
+            continue
+        t1, t2 = (a + 1) % 3, (a + 2) % 3
+        tf, ts = min(t1, t2), max(t1, t2)
+        regions[f] = FaceRegion(nb[a] - 1 - OFF if hi else OFF, shift[tf], nt[tf], shift[ts],
+                                nt[ts])
+    table = MaterialTable.vacuum_filled(dt, dx)
+    srcs = [Source(s.x + shift[0], s.y + shift[1], s.z + shift[2], s.component, s.amplitude,
+                   s.waveform, hard=s.hard) for s in sources]
+    rows = steps // every
+    out = {}
+    with DeviceEngine(big, None, table, thick_context(big, dt, dx, periodic), periodic=periodic,
+                      device=device) as eng:
+        eng.set_sources(srcs, dt, steps)
+        check(lib().wf_record_face_samples(eng.handle, mask, regions, every, rows))
+        eng.step(steps)
+        for f in range(6):
+            if mask >> f & 1:
+                n = int(lib().wf_face_sample_count(eng.handle, f))
+                buf = np.zeros((n, STRIDE), f32)
+                check(lib().wf_read_face_samples(eng.handle, f, ptr(buf, c_float)))
+                out[FACES[f]] = buf.reshape(rows, 2, -1, STRIDE)
+    return out
+
+
+def training_rows(rec: np.ndarray, floor: float = 1e-2):
+    """Scaled inputs and targets of the samples whose input scale s exceeds
+    `floor` times the largest s of the recording: the quiet samples before
+    a wave arrives carry no information about the boundary and would
+    dominate a scale-free loss."""
+    r = rec.reshape(-1, STRIDE)
+    s = r[:, 18]
+    keep = s > floor * float(s.max()) if s.size and s.max() > 0 else np.zeros(len(s), bool)
+    return r[keep, :16].copy(), r[keep, 16].copy()
+
+
+def plane_wave_sources(axis: int, n_axis: int, pos: int, kind: str, freq: float, pol: int,
+                       amp: float = 1.0) -> list[Source]:
+    """A soft sheet source across a block-wide periodic cross-section."""
+    wave = (Waveform.gaussian_pulse(*_gauss(freq)) if kind == "gaussian" else
+            Waveform.ricker_wavelet(freq, float(f32(1.0) / f32(freq))) if kind == "ricker" else
+            Waveform.sinusoid(freq))
+    t1, t2 = (axis + 1) % 3, (axis + 2) % 3
+    comp = [SC.Ex, SC.Ey, SC.Ez][t1 if pol == 0 else t2]
+    out = []
+    for i in range(BLOCK_DIM):
+        for j in range(BLOCK_DIM):
+            c = [0, 0, 0]
+            c[axis], c[t1], c[t2] = pos, i, j
+            out.append(Source(c[0], c[1], c[2], comp, amp, wave))
+    return out
+
+
+def _gauss(freq):
+    spread = float(f32(0.5) / f32(freq))
+    return float(f32(4.0) * f32(spread)), spread
+
+
+# ---------------------------------------------------------------------------
+# Network and training
+# ---------------------------------------------------------------------------
+@dataclass
+class FaceNets:
+    """Parameters of the six face networks, [6, param_count()] float32."""
+    params: np.ndarray
+    dt: float
+    dx: float
+
+    def save(self, path: str) -> None:
+        np.savez(path, params=self.params, dt=self.dt, dx=self.dx)
+
+    @classmethod
+    def load(cls, path: str) -> "FaceNets":
+        z = np.load(path)
+        return cls(z["params"].astype(f32), float(z["dt"]), float(z["dx"]))
+
+
+def init_params(rng: np.random.Generator, wlin=None) -> np.ndarray:
+    """W1, W2 ~ N(0, 1/fan_in), W3 = 0 (the network starts as its linear
+    skip), biases 0, wlin = `wlin` or 0."""
+    p = np.zeros(param_count(), f32)
+    o = 0
+    p[o:o + 16 * 64] = rng.normal(0, 1 / 4.0, 16 * 64)
+    o += 16 * 64 + 64
+    p[o:o + 64 * 64] = rng.normal(0, 1 / 8.0, 64 * 64)
+    o += 64 * 64 + 64 + 64 + 1
+    if wlin is not None:
+        p[o:o + 16] = wlin
+    return p
+
+
+def least_squares_skip(u: np.ndarray, t: np.ndarray) -> np.ndarray:
+    """Initial linear skip: the least-squares fit of the targets (FP64)."""
+    w, *_ = np.linalg.lstsq(u.astype(np.float64), t.astype(np.float64), rcond=None)
+    return w.astype(f32)
+
+
+class Trainer:
+    """A GPU training session over one face's samples."""
+
+    def __init__(self, u: np.ndarray, t: np.ndarray, device: int = 0):
+        self.u = np.ascontiguousarray(u, f32)
+        self.t = np.ascontiguousarray(t, f32)
+        h = c_void_p()
+        check(lib().wf_mlp_trainer_create(device, len(self.t), ptr(self.u, c_float),
+                                          ptr(self.t, c_float), byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().wf_mlp_trainer_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, f32)
+        check(lib().wf_mlp_set_params(self._h, ptr(p, c_float)))
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(param_count(), f32)
+        check(lib().wf_mlp_get_params(self._h, ptr(p, c_float)))
+        return p
+
+    def gradient(self):
+        g = np.zeros(param_count(), f32)
+        loss = c_double()
+        check(lib().wf_mlp_gradient(self._h, ptr(g, c_float), byref(loss)))
+        return g, loss.value
+
+    def train(self, iters: int, lr: float, lr_skip: float | None = None, odd: bool = False) -> float:
+        """Adam steps; the linear skip moves at lr_skip (default lr, 0 = fixed);
+        odd keeps the biases fixed."""
+        loss = c_double()
+        check(lib().wf_mlp_train(self._h, iters, lr, lr if lr_skip is None else lr_skip,
+                                 1 if odd else 0, byref(loss)))
+        return loss.value
+
+    def predict(self, u: np.ndarray) -> np.ndarray:
+        u = np.ascontiguousarray(u, f32)
+        out = np.zeros(len(u), f32)
+        check(lib().wf_mlp_predict(self._h, len(u), ptr(u, c_float), ptr(out, c_float)))
+        return out
+
+
+def train_face(u: np.ndarray, t: np.ndarray, seed: int = 0,
+               schedule=((2000, 1e-4), (1000, 3e-5)), lr_skip: float = 1e-5,
+               device: int = 0, log=None):
+    """Linear skip from least squares, then Adam on the GPU: the MLP at the
+    scheduled rates, the skip at lr_skip, biases held at zero so the network
+    stays an odd function of its inputs (a linear field update maps -x to
+    -y; a bias would add a rectified error).  Returns (params, history of
+    (iters, loss))."""
+    rng = np.random.default_rng(seed)
+    tr = Trainer(u, t, device)
+    tr.set_params(init_params(rng, least_squares_skip(u, t)))
+    hist = [(0, tr.gradient()[1])]
+    done = 0
+    for iters, lr in schedule:
+        loss = tr.train(iters, lr, lr_skip, odd=True)
+        done += iters
+        hist.append((done, loss))
+        if log:
+            log(f"  {done} iters (lr {lr:g}): loss {loss:.3e}")
+    p = tr.params()
+    tr.close()
+    return p, hist
+
+
+# ---------------------------------------------------------------------------
+# Thin-layer engine
+# ---------------------------------------------------------------------------
+def thin_engine(dims: GridDims, nets: FaceNets | None, dt: float, dx: float, periodic: int = 0,
+                ids=None, table=None, device: int = 0, mode: str = "learned",
+                replay: dict | None = None) -> DeviceEngine:
+    """A DeviceEngine of `dims` ending in the thin layer on every non-periodic
+    face.  mode: "learned" (predicted ghosts), "zero" (the thin layer with
+    the reference's zero boundary: the method without the network), or
+    "replay" (ghosts recorded by the thick twin, `replay` = record_run
+    output)."""
+    table = table or MaterialTable.vacuum_filled(dt, dx)
+    eng = DeviceEngine(dims, ids, table, thin_context(dims, dt, dx, periodic), periodic=periodic,
+                       device=device)
+    mask = face_mask(periodic)
+    if mode == "learned":
+        p = np.ascontiguousarray(nets.params, f32).reshape(-1)
+        check(lib().wf_set_learned_faces(eng.handle, mask, ptr(p, c_float)))
+    elif mode == "replay":
+        keep = []
+        arr = (POINTER(c_float) * 6)()
+        rows = None
+        for f in range(6):
+            if mask >> f & 1:
+                r = np.ascontiguousarray(replay[FACES[f]], f32)
+                rows = r.shape[0]
+                keep.append(r)
+                arr[f] = ptr(r, c_float)
+        check(lib().wf_replay_faces(eng.handle, mask, rows, arr))
+        eng._replay_keep = keep
+    return eng
+
+
+# ---------------------------------------------------------------------------
+# Evaluation: the reference's two-run reflection measurement
+# ---------------------------------------------------------------------------
+def study_grid():
+    """dx, dt, frequency of examples/pml_reflection_study.rs:237-244
+    (30 GHz, 20 points per wavelength, Courant number 0.4)."""
+    c0 = f32(299792458.0)
+    freq = f32(3.0e10)
+    dx = (c0 / freq) / f32(20.0)
+    dt = f32(0.4) * dx / c0
+    return float(dx), float(dt), float(freq)
+
+
+def plane_wave_trace(nx: int, dx: float, dt: float, freq: float, x_source: int, x_probe: int,
+                     steps: int, mode: str, nets: FaceNets | None = None,
+                     cpml_voxels: int = 0, device: int = 0) -> np.ndarray:
+    """The study's hard sheet-source plane wave (periodic Y/Z, one block wide)
+    ending in: the learned layer ("learned"), the thin layer with a zero
+    boundary ("zero"), a CPML of cpml_voxels ("cpml", 0 = none)."""
+    from .engine import PERIODIC_Y, PERIODIC_Z
+    from .probe import FieldComponent as PC
+    from .probe import Probe
+    from .studies import run_plane_wave
+    if mode == "cpml":
+        return run_plane_wave(nx, dx, dt, freq, x_source, x_probe, steps, cpml_voxels, device)
+    dims = GridDims(nx, BLOCK_DIM, BLOCK_DIM)
+    per = PERIODIC_Y | PERIODIC_Z
+    wave = Waveform.sinusoid(freq)
+    sources = [Source(x_source, ly, lz, SC.Ez, 1.0, wave, hard=True)
+               for ly in range(BLOCK_DIM) for lz in range(BLOCK_DIM)]
+    probe = Probe(x_probe, 0, 0, PC.Ez, [], 0.0)
+    with thin_engine(dims, nets, dt, dx, periodic=per, device=device, mode=mode) as eng:
+        eng.set_sources(sources, dt, steps)
+        eng.set_probes([probe], dt, steps, trace=True)
+        eng.step(steps)
+        return eng.read_trace(steps)[:, 0].copy()
+
+
+def reflection_geometry(thickness: int):
+    """Test/reference geometry of pml_reflection_study.rs:246-313 for a layer of
+    `thickness` voxels: (nx_test, x_source, x_probe, steps, skip_before_s,
+    nx_ref, x_source_ref, x_probe_ref)."""
+    dx, dt, freq = study_grid()
+    c0 = f32(299792458.0)
+    ppw = f32(20.0)
+    probe_off = int(math.floor(float(f32(3.0) * ppw) + 0.5))
+    gap = int(math.floor(float(f32(2.0) * ppw) + 0.5))
+    xs = thickness + gap
+    xp = xs + probe_off
+    face = xp + gap
+    nx = -(-(face + thickness) // BLOCK_DIM) * BLOCK_DIM
+    period = f32(1.0) / f32(freq)
+    inc = f32(probe_off) * f32(dx) / c0
+    rt = f32(2 * (face - xp)) * f32(dx) / c0
+    skip = inc + rt + f32(6.0) * period
+    steps = int(math.ceil(float((skip + f32(10.0) * period) / f32(dt)))) + 4
+    req = int(math.ceil(float((f32(steps) * f32(dt) + f32(2.0) * period) * c0 / f32(dx)))) + BLOCK_DIM
+    xpr = req + probe_off
+    xsr = xpr - probe_off
+    nxr = -(-(xpr + req) // BLOCK_DIM) * BLOCK_DIM
+    return nx, xs, xp, steps, skip, nxr, xsr, xpr
+
+
+def quadrature_amplitude(trace: np.ndarray, dt: float, freq: float, skip) -> float:
+    """pml_reflection_study.rs:204-224."""
+    omega = 2.0 * math.pi * float(f32(freq))
+    sc = ss = 0.0
+    count = 0
+    for i, v in enumerate(trace):
+        t = (f32(i) + f32(1.0)) * f32(dt)
+        if t > skip:
+            ph = omega * float(t)
+            sc += float(v) * math.cos(ph)
+            ss += float(v) * math.sin(ph)
+            count += 1
+    a = 2.0 * sc / count
+    b = 2.0 * ss / count
+    return math.sqrt(a * a + b * b)
+
+
+def reflection(mode: str, nets: FaceNets | None = None, cpml_voxels: int = 0,
+               reference: np.ndarray | None = None, device: int = 0):
+    """Reflection coefficient of a boundary by the study's two-run method.
+    Returns (R, test trace, reference trace)."""
+    dx, dt, freq = study_grid()
+    thickness = cpml_voxels if mode == "cpml" else LAYER
+    nx, xs, xp, steps, skip, nxr, xsr, xpr = reflection_geometry(thickness)
+    test = plane_wave_trace(nx, dx, dt, freq, xs, xp, steps, mode, nets, cpml_voxels, device)
+    if reference is None:
+        reference = plane_wave_trace(nxr, dx, dt, freq, xsr, xpr, steps, "cpml", None, 0, device)
+    inc = quadrature_amplitude(reference, dt, freq, skip)
+    refl = quadrature_amplitude(test - reference, dt, freq, skip)
+    return refl / inc, test, reference
