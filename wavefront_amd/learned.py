@@ -197,12 +197,20 @@ class FaceNets:
         return cls(z["params"].astype(f32), float(z["dt"]), float(z["dx"]))
 
 
-def init_params(rng: np.random.Generator, wlin=None) -> np.ndarray:
+def init_params(rng: np.random.Generator, wlin=None, features=None) -> np.ndarray:
     """W1, W2 ~ N(0, 1/fan_in), W3 = 0 (the network starts as its linear
-    skip), biases 0, wlin = `wlin` or 0."""
+    skip), biases 0, wlin = `wlin` or 0.  Inputs outside `features` (a list
+    of input indices, default all) get zero W1 rows; trained on data whose
+    masked columns are zero, those rows keep a zero gradient and stay zero,
+    so the network ignores those inputs."""
     p = np.zeros(param_count(), f32)
     o = 0
-    p[o:o + 16 * 64] = rng.normal(0, 1 / 4.0, 16 * 64)
+    w1 = rng.normal(0, 1 / 4.0, (16, 64))
+    if features is not None:
+        off = np.ones(16, bool)
+        off[list(features)] = False
+        w1[off] = 0.0
+    p[o:o + 16 * 64] = w1.reshape(-1)
     o += 16 * 64 + 64
     p[o:o + 64 * 64] = rng.normal(0, 1 / 8.0, 64 * 64)
     o += 64 * 64 + 64 + 64 + 1
@@ -211,10 +219,14 @@ def init_params(rng: np.random.Generator, wlin=None) -> np.ndarray:
     return p
 
 
-def least_squares_skip(u: np.ndarray, t: np.ndarray) -> np.ndarray:
-    """Initial linear skip: the least-squares fit of the targets (FP64)."""
-    w, *_ = np.linalg.lstsq(u.astype(np.float64), t.astype(np.float64), rcond=None)
-    return w.astype(f32)
+def least_squares_skip(u: np.ndarray, t: np.ndarray, features=None) -> np.ndarray:
+    """Initial linear skip: the least-squares fit of the targets (FP64) over
+    the input columns `features` (default all; the others get 0)."""
+    cols = list(range(16)) if features is None else list(features)
+    w = np.zeros(16, f32)
+    sol, *_ = np.linalg.lstsq(u[:, cols].astype(np.float64), t.astype(np.float64), rcond=None)
+    w[cols] = sol
+    return w
 
 
 class Trainer:
@@ -269,17 +281,29 @@ class Trainer:
         return out
 
 
+# Inputs used by default: everything but the transverse Laplacian (8, 9).
+# Fitted freely, its weights let checkerboard modes across the face grow in
+# closed-loop 3D runs (the data never contains them); without it the
+# learned faces are stable in 3D.
+FEATURES = (0, 1, 2, 3, 4, 5, 6, 7, 10, 11, 12, 13, 14, 15)
+
+
 def train_face(u: np.ndarray, t: np.ndarray, seed: int = 0,
                schedule=((2000, 1e-4), (1000, 3e-5)), lr_skip: float = 1e-5,
-               device: int = 0, log=None):
+               features=FEATURES, device: int = 0, log=None):
     """Linear skip from least squares, then Adam on the GPU: the MLP at the
     scheduled rates, the skip at lr_skip, biases held at zero so the network
     stays an odd function of its inputs (a linear field update maps -x to
     -y; a bias would add a rectified error).  Returns (params, history of
     (iters, loss))."""
     rng = np.random.default_rng(seed)
+    u = np.array(u, f32)
+    if features is not None:
+        off = np.ones(16, bool)
+        off[list(features)] = False
+        u[:, off] = 0.0
     tr = Trainer(u, t, device)
-    tr.set_params(init_params(rng, least_squares_skip(u, t)))
+    tr.set_params(init_params(rng, least_squares_skip(u, t, features), features))
     hist = [(0, tr.gradient()[1])]
     done = 0
     for iters, lr in schedule:
@@ -419,3 +443,157 @@ def reflection(mode: str, nets: FaceNets | None = None, cpml_voxels: int = 0,
     inc = quadrature_amplitude(reference, dt, freq, skip)
     refl = quadrature_amplitude(test - reference, dt, freq, skip)
     return refl / inc, test, reference
+
+
+# ---------------------------------------------------------------------------
+# Six-face training from 3D runs
+# ---------------------------------------------------------------------------
+def point_sources(rng, thin: GridDims, n: int, margin: int = 12) -> list[Source]:
+    """n Gaussian / Ricker point sources (10-45 GHz) at random interior
+    positions, random E component and amplitude."""
+    out = []
+    for k in range(n):
+        freq = float(f32(rng.uniform(1.0e10, 4.5e10)))
+        wave = (Waveform.gaussian_pulse(*_gauss(freq)) if k % 2 == 0 else
+                Waveform.ricker_wavelet(freq, float(f32(1.0) / f32(freq))))
+        c = [int(rng.integers(margin, m - margin)) for m in (thin.nx, thin.ny, thin.nz)]
+        comp = [SC.Ex, SC.Ey, SC.Ez][int(rng.integers(0, 3))]
+        out.append(Source(c[0], c[1], c[2], comp, float(rng.uniform(0.3, 1.0)), wave))
+    return out
+
+
+def collect_3d(thin: GridDims, runs: int, steps: int, every: int, per_face: int, seed: int,
+               dt: float, dx: float, plane_runs: int = 8, log=None, device: int = 0):
+    """Training samples for all six faces: `runs` thick-CPML runs of `thin`
+    with random point sources (oblique incidence), plus `plane_runs`
+    plane-wave runs along each axis (normal incidence), each face capped at
+    `per_face` samples drawn at random."""
+    from .engine import PERIODIC_X, PERIODIC_Y, PERIODIC_Z
+    rng = np.random.default_rng(seed)
+    pools = {f: [] for f in FACES}
+    for r in range(runs):
+        rec = record_run(thin, point_sources(rng, thin, 3), steps, dt, dx, every=every,
+                         device=device)
+        for f in FACES:
+            u, t = training_rows(rec[f])
+            pools[f].append((u, t))
+        if log:
+            log(f"3D run {r}: " + ", ".join(f"{f} {len(pools[f][-1][1])}" for f in FACES))
+    per = (PERIODIC_Y | PERIODIC_Z, PERIODIC_X | PERIODIC_Z, PERIODIC_X | PERIODIC_Y)
+    for a in range(3):
+        n = 160
+        dims = [BLOCK_DIM] * 3
+        dims[a] = n
+        pd = GridDims(*dims)
+        for r in range(plane_runs):
+            srcs = []
+            for pol in (0, 1):
+                kind = ("gaussian", "ricker")[(r + pol) % 2]
+                freq = float(f32(rng.uniform(1.0e10, 4.5e10)))
+                srcs += plane_wave_sources(a, n, int(rng.integers(8, n - 8)), kind, freq, pol,
+                                           float(rng.uniform(0.3, 1.0)))
+            rec = record_run(pd, srcs, 1100, dt, dx, periodic=per[a], device=device)
+            for f in FACES[2 * a:2 * a + 2]:
+                pools[f].append(training_rows(rec[f][:, :, :1]))
+    out = {}
+    for f in FACES:
+        u = np.concatenate([p[0] for p in pools[f]])
+        t = np.concatenate([p[1] for p in pools[f]])
+        if len(t) > per_face:
+            keep = rng.choice(len(t), per_face, replace=False)
+            u, t = u[keep], t[keep]
+        out[f] = (u, t)
+    return out
+
+
+def train_all(samples: dict, dt: float, dx: float, log=None, device: int = 0) -> FaceNets:
+    p = np.zeros((6, param_count()), f32)
+    for i, f in enumerate(FACES):
+        u, t = samples[f]
+        if log:
+            log(f"{f}: {len(t)} samples")
+        p[i], _ = train_face(u, t, seed=i, device=device, log=log)
+    return FaceNets(p, dt, dx)
+
+
+# ---------------------------------------------------------------------------
+# 3D evaluation: reflection at probes near the faces, and throughput
+# ---------------------------------------------------------------------------
+BOUNDARIES = ("learned", "zero", "cpml8", "cpml32", "cpml64")
+
+
+def _boundary_engine(kind: str, interior: int, nets: FaceNets | None, dt: float, dx: float,
+                     device: int = 0):
+    """(engine, margin): a cube of `interior` cells plus the boundary layer
+    `kind`, and the offset of the interior's first cell."""
+    table = MaterialTable.vacuum_filled(dt, dx)
+    if kind in ("learned", "zero"):
+        n = interior + 2 * LAYER
+        eng = thin_engine(GridDims(n, n, n), nets, dt, dx, device=device,
+                          mode="learned" if kind == "learned" else "zero")
+        return eng, LAYER
+    t = int(kind[4:])
+    n = interior + 2 * t
+    dims = GridDims(n, n, n)
+    prof = [PmlProfile1D.build(n, PmlConfig(thickness=t), dt, dx, t)] * 3
+    ctx = PmlContext(prof, (t // BLOCK_DIM,) * 3)
+    return DeviceEngine(dims, None, table, ctx, device=device), t
+
+
+def probe_points(interior: int, inset: int = 10):
+    """Probes `inset` cells inside the interior: the six face centres, the
+    twelve edge midpoints and the eight corners (normal to grazing)."""
+    lo, mid, hi = inset, interior // 2, interior - 1 - inset
+    pts = set()
+    for x in (lo, mid, hi):
+        for y in (lo, mid, hi):
+            for z in (lo, mid, hi):
+                if (x, y, z) != (mid, mid, mid):
+                    pts.add((x, y, z))
+    return sorted(pts)
+
+
+def reflection_3d(nets: FaceNets, interior: int = 128, steps: int = 520, freq: float = 3.0e10,
+                  kinds=BOUNDARIES, wave_kind: str = "ricker", device: int = 0) -> dict:
+    """A Ricker (or Gaussian) Ez point source at the centre of a vacuum cube; the Ez
+    trace at 26 probes near the faces, edges and corners, for each boundary
+    kind.  Returns per kind the reflected-to-incident energy ratio
+    sum (trace - trace_ref)^2 / sum trace_ref^2 over all probes, with the
+    64-cell CPML run as the reference (its own reflection is the floor)."""
+    from .probe import FieldComponent as PC
+    from .probe import Probe
+    dx, dt = nets.dx, nets.dt
+    c = interior // 2
+    wave = (Waveform.ricker_wavelet(freq, float(f32(1.0) / f32(freq))) if wave_kind == "ricker"
+            else Waveform.gaussian_pulse(*_gauss(freq)))
+    pts = probe_points(interior)
+    traces = {}
+    for kind in dict.fromkeys(("cpml64",) + tuple(kinds)):
+        eng, m = _boundary_engine(kind, interior, nets, dt, dx, device)
+        with eng:
+            eng.set_sources([Source(c + m, c + m, c + m, SC.Ez, 1.0, wave)], dt, steps)
+            eng.set_probes([Probe(x + m, y + m, z + m, PC.Ez, [], 0.0) for x, y, z in pts], dt,
+                           steps, trace=True)
+            eng.step(steps)
+            traces[kind] = eng.read_trace(steps).astype(np.float64)
+    ref = traces["cpml64"]
+    e_ref = float((ref ** 2).sum())
+    return {k: float(((traces[k] - ref) ** 2).sum()) / e_ref for k in kinds if k != "cpml64"}
+
+
+def throughput(kind: str, nets: FaceNets | None, interior: int = 256, steps: int = 100,
+               warmup: int = 16, device: int = 0):
+    """Useful (interior) cell updates per second of a vacuum cube of
+    `interior` cells ending in the boundary `kind`, with a source running.
+    Returns (Mcell/s of the interior, ms per step, cells in the domain)."""
+    dx, dt = nets.dx, nets.dt
+    eng, m = _boundary_engine(kind, interior, nets, dt, dx, device)
+    with eng:
+        c = interior // 2 + m
+        eng.set_sources([Source(c, c, c, SC.Ez, 1.0, Waveform.gaussian_pulse(*_gauss(3.0e10)))], dt,
+                        warmup + steps)
+        eng.step(warmup)
+        eng.sync()
+        ms = eng.time_steps(steps)
+        n = eng.dims.voxel_count()
+    return interior ** 3 * steps / (ms / 1e3) / 1e6, ms / steps, n
