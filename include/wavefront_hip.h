@@ -180,6 +180,12 @@ int wf_set_probes(wf_ctx *ctx, uint32_t n, const wf_probe_site *sites, uint64_t 
 int wf_step(wf_ctx *ctx, uint64_t nsteps);
 /* Waits for all work queued on the context. */
 int wf_sync(wf_ctx *ctx);
+/* Selects the step kernels: on = 0 (default) the exact set, compiled
+ * without FP contraction and bit-identical to the CPU engine (the reference
+ * never contracts, src/fdtd.rs); on != 0 the same kernels compiled with
+ * contraction (a*b+c as fused multiply-adds), within rounding of the exact
+ * results.  WF_FP_CONTRACT=1 at wf_create selects the contracted set. */
+int wf_set_fp_contract(wf_ctx *ctx, int on);
 /* Steps advanced since creation. */
 uint64_t wf_step_count(const wf_ctx *ctx);
 

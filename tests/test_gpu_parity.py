@@ -279,3 +279,27 @@ def test_pml_reflection_study_on_gpu():
     assert got == gold
     assert all(b < a for a, b in zip(got, got[1:]))
     assert got[-1] < 1e-2
+
+
+@pytest.mark.parametrize("case", ["plain_64", "random_fill_128"])
+def test_contracted_kernels_within_tolerance(case):
+    """The contracted kernel set (a*b+c as fused multiply-adds): relative L2
+    field error against the CPU engine <= 1e-6 after the whole run, and not
+    bit-identical (the contracted kernels did run)."""
+    w = workloads.plain_64() if case == "plain_64" else workloads.random_1024(steps=40, n=128)
+    steps = w.num_steps if case == "plain_64" else 40
+    _say(f"contracted {case} steps={steps}")
+    eng = DeviceEngine(w.dims, w.ids, w.table, w.pml())
+    eng.set_fp_contract(True)
+    eng.set_sources(w.sources, w.dt, steps)
+    eng.set_probes(w.probes, w.dt, steps)
+    eng.step(steps)
+    got = eng.read_fields().astype(np.float64)
+    eng.close()
+    sim = oracle_sim_for(w)
+    sim.run(steps, w.dt, threads=4)
+    ref = sim.read_fields().astype(np.float64)
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"contracted {case}: relative L2 error {rel:.3e}")
+    assert rel <= 1e-6
+    assert not np.array_equal(got, ref)

@@ -262,6 +262,13 @@ struct wf_ctx {
     float *d_samples[6] = {};   // [rows][2][npos][kSampleStride]
     MlpPacked *d_net[6] = {};
     bool ghost_reads = false;   // some predicted face feeds the step kernels
+
+    // step kernel set: exact (contraction off, bit-identical to the CPU
+    // engine) or contracted (fused multiply-adds)
+    bool contract = false;
+    hipError_t (*lh)(const StepLaunch &, const BoxList &, hipStream_t) = launch_h;
+    hipError_t (*le)(const StepLaunch &, const BoxList &, hipStream_t) = launch_e;
+    hipError_t (*lf)(const StepLaunch &, const Box &, hipStream_t) = launch_fused;
 };
 
 namespace {
@@ -428,21 +435,21 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
         if (overlap) {
             WF_HIP(hipEventRecord(c->fork, st));
             WF_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
-            WF_HIP(launch_h(s, c->frame, c->side));
+            WF_HIP(c->lh(s, c->frame, c->side));
             WF_HIP(hipEventRecord(c->join, c->side));
-            WF_HIP(launch_fused(s, c->interior, st));
+            WF_HIP(c->lf(s, c->interior, st));
             WF_HIP(hipStreamWaitEvent(st, c->join, 0));
         } else {
             if (ev) WF_HIP(hipEventRecord(ev[0], st));
-            WF_HIP(launch_fused(s, c->interior, st));
+            WF_HIP(c->lf(s, c->interior, st));
             if (ev) WF_HIP(hipEventRecord(ev[1], st));
             if (ev) WF_HIP(hipEventRecord(ev[2], st));
-            WF_HIP(launch_h(s, c->frame, st));
+            WF_HIP(c->lh(s, c->frame, st));
             if (ev) WF_HIP(hipEventRecord(ev[3], st));
         }
         if (int rc = run_hooks(c, false, c->p, c->alt, st)) return rc;
         if (ev) WF_HIP(hipEventRecord(ev[4], st));
-        WF_HIP(launch_e(s, c->frame, st));
+        WF_HIP(c->le(s, c->frame, st));
         if (ev) WF_HIP(hipEventRecord(ev[5], st));
         std::swap(c->p, c->alt);
     } else {
@@ -454,11 +461,11 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
             WF_HIP(hipEventRecord(ev[1], st));
             WF_HIP(hipEventRecord(ev[2], st));
         }
-        WF_HIP(launch_h(s, c->whole, st));
+        WF_HIP(c->lh(s, c->whole, st));
         if (ev) WF_HIP(hipEventRecord(ev[3], st));
         if (int rc = run_hooks(c, false, c->p, c->p, st)) return rc;
         if (ev) WF_HIP(hipEventRecord(ev[4], st));
-        WF_HIP(launch_e(s, c->whole, st));
+        WF_HIP(c->le(s, c->whole, st));
         if (ev) WF_HIP(hipEventRecord(ev[5], st));
     }
     if (c->nsrc) {
@@ -752,6 +759,13 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     }
     if (const char *ov = std::getenv("WF_OVERLAP")) c->overlap = std::atoi(ov) != 0;
     if (const char *gr = std::getenv("WF_GRAPH")) c->use_graph = std::atoi(gr) != 0;
+    if (const char *fc = std::getenv("WF_FP_CONTRACT"))
+        if (std::atoi(fc) != 0) {
+            c->contract = true;
+            c->lh = fma::launch_h;
+            c->le = fma::launch_e;
+            c->lf = fma::launch_fused;
+        }
     plan_boxes(c);
     *out = c;
     return WF_OK;
@@ -1250,6 +1264,20 @@ void update_ghost_reads(wf_ctx *c)
 }  // namespace
 
 extern "C" {
+
+int wf_set_fp_contract(wf_ctx *c, int on)
+{
+    clear_error();
+    if (!c) return fail(WF_ERR_INVALID, "wf_set_fp_contract: null context");
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    drop_graph_exec(c);
+    c->contract = on != 0;
+    c->lh = on ? fma::launch_h : launch_h;
+    c->le = on ? fma::launch_e : launch_e;
+    c->lf = on ? fma::launch_fused : launch_fused;
+    return WF_OK;
+}
 
 int wf_clear_faces(wf_ctx *c)
 {

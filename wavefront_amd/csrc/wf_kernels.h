@@ -192,4 +192,15 @@ hipError_t launch_batch_h(const BatchScene *scenes, const uint32_t *wg_scene, in
 hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, int total,
                           int zchunk, hipStream_t st);
 
+// The same step kernels built with FP contraction (fused multiply-adds).
+namespace fma {
+hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st);
+hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
+hipError_t launch_fused(const StepLaunch &s, const Box &interior, hipStream_t st);
+hipError_t launch_batch_h(const BatchScene *scenes, const uint32_t *wg_scene, int total,
+                          int zchunk, hipStream_t st);
+hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, int total,
+                          int zchunk, hipStream_t st);
+}  // namespace fma
+
 }  // namespace wf

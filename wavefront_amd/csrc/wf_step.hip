@@ -14,11 +14,18 @@
 // Arithmetic: built with -ffp-contract=off; every f32 operation is a single
 // rounded IEEE operation in the order of src/fdtd.rs, so the results are
 // bit-identical to the CPU engine (vector operations act per element).
+// The same file is built a second time with WF_STEP_FMA and
+// -ffp-contract=fast: the contracted kernel set (namespace wf::fma), whose
+// a*b+c become fused multiply-adds (results within rounding of the exact
+// ones, no longer bit-identical).
 #include <hip/hip_runtime.h>
 
 #include "wf_kernels.h"
 
 namespace wf {
+#ifdef WF_STEP_FMA
+namespace fma {
+#endif
 namespace {
 
 __device__ __forceinline__ f4 ld4(const float *p) { return *reinterpret_cast<const f4 *>(p); }
@@ -635,8 +642,10 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
 
 }  // namespace
 
+#ifndef WF_STEP_FMA
 int fused_rows_per_tile() { return kFY - 1; }
 int split_rows_per_wg() { return kSY; }
+#endif
 
 hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st)
 {
@@ -694,4 +703,7 @@ hipError_t launch_fused(const StepLaunch &s, const Box &interior, hipStream_t st
     return hipGetLastError();
 }
 
+#ifdef WF_STEP_FMA
+}  // namespace fma
+#endif
 }  // namespace wf

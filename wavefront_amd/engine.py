@@ -129,6 +129,10 @@ class DeviceEngine:
     def sync(self) -> None:
         check(lib().wf_sync(self._h))
 
+    def set_fp_contract(self, on: bool) -> None:
+        """Contracted (fused multiply-add) step kernels instead of the exact ones."""
+        check(lib().wf_set_fp_contract(self._h, 1 if on else 0))
+
     def run(self, num_steps: int, snapshot_every: int, output_path: str) -> float:
         """engine::run's loop with snapshot egress (wf_run): num_steps steps,
         a snapshot appended to output_path after every step whose loop index
