@@ -143,13 +143,14 @@ def run_batch_96(args):
     engs = engines()
     with DeviceBatch(engs) as b:
         t0 = time.perf_counter()
-        z, lpr, taken = b.tune(4)
+        z, lpr, grid, taken = b.tune(4)
         ms_rest = b.time_steps(steps - taken)
         b.sync()
         wall = time.perf_counter() - t0
     for e in engs:
         e.close()
-    log(f"batched: tuned zchunk={z} lanes/row={lpr} on {taken} steps; {wall:.3f}s total, "
+    log(f"batched: tuned zchunk={z} lanes/row={lpr} grid={grid or 'per tile'} on {taken} steps; "
+        f"{wall:.3f}s total, "
         f"{ms_rest / (steps - taken):.3f} ms/step after tuning")
     # the same scenes one at a time on their own engines (graph-replayed steps)
     engs = engines()
@@ -178,6 +179,7 @@ def run_batch_96(args):
             "cells": cells,
             "tuned_zchunk": z,
             "tuned_lanes_per_row": lpr,
+            "tuned_grid": grid,
             "tuning_steps": taken,
             "batched_ms_per_step_after_tuning": round(ms_rest / (steps - taken), 4),
             "solo_sequential_seconds": round(solo_ms / 1e3, 3),

@@ -230,12 +230,14 @@ int wf_batch_sync(wf_batch *batch);
 /* nsteps steps between two events on the batch stream; *total_ms = their time. */
 int wf_batch_time_steps(wf_batch *batch, uint64_t nsteps, double *total_ms);
 /* Autotuner: runs steps_per_candidate real steps (the first untimed) under
- * each candidate tiling (z-chunk, lanes-per-row cap), keeps the fastest and
- * reports it; *steps_taken = steps advanced (results do not depend on the
- * tiling).  WF_BATCH_ZCHUNK sets the initial z-chunk (default 8). */
+ * each candidate tiling and schedule (z-chunk, lanes-per-row cap, grid: 0 =
+ * one workgroup per tile, n = n persistent workgroups striding over the
+ * tiles), keeps the fastest and reports it; *steps_taken = steps advanced
+ * (results do not depend on the tiling).  WF_BATCH_ZCHUNK sets the initial
+ * z-chunk (default 8). */
 int wf_batch_tune(wf_batch *batch, uint32_t steps_per_candidate, int *zchunk,
-                  int *lanes_per_row, uint64_t *steps_taken);
-int wf_batch_set_tiling(wf_batch *batch, int zchunk, int lanes_per_row);
+                  int *lanes_per_row, int *grid, uint64_t *steps_taken);
+int wf_batch_set_tiling(wf_batch *batch, int zchunk, int lanes_per_row, int grid);
 /* Sum of the scenes' cells. */
 uint64_t wf_batch_cells(const wf_batch *batch);
 

@@ -93,7 +93,7 @@ def test_batch_argument_errors():
     assert lib.wf_batch_create(2, arr, byref(h)) == _lib.WF_ERR_INVALID
     assert "null" in _lib.last_error()
     assert lib.wf_batch_step(None, 1) == _lib.WF_ERR_INVALID
-    assert lib.wf_batch_set_tiling(None, 8, 64) == _lib.WF_ERR_INVALID
+    assert lib.wf_batch_set_tiling(None, 8, 64, 0) == _lib.WF_ERR_INVALID
     assert lib.wf_batch_cells(None) == 0
 
 

@@ -20,7 +20,7 @@ def _engine(w, steps):
 
 def test_batch_matches_oracle_with_tuning():
     """The first six scenes of the 96-scene set plus two hand-picked shapes,
-    36 steps: 6 tuner candidates x 3 real steps each, then 18 more."""
+    36 steps: 9 tuner candidates x 3 real steps each, then 9 more."""
     ws = workloads.batch_scenes(nscenes=6, steps=36)
     ws += workloads.batch_scenes(nscenes=2, steps=36, seed=11,
                                  edges=[(32, 48, 40), (200, 32, 64)], max_edge=200)
@@ -28,8 +28,8 @@ def test_batch_matches_oracle_with_tuning():
           + ", ".join(w.name for w in ws), flush=True)
     engs = [_engine(w, 36) for w in ws]
     with DeviceBatch(engs) as b:
-        z, lpr, taken = b.tune(3)
-        assert taken == 18 and z >= 1 and lpr in (16, 64)
+        z, lpr, grid, taken = b.tune(3)
+        assert taken == 27 and z >= 1 and lpr in (16, 64) and grid in (0, 2048, 8192)
         b.step(36 - taken)
         b.sync()
     for w, eng in zip(ws, engs):
@@ -54,9 +54,11 @@ def test_batch_96_matches_solo_engines():
     engs = [_engine(w, 12) for w in ws]
     with DeviceBatch(engs) as b:
         b.set_tiling(8, 64)
-        b.step(5)
+        b.step(4)
         b.set_tiling(16, 16)
-        b.step(7)
+        b.step(4)
+        b.set_tiling(8, 64, 1024)  # persistent: 1024 workgroups stride over the tiles
+        b.step(4)
         b.sync()
     for w, eng in zip(ws, engs):
         solo = _engine(w, 12)

@@ -1502,6 +1502,7 @@ struct wf_batch {
     int total = 0;
     int zchunk = 8;
     int lpr_cap = 64;
+    int grid = 0;  // 0: one workgroup per tile; else persistent workgroups
     int tiled_zchunk = -1, tiled_lpr = -1;  // tiling the wg table was built for
     std::vector<Box> boxes;
 };
@@ -1574,8 +1575,8 @@ int batch_launch_steps(wf_batch *b, uint64_t nsteps)
 {
     const unsigned n = (unsigned)b->scenes.size();
     for (uint64_t s = 0; s < nsteps; ++s) {
-        WF_HIP(launch_batch_h(b->d_scenes, b->d_wg, b->total, b->zchunk, b->stream));
-        WF_HIP(launch_batch_e(b->d_scenes, b->d_wg, b->total, b->zchunk, b->stream));
+        WF_HIP(launch_batch_h(b->d_scenes, b->d_wg, b->total, b->zchunk, b->grid, b->stream));
+        WF_HIP(launch_batch_e(b->d_scenes, b->d_wg, b->total, b->zchunk, b->grid, b->stream));
         hipLaunchKernelGGL(k_sources_batch, dim3(n), dim3(64), 0, b->stream, b->d_scenes);
         hipLaunchKernelGGL(k_probes_batch, dim3(n), dim3(64), 0, b->stream, b->d_scenes);
         WF_HIP(hipGetLastError());
@@ -1699,27 +1700,32 @@ int wf_batch_time_steps(wf_batch *b, uint64_t nsteps, double *total_ms)
 }
 
 int wf_batch_tune(wf_batch *b, uint32_t steps_per_candidate, int *zchunk, int *lanes_per_row,
-                  uint64_t *steps_taken)
+                  int *grid, uint64_t *steps_taken)
 {
     clear_error();
     if (!b) return fail(WF_ERR_INVALID, "wf_batch_tune: null batch");
     if (steps_per_candidate < 2)
         return fail(WF_ERR_INVALID, "wf_batch_tune: need at least 2 steps per candidate");
-    // (z-chunk, lanes-per-row cap): short z-marches fill the chip with small
-    // scenes; a 16-lane cap avoids idle lanes on rows of 33-48 quads
-    static const int cand[][2] = {{4, 64}, {8, 64}, {16, 64}, {32, 64}, {8, 16}, {16, 16}};
+    // (z-chunk, lanes-per-row cap, grid): short z-marches fill the chip with
+    // small scenes; a 16-lane cap avoids idle lanes on rows of 33-48 quads;
+    // a persistent grid (8 or 32 workgroups per CU) trades launch width for
+    // fewer, longer-lived workgroups
+    static const int cand[][3] = {{4, 64, 0},    {8, 64, 0},    {16, 64, 0},
+                                  {32, 64, 0},   {8, 16, 0},    {16, 16, 0},
+                                  {8, 64, 2048}, {16, 64, 2048}, {8, 64, 8192}};
     const int ncand = (int)(sizeof cand / sizeof cand[0]);
     if (int rc = batch_prepare(b, (uint64_t)ncand * steps_per_candidate)) return rc;
     hipEvent_t e0, e1;
     WF_HIP(hipEventCreate(&e0));
     WF_HIP(hipEventCreate(&e1));
     double best = 0.0;
-    int bz = b->zchunk, bl = b->lpr_cap, rc = WF_OK;
+    int bz = b->zchunk, bl = b->lpr_cap, bg = b->grid, rc = WF_OK;
     uint64_t taken = 0;
     for (int k = 0; k < ncand && !rc; ++k) {
         // every trial step is a real step: results do not depend on tiling
         b->zchunk = cand[k][0];
         b->lpr_cap = cand[k][1];
+        b->grid = cand[k][2];
         rc = batch_upload(b);
         if (!rc) rc = batch_launch_steps(b, 1);
         if (!rc && hipEventRecord(e0, b->stream) != hipSuccess) rc = fail(WF_ERR_HIP, "event");
@@ -1734,26 +1740,31 @@ int wf_batch_tune(wf_batch *b, uint32_t steps_per_candidate, int *zchunk, int *l
             best = ms;
             bz = cand[k][0];
             bl = cand[k][1];
+            bg = cand[k][2];
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->zchunk = bz;
     b->lpr_cap = bl;
+    b->grid = bg;
     if (zchunk) *zchunk = bz;
     if (lanes_per_row) *lanes_per_row = bl;
+    if (grid) *grid = bg;
     if (steps_taken) *steps_taken = taken;
     return rc;
 }
 
-int wf_batch_set_tiling(wf_batch *b, int zchunk, int lanes_per_row)
+int wf_batch_set_tiling(wf_batch *b, int zchunk, int lanes_per_row, int grid)
 {
     if (!b) return fail(WF_ERR_INVALID, "wf_batch_set_tiling: null batch");
     if (zchunk < 1 || lanes_per_row < 1 || lanes_per_row > 64 ||
-        (lanes_per_row & (lanes_per_row - 1)))
-        return fail(WF_ERR_INVALID, "wf_batch_set_tiling: zchunk >= 1, lanes per row a power of two <= 64");
+        (lanes_per_row & (lanes_per_row - 1)) || grid < 0)
+        return fail(WF_ERR_INVALID,
+                    "wf_batch_set_tiling: zchunk >= 1, lanes per row a power of two <= 64, grid >= 0");
     b->zchunk = zchunk;
     b->lpr_cap = lanes_per_row;
+    b->grid = grid;
     return WF_OK;
 }
 

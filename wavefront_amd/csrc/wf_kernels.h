@@ -185,12 +185,13 @@ struct BatchScene {
     int nsrc, nprobe, nfreq, pad;
 };
 
-// H / E kernels over every scene of a batch in one launch: workgroup t
-// (after XCD renumbering) belongs to scene wg_scene[t].
+// H / E kernels over every scene of a batch in one launch: tile t (after
+// XCD renumbering) belongs to scene wg_scene[t].  grid = 0: one workgroup
+// per tile; otherwise `grid` persistent workgroups stride over the tiles.
 hipError_t launch_batch_h(const BatchScene *scenes, const uint32_t *wg_scene, int total,
-                          int zchunk, hipStream_t st);
+                          int zchunk, int grid, hipStream_t st);
 hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, int total,
-                          int zchunk, hipStream_t st);
+                          int zchunk, int grid, hipStream_t st);
 
 // The same step kernels built with FP contraction (fused multiply-adds).
 namespace fma {
@@ -198,9 +199,9 @@ hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 hipError_t launch_fused(const StepLaunch &s, const Box &interior, hipStream_t st);
 hipError_t launch_batch_h(const BatchScene *scenes, const uint32_t *wg_scene, int total,
-                          int zchunk, hipStream_t st);
+                          int zchunk, int grid, hipStream_t st);
 hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, int total,
-                          int zchunk, hipStream_t st);
+                          int zchunk, int grid, hipStream_t st);
 }  // namespace fma
 
 }  // namespace wf

@@ -238,20 +238,24 @@ __global__ __launch_bounds__(64 * kSY) void k_h4(Geo g, Ptrs src, Ptrs dst, BoxL
     h4_tile<PML, GHOST>(g, src, dst, b, local, zchunk, s_co);
 }
 
-// Batched H kernel: every scene in place, one launch.
+// Batched H kernel: every scene in place, one launch.  A grid smaller than
+// the tile count makes it persistent: workgroup i takes tiles i, i + grid, ...
 template <bool PML>
 __global__ __launch_bounds__(64 * kSY) void k_h4_batch(const BatchScene *__restrict__ sc,
                                                        const uint32_t *__restrict__ wg_scene,
                                                        int total, int zchunk)
 {
     __shared__ float2 s_co[256];  // (da, db)
-    const int t = xcd_tile(blockIdx.x, total);
-    const BatchScene *d = sc + wg_scene[t];
-    const Geo g = d->g;
-    const Ptrs p = d->p;
-    const Box b = d->box;
-    if (t - b.first >= b.gx * b.gy * b.gz) return;  // tables and grid always agree
-    h4_tile<PML>(g, p, p, b, t - b.first, zchunk, s_co);
+    for (int i = blockIdx.x; i < total; i += gridDim.x) {
+        const int t = xcd_tile(i, total);
+        const BatchScene *d = sc + wg_scene[t];
+        const Geo g = d->g;
+        const Ptrs p = d->p;
+        const Box b = d->box;
+        if (t - b.first >= b.gx * b.gy * b.gz) return;  // tables and grid always agree
+        h4_tile<PML>(g, p, p, b, t - b.first, zchunk, s_co);
+        __syncthreads();  // the next tile reloads s_co
+    }
 }
 
 // One workgroup of the E kernel.  hsrc: where H(n+1/2) lives; esrc: E^n and
@@ -410,13 +414,16 @@ __global__ __launch_bounds__(64 * kSY) void k_e4_batch(const BatchScene *__restr
                                                        int total, int zchunk)
 {
     __shared__ float2 s_co[256];  // (ca, cb)
-    const int t = xcd_tile(blockIdx.x, total);
-    const BatchScene *d = sc + wg_scene[t];
-    const Geo g = d->g;
-    const Ptrs p = d->p;
-    const Box b = d->box;
-    if (t - b.first >= b.gx * b.gy * b.gz) return;  // tables and grid always agree
-    e4_tile<PML>(g, p, p, p, b, t - b.first, zchunk, s_co);
+    for (int i = blockIdx.x; i < total; i += gridDim.x) {
+        const int t = xcd_tile(i, total);
+        const BatchScene *d = sc + wg_scene[t];
+        const Geo g = d->g;
+        const Ptrs p = d->p;
+        const Box b = d->box;
+        if (t - b.first >= b.gx * b.gy * b.gz) return;  // tables and grid always agree
+        e4_tile<PML>(g, p, p, p, b, t - b.first, zchunk, s_co);
+        __syncthreads();  // the next tile reloads s_co
+    }
 }
 
 // ===========================================================================
@@ -677,19 +684,21 @@ hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st)
 }
 
 hipError_t launch_batch_h(const BatchScene *scenes, const uint32_t *wg_scene, int total,
-                          int zchunk, hipStream_t st)
+                          int zchunk, int grid, hipStream_t st)
 {
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_h4_batch<true>, dim3((unsigned)total), dim3(64, kSY), 0, st, scenes,
+    const int n = grid > 0 && grid < total ? grid : total;
+    hipLaunchKernelGGL(k_h4_batch<true>, dim3((unsigned)n), dim3(64, kSY), 0, st, scenes,
                        wg_scene, total, zchunk);
     return hipGetLastError();
 }
 
 hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, int total,
-                          int zchunk, hipStream_t st)
+                          int zchunk, int grid, hipStream_t st)
 {
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_e4_batch<true>, dim3((unsigned)total), dim3(64, kSY), 0, st, scenes,
+    const int n = grid > 0 && grid < total ? grid : total;
+    hipLaunchKernelGGL(k_e4_batch<true>, dim3((unsigned)n), dim3(64, kSY), 0, st, scenes,
                        wg_scene, total, zchunk);
     return hipGetLastError();
 }

@@ -253,14 +253,15 @@ class DeviceBatch:
         return ms.value
 
     def tune(self, steps_per_candidate: int = 4):
-        """Autotunes the tiling on real steps; returns (zchunk, lanes_per_row,
-        steps_taken)."""
-        z, l, n = c_int(), c_int(), c_uint64()
-        check(lib().wf_batch_tune(self._h, steps_per_candidate, byref(z), byref(l), byref(n)))
-        return z.value, l.value, n.value
+        """Autotunes tiling and schedule on real steps; returns (zchunk,
+        lanes_per_row, grid, steps_taken), grid 0 = one workgroup per tile."""
+        z, l, g, n = c_int(), c_int(), c_int(), c_uint64()
+        check(lib().wf_batch_tune(self._h, steps_per_candidate, byref(z), byref(l), byref(g),
+                                  byref(n)))
+        return z.value, l.value, g.value, n.value
 
-    def set_tiling(self, zchunk: int, lanes_per_row: int) -> None:
-        check(lib().wf_batch_set_tiling(self._h, zchunk, lanes_per_row))
+    def set_tiling(self, zchunk: int, lanes_per_row: int, grid: int = 0) -> None:
+        check(lib().wf_batch_set_tiling(self._h, zchunk, lanes_per_row, grid))
 
     def cells(self) -> int:
         return int(lib().wf_batch_cells(self._h))
