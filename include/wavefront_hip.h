@@ -268,6 +268,13 @@ int wf_mlp_trainer_create(int device, uint64_t n, const float *inputs, const flo
 void wf_mlp_trainer_destroy(wf_mlp_trainer *t);
 /* Sets the FP32 master parameters (and resets the Adam moments). */
 int wf_mlp_set_params(wf_mlp_trainer *t, const float *params);
+/* Output head: WF_HEAD_DIRECT (default) -- the network output is the scaled
+ * ghost value; WF_HEAD_ONE_WAY -- the output y sets the coefficient of a
+ * first-order one-way update, ghost/s = P0(n-1) + tanh(y) (P0(n) - g(n-1)),
+ * from sign-normalized inputs (a contractive recursion on the ghost). */
+#define WF_HEAD_DIRECT 0
+#define WF_HEAD_ONE_WAY 1
+int wf_mlp_set_head(wf_mlp_trainer *t, int head);
 int wf_mlp_get_params(wf_mlp_trainer *t, float *params);
 /* Gradient of the loss at the current parameters (the network as it runs:
  * BF16 weights and activations) and the loss itself. */
@@ -281,11 +288,12 @@ int wf_mlp_train(wf_mlp_trainer *t, uint32_t iters, float lr, float lr_skip, int
 int wf_mlp_predict(wf_mlp_trainer *t, uint64_t n, const float *inputs, float *outputs);
 
 /* Predicted faces: the faces in face_mask get ghost planes predicted by the
- * networks params[f * wf_mlp_param_count() ...] each half step (high faces
- * before the H phase, low faces before the E phase); other faces keep their
+ * networks params[f * wf_mlp_param_count() ...] with output head `head` each
+ * half step (high faces before the H phase, low faces before the E phase);
+ * other faces keep their
  * boundary.  A face's axis must not be periodic.  Clears the faces' history.
  * face_mask 0 removes nothing; wf_clear_faces removes every hook. */
-int wf_set_learned_faces(wf_ctx *ctx, uint32_t face_mask, const float *params);
+int wf_set_learned_faces(wf_ctx *ctx, uint32_t face_mask, const float *params, int head);
 int wf_clear_faces(wf_ctx *ctx);
 
 /* Recording faces (training data from a thick-CPML run): a virtual face of

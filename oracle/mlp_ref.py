@@ -13,7 +13,8 @@ check them:
   grad_f64        the exact gradient of the mean squared error of forward_f64
                   with respect to the FP32 master parameters (straight
                   through the bf16 rounding of the weights), checked against
-                  central finite differences in the tests.
+                  central finite differences in the tests;
+  *_one_way_f64   the same for the one-way output head.
 
 Parameter layout: W1[16][64] b1[64] W2[64][64] b2[64] W3[64] b3 wlin[16]
 (W[in][out]).
@@ -88,6 +89,38 @@ def forward_bf16(p, u):
 def loss_f64(p, u, t):
     y, _ = forward_f64(p, u)
     return float(np.mean((y - np.asarray(t, np.float64)) ** 2))
+
+
+def one_way_sign(u):
+    """Sign normalizing the one-way head's input: sign of u0, or of u4 when
+    u0 = 0."""
+    u = np.asarray(u)
+    return np.where((u[:, 0] > 0) | ((u[:, 0] == 0) & (u[:, 4] >= 0)), 1.0, -1.0)
+
+
+def predict_one_way_f64(p, u):
+    """One-way head: u14 + tanh(y(sigma u)) (u0 - u12)."""
+    u = np.asarray(u, np.float64)
+    y, _ = forward_f64(p, u * one_way_sign(u)[:, None])
+    return u[:, 14] + np.tanh(y) * (u[:, 0] - u[:, 12])
+
+
+def loss_one_way_f64(p, u, t):
+    return float(np.mean((predict_one_way_f64(p, u) - np.asarray(t, np.float64)) ** 2))
+
+
+def grad_one_way_f64(p, u, t):
+    """d loss / d p for the one-way head: the direct-head gradient of the
+    network on sigma*u with the per-sample error scaled by
+    (1 - tanh^2 y) (u0 - u12)."""
+    u = np.asarray(u, np.float64)
+    ui = u * one_way_sign(u)[:, None]
+    y, _ = forward_f64(p, ui)
+    th = np.tanh(y)
+    pred = u[:, 14] + th * (u[:, 0] - u[:, 12])
+    dy = (pred - np.asarray(t, np.float64)) * (1.0 - th * th) * (u[:, 0] - u[:, 12])
+    # grad_f64 differentiates mean((y - t')^2); choose t' = y - dy so its error is dy
+    return grad_f64(p, ui, y - dy)
 
 
 def grad_f64(p, u, t):

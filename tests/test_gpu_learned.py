@@ -279,3 +279,34 @@ def test_learned_layer_reflection():
     assert r_learned < r_zero / 1000
     assert r_learned < 1e-4  # the thin layer's target: at least as good as 32-cell CPML's 5.4e-5 class
     assert abs(r_fp64 - r_learned) < 0.1 * r_learned + 2 * floor
+
+
+def test_one_way_head_gradient_and_mur_baseline():
+    """The one-way head: GPU gradient against the FP64 one (same tolerances as
+    the direct head), and the untrained head -- the normal-incidence
+    coefficient (c dt - dx)/(c dt + dx), i.e. first-order Mur -- measured with
+    the two-run method: 4.0e-4, between the bare thin layer and the trained
+    direct head."""
+    _say("one-way head")
+    rng = np.random.default_rng(8)
+    p = _params(rng).astype(np.float32)
+    u = rng.normal(0, 1, (3000, 16)).astype(np.float32)
+    t = rng.normal(0, 1, 3000).astype(np.float32)
+    tr = L.Trainer(u, t, head=L.HEAD_ONE_WAY)
+    tr.set_params(p)
+    g, loss = tr.gradient()
+    y = tr.predict(u)
+    tr.close()
+    assert np.max(np.abs(y - mlp_ref.predict_one_way_f64(p, u))) < 3e-2 * np.max(np.abs(y))
+    g64 = mlp_ref.grad_one_way_f64(p, u, t)
+    assert abs(loss - mlp_ref.loss_one_way_f64(p, u, t)) < 2e-2 * loss
+    rel = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+    print(f"one-way gradient rel L2 {rel:.3e}")
+    assert rel < 3e-2
+    dx, dt, _ = L.study_grid()
+    P = L.param_count()
+    p0 = np.zeros((6, P), np.float32)
+    p0[:, P - 17] = np.arctanh(L.one_way_coefficient(dt, dx))
+    r, _, _ = L.reflection("learned", L.FaceNets(p0, dt, dx, L.HEAD_ONE_WAY))
+    print(f"first-order one-way (Mur) boundary: R {r:.3e}")
+    assert 1e-4 < r < 1e-3

@@ -1291,10 +1291,12 @@ int wf_clear_faces(wf_ctx *c)
     return WF_OK;
 }
 
-int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params)
+int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params, int head)
 {
     clear_error();
     if (!c) return fail(WF_ERR_INVALID, "wf_set_learned_faces: null context");
+    if (head != kHeadDirect && head != kHeadOneWay)
+        return fail(WF_ERR_INVALID, "wf_set_learned_faces: unknown head %d", head);
     if (face_mask >= 64u) return fail(WF_ERR_INVALID, "wf_set_learned_faces: face mask has bits above 5");
     if (face_mask && !params) return fail(WF_ERR_INVALID, "wf_set_learned_faces: null parameters");
     for (int f = 0; f < 6; ++f) {
@@ -1324,7 +1326,7 @@ int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params)
             hipMemset(c->d_mem[f], 0, 2 * npos * 4 * sizeof(float)) != hipSuccess ||
             hipMemcpy(d_par, params + (size_t)f * kMlpParams, kMlpParams * sizeof(float),
                       hipMemcpyHostToDevice) != hipSuccess ||
-            pack_mlp(d_par, c->d_net[f], c->stream) != hipSuccess ||
+            pack_mlp(d_par, head, c->d_net[f], c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess) {
             rc = fail(WF_ERR_HIP, "wf_set_learned_faces: upload failed");
             break;

@@ -51,8 +51,18 @@ struct MlpPacked {
     float w3r[kMlpHid];  // W3 rounded to bf16 (the values the forward uses)
     float wlin[kMlpIn];
     float b3;
-    float pad[3];
+    int head;  // output head (kHeadDirect / kHeadOneWay)
+    float pad[2];
 };
+
+// Output heads.  Direct: the network output is the ghost value / s.
+// One-way: the output y sets the coefficient of a first-order one-way
+// boundary update, ghost / s = P0(n-1) + tanh(y) (P0(n) - g(n-1)) (in scaled
+// inputs u14 + tanh(y) (u0 - u12)); |tanh y| < 1 keeps the recursion on the
+// ghost contractive.  The network then sees sign-normalized inputs (sign of
+// u0, or of u4 when u0 = 0), so y is even and the ghost odd in the fields.
+constexpr int kHeadDirect = 0;
+constexpr int kHeadOneWay = 1;
 
 // One face hook: a face of the domain whose ghost plane is predicted
 // (mode 0), or a virtual face inside a thick-CPML run whose stencils and true
@@ -75,6 +85,6 @@ struct FaceHook {
 hipError_t launch_face_hook(const Geo &g, const Ptrs &esrc, const Ptrs &hsrc, const FaceHook &h,
                             const uint64_t *step, hipStream_t st);
 // FP32 parameter vector (device) -> packed network (device).
-hipError_t pack_mlp(const float *params, MlpPacked *net, hipStream_t st);
+hipError_t pack_mlp(const float *params, int head, MlpPacked *net, hipStream_t st);
 
 }  // namespace wf

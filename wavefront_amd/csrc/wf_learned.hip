@@ -61,6 +61,7 @@ struct MlpRegs {
     fv4 b1[4], b2[4];
     fv4 wl;
     float b3;
+    int head;
 };
 
 __device__ __forceinline__ void load_regs(const MlpPacked *__restrict__ w, int lane, MlpRegs &r)
@@ -79,6 +80,22 @@ __device__ __forceinline__ void load_regs(const MlpPacked *__restrict__ w, int l
     }
     r.wl = fv4{w->wlin[rb], w->wlin[rb + 1], w->wlin[rb + 2], w->wlin[rb + 3]};
     r.b3 = w->b3;
+    r.head = w->head;
+}
+
+// Sign that makes the one-way head's network input even: the sign of u0, or
+// of u4 when u0 = 0 (same value on the four lanes of a sample).
+__device__ __forceinline__ float input_sign(fv4 u, int sl)
+{
+    const float u0 = __shfl(u.x, sl), u4 = __shfl(u.x, sl + 16);
+    return (u0 > 0.f || (u0 == 0.f && u4 >= 0.f)) ? 1.0f : -1.0f;
+}
+
+// Prediction in scaled units from the network output y (lanes 0..15):
+// u0, u12, u14 are the sample's scaled P0(n), g(n-1), P0(n-1).
+__device__ __forceinline__ float head_out(int head, float y, float u0, float u12, float u14)
+{
+    return head == kHeadDirect ? y : u14 + tanh_f(y) * (u0 - u12);
 }
 
 struct Acts {
@@ -118,7 +135,7 @@ __device__ __forceinline__ float mlp_forward(const MlpRegs &r, bfv4 ub, fv4 uf, 
 }
 
 // FP32 parameters -> per-lane MFMA operands.  One wave.
-__global__ void k_pack_mlp(const float *__restrict__ p, MlpPacked *__restrict__ w)
+__global__ void k_pack_mlp(const float *__restrict__ p, int head, MlpPacked *__restrict__ w)
 {
     const int l = threadIdx.x, c = l & 15, rb = 4 * (l >> 4);
     auto bf = [](float v) { return (short)bf16_rne(v); };
@@ -157,7 +174,10 @@ __global__ void k_pack_mlp(const float *__restrict__ p, MlpPacked *__restrict__ 
     w->b2[l] = p[kOffB2 + l];
     w->w3r[l] = bf16_f(bf16_rne(p[kOffW3 + l]));
     if (l < kMlpIn) w->wlin[l] = p[kOffWL + l];
-    if (l == 0) w->b3 = p[kOffB3];
+    if (l == 0) {
+        w->b3 = p[kOffB3];
+        w->head = head;
+    }
 }
 
 // Prediction for a matrix of already-scaled inputs (tests and evaluation).
@@ -176,9 +196,11 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ u
         const int s = grp * 16 + sl;
         fv4 uf = {0.f, 0.f, 0.f, 0.f};
         if (s < n) uf = *reinterpret_cast<const fv4 *>(u + (size_t)s * kMlpIn + rb);
+        const fv4 ui = r.head == kHeadDirect ? uf : uf * input_sign(uf, sl);
         Acts t;
-        const float y = mlp_forward(r, to_bf4(uf), uf, t);
-        if (lane < 16 && s < n) out[s] = y;
+        const float y = mlp_forward(r, to_bf4(ui), ui, t);
+        const float p = head_out(r.head, y, uf.x, __shfl(uf.x, sl + 48), __shfl(uf.z, sl + 48));
+        if (lane < 16 && s < n) out[s] = p;
     }
 }
 
@@ -258,15 +280,22 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float *__restrict__ u,
             uf = *reinterpret_cast<const fv4 *>(u + (size_t)s * kMlpIn + rb);
             t = tgt[s];
         }
-        const bfv4 ub = to_bf4(uf);
+        const fv4 ui = r.head == kHeadDirect ? uf : uf * input_sign(uf, sl);
+        const bfv4 ub = to_bf4(ui);
         Acts a;
-        const float y = mlp_forward(r, ub, uf, a);
-        const float res = (lane < 16 && live) ? y - t : 0.f;
+        const float y = mlp_forward(r, ub, ui, a);
+        const float u12 = __shfl(uf.x, sl + 48), u14 = __shfl(uf.z, sl + 48);
+        const float pred = head_out(r.head, y, uf.x, u12, u14);
+        const float res = (lane < 16 && live) ? pred - t : 0.f;
         loss = loss + res * res;
-        const float g = 2.0f * res * inv_n;  // dL/dy on lanes 0..15
-        const float gs = __shfl(g, sl);      // broadcast to the sample's four lanes
+        float g = 2.0f * res * inv_n;  // dL/dpred on lanes 0..15
+        if (r.head != kHeadDirect) {
+            const float th = tanh_f(y);
+            g = g * (1.0f - th * th) * (uf.x - u12);  // dL/dy
+        }
+        const float gs = __shfl(g, sl);  // broadcast to the sample's four lanes
         if (lane < 16) gb3 = gb3 + g;
-        gwl = gwl + uf * gs;
+        gwl = gwl + ui * gs;
         // layer 3 (rank one): dW3 = a2 g, da2 = W3 g
         fv4 dz2[4];
         bfv4 dz2b[4];
@@ -544,8 +573,11 @@ __global__ __launch_bounds__(256) void k_face_hook(Geo g, Ptrs es, Ptrs hs, Face
         float gnew;
         if (h.mode == 0) {
             Acts t;
-            const float y = mlp_forward(r, to_bf4(u), u, t);
-            gnew = s > 0.f ? s * y : 0.f;
+            const fv4 ui = r.head == kHeadDirect ? u : u * input_sign(u, sl);
+            const float y = mlp_forward(r, to_bf4(ui), ui, t);
+            const float pred =
+                head_out(r.head, y, u.x, __shfl(u.x, sl + 48), __shfl(u.z, sl + 48));
+            gnew = s > 0.f ? s * pred : 0.f;
             if (lane < 16 && live) (pol == 0 ? h.ghost[0] : h.ghost[1])[pos] = gnew;
         } else {
             gnew = live ? Pf[face_cell(g, a, h.b0 - dir, f, sc)] : 0.f;
@@ -565,9 +597,9 @@ __global__ __launch_bounds__(256) void k_face_hook(Geo g, Ptrs es, Ptrs hs, Face
 
 }  // namespace
 
-hipError_t pack_mlp(const float *params, MlpPacked *net, hipStream_t st)
+hipError_t pack_mlp(const float *params, int head, MlpPacked *net, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, st, params, net);
+    hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, st, params, head, net);
     return hipGetLastError();
 }
 
@@ -606,6 +638,7 @@ struct wf_mlp_trainer {
     float *d_p = nullptr, *d_m = nullptr, *d_v = nullptr, *d_grad = nullptr, *d_part = nullptr;
     MlpPacked *d_w = nullptr;
     int blocks = 0;
+    int head = kHeadDirect;
     uint64_t adam_t = 0;
 };
 
@@ -685,7 +718,7 @@ int wf_mlp_trainer_create(int device, uint64_t n, const float *inputs, const flo
                 hipMemset(t->d_v, 0, kMlpParams * sizeof(float)) != hipSuccess))
         rc = fail(WF_ERR_HIP, "wf_mlp_trainer_create: upload failed");
     if (!rc) {
-        hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, t->stream, t->d_p, t->d_w);
+        hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, t->stream, t->d_p, t->head, t->d_w);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(t->stream) != hipSuccess)
             rc = fail(WF_ERR_HIP, "wf_mlp_trainer_create: pack failed");
     }
@@ -706,7 +739,20 @@ int wf_mlp_set_params(wf_mlp_trainer *t, const float *params)
     WF_HIP(hipMemset(t->d_m, 0, kMlpParams * sizeof(float)));
     WF_HIP(hipMemset(t->d_v, 0, kMlpParams * sizeof(float)));
     t->adam_t = 0;
-    hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, t->stream, t->d_p, t->d_w);
+    hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, t->stream, t->d_p, t->head, t->d_w);
+    WF_HIP(hipGetLastError());
+    WF_HIP(hipStreamSynchronize(t->stream));
+    return WF_OK;
+}
+
+int wf_mlp_set_head(wf_mlp_trainer *t, int head)
+{
+    clear_error();
+    if (!t) return fail(WF_ERR_INVALID, "wf_mlp_set_head: null trainer");
+    if (head != kHeadDirect && head != kHeadOneWay) return fail(WF_ERR_INVALID, "wf_mlp_set_head: unknown head %d", head);
+    WF_HIP(hipSetDevice(t->device));
+    t->head = head;
+    hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, t->stream, t->d_p, t->head, t->d_w);
     WF_HIP(hipGetLastError());
     WF_HIP(hipStreamSynchronize(t->stream));
     return WF_OK;
@@ -750,7 +796,7 @@ int wf_mlp_train(wf_mlp_trainer *t, uint32_t iters, float lr, float lr_skip, int
         const float bc2 = 1.0f - std::pow(b2, (float)t->adam_t);
         hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, t->stream, t->d_p,
                            t->d_m, t->d_v, t->d_grad, lr, lr_skip, odd, b1, b2, eps, bc1, bc2);
-        hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, t->stream, t->d_p, t->d_w);
+        hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, t->stream, t->d_p, t->head, t->d_w);
         WF_HIP(hipGetLastError());
     }
     if (loss) {

@@ -181,20 +181,27 @@ def _gauss(freq):
 # ---------------------------------------------------------------------------
 # Network and training
 # ---------------------------------------------------------------------------
+HEAD_DIRECT, HEAD_ONE_WAY = 0, 1
+
+
 @dataclass
 class FaceNets:
-    """Parameters of the six face networks, [6, param_count()] float32."""
+    """Parameters of the six face networks, [6, param_count()] float32, and
+    their output head (HEAD_DIRECT: the scaled ghost value; HEAD_ONE_WAY: the
+    coefficient of a first-order one-way update)."""
     params: np.ndarray
     dt: float
     dx: float
+    head: int = HEAD_DIRECT
 
     def save(self, path: str) -> None:
-        np.savez(path, params=self.params, dt=self.dt, dx=self.dx)
+        np.savez(path, params=self.params, dt=self.dt, dx=self.dx, head=self.head)
 
     @classmethod
     def load(cls, path: str) -> "FaceNets":
         z = np.load(path)
-        return cls(z["params"].astype(f32), float(z["dt"]), float(z["dx"]))
+        return cls(z["params"].astype(f32), float(z["dt"]), float(z["dx"]),
+                   int(z["head"]) if "head" in z else HEAD_DIRECT)
 
 
 def init_params(rng: np.random.Generator, wlin=None, features=None) -> np.ndarray:
@@ -232,13 +239,15 @@ def least_squares_skip(u: np.ndarray, t: np.ndarray, features=None) -> np.ndarra
 class Trainer:
     """A GPU training session over one face's samples."""
 
-    def __init__(self, u: np.ndarray, t: np.ndarray, device: int = 0):
+    def __init__(self, u: np.ndarray, t: np.ndarray, device: int = 0, head: int = HEAD_DIRECT):
         self.u = np.ascontiguousarray(u, f32)
         self.t = np.ascontiguousarray(t, f32)
         h = c_void_p()
         check(lib().wf_mlp_trainer_create(device, len(self.t), ptr(self.u, c_float),
                                           ptr(self.t, c_float), byref(h)))
         self._h = h
+        if head != HEAD_DIRECT:
+            check(lib().wf_mlp_set_head(h, head))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -288,6 +297,42 @@ class Trainer:
 FEATURES = (0, 1, 2, 3, 4, 5, 6, 7, 10, 11, 12, 13, 14, 15)
 
 
+def one_way_coefficient(dt: float, dx: float) -> float:
+    """First-order one-way (Mur) coefficient for normal incidence,
+    (c dt - dx) / (c dt + dx)."""
+    s = 299792458.0 * dt / dx
+    return (s - 1.0) / (s + 1.0)
+
+
+def train_face_one_way(u: np.ndarray, t: np.ndarray, dt: float, dx: float, seed: int = 0,
+                       schedule=((2000, 1e-3), (2000, 3e-4), (1000, 1e-4)), features=None,
+                       device: int = 0, log=None):
+    """The one-way head: the network starts as the normal-incidence
+    coefficient (b3 = atanh k, everything else zero but W1, W2) and learns
+    how it should vary with the local stencil; Adam on all parameters."""
+    rng = np.random.default_rng(seed)
+    u = np.array(u, f32)
+    if features is not None:
+        off = np.ones(16, bool)
+        off[list(features)] = False
+        u[:, off] = 0.0
+    p = init_params(rng, None, features)
+    p[param_count() - 17] = math.atanh(one_way_coefficient(dt, dx))  # b3
+    tr = Trainer(u, t, device, head=HEAD_ONE_WAY)
+    tr.set_params(p)
+    hist = [(0, tr.gradient()[1])]
+    done = 0
+    for iters, lr in schedule:
+        loss = tr.train(iters, lr, lr)
+        done += iters
+        hist.append((done, loss))
+        if log:
+            log(f"  {done} iters (lr {lr:g}): loss {loss:.3e}")
+    p = tr.params()
+    tr.close()
+    return p, hist
+
+
 def train_face(u: np.ndarray, t: np.ndarray, seed: int = 0,
                schedule=((2000, 1e-4), (1000, 3e-5)), lr_skip: float = 1e-5,
                features=FEATURES, device: int = 0, log=None):
@@ -334,7 +379,7 @@ def thin_engine(dims: GridDims, nets: FaceNets | None, dt: float, dx: float, per
     mask = face_mask(periodic)
     if mode == "learned":
         p = np.ascontiguousarray(nets.params, f32).reshape(-1)
-        check(lib().wf_set_learned_faces(eng.handle, mask, ptr(p, c_float)))
+        check(lib().wf_set_learned_faces(eng.handle, mask, ptr(p, c_float), nets.head))
     elif mode == "replay":
         keep = []
         arr = (POINTER(c_float) * 6)()
@@ -506,14 +551,19 @@ def collect_3d(thin: GridDims, runs: int, steps: int, every: int, per_face: int,
     return out
 
 
-def train_all(samples: dict, dt: float, dx: float, log=None, device: int = 0) -> FaceNets:
+def train_all(samples: dict, dt: float, dx: float, log=None, device: int = 0,
+              head: int = HEAD_DIRECT, features=FEATURES) -> FaceNets:
     p = np.zeros((6, param_count()), f32)
     for i, f in enumerate(FACES):
         u, t = samples[f]
         if log:
             log(f"{f}: {len(t)} samples")
-        p[i], _ = train_face(u, t, seed=i, device=device, log=log)
-    return FaceNets(p, dt, dx)
+        if head == HEAD_DIRECT:
+            p[i], _ = train_face(u, t, seed=i, features=features, device=device, log=log)
+        else:
+            p[i], _ = train_face_one_way(u, t, dt, dx, seed=i, features=features, device=device,
+                                         log=log)
+    return FaceNets(p, dt, dx, head)
 
 
 # ---------------------------------------------------------------------------
