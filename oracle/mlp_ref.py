@@ -69,14 +69,16 @@ def forward_f64(p, u):
 
 def forward_bf16(p, u):
     """The GPU's rounding points: bf16 inputs and activations into each
-    matrix product, f32 accumulation (products of bf16 values are exact in
-    f32), f32 tanh, f32 linear skip."""
+    matrix product, f32 accumulation starting from the bias (products of
+    bf16 values are exact in f32), f32 tanh, f32 linear skip."""
     w = _gpu_weights(p)
     u32 = np.asarray(u, np.float32)
     ub = bf16(u32).astype(np.float64)
-    z1 = (ub @ w["W1"]).astype(np.float32) + w["b1"].astype(np.float32)
+    b1 = w["b1"].astype(np.float32).astype(np.float64)
+    b2 = w["b2"].astype(np.float32).astype(np.float64)
+    z1 = (ub @ w["W1"] + b1).astype(np.float32)
     a1 = np.tanh(z1.astype(np.float64)).astype(np.float32)
-    z2 = (bf16(a1).astype(np.float64) @ w["W2"]).astype(np.float32) + w["b2"].astype(np.float32)
+    z2 = (bf16(a1).astype(np.float64) @ w["W2"] + b2).astype(np.float32)
     a2 = np.tanh(z2.astype(np.float64)).astype(np.float32)
     z3 = (bf16(a2).astype(np.float64) @ w["W3"]).astype(np.float32)
     wl = w["wlin"].astype(np.float32)

@@ -101,9 +101,9 @@ def _point_sources(c):
 
 
 def test_recorded_inputs_match_fields():
-    """The hook's inputs (fields at depths 0..3, transverse Laplacian,
-    history) and true ghost values at the last step of a thick run, against
-    the same quantities taken from the fields read back."""
+    """The hook's inputs (fields at depths 0..3, CPML psi at depths 0, 1, history) and
+    true ghost values at the last step of a thick run, against the same
+    quantities taken from the fields and psi read back."""
     _say("face hook record vs fields")
     thin = GridDims(24, 16, 32)
     dx, dt, _ = L.study_grid()
@@ -119,12 +119,13 @@ def test_recorded_inputs_match_fields():
         t1, t2 = (a + 1) % 3, (a + 2) % 3
         tf, ts = min(t1, t2), max(t1, t2)
         regions[f] = L.FaceRegion(nb[a] - 1 - L.OFF if hi else L.OFF, L.OFF, nt[tf], L.OFF, nt[ts])
-    with DeviceEngine(big, None, MaterialTable.vacuum_filled(dt, dx),
-                      L.thick_context(big, dt, dx)) as eng:
+    ctx = L.thick_context(big, dt, dx)
+    with DeviceEngine(big, None, MaterialTable.vacuum_filled(dt, dx), ctx) as eng:
         eng.set_sources(srcs, dt, K)
         L.check(L.lib().wf_record_face_samples(eng.handle, 63, regions, 1, K))
         eng.step(K - 1)
         F1 = aosoa_to_xyz(eng.read_fields(), big)  # (6, z, y, x)
+        psi1 = eng.read_psi()  # (shell blocks, 12, 512), PmlAux order
         eng.step(1)
         F2 = aosoa_to_xyz(eng.read_fields(), big)
         recs = {}
@@ -141,8 +142,30 @@ def test_recorded_inputs_match_fields():
         c[a], c[tf], c[ts] = n, fc, sc
         return F[comp, c[2], c[1], c[0]]
 
+    # shell blocks in the order the psi read-back lists them (z, y, x loops)
+    nbk = (big.nx // 8, big.ny // 8, big.nz // 8)
+    sb = ctx.shell_blocks
+    order = {}
+    for bz in range(nbk[2]):
+        for by in range(nbk[1]):
+            for bx in range(nbk[0]):
+                bc = (bx, by, bz)
+                if any(sb[q] > 0 and (bc[q] < sb[q] or bc[q] >= nbk[q] - sb[q]) for q in range(3)):
+                    order[bc] = len(order)
+    # (normal axis, term) -> PmlAux component: terms 0, 1 of the H phase, 2, 3
+    # of the E phase, for the two tangential derivatives of each axis
+    aux = {(0, 0): 0, (0, 1): 1, (1, 0): 2, (1, 1): 3, (2, 0): 4, (2, 1): 5,
+           (0, 2): 6, (0, 3): 7, (1, 3): 8, (1, 2): 9, (2, 2): 10, (2, 3): 11}
+
+    def psi_at(a, term, n, fc, sc):
+        c = [0, 0, 0]
+        t1, t2 = (a + 1) % 3, (a + 2) % 3
+        c[a], c[min(t1, t2)], c[max(t1, t2)] = n, fc, sc
+        blk = order[(c[0] // 8, c[1] // 8, c[2] // 8)]
+        return psi1[blk, aux[(a, term)], ((c[2] % 8) * 8 + c[1] % 8) * 8 + c[0] % 8]
+
     rng = np.random.default_rng(7)
-    checked = 0
+    checked = psi_seen = 0
     for f in range(6):
         a, hi = f // 2, f % 2
         t1, t2 = (a + 1) % 3, (a + 2) % 3
@@ -154,9 +177,11 @@ def test_recorded_inputs_match_fields():
             if hi:
                 P, Q = (t1 if pol == 0 else t2), 3 + (t2 if pol == 0 else t1)
                 FP, FQ = F1, F1
+                term = pol if a == 1 else 1 - pol
             else:
                 P, Q = 3 + (t1 if pol == 0 else t2), (t2 if pol == 0 else t1)
                 FP, FQ = F2, F1
+                term = 2 + pol if a == 1 else 3 - pol
             for pos in rng.choice(r.nfast * r.nslow, 40, replace=False):
                 fi, si = pos % r.nfast, pos // r.nfast
                 fc, sc = r.fast0 + fi, r.slow0 + si
@@ -164,26 +189,21 @@ def test_recorded_inputs_match_fields():
                 for j in range(4):
                     x[j] = at(FP, P, a, r.b0 + d * j, fc, sc)
                     x[4 + j] = at(FQ, Q, a, r.b0 + d * j, fc, sc)
-                f0, f1, s0, s1 = r.fast0, r.fast0 + r.nfast, r.slow0, r.slow0 + r.nslow
-                for j in range(2):
-                    n = r.b0 + d * j
-                    # neighbours outside the face region read as zero
-                    v = lambda ff, ss: at(FP, P, a, n, ff, ss) if f0 <= ff < f1 and s0 <= ss < s1 else 0.0
-                    x[8 + j] = np.float32(np.float32(v(fc + 1, sc) + v(fc - 1, sc))
-                                          + np.float32(v(fc, sc + 1) + v(fc, sc - 1))) \
-                        - np.float32(4.0) * np.float32(at(FP, P, a, n, fc, sc))
+                x[10], x[11] = (psi_at(a, term, r.b0 + d * j, fc, sc) for j in range(2))
                 row = rec[K - 1, pol, pos]
                 prev1, prev2 = rec[K - 2, pol, pos], rec[K - 3, pol, pos]
                 x[12], x[13] = prev1[17], prev2[17]
                 x[14], x[15] = prev1[0] * prev1[18], prev2[0] * prev2[18]
                 s = row[18]
                 got = row[:16].astype(np.float64) * s
-                for i in list(range(10)) + [12, 13, 14, 15]:
+                psi_seen += bool(np.any(x[10:12] != 0))
+                for i in range(16):
                     assert abs(got[i] - x[i]) <= 1e-5 * max(abs(x[i]), 1e-3 * s) + 1e-30, (f, pol, pos, i)
                 ghost = at(FP, P, a, r.b0 - d, fc, sc)
                 assert row[17] == np.float32(ghost)
                 checked += 1
     assert checked == 6 * 2 * 40
+    assert psi_seen > checked // 2  # the virtual faces lie in the graded layer
 
 
 @pytest.mark.parametrize("kernel", ["fused", "split"])

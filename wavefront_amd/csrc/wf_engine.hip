@@ -218,6 +218,7 @@ struct wf_ctx {
     hipStream_t side = nullptr;            // frame H kernel, overlapped
     hipEvent_t fork = nullptr, join = nullptr;
     bool overlap = false;                  // WF_OVERLAP=1
+    bool hook_overlap = true;              // WF_HOOK_OVERLAP=0 serializes the face hooks
 
     float *d_fields = nullptr;
     float *d_fields2 = nullptr;
@@ -365,8 +366,11 @@ void plan_boxes(wf_ctx *c)
             lo[a] = 0;
             hi[a] = n[a];
         } else if (a == 0) {
-            // halo x-1 outside the layer; rows start on a 128-byte line
-            lo[a] = (g.T[a] + 1 + 31) / 32 * 32;
+            // halo x-1 outside the layer; rows start on a quad, and on a
+            // 128-byte line when that moves at most one more quad into the
+            // (slower, narrow-row) frame kernels
+            const int q = (g.T[a] + 1 + 3) / 4 * 4, l = (g.T[a] + 1 + 31) / 32 * 32;
+            lo[a] = l - q <= 4 ? l : q;
             hi[a] = n[a] - g.T[a];
         } else {
             lo[a] = g.T[a] + 1;
@@ -418,15 +422,52 @@ void plan_boxes(wf_ctx *c)
 // and the E-phase psi in `es`).
 int run_hooks(wf_ctx *c, bool high, const Ptrs &es, const Ptrs &hs, hipStream_t st)
 {
+    const FaceHook *on[3];
+    int n = 0;
     for (int f = high ? 1 : 0; f < 6; f += 2)
-        if (c->hook_on[f]) WF_HIP(launch_face_hook(c->g, es, hs, c->hook[f], c->d_step, st));
+        if (c->hook_on[f]) on[n++] = &c->hook[f];
+    if (n) WF_HIP(launch_face_hooks(c->g, es, hs, on, n, c->d_step, st));
     return WF_OK;
+}
+
+// The hooks may run beside the interior kernel when every one of them reads
+// and writes only frame cells and its own buffers: predicted or replayed
+// domain faces whose normal axis has a shell of at least 4 cells (a hook
+// reads depths 0..3).  Recording hooks may sit on interior cells.
+bool hooks_beside_interior(const wf_ctx *c)
+{
+    bool any = false;
+    for (int f = 0; f < 6; ++f) {
+        if (!c->hook_on[f]) continue;
+        if (c->hook[f].mode == 1 || c->g.T[f / 2] < 4) return false;
+        any = true;
+    }
+    return any;
 }
 
 int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
 {
     hipStream_t st = c->stream;
-    if (c->fused) {
+    if (c->fused && c->hook_overlap && !ev && c->side && c->frame.total > 0 &&
+        c->interior.gx > 0 && hooks_beside_interior(c)) {
+        // Face hooks and the frame's H phase on the side stream, beside the
+        // interior kernel: the hooks touch only frame cells (the high faces
+        // read state A, the low faces the frame's new H in state B), the
+        // interior kernel reads state A and writes interior cells of B.
+        // The frame's E phase waits for both.
+        StepLaunch s{c->g, c->p, c->alt, c->pml, c->zchunk, st};
+        s.ghost = c->ghost_reads;
+        WF_HIP(hipEventRecord(c->fork, st));
+        WF_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
+        if (int rc = run_hooks(c, true, c->p, c->p, c->side)) return rc;
+        WF_HIP(c->lh(s, c->frame, c->side));
+        if (int rc = run_hooks(c, false, c->p, c->alt, c->side)) return rc;
+        WF_HIP(hipEventRecord(c->join, c->side));
+        WF_HIP(c->lf(s, c->interior, st));
+        WF_HIP(hipStreamWaitEvent(st, c->join, 0));
+        WF_HIP(c->le(s, c->frame, st));
+        std::swap(c->p, c->alt);
+    } else if (c->fused) {
         StepLaunch s{c->g, c->p, c->alt, c->pml, c->zchunk, st};
         s.ghost = c->ghost_reads;
         if (int rc = run_hooks(c, true, c->p, c->p, st)) return rc;
@@ -758,6 +799,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
         if (v > 0) c->zchunk = v;
     }
     if (const char *ov = std::getenv("WF_OVERLAP")) c->overlap = std::atoi(ov) != 0;
+    if (const char *ho = std::getenv("WF_HOOK_OVERLAP")) c->hook_overlap = std::atoi(ho) != 0;
     if (const char *gr = std::getenv("WF_GRAPH")) c->use_graph = std::atoi(gr) != 0;
     if (const char *fc = std::getenv("WF_FP_CONTRACT"))
         if (std::atoi(fc) != 0) {
@@ -1305,6 +1347,9 @@ int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params, int
         if (c->g.periodic >> fg.a & 1u)
             return fail(WF_ERR_INVALID, "face %d lies on a periodic axis", f);
         if (fg.na < 8) return fail(WF_ERR_INVALID, "face %d: axis shorter than 8 cells", f);
+        if (c->g.T[fg.a] < 4)
+            return fail(WF_ERR_INVALID, "face %d: its axis needs a CPML shell of at least 4 cells "
+                        "(the layer in front of the learned boundary)", f);
     }
     WF_HIP(hipSetDevice(c->device));
     WF_HIP(hipStreamSynchronize(c->stream));
@@ -1427,6 +1472,8 @@ int wf_replay_faces(wf_ctx *c, uint32_t face_mask, uint64_t rows, const float *c
         if (!(face_mask >> f & 1u)) continue;
         if (!samples[f]) return fail(WF_ERR_INVALID, "wf_replay_faces: face %d has no samples", f);
         if (c->g.periodic >> (f / 2) & 1u) return fail(WF_ERR_INVALID, "face %d lies on a periodic axis", f);
+        if (c->g.T[f / 2] < 4)
+            return fail(WF_ERR_INVALID, "face %d: its axis needs a CPML shell of at least 4 cells", f);
     }
     WF_HIP(hipSetDevice(c->device));
     WF_HIP(hipStreamSynchronize(c->stream));

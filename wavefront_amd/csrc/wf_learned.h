@@ -82,8 +82,12 @@ struct FaceHook {
     const MlpPacked *net;
 };
 
-hipError_t launch_face_hook(const Geo &g, const Ptrs &esrc, const Ptrs &hsrc, const FaceHook &h,
-                            const uint64_t *step, hipStream_t st);
+// n (1..3) faces in one launch; the hooks must not write what another
+// reads (true for the faces of one side: each writes its own ghost planes
+// and history, and reads only the fields).
+hipError_t launch_face_hooks(const Geo &g, const Ptrs &esrc, const Ptrs &hsrc,
+                             const FaceHook *const *hooks, int n, const uint64_t *step,
+                             hipStream_t st);
 // FP32 parameter vector (device) -> packed network (device).
 hipError_t pack_mlp(const float *params, int head, MlpPacked *net, hipStream_t st);
 

@@ -10,6 +10,7 @@
 // the B layout the next layer consumes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -30,13 +31,21 @@ __device__ __forceinline__ unsigned short bf16_rne(float f)
 
 __device__ __forceinline__ float bf16_f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// Four floats to bf16, round to nearest even: two v_cvt_pk_bf16_f32 (the
+// same bits as bf16_rne for finite inputs).
 __device__ __forceinline__ bfv4 to_bf4(fv4 v)
 {
+    const bf16x2 lo = __builtin_convertvector((f32x2{v.x, v.y}), bf16x2);
+    const bf16x2 hi = __builtin_convertvector((f32x2{v.z, v.w}), bf16x2);
+    const uint32_t a = __builtin_bit_cast(uint32_t, lo), b = __builtin_bit_cast(uint32_t, hi);
     bfv4 r;
-    r.x = (short)bf16_rne(v.x);
-    r.y = (short)bf16_rne(v.y);
-    r.z = (short)bf16_rne(v.z);
-    r.w = (short)bf16_rne(v.w);
+    r.x = (short)(a & 0xffffu);
+    r.y = (short)(a >> 16);
+    r.z = (short)(b & 0xffffu);
+    r.w = (short)(b >> 16);
     return r;
 }
 
@@ -45,11 +54,13 @@ __device__ __forceinline__ fv4 mfma(bfv4 a, bfv4 b, fv4 c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// tanh(x) = sign(x) (1 - 2 / (exp(2|x|) + 1))
+// tanh(x) = sign(x) (1 - 2 / (exp(2|x|) + 1)), with the hardware exp2 and
+// reciprocal (v_exp_f32, v_rcp_f32: about 1 ulp each).  Large |x|: exp2
+// gives +inf, the reciprocal 0, tanh +-1.
 __device__ __forceinline__ float tanh_f(float x)
 {
-    const float e = __expf(2.0f * fabsf(x));
-    const float t = 1.0f - __fdividef(2.0f, e + 1.0f);
+    const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * fabsf(x));  // 2 log2(e) |x|
+    const float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
     return copysignf(t, x);
 }
 
@@ -109,18 +120,18 @@ struct Acts {
 __device__ __forceinline__ float mlp_forward(const MlpRegs &r, bfv4 ub, fv4 uf, Acts &t)
 {
     const fv4 zero = {0.f, 0.f, 0.f, 0.f};
+    // biases enter as the accumulator of the first product of each layer
 #pragma unroll
     for (int o = 0; o < 4; ++o) {
-        const fv4 z = mfma(r.A1[o], ub, zero) + r.b1[o];
+        const fv4 z = mfma(r.A1[o], ub, r.b1[o]);
         t.a1[o] = tanh4(z);
         t.a1b[o] = to_bf4(t.a1[o]);
     }
 #pragma unroll
     for (int o = 0; o < 4; ++o) {
-        fv4 z = zero;
+        fv4 z = r.b2[o];
 #pragma unroll
         for (int k = 0; k < 4; ++k) z = mfma(r.A2[o][k], t.a1b[k], z);
-        z = z + r.b2[o];
         t.a2[o] = tanh4(z);
         t.a2b[o] = to_bf4(t.a2[o]);
     }
@@ -470,14 +481,14 @@ __device__ __forceinline__ const float *psi_ptr(const Ptrs &p, int a, int t)
 // ghost value the boundary cell's update reads, Q = its partner in the
 // curl, d = depth from the boundary cell inward:
 //   0..3   P(d)                     4..7  Q(d)
-//   8, 9   transverse Laplacian of P at d = 0, 1 (within the face region)
-//   10, 11 CPML psi of P's normal derivative at d = 0, 1
+//   8, 9   zero (unused)
+//   10, 11 CPML psi of P's normal derivative at d = 0, 1 (0 outside the grading)
 //   12..15 g(n-1), g(n-2), P(0)(n-1), P(0)(n-2)   (history)
 // High faces (before the H phase): P = E_t1 / E_t2, Q = H_t2 / H_t1, ghost
 // E at b0+1.  Low faces (before the E phase): P = H_t1 / H_t2, Q = E_t2 /
 // E_t1, ghost H at b0-1.  t1 = (a+1)%3, t2 = (a+2)%3.
-__global__ __launch_bounds__(256) void k_face_hook(Geo g, Ptrs es, Ptrs hs, FaceHook h,
-                                                   const uint64_t *__restrict__ step)
+__device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Ptrs &hs,
+                                          const FaceHook &h, const uint64_t *__restrict__ step)
 {
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -488,8 +499,6 @@ __global__ __launch_bounds__(256) void k_face_hook(Geo g, Ptrs es, Ptrs hs, Face
     const int q4 = lane >> 4, sl = lane & 15;
     const int dir = h.hi ? -1 : 1;
     const int na = h.na, Ta = h.Ta;
-    const bool pf = (g.periodic >> (t1 < t2 ? t1 : t2)) & 1u;  // fast axis wraps
-    const bool ps = (g.periodic >> (t1 < t2 ? t2 : t1)) & 1u;  // slow axis wraps
 
     MlpRegs r;
     if (h.mode == 0) load_regs(h.net, lane, r);
@@ -533,30 +542,18 @@ __global__ __launch_bounds__(256) void k_face_hook(Geo g, Ptrs es, Ptrs hs, Face
 
         fv4 x = {0.f, 0.f, 0.f, 0.f};
         if (live) {
-            if (q4 == 0 || q4 == 1) {
-                const float *F = q4 == 0 ? Pf : Qf;
+            if (q4 < 3) {
+                // P, Q at depths 0..3, or (inputs 8..11) zero, zero and the
+                // psi of dP/dn at depths 0, 1 (psi exists only in graded
+                // cells; a face of the domain has its first four graded)
+                const float *F = q4 == 0 ? Pf : (q4 == 1 ? Qf : Ps);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = F[face_cell(g, a, h.b0 + dir * j, f, sc)];
-            } else if (q4 == 2) {
-                // transverse neighbours within the face region: zero beyond
-                // it (the thin domain ends there), wrapped on periodic axes
-                // (whose region is the whole extent)
-                const int f1 = h.fast0 + h.nfast - 1, s1 = h.slow0 + h.nslow - 1;
-                const int fpi = f < f1 ? f + 1 : (pf ? h.fast0 : -1);
-                const int fmi = f > h.fast0 ? f - 1 : (pf ? f1 : -1);
-                const int spi = sc < s1 ? sc + 1 : (ps ? h.slow0 : -1);
-                const int smi = sc > h.slow0 ? sc - 1 : (ps ? s1 : -1);
-#pragma unroll
-                for (int d = 0; d < 2; ++d) {
-                    const int n = h.b0 + dir * d;
-                    const float p0 = Pf[face_cell(g, a, n, f, sc)];
-                    const float fp = fpi >= 0 ? Pf[face_cell(g, a, n, fpi, sc)] : 0.f;
-                    const float fm = fmi >= 0 ? Pf[face_cell(g, a, n, fmi, sc)] : 0.f;
-                    const float sp = spi >= 0 ? Pf[face_cell(g, a, n, f, spi)] : 0.f;
-                    const float sm = smi >= 0 ? Pf[face_cell(g, a, n, f, smi)] : 0.f;
-                    x[d] = ((fp + fm) + (sp + sm)) - 4.0f * p0;
-                    const bool gr = Ta > 0 && (n < Ta || n >= na - Ta);
-                    x[2 + d] = gr ? Ps[face_psi(g, a, Ta, na, n, f, sc)] : 0.f;
+                for (int j = 0; j < 4; ++j) {
+                    const int n = h.b0 + dir * (q4 < 2 ? j : j - 2);
+                    const bool gr = j >= 2 && Ta > 0 && (n < Ta || n >= na - Ta);
+                    const size_t i = q4 < 2 ? face_cell(g, a, n, f, sc)
+                                            : (gr ? face_psi(g, a, Ta, na, n, f, sc) : 0);
+                    x[j] = (q4 < 2 || gr) ? F[i] : 0.f;
                 }
             } else {
                 x = *reinterpret_cast<const fv4 *>(mem);
@@ -567,7 +564,12 @@ __global__ __launch_bounds__(256) void k_face_hook(Geo g, Ptrs es, Ptrs hs, Face
         float s = fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w)));
         s = fmaxf(s, __shfl_xor(s, 16));
         s = fmaxf(s, __shfl_xor(s, 32));
-        const fv4 u = s > 0.f ? x / s : fv4{0.f, 0.f, 0.f, 0.f};
+        // at least 2^-100 (a normal number whose reciprocal is finite and
+        // exact): the first values of a wave front reaching the face can be
+        // denormal, and their reciprocal would overflow
+        s = fmaxf(s, 0x1p-100f);
+        const float rs = 1.0f / s;
+        const fv4 u = x * rs;
         // history on lanes 0..15: their own P(0) and the memory lanes' values
         const float g1 = __shfl(x.x, sl + 48), p1 = __shfl(x.z, sl + 48);
         float gnew;
@@ -577,7 +579,7 @@ __global__ __launch_bounds__(256) void k_face_hook(Geo g, Ptrs es, Ptrs hs, Face
             const float y = mlp_forward(r, to_bf4(ui), ui, t);
             const float pred =
                 head_out(r.head, y, u.x, __shfl(u.x, sl + 48), __shfl(u.z, sl + 48));
-            gnew = s > 0.f ? s * pred : 0.f;
+            gnew = s * pred;
             if (lane < 16 && live) (pol == 0 ? h.ghost[0] : h.ghost[1])[pos] = gnew;
         } else {
             gnew = live ? Pf[face_cell(g, a, h.b0 - dir, f, sc)] : 0.f;
@@ -585,7 +587,7 @@ __global__ __launch_bounds__(256) void k_face_hook(Geo g, Ptrs es, Ptrs hs, Face
                 float *o = h.samples + (((size_t)row * 2 + pol) * npos + pos) * kSampleStride;
                 *reinterpret_cast<fv4 *>(o + 4 * q4) = u;  // 80-byte records: 16-byte aligned
                 if (lane < 16) {
-                    o[16] = s > 0.f ? gnew / s : 0.f;
+                    o[16] = gnew * rs;
                     o[17] = gnew;  // raw ghost value (replay)
                     o[18] = s;
                 }
@@ -593,6 +595,16 @@ __global__ __launch_bounds__(256) void k_face_hook(Geo g, Ptrs es, Ptrs hs, Face
         }
         if (lane < 16 && live) *reinterpret_cast<fv4 *>(mem) = fv4{gnew, g1, x.x, p1};
     }
+}
+
+// Up to three faces of one side in one launch: blockIdx.y picks the face.
+__global__ __launch_bounds__(256) void k_face_hooks(Geo g, Ptrs es, Ptrs hs, FaceHook h0,
+                                                    FaceHook h1, FaceHook h2,
+                                                    const uint64_t *__restrict__ step)
+{
+    const int k = blockIdx.y;
+    const FaceHook h = k == 0 ? h0 : (k == 1 ? h1 : h2);
+    face_hook(g, es, hs, h, step);
 }
 
 }  // namespace
@@ -603,15 +615,21 @@ hipError_t pack_mlp(const float *params, int head, MlpPacked *net, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_face_hook(const Geo &g, const Ptrs &esrc, const Ptrs &hsrc, const FaceHook &h,
-                            const uint64_t *step, hipStream_t st)
+hipError_t launch_face_hooks(const Geo &g, const Ptrs &esrc, const Ptrs &hsrc,
+                             const FaceHook *const *hooks, int n, const uint64_t *step,
+                             hipStream_t st)
 {
-    const int npos = h.nfast * h.nslow;
-    const int groups = 2 * ((npos + 15) / 16);
-    int blocks = (groups + 3) / 4;
-    if (blocks > 1024) blocks = 1024;
-    if (blocks < 1) return hipSuccess;
-    hipLaunchKernelGGL(k_face_hook, dim3(blocks), dim3(256), 0, st, g, esrc, hsrc, h, step);
+    if (n < 1) return hipSuccess;
+    if (n > 3) return hipErrorInvalidValue;
+    int blocks = 1;
+    for (int k = 0; k < n; ++k) {
+        const int npos = hooks[k]->nfast * hooks[k]->nslow;
+        const int groups = 2 * ((npos + 15) / 16);
+        blocks = std::max(blocks, std::min((groups + 3) / 4, 1024));
+    }
+    const FaceHook &h0 = *hooks[0], &h1 = *hooks[n > 1 ? 1 : 0], &h2 = *hooks[n > 2 ? 2 : 0];
+    hipLaunchKernelGGL(k_face_hooks, dim3(blocks, n), dim3(256), 0, st, g, esrc, hsrc, h0, h1,
+                       h2, step);
     return hipGetLastError();
 }
 

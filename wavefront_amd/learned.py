@@ -290,10 +290,11 @@ class Trainer:
         return out
 
 
-# Inputs used by default: everything but the transverse Laplacian (8, 9).
-# Fitted freely, its weights let checkerboard modes across the face grow in
-# closed-loop 3D runs (the data never contains them); without it the
-# learned faces are stable in 3D.
+# Inputs used by default: P and Q at depths 0..3, the normal-derivative psi
+# at depths 0 and 1, and the history (inputs 8, 9 are always zero).  Purely
+# per-column: no input couples neighbouring face positions.  (With the psi of
+# depths 2 and 3 added, the least-squares skip fits the 1D data as well but
+# is unstable in closed loop.)
 FEATURES = (0, 1, 2, 3, 4, 5, 6, 7, 10, 11, 12, 13, 14, 15)
 
 
