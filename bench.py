@@ -248,7 +248,7 @@ def run_learned(args):
         "data": "synthetic",
         "config": {
             "workload": "learned thin boundary: 4-cell layer cut from a 64-cell CPML, per-face "
-                        "MLP 16-64-64-1 (tanh, odd) + linear skip, trained on 32 plane-wave runs "
+                        "MLP 16-64-64-1 (hardtanh, odd) + linear skip, trained on 32 plane-wave runs "
                         "per axis from the 64-cell-CPML twin",
             "reflection": {"learned": r_learned, "linear_skip_only": r_lin,
                            "no_network": r_zero, "cpml8": r_c8, "cpml32": r_c32},

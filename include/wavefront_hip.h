@@ -250,7 +250,7 @@ uint64_t wf_batch_cells(const wf_batch *batch);
  * thin layer whose outside "ghost" fields (the zero the boundary cell's curl
  * update reads) are predicted each half step by a small per-face MLP:
  * 16 inputs (4-cell-deep E/H/psi stencil and a short history, scaled by
- * their largest magnitude) -> 64 -> 64 -> 1, tanh, BF16 weights and
+ * their largest magnitude) -> 64 -> 64 -> 1, hardtanh, BF16 weights and
  * activations on MFMA with FP32 accumulation, plus an FP32 linear skip.
  *
  * Parameters of one network: wf_mlp_param_count() floats, laid out

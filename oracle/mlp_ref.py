@@ -10,6 +10,9 @@ check them:
                   of activations;
   forward_bf16    the same with the GPU's rounding points emulated (bf16
                   inputs and hidden activations, f32 accumulation);
+
+The hidden activation is hardtanh, clip(z, -1, 1), with derivative 1 inside
+(-1, 1) and 0 where it clips.
   grad_f64        the exact gradient of the mean squared error of forward_f64
                   with respect to the FP32 master parameters (straight
                   through the bf16 rounding of the weights), checked against
@@ -59,27 +62,36 @@ def _gpu_weights(p):
             "wlin": w["wlin"].astype(np.float64)}
 
 
+def act(z):
+    return np.clip(z, -1.0, 1.0)
+
+
+def dact(a):
+    """Derivative of the activation from its value."""
+    return (np.abs(a) < 1.0).astype(np.float64)
+
+
 def forward_f64(p, u):
     w = _gpu_weights(p)
     u = np.asarray(u, np.float64)
-    a1 = np.tanh(u @ w["W1"] + w["b1"])
-    a2 = np.tanh(a1 @ w["W2"] + w["b2"])
+    a1 = act(u @ w["W1"] + w["b1"])
+    a2 = act(a1 @ w["W2"] + w["b2"])
     return a2 @ w["W3"] + w["b3"] + u @ w["wlin"], (a1, a2)
 
 
 def forward_bf16(p, u):
     """The GPU's rounding points: bf16 inputs and activations into each
     matrix product, f32 accumulation starting from the bias (products of
-    bf16 values are exact in f32), f32 tanh, f32 linear skip."""
+    bf16 values are exact in f32), f32 activation, f32 linear skip."""
     w = _gpu_weights(p)
     u32 = np.asarray(u, np.float32)
     ub = bf16(u32).astype(np.float64)
     b1 = w["b1"].astype(np.float32).astype(np.float64)
     b2 = w["b2"].astype(np.float32).astype(np.float64)
     z1 = (ub @ w["W1"] + b1).astype(np.float32)
-    a1 = np.tanh(z1.astype(np.float64)).astype(np.float32)
+    a1 = np.clip(z1, np.float32(-1), np.float32(1))
     z2 = (bf16(a1).astype(np.float64) @ w["W2"] + b2).astype(np.float32)
-    a2 = np.tanh(z2.astype(np.float64)).astype(np.float32)
+    a2 = np.clip(z2, np.float32(-1), np.float32(1))
     z3 = (bf16(a2).astype(np.float64) @ w["W3"]).astype(np.float32)
     wl = w["wlin"].astype(np.float32)
     lin = (u32 * wl).astype(np.float32)
@@ -133,10 +145,10 @@ def grad_f64(p, u, t):
     y, (a1, a2) = forward_f64(p, u)
     g = 2.0 * (y - np.asarray(t, np.float64)) / n
     d = {"b3": np.array([g.sum()]), "W3": a2.T @ g, "wlin": u.T @ g}
-    dz2 = np.outer(g, w["W3"]) * (1.0 - a2 * a2)
+    dz2 = np.outer(g, w["W3"]) * dact(a2)
     d["W2"] = a1.T @ dz2
     d["b2"] = dz2.sum(0)
-    dz1 = (dz2 @ w["W2"].T) * (1.0 - a1 * a1)
+    dz1 = (dz2 @ w["W2"].T) * dact(a1)
     d["W1"] = u.T @ dz1
     d["b1"] = dz1.sum(0)
     return join(d)
@@ -161,15 +173,15 @@ def grad_exact(p, u, t):
     w = split(np.asarray(p, np.float64))
     u = np.asarray(u, np.float64)
     n = len(u)
-    a1 = np.tanh(u @ w["W1"] + w["b1"])
-    a2 = np.tanh(a1 @ w["W2"] + w["b2"])
+    a1 = act(u @ w["W1"] + w["b1"])
+    a2 = act(a1 @ w["W2"] + w["b2"])
     y = a2 @ w["W3"] + w["b3"][0] + u @ w["wlin"]
     g = 2.0 * (y - np.asarray(t, np.float64)) / n
     d = {"b3": np.array([g.sum()]), "W3": a2.T @ g, "wlin": u.T @ g}
-    dz2 = np.outer(g, w["W3"]) * (1.0 - a2 * a2)
+    dz2 = np.outer(g, w["W3"]) * dact(a2)
     d["W2"] = a1.T @ dz2
     d["b2"] = dz2.sum(0)
-    dz1 = (dz2 @ w["W2"].T) * (1.0 - a1 * a1)
+    dz1 = (dz2 @ w["W2"].T) * dact(a1)
     d["W1"] = u.T @ dz1
     d["b1"] = dz1.sum(0)
     return join(d)
@@ -178,7 +190,7 @@ def grad_exact(p, u, t):
 def _loss_exact(p, u, t):
     w = split(np.asarray(p, np.float64))
     u = np.asarray(u, np.float64)
-    a1 = np.tanh(u @ w["W1"] + w["b1"])
-    a2 = np.tanh(a1 @ w["W2"] + w["b2"])
+    a1 = act(u @ w["W1"] + w["b1"])
+    a2 = act(a1 @ w["W2"] + w["b2"])
     y = a2 @ w["W3"] + w["b3"][0] + u @ w["wlin"]
     return float(np.mean((y - np.asarray(t, np.float64)) ** 2))

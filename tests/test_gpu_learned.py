@@ -25,7 +25,7 @@ def _say(msg):
 def test_mfma_forward_matches_cpu():
     """Pins the MFMA operand layouts: the GPU network equals the CPU
     emulation of its rounding points (tolerance 2e-3 of the output scale,
-    from f32 accumulation order and tanh), and the FP64 network within the
+    from f32 accumulation order and bf16 ties at the clamp), and the FP64 network within the
     bf16 activation error (3e-2)."""
     _say("mlp forward vs cpu")
     rng = np.random.default_rng(3)

@@ -8,7 +8,7 @@
 // src/engine.rs:83) are predicted each half step by a small per-face MLP
 // from the 4-cell-deep E/H/psi stencil at the face and a short history.
 //
-// MLP: 16 inputs -> 64 -> 64 -> 1 with tanh, BF16 weights and activations on
+// MLP: 16 inputs -> 64 -> 64 -> 1 with hardtanh, BF16 weights and activations on
 // MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation, plus an FP32
 // linear skip from the inputs.  Inputs are scaled by their largest
 // magnitude s and the output is scaled back by s, so the predictor is

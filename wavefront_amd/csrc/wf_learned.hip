@@ -64,7 +64,13 @@ __device__ __forceinline__ float tanh_f(float x)
     return copysignf(t, x);
 }
 
-__device__ __forceinline__ fv4 tanh4(fv4 z) { return fv4{tanh_f(z.x), tanh_f(z.y), tanh_f(z.z), tanh_f(z.w)}; }
+// Hidden activation: hardtanh, clamp(z, -1, 1) (one v_med3_f32; odd and
+// bounded like tanh at a twelfth of its cost), and its derivative from the
+// activation's value: 1 inside (-1, 1), 0 where it clamps.
+__device__ __forceinline__ float act_f(float z) { return __builtin_amdgcn_fmed3f(z, -1.0f, 1.0f); }
+__device__ __forceinline__ fv4 act4(fv4 z) { return fv4{act_f(z.x), act_f(z.y), act_f(z.z), act_f(z.w)}; }
+__device__ __forceinline__ float dact_f(float a) { return fabsf(a) < 1.0f ? 1.0f : 0.0f; }
+__device__ __forceinline__ fv4 dact4(fv4 a) { return fv4{dact_f(a.x), dact_f(a.y), dact_f(a.z), dact_f(a.w)}; }
 
 // The network in registers: A operands and the per-lane biases.
 struct MlpRegs {
@@ -124,7 +130,7 @@ __device__ __forceinline__ float mlp_forward(const MlpRegs &r, bfv4 ub, fv4 uf, 
 #pragma unroll
     for (int o = 0; o < 4; ++o) {
         const fv4 z = mfma(r.A1[o], ub, r.b1[o]);
-        t.a1[o] = tanh4(z);
+        t.a1[o] = act4(z);
         t.a1b[o] = to_bf4(t.a1[o]);
     }
 #pragma unroll
@@ -132,7 +138,7 @@ __device__ __forceinline__ float mlp_forward(const MlpRegs &r, bfv4 ub, fv4 uf, 
         fv4 z = r.b2[o];
 #pragma unroll
         for (int k = 0; k < 4; ++k) z = mfma(r.A2[o][k], t.a1b[k], z);
-        t.a2[o] = tanh4(z);
+        t.a2[o] = act4(z);
         t.a2b[o] = to_bf4(t.a2[o]);
     }
     fv4 z3 = zero;
@@ -314,7 +320,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float *__restrict__ u,
         for (int k = 0; k < 4; ++k) {
             gw3[k] = gw3[k] + a.a2[k] * gs;
             const fv4 da2 = w3r[k] * gs;
-            dz2[k] = da2 * (1.0f - a.a2[k] * a.a2[k]);
+            dz2[k] = da2 * dact4(a.a2[k]);
             gb2[k] = gb2[k] + dz2[k];
             dz2b[k] = to_bf4(dz2[k]);
         }
@@ -329,14 +335,14 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float *__restrict__ u,
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int o = 0; o < 4; ++o) gW2[i][o] = mfma(a1T[i], dz2T[o], gW2[i][o]);
-        // da1 = W2 dz2 on MFMA; dz1 = da1 (1 - a1^2)
+        // da1 = W2 dz2 on MFMA; dz1 = da1 act'(a1)
         bfv4 dz1b[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             fv4 d = zero;
 #pragma unroll
             for (int o = 0; o < 4; ++o) d = mfma(A2B[i][o], dz2b[o], d);
-            const fv4 dz1 = d * (1.0f - a.a1[i] * a.a1[i]);
+            const fv4 dz1 = d * dact4(a.a1[i]);
             gb1[i] = gb1[i] + dz1;
             dz1b[i] = to_bf4(dz1);
         }
@@ -500,6 +506,23 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
     const int dir = h.hi ? -1 : 1;
     const int na = h.na, Ta = h.Ta;
 
+    // per polarization: the P and Q fields and the psi slab, chosen once
+    // (the switches stay out of the loop)
+    // high faces: P = E_t1 / E_t2, Q = H_t2 / H_t1, H-phase psi terms;
+    // low faces: P = H_t1 / H_t2, Q = E_t2 / E_t1, E-phase psi terms
+    const int P0 = (h.hi ? 0 : 3) + t1, P1 = (h.hi ? 0 : 3) + t2;
+    const int Q0 = (h.hi ? 3 : 0) + t2, Q1 = (h.hi ? 3 : 0) + t1;
+    const int T0 = h.hi ? (a == 1 ? 0 : 1) : (a == 1 ? 2 : 3);
+    const int T1 = h.hi ? (a == 1 ? 1 : 0) : (a == 1 ? 3 : 2);
+    const float *const Pf_0 = h.hi ? field_ptr(es, P0) : field_ptr(hs, P0);
+    const float *const Pf_1 = h.hi ? field_ptr(es, P1) : field_ptr(hs, P1);
+    const float *const Qf_0 = h.hi ? field_ptr(hs, Q0) : field_ptr(es, Q0);
+    const float *const Qf_1 = h.hi ? field_ptr(hs, Q1) : field_ptr(es, Q1);
+    const float *const Ps_0 = psi_ptr(es, a, T0);
+    const float *const Ps_1 = psi_ptr(es, a, T1);
+    // step of one cell inward along the normal, in the field arrays
+    const long sa = (long)dir * (a == 0 ? 1l : (a == 1 ? (long)g.nx : (long)g.nx * g.ny));
+
     MlpRegs r;
     if (h.mode == 0) load_regs(h.net, lane, r);
     long row = -1;
@@ -508,57 +531,63 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
         if (k % h.emit_every == 0 && k / h.emit_every < h.emit_rows) row = (long)(k / h.emit_every);
     }
 
-    for (int grp = wave; grp < 2 * gpp; grp += nwaves) {
-        const int pol = grp / gpp;
-        const int pos = (grp - pol * gpp) * 16 + sl;
-        const bool live = pos < npos;
-        if (h.mode == 2) {
-            // replay: ghost values recorded by a thick-CPML run, step by step
-            if (lane < 16 && live) {
-                const uint64_t k = *step - h.emit_base;
+    if (h.mode == 2) {
+        // replay: ghost values recorded by a thick-CPML run, step by step
+        const uint64_t k = *step - h.emit_base;
+        for (int grp = wave; grp < 2 * gpp; grp += nwaves) {
+            const int pol = grp / gpp;
+            const int pos = (grp - pol * gpp) * 16 + sl;
+            if (lane < 16 && pos < npos) {
                 const float v = k < h.emit_rows
                                     ? h.samples[((k * 2 + pol) * npos + pos) * kSampleStride + 17]
                                     : 0.f;
                 (pol == 0 ? h.ghost[0] : h.ghost[1])[pos] = v;
             }
-            continue;
         }
-        const int f = h.fast0 + (live ? pos % h.nfast : 0);
-        const int sc = h.slow0 + (live ? pos / h.nfast : 0);
-        int P, Q, term;
-        if (h.hi) {
-            P = pol == 0 ? t1 : t2;
-            Q = 3 + (pol == 0 ? t2 : t1);
-            term = (a == 1) ? pol : 1 - pol;  // H-phase psi of dP/da
-        } else {
-            P = 3 + (pol == 0 ? t1 : t2);
-            Q = pol == 0 ? t2 : t1;
-            term = (a == 1) ? 2 + pol : 3 - pol;  // E-phase psi of dP/da
-        }
-        const float *Pf = h.hi ? field_ptr(es, P) : field_ptr(hs, P);
-        const float *Qf = h.hi ? field_ptr(hs, Q) : field_ptr(es, Q);
-        const float *Ps = psi_ptr(es, a, term);
-        float *mem = h.mem + ((size_t)pol * npos + (live ? pos : 0)) * 4;
+        return;
+    }
 
+    // This lane's four inputs of group grp (zero past the face).
+    auto gather = [&](int grp) __attribute__((always_inline)) -> fv4 {
         fv4 x = {0.f, 0.f, 0.f, 0.f};
-        if (live) {
-            if (q4 < 3) {
-                // P, Q at depths 0..3, or (inputs 8..11) zero, zero and the
-                // psi of dP/dn at depths 0, 1 (psi exists only in graded
-                // cells; a face of the domain has its first four graded)
-                const float *F = q4 == 0 ? Pf : (q4 == 1 ? Qf : Ps);
+        const int pol = grp / gpp;
+        const int pos = (grp - pol * gpp) * 16 + sl;
+        if (pos >= npos) return x;
+        const int f = h.fast0 + pos % h.nfast, sc = h.slow0 + pos / h.nfast;
+        if (q4 < 2) {
+            // P or Q at depths 0..3
+            const float *Pp = pol ? Pf_1 : Pf_0, *Qp = pol ? Qf_1 : Qf_0;  // uniform
+            const float *F = q4 == 0 ? Pp : Qp;
+            const long c0 = (long)face_cell(g, a, h.b0, f, sc);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int n = h.b0 + dir * (q4 < 2 ? j : j - 2);
-                    const bool gr = j >= 2 && Ta > 0 && (n < Ta || n >= na - Ta);
-                    const size_t i = q4 < 2 ? face_cell(g, a, n, f, sc)
-                                            : (gr ? face_psi(g, a, Ta, na, n, f, sc) : 0);
-                    x[j] = (q4 < 2 || gr) ? F[i] : 0.f;
-                }
-            } else {
-                x = *reinterpret_cast<const fv4 *>(mem);
+            for (int j = 0; j < 4; ++j) x[j] = F[c0 + j * sa];
+        } else if (q4 == 2) {
+            // inputs 8..11: zero, zero and the psi of dP/dn at depths 0, 1
+            // (psi exists only in graded cells; a face of the domain has its
+            // first four graded)
+            const float *Ps = pol ? Ps_1 : Ps_0;
+#pragma unroll
+            for (int j = 2; j < 4; ++j) {
+                const int n = h.b0 + dir * (j - 2);
+                const bool gr = Ta > 0 && (n < Ta || n >= na - Ta);
+                x[j] = gr ? Ps[face_psi(g, a, Ta, na, n, f, sc)] : 0.f;
             }
+        } else {
+            x = *reinterpret_cast<const fv4 *>(h.mem + ((size_t)pol * npos + pos) * 4);
         }
+        return x;
+    };
+
+    // Each wave takes groups wave, wave + nwaves, ...; the next group's
+    // inputs are loaded before the current one is evaluated.
+    fv4 xn = wave < 2 * gpp ? gather(wave) : fv4{0.f, 0.f, 0.f, 0.f};
+    for (int grp = wave; grp < 2 * gpp; grp += nwaves) {
+        const fv4 x = xn;
+        if (grp + nwaves < 2 * gpp) xn = gather(grp + nwaves);
+        const int pol = grp / gpp;
+        const int pos = (grp - pol * gpp) * 16 + sl;
+        const bool live = pos < npos;
+        float *mem = h.mem + ((size_t)pol * npos + (live ? pos : 0)) * 4;
         // scale: the largest magnitude of the 16 inputs (no squares, so no
         // underflow for the tiny fields ahead of a wave)
         float s = fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w)));
@@ -582,7 +611,11 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
             gnew = s * pred;
             if (lane < 16 && live) (pol == 0 ? h.ghost[0] : h.ghost[1])[pos] = gnew;
         } else {
-            gnew = live ? Pf[face_cell(g, a, h.b0 - dir, f, sc)] : 0.f;
+            gnew = 0.f;
+            if (live) {
+                const int f = h.fast0 + pos % h.nfast, sc = h.slow0 + pos / h.nfast;
+                gnew = (pol ? Pf_1 : Pf_0)[(long)face_cell(g, a, h.b0, f, sc) - sa];
+            }
             if (row >= 0 && live) {
                 float *o = h.samples + (((size_t)row * 2 + pol) * npos + pos) * kSampleStride;
                 *reinterpret_cast<fv4 *>(o + 4 * q4) = u;  // 80-byte records: 16-byte aligned
@@ -621,11 +654,21 @@ hipError_t launch_face_hooks(const Geo &g, const Ptrs &esrc, const Ptrs &hsrc,
 {
     if (n < 1) return hipSuccess;
     if (n > 3) return hipErrorInvalidValue;
+    // One resident wave set: the kernel holds 3 waves per SIMD (about 130
+    // VGPRs), i.e. three 4-wave blocks per CU, shared by the n faces; every
+    // wave then loads the network once and strides over many groups.
+    static int cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    if (!cus[dev] &&
+        hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return hipErrorInvalidDevice;
+    const int cap = std::max(1, 3 * cus[dev] / n);
     int blocks = 1;
     for (int k = 0; k < n; ++k) {
         const int npos = hooks[k]->nfast * hooks[k]->nslow;
         const int groups = 2 * ((npos + 15) / 16);
-        blocks = std::max(blocks, std::min((groups + 3) / 4, 1024));
+        blocks = std::max(blocks, std::min((groups + 3) / 4, cap));
     }
     const FaceHook &h0 = *hooks[0], &h1 = *hooks[n > 1 ? 1 : 0], &h2 = *hooks[n > 2 ? 2 : 0];
     hipLaunchKernelGGL(k_face_hooks, dim3(blocks, n), dim3(256), 0, st, g, esrc, hsrc, h0, h1,
