@@ -18,16 +18,20 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
-@pytest.fixture(autouse=True, params=["fused", "split"])
+@pytest.fixture(autouse=True, params=["fused", "split", "single"])
 def kernel_variant(request, monkeypatch):
-    """Every case runs on both step kernels: the single-pass fused kernel and
-    the separate in-place H and E kernels."""
-    monkeypatch.setenv("WF_KERNEL", request.param)
+    """Every case runs on each step-kernel scheme: the fused interior kernel
+    with the CPML shell in separate H and E frame kernels (the default), the
+    in-place H and E kernels over the whole domain, and the fused kernel
+    over the whole domain, shell included (opt-in)."""
+    monkeypatch.setenv("WF_KERNEL", "split" if request.param == "split" else "fused")
+    monkeypatch.setenv("WF_PML_FUSED", "1" if request.param == "single" else "0")
     return request.param
 
 
 def _say(msg):
-    print(f"[gpu-case] {msg} kernel={os.environ.get('WF_KERNEL')}", flush=True)
+    print(f"[gpu-case] {msg} kernel={os.environ.get('WF_KERNEL')} "
+          f"single_pass={os.environ.get('WF_PML_FUSED')}", flush=True)
 
 
 def _run_workload_gpu(w, steps):

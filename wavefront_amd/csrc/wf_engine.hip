@@ -214,6 +214,8 @@ struct wf_ctx {
     int zchunk = 32;
     Box interior{};     // fused variant: cells swept by the single-pass kernel
     BoxList frame{};    // fused variant: the rest (holds the whole CPML shell)
+    Box all{};          // fused variant with a CPML: the whole domain, one pass
+    bool pml_fused = false;  // WF_PML_FUSED=1: single pass over the shell too (slower, see launch_step)
     BoxList whole{};    // split variant: the whole domain
     hipStream_t side = nullptr;            // frame H kernel, overlapped
     hipEvent_t fork = nullptr, join = nullptr;
@@ -269,7 +271,7 @@ struct wf_ctx {
     bool contract = false;
     hipError_t (*lh)(const StepLaunch &, const BoxList &, hipStream_t) = launch_h;
     hipError_t (*le)(const StepLaunch &, const BoxList &, hipStream_t) = launch_e;
-    hipError_t (*lf)(const StepLaunch &, const Box &, hipStream_t) = launch_fused;
+    hipError_t (*lf)(const StepLaunch &, const Box &, bool, hipStream_t) = launch_fused;
 };
 
 namespace {
@@ -405,6 +407,11 @@ void plan_boxes(wf_ctx *c)
         add_box(f, split_box(hi[0], n[0], lo[1], hi[1], lo[2], hi[2], zc));
     }
     c->frame = f;
+    Box all{0, n[0], 0, n[1], 0, n[2], 64, 0, 0, 0, 0};
+    all.gx = (n[0] + 255) / 256;
+    all.gy = (n[1] + fr - 1) / fr;
+    all.gz = (n[2] + zc - 1) / zc;
+    c->all = all;
     BoxList w{};
     add_box(w, split_box(0, n[0], 0, n[1], 0, n[2], zc));
     c->whole = w;
@@ -445,10 +452,33 @@ bool hooks_beside_interior(const wf_ctx *c)
     return any;
 }
 
+// The whole domain in one pass of the fused kernel (CPML shell included),
+// when enabled (WF_PML_FUSED=1) and no learned face feeds ghost planes (its
+// low faces need the H phase done before their E update).  Bit-identical
+// to the frame kernels but slower on MI355X (1024^3 with CPML 24: 15.8 vs
+// 12.8 ms per step): a wave with any shell lane takes the CPML path, whose
+// psi loads are issued in the plane's arithmetic instead of a plane ahead,
+// and the kernel needs 232 VGPRs; the frame kernels stay the default.
+bool single_pass(const wf_ctx *c) { return c->fused && c->pml && c->pml_fused && !c->ghost_reads; }
+
 int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
 {
     hipStream_t st = c->stream;
-    if (c->fused && c->hook_overlap && !ev && c->side && c->frame.total > 0 &&
+    if (single_pass(c)) {
+        // hooks (recording only, no ghost planes): high faces before the
+        // step (E^n, H^{n-1/2} in state A), low faces after it (H^{n+1/2} in
+        // state B; E^n and the E-phase psi of step n still in state A)
+        StepLaunch s{c->g, c->p, c->alt, c->pml, c->zchunk, st};
+        if (int rc = run_hooks(c, true, c->p, c->p, st)) return rc;
+        if (ev) WF_HIP(hipEventRecord(ev[0], st));
+        WF_HIP(c->lf(s, c->all, true, st));
+        if (ev) {
+            WF_HIP(hipEventRecord(ev[1], st));
+            for (int k = 2; k < 6; ++k) WF_HIP(hipEventRecord(ev[k], st));
+        }
+        if (int rc = run_hooks(c, false, c->p, c->alt, st)) return rc;
+        std::swap(c->p, c->alt);
+    } else if (c->fused && c->hook_overlap && !ev && c->side && c->frame.total > 0 &&
         c->interior.gx > 0 && hooks_beside_interior(c)) {
         // Face hooks and the frame's H phase on the side stream, beside the
         // interior kernel: the hooks touch only frame cells (the high faces
@@ -463,7 +493,7 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
         WF_HIP(c->lh(s, c->frame, c->side));
         if (int rc = run_hooks(c, false, c->p, c->alt, c->side)) return rc;
         WF_HIP(hipEventRecord(c->join, c->side));
-        WF_HIP(c->lf(s, c->interior, st));
+        WF_HIP(c->lf(s, c->interior, false, st));
         WF_HIP(hipStreamWaitEvent(st, c->join, 0));
         WF_HIP(c->le(s, c->frame, st));
         std::swap(c->p, c->alt);
@@ -478,11 +508,11 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
             WF_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
             WF_HIP(c->lh(s, c->frame, c->side));
             WF_HIP(hipEventRecord(c->join, c->side));
-            WF_HIP(c->lf(s, c->interior, st));
+            WF_HIP(c->lf(s, c->interior, false, st));
             WF_HIP(hipStreamWaitEvent(st, c->join, 0));
         } else {
             if (ev) WF_HIP(hipEventRecord(ev[0], st));
-            WF_HIP(c->lf(s, c->interior, st));
+            WF_HIP(c->lf(s, c->interior, false, st));
             if (ev) WF_HIP(hipEventRecord(ev[1], st));
             if (ev) WF_HIP(hipEventRecord(ev[2], st));
             WF_HIP(c->lh(s, c->frame, st));
@@ -800,6 +830,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     }
     if (const char *ov = std::getenv("WF_OVERLAP")) c->overlap = std::atoi(ov) != 0;
     if (const char *ho = std::getenv("WF_HOOK_OVERLAP")) c->hook_overlap = std::atoi(ho) != 0;
+    if (const char *pf = std::getenv("WF_PML_FUSED")) c->pml_fused = std::atoi(pf) != 0;
     if (const char *gr = std::getenv("WF_GRAPH")) c->use_graph = std::atoi(gr) != 0;
     if (const char *fc = std::getenv("WF_FP_CONTRACT"))
         if (std::atoi(fc) != 0) {
@@ -1227,6 +1258,13 @@ int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
     // non-identity psi terms per phase: 2 per graded axis per cell
     const uint64_t terms = 2ull * (2ull * c->g.T[0] * ny * nz + 2ull * c->g.T[1] * nx * nz +
                                    2ull * c->g.T[2] * nx * ny);
+    if (single_pass(c)) {
+        // one pass: 49 B per cell, and per psi term of each phase 4 B read
+        // and 4 B written
+        *fused_bytes = 49ull * nx * ny * nz + 16ull * terms;
+        *h_bytes = *e_bytes = 0;
+        return WF_OK;
+    }
     const BoxList &bl = c->fused ? c->frame : c->whole;
     uint64_t cells = 0;
     for (int k = 0; k < bl.n; ++k) {

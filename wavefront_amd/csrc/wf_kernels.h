@@ -157,8 +157,10 @@ hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 // E kernel over the boxes: reads H(n+1/2) from alt and E^n / psi from cur;
 // writes E(n+1) and the E-phase psi to alt.
 hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
-// Single-pass kernel over the interior box, cur -> alt (no CPML inside).
-hipError_t launch_fused(const StepLaunch &s, const Box &interior, hipStream_t st);
+// Single-pass kernel over a box, cur -> alt.  pml_cells: the box may hold
+// CPML shell cells (the kernel then updates their psi too); false selects
+// the plain variant for a box wholly outside the shell.
+hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st);
 // Tile shapes of the two kernels (owned rows per fused tile, rows per split
 // workgroup of lpr = 64).
 int fused_rows_per_tile();
@@ -197,7 +199,7 @@ hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, in
 namespace fma {
 hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
-hipError_t launch_fused(const StepLaunch &s, const Box &interior, hipStream_t st);
+hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st);
 hipError_t launch_batch_h(const BatchScene *scenes, const uint32_t *wg_scene, int total,
                           int zchunk, int grid, hipStream_t st);
 hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, int total,

@@ -447,6 +447,29 @@ __global__ __launch_bounds__(64 * kSY) void k_e4_batch(const BatchScene *__restr
 #endif
 constexpr int kFY = WF_FUSED_FY;
 
+// CPML of one derivative of a cell (quad) in the PML shell on axis `ax`:
+// graded -> the one-tap recursion with psi read from `pin` and, when
+// `store`, the new psi written to `pout`; otherwise raw + 0.0f, the
+// reference's identity profile (src/fdtd.rs:136-163).
+__device__ __forceinline__ f4 pml_d4(f4 raw, bool gr, f4 pb, f4 pa, f4 pk, const float *pin,
+                                     float *pout, size_t i, bool store)
+{
+    if (!gr) return raw + 0.0f;
+    f4 p = ld4(pin + i);
+    const f4 d = cpml4(raw, pb, pa, pk, p);
+    if (store) st4(pout + i, p);
+    return d;
+}
+
+__device__ __forceinline__ float pml_d1(float raw, bool gr, float pb, float pa, float pk,
+                                        const float *pin, size_t i)
+{
+    if (!gr) return raw + 0.0f;
+    float p = pin[i];
+    return cpml(raw, pb, pa, pk, p);
+}
+
+template <bool PML>
 __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box box, int zchunk)
 {
     __shared__ float2 s_ce[256];       // (ca, cb)
@@ -483,6 +506,42 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
     const bool hlive = own && lane == 0 && xh >= 0;
     const size_t rowh = (size_t)yc * nx + (xh >= 0 ? xh : 0);
     const size_t rowh_py = (size_t)(yp >= 0 ? yp : 0) * nx + (xh >= 0 ? xh : 0);
+
+    // CPML shell and grading of this lane's quad (x), this wave's row (y),
+    // and the halo cell; profiles of graded positions (identity elsewhere).
+    bool sh_x = false, sh_y = false, sh_h = false, gx = false, gy = false, gh = false;
+    int gxi = 0, gyi = 0, ghi = 0;
+    f4 pbx = splat(1.f), pax = splat(0.f), pkx = splat(1.f);
+    float pby = 1.f, pay = 0.f, pky = 1.f, pbh = 1.f, pah = 0.f, pkh = 1.f;
+    if (PML && live) {
+        sh_x = in_shell_axis(g, 0, xc);
+        sh_y = in_shell_axis(g, 1, yc);
+        gx = graded(g, 0, xc, g.nx);
+        gy = graded(g, 1, yc, g.ny);
+        gxi = slab_index(g, 0, xc, g.nx);
+        gyi = slab_index(g, 1, yc, g.ny);
+        if (gx) {
+            pbx = ld4(a.prof[0][0] + xc);
+            pax = ld4(a.prof[0][1] + xc);
+            pkx = ld4(a.prof[0][2] + xc);
+        }
+        if (gy) {
+            pby = a.prof[1][0][yc];
+            pay = a.prof[1][1][yc];
+            pky = a.prof[1][2][yc];
+        }
+        if (hlive) {
+            sh_h = in_shell_axis(g, 0, xh);
+            gh = graded(g, 0, xh, g.nx);
+            ghi = slab_index(g, 0, xh, g.nx);
+            if (gh) {
+                pbh = a.prof[0][0][xh];
+                pah = a.prof[0][1][xh];
+                pkh = a.prof[0][2][xh];
+            }
+        }
+    }
+    const size_t Tx2 = 2 * (size_t)g.T[0], Ty2 = 2 * (size_t)g.T[1];
 
     const float *__restrict__ aex = a.f[0];
     const float *__restrict__ aey = a.f[1];
@@ -576,9 +635,30 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
     load(cur, it0);
 
     for (int it = it0; it < nit; ++it) {
-        const size_t zo = (size_t)plane_z(it) * plane;
+        const int z = plane_z(it);
+        const size_t zo = (size_t)z * plane;
         Plane nxt;
         if (it + 1 < nit) load(nxt, it + 1);
+
+        // CPML of this plane (uniform): shell on z, grading, z profile
+        bool sh_z = false, gz = false, pml_q = false;
+        size_t gzo = 0;
+        float pbz = 1.f, paz = 0.f, pkz = 1.f;
+        if (PML) {
+            sh_z = in_shell_axis(g, 2, z);
+            gz = graded(g, 2, z, g.nz);
+            if (gz) {
+                gzo = (size_t)slab_index(g, 2, z, g.nz) * plane;
+                pbz = a.prof[2][0][z];
+                paz = a.prof[2][1][z];
+                pkz = a.prof[2][2][z];
+            }
+            pml_q = live && (sh_x || sh_y || sh_z);
+        }
+        const bool store = own && it >= 0;  // the pre-plane and halo row only read psi
+        const size_t iy = ((size_t)z * Ty2 + gyi) * nx + xc;  // psi indices of the quad
+        const size_t iz = gzo + row;
+        const size_t ix = ((size_t)z * g.ny + yc) * Tx2 + gxi;
 
         float sez = __shfl_down(ez_c.x, 1, 64), sey = __shfl_down(ey_c.x, 1, 64);
         if (edge) {
@@ -586,14 +666,29 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
             sey = cur.sey;
         }
 
-        // ---- H(n+1/2) of the own quad: update_h_field -----------------------
+        // ---- H(n+1/2) of the own quad: update_h_field[_pml] ------------------
         const f4 ez_px = f4{ez_c.y, ez_c.z, ez_c.w, sez};
         const f4 ey_px = f4{ey_c.y, ey_c.z, ey_c.w, sey};
         f4 da, db;
         coef4(s_ch, cur.ids, da, db);
-        f4 hxn = da * cur.hx_o - db * ((cur.ez_py - ez_c) - (cur.ey_n - ey_c));
-        f4 hyn = da * cur.hy_o - db * ((cur.ex_n - ex_c) - (ez_px - ez_c));
-        f4 hzn = da * cur.hz_o - db * ((ey_px - ey_c) - (cur.ex_py - ex_c));
+        f4 d_ez_dy = cur.ez_py - ez_c, d_ey_dz = cur.ey_n - ey_c;
+        f4 d_ex_dz = cur.ex_n - ex_c, d_ez_dx = ez_px - ez_c;
+        f4 d_ey_dx = ey_px - ey_c, d_ex_dy = cur.ex_py - ex_c;
+        if (PML && pml_q) {
+            d_ez_dy = pml_d4(d_ez_dy, gy, splat(pby), splat(pay), splat(pky), a.psi[1][0],
+                             b.psi[1][0], iy, store);
+            d_ex_dy = pml_d4(d_ex_dy, gy, splat(pby), splat(pay), splat(pky), a.psi[1][1],
+                             b.psi[1][1], iy, store);
+            d_ey_dz = pml_d4(d_ey_dz, gz, splat(pbz), splat(paz), splat(pkz), a.psi[2][0],
+                             b.psi[2][0], iz, store);
+            d_ex_dz = pml_d4(d_ex_dz, gz, splat(pbz), splat(paz), splat(pkz), a.psi[2][1],
+                             b.psi[2][1], iz, store);
+            d_ez_dx = pml_d4(d_ez_dx, gx, pbx, pax, pkx, a.psi[0][0], b.psi[0][0], ix, store);
+            d_ey_dx = pml_d4(d_ey_dx, gx, pbx, pax, pkx, a.psi[0][1], b.psi[0][1], ix, store);
+        }
+        f4 hxn = da * cur.hx_o - db * (d_ez_dy - d_ey_dz);
+        f4 hyn = da * cur.hy_o - db * (d_ex_dz - d_ez_dx);
+        f4 hzn = da * cur.hz_o - db * (d_ey_dx - d_ex_dy);
         if (!live) {
             hxn = splat(0.f);
             hyn = splat(0.f);
@@ -605,8 +700,18 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
             float hy_h = 0.f, hz_h = 0.f;
             if (hlive) {
                 const float2 c = s_ch[cur.hid];
-                hy_h = c.x * cur.hhy - c.y * ((cur.hexn - hex) - (ez_c.x - hez));
-                hz_h = c.x * cur.hhz - c.y * ((ey_c.x - hey) - (cur.hexpy - hex));
+                float h_ex_dz = cur.hexn - hex, h_ez_dx = ez_c.x - hez;
+                float h_ey_dx = ey_c.x - hey, h_ex_dy = cur.hexpy - hex;
+                if (PML && (sh_h || sh_y || sh_z)) {
+                    const size_t jx = ((size_t)z * g.ny + yc) * Tx2 + ghi;
+                    const size_t jy = ((size_t)z * Ty2 + gyi) * nx + xh;
+                    h_ex_dz = pml_d1(h_ex_dz, gz, pbz, paz, pkz, a.psi[2][1], gzo + rowh);
+                    h_ez_dx = pml_d1(h_ez_dx, gh, pbh, pah, pkh, a.psi[0][0], jx);
+                    h_ey_dx = pml_d1(h_ey_dx, gh, pbh, pah, pkh, a.psi[0][1], jx);
+                    h_ex_dy = pml_d1(h_ex_dy, gy, pby, pay, pky, a.psi[1][1], jy);
+                }
+                hy_h = c.x * cur.hhy - c.y * (h_ex_dz - h_ez_dx);
+                hz_h = c.x * cur.hhz - c.y * (h_ey_dx - h_ex_dy);
             }
             // ---- neighbours of H(n+1/2) at x-1 (lane-1) and y-1 (LDS) -------
             float hz_l = __shfl_up(hzn.w, 1, 64), hy_l = __shfl_up(hyn.w, 1, 64);
@@ -623,13 +728,30 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
                 const f4 hx_my = s_h[buf][1][w - 1][lane];
                 const f4 hz_mx = f4{hz_l, hzn.x, hzn.y, hzn.z};
                 const f4 hy_mx = f4{hy_l, hyn.x, hyn.y, hyn.z};
-                // ---- E(n+1): update_e_field ---------------------------------
+                // ---- E(n+1): update_e_field[_pml] ---------------------------
                 f4 ca, cb;
                 coef4(s_ce, cur.ids, ca, cb);
+                f4 d_hz_dy = hzn - hz_my, d_hy_dz = hyn - hy_mz;
+                f4 d_hx_dz = hxn - hx_mz, d_hz_dx = hzn - hz_mx;
+                f4 d_hy_dx = hyn - hy_mx, d_hx_dy = hxn - hx_my;
+                if (PML && pml_q) {
+                    d_hz_dy = pml_d4(d_hz_dy, gy, splat(pby), splat(pay), splat(pky),
+                                     a.psi[1][2], b.psi[1][2], iy, true);
+                    d_hx_dy = pml_d4(d_hx_dy, gy, splat(pby), splat(pay), splat(pky),
+                                     a.psi[1][3], b.psi[1][3], iy, true);
+                    d_hy_dz = pml_d4(d_hy_dz, gz, splat(pbz), splat(paz), splat(pkz),
+                                     a.psi[2][2], b.psi[2][2], iz, true);
+                    d_hx_dz = pml_d4(d_hx_dz, gz, splat(pbz), splat(paz), splat(pkz),
+                                     a.psi[2][3], b.psi[2][3], iz, true);
+                    d_hz_dx = pml_d4(d_hz_dx, gx, pbx, pax, pkx, a.psi[0][2], b.psi[0][2], ix,
+                                     true);
+                    d_hy_dx = pml_d4(d_hy_dx, gx, pbx, pax, pkx, a.psi[0][3], b.psi[0][3], ix,
+                                     true);
+                }
                 const size_t o = zo + row;
-                st4(b.f[0] + o, ca * ex_c + cb * ((hzn - hz_my) - (hyn - hy_mz)));
-                st4(b.f[1] + o, ca * ey_c + cb * ((hxn - hx_mz) - (hzn - hz_mx)));
-                st4(b.f[2] + o, ca * ez_c + cb * ((hyn - hy_mx) - (hxn - hx_my)));
+                st4(b.f[0] + o, ca * ex_c + cb * (d_hz_dy - d_hy_dz));
+                st4(b.f[1] + o, ca * ey_c + cb * (d_hx_dz - d_hz_dx));
+                st4(b.f[2] + o, ca * ez_c + cb * (d_hy_dx - d_hx_dy));
                 st4(b.f[3] + o, hxn);
                 st4(b.f[4] + o, hyn);
                 st4(b.f[5] + o, hzn);
@@ -703,12 +825,15 @@ hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, in
     return hipGetLastError();
 }
 
-hipError_t launch_fused(const StepLaunch &s, const Box &interior, hipStream_t st)
+hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st)
 {
-    const int n = interior.gx * interior.gy * interior.gz;
-    if (n > 0)
-        hipLaunchKernelGGL(k_fused4, dim3((unsigned)n), dim3(64, kFY), 0, st, s.g, s.cur, s.alt,
-                           interior, s.zchunk);
+    const int n = box.gx * box.gy * box.gz;
+    if (n > 0 && pml_cells)
+        hipLaunchKernelGGL(k_fused4<true>, dim3((unsigned)n), dim3(64, kFY), 0, st, s.g, s.cur,
+                           s.alt, box, s.zchunk);
+    else if (n > 0)
+        hipLaunchKernelGGL(k_fused4<false>, dim3((unsigned)n), dim3(64, kFY), 0, st, s.g, s.cur,
+                           s.alt, box, s.zchunk);
     return hipGetLastError();
 }
 
