@@ -109,6 +109,8 @@ def test_recorded_inputs_match_fields():
     dx, dt, _ = L.study_grid()
     big = L.thick_dims(thin)
     K = 44
+    # virtual faces LAYER cells inside the CPML, where psi is live
+    off = L.THICK - L.LAYER
     srcs = [Source(s.x + L.OFF, s.y + L.OFF, s.z + L.OFF, s.component, s.amplitude, s.waveform)
             for s in _point_sources((18, 10, 26))]
     regions = (L.FaceRegion * 6)()
@@ -118,7 +120,8 @@ def test_recorded_inputs_match_fields():
         a, hi = f // 2, f % 2
         t1, t2 = (a + 1) % 3, (a + 2) % 3
         tf, ts = min(t1, t2), max(t1, t2)
-        regions[f] = L.FaceRegion(nb[a] - 1 - L.OFF if hi else L.OFF, L.OFF, nt[tf], L.OFF, nt[ts])
+        regions[f] = L.FaceRegion(nb[a] - 1 - off if hi else off, off, nt[tf] + 2 * L.LAYER, off,
+                                  nt[ts] + 2 * L.LAYER)
     ctx = L.thick_context(big, dt, dx)
     with DeviceEngine(big, None, MaterialTable.vacuum_filled(dt, dx), ctx) as eng:
         eng.set_sources(srcs, dt, K)
@@ -211,7 +214,8 @@ def test_replayed_ghosts_reproduce_thick_run(monkeypatch, kernel):
     """A thin run fed, step by step, the ghost values the thick-CPML run
     recorded is bit-identical to the thick run inside the thin domain: the
     ghost planes reach the right cells of all six faces at the right half
-    step, and the thin layer is exactly the thick layer's first cells."""
+    step, in the single-pass schedule (the fused kernel, then the low faces'
+    boundary E redone with their ghosts) and in the split one."""
     monkeypatch.setenv("WF_KERNEL", kernel)
     _say(f"replay thin vs thick kernel={kernel}")
     thin = GridDims(32, 24, 40)

@@ -1,7 +1,7 @@
 """CPU checks of the learned-boundary test infrastructure: the FP64 network
 reference's gradient against finite differences, its bf16 emulation against
 its f64 form, the FP64 1D plane-wave oracle against the C oracle's 3D run,
-and the thin-layer profile construction."""
+and the thin domain's place beside its twin's CPML."""
 import numpy as np
 
 from oracle import fdtd1d, mlp_ref, wfo
@@ -63,16 +63,16 @@ def test_fp64_plane_wave_oracle_matches_c_oracle():
     assert np.max(np.abs(tr64)) > 0.1
 
 
-def test_thin_profile_is_the_thick_profile_cut():
+def test_thin_domain_lies_beyond_the_twin_cpml():
+    """The thin domain starts right after the twin's CPML on each learned
+    axis: the twin's profile is graded in its first THICK cells (and last)
+    and identity on every cell of the thin domain, so a thin run with exact
+    ghosts is the twin's run restricted to the thin domain."""
     dx, dt, _ = L.study_grid()
     n = 152
-    thin = L.thin_profile(n, dt, dx)
-    thick = L.thick_profile(n + 2 * L.OFF, dt, dx)
-    for p in range(L.LAYER):
-        assert thin.get(p) == thick.get(p + L.OFF)
-        assert thin.get(n - 1 - p) == thick.get(n + 2 * L.OFF - 1 - (p + L.OFF))
-    assert thin.get(L.LAYER) == (1.0, 0.0, 1.0)
-    assert all(thin.get(i) == (1.0, 0.0, 1.0) for i in range(L.LAYER, n - L.LAYER))
-    # every layer cell is graded (non-identity), at both ends
-    for p in list(range(L.LAYER)) + list(range(n - L.LAYER, n)):
-        assert thin.get(p) != (1.0, 0.0, 1.0)
+    nb = n + 2 * L.OFF
+    thick = L.thick_profile(nb, dt, dx)
+    assert L.OFF == L.THICK
+    for p in range(L.THICK):
+        assert thick.get(p) != (1.0, 0.0, 1.0) and thick.get(nb - 1 - p) != (1.0, 0.0, 1.0)
+    assert all(thick.get(L.OFF + i) == (1.0, 0.0, 1.0) for i in range(n))

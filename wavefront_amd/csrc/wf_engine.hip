@@ -215,6 +215,7 @@ struct wf_ctx {
     Box interior{};     // fused variant: cells swept by the single-pass kernel
     BoxList frame{};    // fused variant: the rest (holds the whole CPML shell)
     Box all{};          // fused variant with a CPML: the whole domain, one pass
+    BoxList lofix{};    // learned faces without a CPML: the low faces' boundary cells
     bool pml_fused = false;  // WF_PML_FUSED=1: single pass over the shell too (slower, see launch_step)
     BoxList whole{};    // split variant: the whole domain
     hipStream_t side = nullptr;            // frame H kernel, overlapped
@@ -461,10 +462,33 @@ bool hooks_beside_interior(const wf_ctx *c)
 // and the kernel needs 232 VGPRs; the frame kernels stay the default.
 bool single_pass(const wf_ctx *c) { return c->fused && c->pml && c->pml_fused && !c->ghost_reads; }
 
+// Learned faces on a domain without a CPML: one pass of the fused kernel
+// over the whole domain (E beyond the high faces from their ghost planes,
+// predicted before the pass), then the low faces' hooks (which read the
+// new H) and the E kernel over the low faces' boundary cells, now with
+// their ghost H.  No frame kernels.
+bool learned_single_pass(const wf_ctx *c) { return c->fused && !c->pml && c->ghost_reads; }
+
 int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
 {
     hipStream_t st = c->stream;
-    if (single_pass(c)) {
+    if (learned_single_pass(c)) {
+        StepLaunch s{c->g, c->p, c->alt, c->pml, c->zchunk, st};
+        s.ghost = true;
+        if (int rc = run_hooks(c, true, c->p, c->p, st)) return rc;
+        if (ev) WF_HIP(hipEventRecord(ev[0], st));
+        WF_HIP(c->lf(s, c->all, false, st));
+        if (ev) {
+            WF_HIP(hipEventRecord(ev[1], st));
+            WF_HIP(hipEventRecord(ev[2], st));
+            WF_HIP(hipEventRecord(ev[3], st));
+        }
+        if (int rc = run_hooks(c, false, c->p, c->alt, st)) return rc;
+        if (ev) WF_HIP(hipEventRecord(ev[4], st));
+        WF_HIP(c->le(s, c->lofix, st));
+        if (ev) WF_HIP(hipEventRecord(ev[5], st));
+        std::swap(c->p, c->alt);
+    } else if (single_pass(c)) {
         // hooks (recording only, no ghost planes): high faces before the
         // step (E^n, H^{n-1/2} in state A), low faces after it (H^{n+1/2} in
         // state B; E^n and the E-phase psi of step n still in state A)
@@ -1265,6 +1289,19 @@ int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
         *h_bytes = *e_bytes = 0;
         return WF_OK;
     }
+    if (learned_single_pass(c)) {
+        // one pass, then the E kernel over the low faces' boundary cells
+        // (37 B per cell)
+        uint64_t cells = 0;
+        for (int k = 0; k < c->lofix.n; ++k) {
+            const Box &b = c->lofix.b[k];
+            cells += (uint64_t)(b.x1 - b.x0) * (b.y1 - b.y0) * (b.z1 - b.z0);
+        }
+        *fused_bytes = 49ull * nx * ny * nz;
+        *h_bytes = 0;
+        *e_bytes = 37ull * cells;
+        return WF_OK;
+    }
     const BoxList &bl = c->fused ? c->frame : c->whole;
     uint64_t cells = 0;
     for (int k = 0; k < bl.n; ++k) {
@@ -1339,6 +1376,17 @@ void update_ghost_reads(wf_ctx *c)
     c->ghost_reads = false;
     for (int f = 0; f < 6; ++f)
         if (c->hook_on[f] && c->hook[f].mode != 1) c->ghost_reads = true;
+    // boundary cells of the low faces with ghost planes (x: the first quad
+    // of every row; cells on two such faces are computed twice, with the
+    // same result, since the E kernel reads only H and the old E)
+    const Geo &g = c->g;
+    const int zc = c->zchunk;
+    BoxList lf{};
+    auto ghosted = [&](int f) { return c->hook_on[f] && c->hook[f].mode != 1; };
+    if (ghosted(4)) add_box(lf, split_box(0, g.nx, 0, g.ny, 0, 1, zc));
+    if (ghosted(2)) add_box(lf, split_box(0, g.nx, 0, 1, 0, g.nz, zc));
+    if (ghosted(0)) add_box(lf, split_box(0, 4, 0, g.ny, 0, g.nz, zc));
+    c->lofix = lf;
 }
 
 }  // namespace
@@ -1385,9 +1433,9 @@ int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params, int
         if (c->g.periodic >> fg.a & 1u)
             return fail(WF_ERR_INVALID, "face %d lies on a periodic axis", f);
         if (fg.na < 8) return fail(WF_ERR_INVALID, "face %d: axis shorter than 8 cells", f);
-        if (c->g.T[fg.a] < 4)
-            return fail(WF_ERR_INVALID, "face %d: its axis needs a CPML shell of at least 4 cells "
-                        "(the layer in front of the learned boundary)", f);
+        if (c->pml && c->g.T[fg.a] < 4)
+            return fail(WF_ERR_INVALID, "face %d: in a domain with a CPML its axis needs a shell "
+                        "of at least 4 cells (the frame kernels then hold the face)", f);
     }
     WF_HIP(hipSetDevice(c->device));
     WF_HIP(hipStreamSynchronize(c->stream));
@@ -1510,8 +1558,9 @@ int wf_replay_faces(wf_ctx *c, uint32_t face_mask, uint64_t rows, const float *c
         if (!(face_mask >> f & 1u)) continue;
         if (!samples[f]) return fail(WF_ERR_INVALID, "wf_replay_faces: face %d has no samples", f);
         if (c->g.periodic >> (f / 2) & 1u) return fail(WF_ERR_INVALID, "face %d lies on a periodic axis", f);
-        if (c->g.T[f / 2] < 4)
-            return fail(WF_ERR_INVALID, "face %d: its axis needs a CPML shell of at least 4 cells", f);
+        if (c->pml && c->g.T[f / 2] < 4)
+            return fail(WF_ERR_INVALID, "face %d: in a domain with a CPML its axis needs a shell "
+                        "of at least 4 cells", f);
     }
     WF_HIP(hipSetDevice(c->device));
     WF_HIP(hipStreamSynchronize(c->stream));

@@ -159,7 +159,9 @@ hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 // Single-pass kernel over a box, cur -> alt.  pml_cells: the box may hold
 // CPML shell cells (the kernel then updates their psi too); false selects
-// the plain variant for a box wholly outside the shell.
+// the plain variant for a box wholly outside the shell.  s.ghost: E beyond
+// the high faces comes from the learned ghost planes (with pml_cells false);
+// the low faces' boundary E is left for the E kernel to redo.
 hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st);
 // Tile shapes of the two kernels (owned rows per fused tile, rows per split
 // workgroup of lpr = 64).

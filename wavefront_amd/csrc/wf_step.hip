@@ -469,7 +469,12 @@ __device__ __forceinline__ float pml_d1(float raw, bool gr, float pb, float pa, 
     return cpml(raw, pb, pa, pk, p);
 }
 
-template <bool PML>
+// GHOST: the E planes just beyond the high faces come from learned ghost
+// planes where present (a.ghost[1], [3], [5]: E that the H phase of the
+// boundary cells reads); the low faces' ghost H is not known yet when this
+// kernel runs, so their boundary cells' E is redone afterwards by the E
+// kernel (see the engine's learned-face schedule).
+template <bool PML, bool GHOST>
 __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box box, int zchunk)
 {
     __shared__ float2 s_ce[256];       // (ca, cb)
@@ -583,10 +588,16 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
                 P.ex_n = ld4(aex + on);
                 P.ey_n = ld4(aey + on);
                 P.ez_n = ld4(aez + on);
+            } else if (GHOST && a.ghost[5][0]) {
+                P.ex_n = ld4(a.ghost[5][0] + (size_t)yc * nx + xc);
+                P.ey_n = ld4(a.ghost[5][1] + (size_t)yc * nx + xc);
             }
             if (yp >= 0) {
                 P.ez_py = ld4(aez + zo + rowp_y);
                 P.ex_py = ld4(aex + zo + rowp_y);
+            } else if (GHOST && a.ghost[3][0]) {
+                P.ez_py = ld4(a.ghost[3][0] + (size_t)z * nx + xc);
+                P.ex_py = ld4(a.ghost[3][1] + (size_t)z * nx + xc);
             }
             P.hx_o = ld4(ahx + zo + row);
             P.hy_o = ld4(ahy + zo + row);
@@ -595,6 +606,9 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
             if (edge && xq >= 0) {
                 P.sez = aez[zo + rowq];
                 P.sey = aey[zo + rowq];
+            } else if (GHOST && edge && a.ghost[1][0]) {
+                P.sey = a.ghost[1][0][(size_t)z * g.ny + yc];
+                P.sez = a.ghost[1][1][(size_t)z * g.ny + yc];
             }
         }
         if (hlive) {
@@ -603,9 +617,13 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
                 P.hexn = aex[on];
                 P.heyn = aey[on];
                 P.hezn = aez[on];
+            } else if (GHOST && a.ghost[5][0]) {
+                P.hexn = a.ghost[5][0][(size_t)yc * nx + xh];
+                P.heyn = a.ghost[5][1][(size_t)yc * nx + xh];
             }
             if (it >= 0) {
-                P.hexpy = yp >= 0 ? aex[zo + rowh_py] : 0.f;
+                P.hexpy = yp >= 0 ? aex[zo + rowh_py]
+                                  : ((GHOST && a.ghost[3][1]) ? a.ghost[3][1][(size_t)z * nx + xh] : 0.f);
                 P.hid = a.id[zo + rowh];
                 P.hhy = ahy[zo + rowh];
                 P.hhz = ahz[zo + rowh];
@@ -828,12 +846,17 @@ hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, in
 hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st)
 {
     const int n = box.gx * box.gy * box.gz;
-    if (n > 0 && pml_cells)
-        hipLaunchKernelGGL(k_fused4<true>, dim3((unsigned)n), dim3(64, kFY), 0, st, s.g, s.cur,
-                           s.alt, box, s.zchunk);
-    else if (n > 0)
-        hipLaunchKernelGGL(k_fused4<false>, dim3((unsigned)n), dim3(64, kFY), 0, st, s.g, s.cur,
-                           s.alt, box, s.zchunk);
+    if (n <= 0) return hipSuccess;
+    if (pml_cells && s.ghost) return hipErrorInvalidValue;  // not a supported combination
+    if (pml_cells)
+        hipLaunchKernelGGL((k_fused4<true, false>), dim3((unsigned)n), dim3(64, kFY), 0, st, s.g,
+                           s.cur, s.alt, box, s.zchunk);
+    else if (s.ghost)
+        hipLaunchKernelGGL((k_fused4<false, true>), dim3((unsigned)n), dim3(64, kFY), 0, st, s.g,
+                           s.cur, s.alt, box, s.zchunk);
+    else
+        hipLaunchKernelGGL((k_fused4<false, false>), dim3((unsigned)n), dim3(64, kFY), 0, st, s.g,
+                           s.cur, s.alt, box, s.zchunk);
     return hipGetLastError();
 }
 

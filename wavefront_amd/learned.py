@@ -2,20 +2,21 @@
 
 The reference ends every face with a graded CPML backed by the zero field
 (src/layout.rs:471-513, OUTER_ZERO_BLOCK in src/engine.rs:83).  Here a face
-can instead end after LAYER cells: the layer is the first LAYER cells
-(counted from the interior) of a THICK-cell CPML profile, and the fields just
-outside it -- the "ghost" values the boundary cell's curl update reads,
-zero in the reference -- are predicted each half step by a small per-face
-network from the 4-cell-deep E/H/psi stencil at the face and a two-step
-history (the C ABI's wf_set_learned_faces).
+can instead end after a LAYER-cell layer of plain cells, beyond which the
+fields -- the "ghost" values the boundary cell's curl update reads, zero in
+the reference -- are predicted each half step by a small per-face network
+from the 4-cell-deep E/H/psi stencil of the layer and a two-step history
+(the C ABI's wf_set_learned_faces).  With no CPML anywhere, the whole domain
+runs in the single-pass kernel.
 
-Training data come from the engine itself: a run of the same physics with
-the full THICK-cell CPML, whose first LAYER cells coincide with the thin
-layer, records at the virtual thin-domain faces the network inputs and the
-true ghost values (wf_record_face_samples).  A predictor that reproduced the
-ghosts exactly would make the thin run equal the thick run inside the thin
-domain (wf_replay_faces checks exactly that).  The network is trained on the
-GPU by the library's BF16-MFMA forward/backward kernels (wf_mlp_train).
+Training data come from the engine itself: the same physics run with a
+THICK-cell CPML around the thin domain (its twin) records at the virtual
+thin-domain faces the network inputs and the true ghost values, i.e. the
+fields of the CPML's first cells (wf_record_face_samples).  A predictor that
+reproduced the ghosts exactly would make the thin run equal the thick run
+inside the thin domain (wf_replay_faces checks exactly that).  The network
+is trained on the GPU by the library's BF16-MFMA forward/backward kernels
+(wf_mlp_train).
 """
 from __future__ import annotations
 
@@ -31,9 +32,9 @@ from .layout import BLOCK_DIM, GridDims, MaterialTable, PmlConfig, PmlContext, P
 from .source import FieldComponent as SC
 from .source import Source, Waveform
 
-THICK = 64           # the CPML the thin layer is cut from (voxels)
-LAYER = 4            # thin layer depth (voxels)
-OFF = THICK - LAYER  # thin-domain origin inside the thick run, per learned axis
+THICK = 64           # the twin's CPML (voxels)
+LAYER = 4            # the layer between the physical interior and a learned face (voxels)
+OFF = THICK          # thin-domain origin inside the twin, per learned axis: right after its CPML
 FACES = ("x_lo", "x_hi", "y_lo", "y_hi", "z_lo", "z_hi")
 STRIDE = 20          # floats per recorded sample
 f32 = np.float32
@@ -52,33 +53,8 @@ def thick_profile(n: int, dt: float, dx: float, cfg: PmlConfig | None = None) ->
     return PmlProfile1D.build(n, cfg, dt, dx, THICK)
 
 
-def thin_profile(n: int, dt: float, dx: float, cfg: PmlConfig | None = None) -> PmlProfile1D:
-    """The first LAYER cells (from the interior) of the thick profile of the
-    corresponding thick domain (n + 2*OFF cells), mirrored at both ends;
-    identity elsewhere."""
-    nb = n + 2 * OFF
-    big = thick_profile(nb, dt, dx, cfg)
-    b = np.ones(n, f32)
-    a = np.zeros(n, f32)
-    k = np.ones(n, f32)
-    for p in range(LAYER):
-        b[p], a[p], k[p] = big.b[p + OFF], big.a[p + OFF], big.inv_kappa[p + OFF]
-        q = nb - 1 - (p + OFF)
-        b[n - 1 - p], a[n - 1 - p], k[n - 1 - p] = big.b[q], big.a[q], big.inv_kappa[q]
-    return PmlProfile1D(b, a, k)
-
-
 def _axes_mask(periodic: int):
     return [not (periodic >> a & 1) for a in range(3)]
-
-
-def thin_context(dims: GridDims, dt: float, dx: float, periodic: int = 0) -> PmlContext:
-    """Shell of one block and the thin profile on every non-periodic axis."""
-    n = (dims.nx, dims.ny, dims.nz)
-    learned = _axes_mask(periodic)
-    profs = [thin_profile(n[a], dt, dx) if learned[a] else PmlProfile1D.identity(n[a])
-             for a in range(3)]
-    return PmlContext(profs, tuple(1 if learned[a] else 0 for a in range(3)))
 
 
 def thick_dims(dims: GridDims, periodic: int = 0) -> GridDims:
@@ -369,14 +345,13 @@ def train_face(u: np.ndarray, t: np.ndarray, seed: int = 0,
 def thin_engine(dims: GridDims, nets: FaceNets | None, dt: float, dx: float, periodic: int = 0,
                 ids=None, table=None, device: int = 0, mode: str = "learned",
                 replay: dict | None = None) -> DeviceEngine:
-    """A DeviceEngine of `dims` ending in the thin layer on every non-periodic
-    face.  mode: "learned" (predicted ghosts), "zero" (the thin layer with
-    the reference's zero boundary: the method without the network), or
-    "replay" (ghosts recorded by the thick twin, `replay` = record_run
-    output)."""
+    """A DeviceEngine of `dims` (no CPML) whose non-periodic faces end in
+    ghost planes.  mode: "learned" (predicted ghosts), "zero" (the
+    reference's zero boundary right behind the layer: the method without the
+    network), or "replay" (ghosts recorded by the thick twin, `replay` =
+    record_run output)."""
     table = table or MaterialTable.vacuum_filled(dt, dx)
-    eng = DeviceEngine(dims, ids, table, thin_context(dims, dt, dx, periodic), periodic=periodic,
-                       device=device)
+    eng = DeviceEngine(dims, ids, table, None, periodic=periodic, device=device)
     mask = face_mask(periodic)
     if mode == "learned":
         p = np.ascontiguousarray(nets.params, f32).reshape(-1)
@@ -632,10 +607,11 @@ def reflection_3d(nets: FaceNets, interior: int = 128, steps: int = 520, freq: f
     return {k: float(((traces[k] - ref) ** 2).sum()) / e_ref for k in kinds if k != "cpml64"}
 
 
-def throughput(kind: str, nets: FaceNets | None, interior: int = 256, steps: int = 100,
+def throughput(kind: str, nets: FaceNets | None, interior: int = 248, steps: int = 100,
                warmup: int = 16, device: int = 0):
     """Useful (interior) cell updates per second of a vacuum cube of
-    `interior` cells ending in the boundary `kind`, with a source running.
+    `interior` cells ending in the boundary `kind`, with a source running
+    (248: the learned layer's domain is 256^3, the 32-cell CPML's 312^3).
     Returns (Mcell/s of the interior, ms per step, cells in the domain)."""
     dx, dt = nets.dx, nets.dt
     eng, m = _boundary_engine(kind, interior, nets, dt, dx, device)
