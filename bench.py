@@ -195,10 +195,11 @@ def run_learned(args):
     networks trained on the GPU, then the reference's two-run reflection
     measurement (30 GHz, 20 points per wavelength, Courant 0.4) for the
     learned 4-cell layer, its linear skip alone, the same layer without the
-    network (zero boundary) and 32-cell CPML; then the same networks in a
-    3D vacuum cube (Ricker point source, 26 probes) after 520 and 1500
-    steps, and the step throughput of each boundary around a 248^3 interior
-    (256^3 with the learned layer)."""
+    network (zero boundary) and 32-cell CPML; then the same networks and the
+    3D-stable configuration (a linear predictor fitted to 3D data) in a 3D
+    vacuum cube (Ricker point source, 26 probes) after 520 and 1500 steps,
+    and the step throughput of each boundary around a 248^3 interior (256^3
+    with the learned layer)."""
     import numpy as np
 
     from wavefront_amd import learned as L
@@ -222,12 +223,24 @@ def run_learned(args):
     r_c32, _, _ = L.reflection("cpml", cpml_voxels=32, reference=ref)
     log(f"R learned {r_learned:.3e}, linear {r_lin:.3e}, no network {r_zero:.3e}, "
         f"CPML-8 {r_c8:.3e}, CPML-32 {r_c32:.3e}")
+    # 3D: the normal-incidence networks, and the 3D-stable configuration
+    # (linear skip on P and history, fitted to 3D point-source and
+    # plane-wave data of the 64-cell-CPML twin)
+    t0 = time.perf_counter()
+    S3 = L.collect_3d(GridDims(96, 96, 96), runs=6, steps=600, every=10, per_face=300000,
+                      seed=23, dt=dt, dx=dx, plane_runs=24)
+    nets3 = L.linear_all(S3, dt, dx)
+    t_data3 = time.perf_counter() - t0
+    r_3lin, _, _ = L.reflection("learned", nets3, reference=ref)
     r3 = {}
     for steps in (520, 1500):
         # a run that blew up is reported as None (non-finite)
-        r3[steps] = {k: (v if np.isfinite(v) else None)
-                     for k, v in L.reflection_3d(nets, steps=steps).items()}
+        fin = lambda v: v if np.isfinite(v) else None
+        r3[steps] = {k: fin(v) for k, v in L.reflection_3d(nets, steps=steps).items()}
+        r3[steps]["learned_3d_linear"] = fin(
+            L.reflection_3d(nets3, steps=steps, kinds=("learned", "cpml64"))["learned"])
         log(f"3D, {steps} steps: " + ", ".join(f"{k} {v}" for k, v in r3[steps].items()))
+    log(f"3D-stable configuration at normal incidence: R {r_3lin:.3e}")
     tp = {}
     for kind in ("learned", "zero", "cpml32"):
         mc, ms, cells = L.throughput(kind, nets, interior=248, steps=args.steps if args.steps < 300 else 100)
@@ -252,8 +265,12 @@ def run_learned(args):
                         "MLP 16-64-64-1 (hardtanh, odd) + linear skip, trained on 32 plane-wave runs "
                         "per axis from the 64-cell-CPML twin",
             "reflection": {"learned": r_learned, "linear_skip_only": r_lin,
-                           "no_network": r_zero, "cpml8": r_c8, "cpml32": r_c32},
+                           "no_network": r_zero, "cpml8": r_c8, "cpml32": r_c32,
+                           "learned_3d_linear": r_3lin},
             "reflection_3d_energy": {str(k): v for k, v in r3.items()},
+            "learned_3d_linear": "linear skip on P(depth 0..3) and history, least squares on "
+                                 "6 3D point-source runs + 24 plane-wave runs per axis",
+            "seconds_data_3d": round(t_data3, 2),
             "throughput_interior_248": tp,
             "seconds_data": round(t_data, 2),
             "seconds_training": round(t_train, 2),

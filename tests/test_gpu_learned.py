@@ -334,3 +334,22 @@ def test_one_way_head_gradient_and_mur_baseline():
     r, _, _ = L.reflection("learned", L.FaceNets(p0, dt, dx, L.HEAD_ONE_WAY))
     print(f"first-order one-way (Mur) boundary: R {r:.3e}")
     assert 1e-4 < r < 1e-3
+
+
+def test_3d_linear_configuration_is_stable():
+    """The 3D-stable configuration -- the linear predictor on P (depths 0..3)
+    and the history, fitted to 3D point-source and plane-wave data of the
+    64-cell-CPML twin -- in a 128^3 vacuum cube with a Ricker point source:
+    bounded after 1500 steps, with a reflected energy between the zero
+    boundary's and 32-cell CPML's (the zero boundary reflects everything)."""
+    _say("3D-stable linear configuration, 1500 steps")
+    dx, dt, _ = L.study_grid()
+    S = L.collect_3d(GridDims(96, 96, 96), runs=6, steps=600, every=10, per_face=300000,
+                     seed=23, dt=dt, dx=dx, plane_runs=24)
+    nets = L.linear_all(S, dt, dx)
+    assert not np.any(nets.params[:, :L.param_count() - 16])  # no MLP weights
+    r = L.reflection_3d(nets, steps=1500, kinds=("learned", "zero", "cpml32", "cpml64"))
+    print(f"3D energy ratio after 1500 steps: {r}")
+    assert np.isfinite(r["learned"])
+    assert r["cpml32"] < r["learned"] < 2e-2
+    assert r["learned"] < r["zero"] / 100
