@@ -274,6 +274,10 @@ int wf_mlp_set_params(wf_mlp_trainer *t, const float *params);
  * from sign-normalized inputs (a contractive recursion on the ghost). */
 #define WF_HEAD_DIRECT 0
 #define WF_HEAD_ONE_WAY 1
+/* WF_HEAD_DELTA -- the linear skip plus the network's output (on
+ * sign-normalized inputs) times the boundary field's step, ghost/s =
+ * skip(u) + m (P0(n) - P0(n-1)): the network adds nothing at zero frequency. */
+#define WF_HEAD_DELTA 2
 int wf_mlp_set_head(wf_mlp_trainer *t, int head);
 int wf_mlp_get_params(wf_mlp_trainer *t, float *params);
 /* Gradient of the loss at the current parameters (the network as it runs:

@@ -17,7 +17,8 @@ The hidden activation is hardtanh, clip(z, -1, 1), with derivative 1 inside
                   with respect to the FP32 master parameters (straight
                   through the bf16 rounding of the weights), checked against
                   central finite differences in the tests;
-  *_one_way_f64   the same for the one-way output head.
+  *_one_way_f64   the same for the one-way output head;
+  *_delta_f64     the same for the delta head.
 
 Parameter layout: W1[16][64] b1[64] W2[64][64] b2[64] W3[64] b3 wlin[16]
 (W[in][out]).
@@ -135,6 +136,56 @@ def grad_one_way_f64(p, u, t):
     dy = (pred - np.asarray(t, np.float64)) * (1.0 - th * th) * (u[:, 0] - u[:, 12])
     # grad_f64 differentiates mean((y - t')^2); choose t' = y - dy so its error is dy
     return grad_f64(p, ui, y - dy)
+
+
+def _weights(p, rounded):
+    if rounded:
+        return _gpu_weights(p)
+    w = split(np.asarray(p, np.float64))
+    w["b3"] = float(w["b3"][0])
+    return w
+
+
+def _mlp_f64(p, v, rounded=True):
+    """The MLP part alone (no linear skip): a2 @ W3 + b3."""
+    w = _weights(p, rounded)
+    a1 = act(v @ w["W1"] + w["b1"])
+    a2 = act(a1 @ w["W2"] + w["b2"])
+    return a2 @ w["W3"] + w["b3"], (a1, a2)
+
+
+def predict_delta_f64(p, u, rounded=True):
+    """Delta head: u @ wlin + m(sigma u) (u0 - u14), m = the MLP part.
+    rounded=False: unrounded weights (for finite differences)."""
+    u = np.asarray(u, np.float64)
+    w = _weights(p, rounded)
+    m, _ = _mlp_f64(p, u * one_way_sign(u)[:, None], rounded)
+    return u @ w["wlin"] + m * (u[:, 0] - u[:, 14])
+
+
+def loss_delta_f64(p, u, t, rounded=True):
+    return float(np.mean((predict_delta_f64(p, u, rounded) - np.asarray(t, np.float64)) ** 2))
+
+
+def grad_delta_f64(p, u, t, rounded=True):
+    """d loss / d p for the delta head: the skip's gradient from the error
+    itself, the MLP's from the error times (u0 - u14)."""
+    u = np.asarray(u, np.float64)
+    n = len(u)
+    w = _weights(p, rounded)
+    v = u * one_way_sign(u)[:, None]
+    m, (a1, a2) = _mlp_f64(p, v, rounded)
+    d = u[:, 0] - u[:, 14]
+    e = 2.0 * (u @ w["wlin"] + m * d - np.asarray(t, np.float64)) / n
+    gm = e * d
+    out = {"wlin": u.T @ e, "b3": np.array([gm.sum()]), "W3": a2.T @ gm}
+    dz2 = np.outer(gm, w["W3"]) * dact(a2)
+    out["W2"] = a1.T @ dz2
+    out["b2"] = dz2.sum(0)
+    dz1 = (dz2 @ w["W2"].T) * dact(a1)
+    out["W1"] = v.T @ dz1
+    out["b1"] = dz1.sum(0)
+    return join(out)
 
 
 def grad_f64(p, u, t):

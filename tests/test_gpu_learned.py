@@ -336,6 +336,27 @@ def test_one_way_head_gradient_and_mur_baseline():
     assert 1e-4 < r < 1e-3
 
 
+def test_delta_head_gradient():
+    """The delta head (skip + network x boundary step): GPU prediction and
+    gradient against the FP64 ones (the tolerances of the other heads)."""
+    _say("delta head")
+    rng = np.random.default_rng(9)
+    p = _params(rng).astype(np.float32)
+    u = rng.normal(0, 1, (3000, 16)).astype(np.float32)
+    t = rng.normal(0, 1, 3000).astype(np.float32)
+    tr = L.Trainer(u, t, head=L.HEAD_DELTA)
+    tr.set_params(p)
+    g, loss = tr.gradient()
+    y = tr.predict(u)
+    tr.close()
+    assert np.max(np.abs(y - mlp_ref.predict_delta_f64(p, u))) < 3e-2 * np.max(np.abs(y))
+    g64 = mlp_ref.grad_delta_f64(p, u, t)
+    assert abs(loss - mlp_ref.loss_delta_f64(p, u, t)) < 2e-2 * loss
+    rel = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+    print(f"delta gradient rel L2 {rel:.3e}")
+    assert rel < 3e-2
+
+
 def test_3d_linear_configuration_is_stable():
     """The 3D-stable configuration -- the linear predictor on P (depths 0..3)
     and the history, fitted to 3D point-source and plane-wave data of the

@@ -34,6 +34,32 @@ def test_gradient_formulas_match_finite_differences():
     np.testing.assert_allclose(g[idx], fd, rtol=2e-5, atol=1e-9)
 
 
+def test_delta_head_gradient_matches_finite_differences():
+    """The delta head's FP64 gradient (unrounded weights) against central
+    differences of its loss, over MLP weights, biases and the skip."""
+    rng = np.random.default_rng(5)
+    p = _params(rng)
+    u = rng.normal(0, 1, (40, 16))
+    t = rng.normal(0, 1, 40)
+    g = mlp_ref.grad_delta_f64(p, u, t, rounded=False)
+    idx = np.concatenate([rng.choice(mlp_ref.P, 60, replace=False),
+                          [mlp_ref.P - 17, mlp_ref.P - 1, mlp_ref.P - 18]])
+    h = 1e-6
+    fd = []
+    for i in idx:
+        q1, q2 = p.copy(), p.copy()
+        q1[i] += h
+        q2[i] -= h
+        fd.append((mlp_ref.loss_delta_f64(q1, u, t, False) - mlp_ref.loss_delta_f64(q2, u, t, False))
+                  / (2 * h))
+    np.testing.assert_allclose(g[idx], fd, rtol=2e-5, atol=1e-9)
+    # a static input (P0(n) = P0(n-1)) gets the skip alone
+    us = u.copy()
+    us[:, 14] = us[:, 0]
+    np.testing.assert_allclose(mlp_ref.predict_delta_f64(p, us, False),
+                               us @ mlp_ref.split(p)["wlin"], rtol=0, atol=1e-12)
+
+
 def test_bf16_rounding_and_emulation():
     x = np.array([1.0, 1.00390625, 1.005859375, -3.14159, 0.0], np.float32)
     # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: rounds to even (1.0)

@@ -1423,7 +1423,7 @@ int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params, int
 {
     clear_error();
     if (!c) return fail(WF_ERR_INVALID, "wf_set_learned_faces: null context");
-    if (head != kHeadDirect && head != kHeadOneWay)
+    if (head != kHeadDirect && head != kHeadOneWay && head != kHeadDelta)
         return fail(WF_ERR_INVALID, "wf_set_learned_faces: unknown head %d", head);
     if (face_mask >= 64u) return fail(WF_ERR_INVALID, "wf_set_learned_faces: face mask has bits above 5");
     if (face_mask && !params) return fail(WF_ERR_INVALID, "wf_set_learned_faces: null parameters");

@@ -61,8 +61,14 @@ struct MlpPacked {
 // inputs u14 + tanh(y) (u0 - u12)); |tanh y| < 1 keeps the recursion on the
 // ghost contractive.  The network then sees sign-normalized inputs (sign of
 // u0, or of u4 when u0 = 0), so y is even and the ghost odd in the fields.
+// Delta: the linear skip plus the MLP's output times the boundary field's
+// step, ghost / s = skip(u) + m(sigma u) (P0(n) - P0(n-1)) (scaled: u0 -
+// u14), with the MLP on sign-normalized inputs (even, so the product is odd
+// in the fields).  A static field gets the skip alone: the network cannot
+// add a response at zero frequency.
 constexpr int kHeadDirect = 0;
 constexpr int kHeadOneWay = 1;
+constexpr int kHeadDelta = 2;
 
 // One face hook: a face of the domain whose ghost plane is predicted
 // (mode 0), or a virtual face inside a thick-CPML run whose stencils and true

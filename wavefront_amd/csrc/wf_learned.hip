@@ -108,11 +108,15 @@ __device__ __forceinline__ float input_sign(fv4 u, int sl)
     return (u0 > 0.f || (u0 == 0.f && u4 >= 0.f)) ? 1.0f : -1.0f;
 }
 
-// Prediction in scaled units from the network output y (lanes 0..15):
-// u0, u12, u14 are the sample's scaled P0(n), g(n-1), P0(n-1).
-__device__ __forceinline__ float head_out(int head, float y, float u0, float u12, float u14)
+// Prediction in scaled units from the network's MLP output m and linear
+// skip p (lanes 0..15): u0, u12, u14 are the sample's scaled P0(n), g(n-1),
+// P0(n-1).
+__device__ __forceinline__ float head_out(int head, float m, float p, float u0, float u12,
+                                          float u14)
 {
-    return head == kHeadDirect ? y : u14 + tanh_f(y) * (u0 - u12);
+    if (head == kHeadDirect) return m + p;
+    if (head == kHeadOneWay) return u14 + tanh_f(m + p) * (u0 - u12);
+    return p + m * (u0 - u14);  // kHeadDelta
 }
 
 struct Acts {
@@ -120,10 +124,11 @@ struct Acts {
     bfv4 a1b[4], a2b[4];
 };
 
-// Forward pass for the wave's 16 samples.  ub/uf: this lane's 4 inputs
-// (features 4(l/16)..+3 of sample l%16).  Returns the prediction, valid on
-// lanes 0..15 (sample = lane).
-__device__ __forceinline__ float mlp_forward(const MlpRegs &r, bfv4 ub, fv4 uf, Acts &t)
+// Forward pass for the wave's 16 samples.  ub: this lane's 4 MLP inputs in
+// bf16, uf: its 4 linear-skip inputs (features 4(l/16)..+3 of sample l%16).
+// Returns the MLP output and sets p to the skip, valid on lanes 0..15
+// (sample = lane).
+__device__ __forceinline__ float mlp_forward(const MlpRegs &r, bfv4 ub, fv4 uf, Acts &t, float &p)
 {
     const fv4 zero = {0.f, 0.f, 0.f, 0.f};
     // biases enter as the accumulator of the first product of each layer
@@ -145,11 +150,20 @@ __device__ __forceinline__ float mlp_forward(const MlpRegs &r, bfv4 ub, fv4 uf, 
 #pragma unroll
     for (int k = 0; k < 4; ++k) z3 = mfma(r.A3[k], t.a2b[k], z3);
     // linear skip, summed over the four lanes that hold one sample
-    float p = r.wl.x * uf.x + r.wl.y * uf.y + r.wl.z * uf.z + r.wl.w * uf.w;
+    p = r.wl.x * uf.x + r.wl.y * uf.y + r.wl.z * uf.z + r.wl.w * uf.w;
     p = p + __shfl_xor(p, 16);
     p = p + __shfl_xor(p, 32);
-    return (z3.x + r.b3) + p;
+    return z3.x + r.b3;
 }
+
+// MLP input of a sample: the scaled inputs, sign-normalized for the heads
+// whose network must be even (one-way, delta); skip input: sign-normalized
+// for the one-way head only (its skip is inside the tanh).
+__device__ __forceinline__ fv4 mlp_in(int head, fv4 u, int sl)
+{
+    return head == kHeadDirect ? u : u * input_sign(u, sl);
+}
+__device__ __forceinline__ fv4 skip_in(int head, fv4 u, fv4 ui) { return head == kHeadOneWay ? ui : u; }
 
 // FP32 parameters -> per-lane MFMA operands.  One wave.
 __global__ void k_pack_mlp(const float *__restrict__ p, int head, MlpPacked *__restrict__ w)
@@ -213,10 +227,11 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ u
         const int s = grp * 16 + sl;
         fv4 uf = {0.f, 0.f, 0.f, 0.f};
         if (s < n) uf = *reinterpret_cast<const fv4 *>(u + (size_t)s * kMlpIn + rb);
-        const fv4 ui = r.head == kHeadDirect ? uf : uf * input_sign(uf, sl);
+        const fv4 ui = mlp_in(r.head, uf, sl);
         Acts t;
-        const float y = mlp_forward(r, to_bf4(ui), ui, t);
-        const float p = head_out(r.head, y, uf.x, __shfl(uf.x, sl + 48), __shfl(uf.z, sl + 48));
+        float sk;
+        const float m = mlp_forward(r, to_bf4(ui), skip_in(r.head, uf, ui), t, sk);
+        const float p = head_out(r.head, m, sk, uf.x, __shfl(uf.x, sl + 48), __shfl(uf.z, sl + 48));
         if (lane < 16 && s < n) out[s] = p;
     }
 }
@@ -297,22 +312,27 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float *__restrict__ u,
             uf = *reinterpret_cast<const fv4 *>(u + (size_t)s * kMlpIn + rb);
             t = tgt[s];
         }
-        const fv4 ui = r.head == kHeadDirect ? uf : uf * input_sign(uf, sl);
+        const fv4 ui = mlp_in(r.head, uf, sl);
+        const fv4 us = skip_in(r.head, uf, ui);
         const bfv4 ub = to_bf4(ui);
         Acts a;
-        const float y = mlp_forward(r, ub, ui, a);
+        float sk;
+        const float m = mlp_forward(r, ub, us, a, sk);
         const float u12 = __shfl(uf.x, sl + 48), u14 = __shfl(uf.z, sl + 48);
-        const float pred = head_out(r.head, y, uf.x, u12, u14);
+        const float pred = head_out(r.head, m, sk, uf.x, u12, u14);
         const float res = (lane < 16 && live) ? pred - t : 0.f;
         loss = loss + res * res;
         float g = 2.0f * res * inv_n;  // dL/dpred on lanes 0..15
-        if (r.head != kHeadDirect) {
-            const float th = tanh_f(y);
-            g = g * (1.0f - th * th) * (uf.x - u12);  // dL/dy
+        float gm = g, gp = g;           // dL/dm, dL/dp
+        if (r.head == kHeadOneWay) {
+            const float th = tanh_f(m + sk);
+            gm = gp = g * (1.0f - th * th) * (uf.x - u12);
+        } else if (r.head == kHeadDelta) {
+            gm = g * (uf.x - u14);
         }
-        const float gs = __shfl(g, sl);  // broadcast to the sample's four lanes
-        if (lane < 16) gb3 = gb3 + g;
-        gwl = gwl + ui * gs;
+        const float gs = __shfl(gm, sl);  // broadcast to the sample's four lanes
+        if (lane < 16) gb3 = gb3 + gm;
+        gwl = gwl + us * __shfl(gp, sl);
         // layer 3 (rank one): dW3 = a2 g, da2 = W3 g
         fv4 dz2[4];
         bfv4 dz2b[4];
@@ -604,10 +624,11 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
         float gnew;
         if (h.mode == 0) {
             Acts t;
-            const fv4 ui = r.head == kHeadDirect ? u : u * input_sign(u, sl);
-            const float y = mlp_forward(r, to_bf4(ui), ui, t);
+            const fv4 ui = mlp_in(r.head, u, sl);
+            float sk;
+            const float m = mlp_forward(r, to_bf4(ui), skip_in(r.head, u, ui), t, sk);
             const float pred =
-                head_out(r.head, y, u.x, __shfl(u.x, sl + 48), __shfl(u.z, sl + 48));
+                head_out(r.head, m, sk, u.x, __shfl(u.x, sl + 48), __shfl(u.z, sl + 48));
             gnew = s * pred;
             if (lane < 16 && live) (pol == 0 ? h.ghost[0] : h.ghost[1])[pos] = gnew;
         } else {
@@ -810,7 +831,8 @@ int wf_mlp_set_head(wf_mlp_trainer *t, int head)
 {
     clear_error();
     if (!t) return fail(WF_ERR_INVALID, "wf_mlp_set_head: null trainer");
-    if (head != kHeadDirect && head != kHeadOneWay) return fail(WF_ERR_INVALID, "wf_mlp_set_head: unknown head %d", head);
+    if (head != kHeadDirect && head != kHeadOneWay && head != kHeadDelta)
+        return fail(WF_ERR_INVALID, "wf_mlp_set_head: unknown head %d", head);
     WF_HIP(hipSetDevice(t->device));
     t->head = head;
     hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, t->stream, t->d_p, t->head, t->d_w);

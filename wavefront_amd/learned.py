@@ -157,7 +157,7 @@ def _gauss(freq):
 # ---------------------------------------------------------------------------
 # Network and training
 # ---------------------------------------------------------------------------
-HEAD_DIRECT, HEAD_ONE_WAY = 0, 1
+HEAD_DIRECT, HEAD_ONE_WAY, HEAD_DELTA = 0, 1, 2
 
 
 @dataclass
@@ -320,7 +320,7 @@ def train_face_one_way(u: np.ndarray, t: np.ndarray, dt: float, dx: float, seed:
 
 def train_face(u: np.ndarray, t: np.ndarray, seed: int = 0,
                schedule=((2000, 1e-4), (1000, 3e-5)), lr_skip: float = 1e-5,
-               features=FEATURES, device: int = 0, log=None):
+               features=FEATURES, device: int = 0, log=None, head: int = HEAD_DIRECT):
     """Linear skip from least squares, then Adam on the GPU: the MLP at the
     scheduled rates, the skip at lr_skip, biases held at zero so the network
     stays an odd function of its inputs (a linear field update maps -x to
@@ -332,7 +332,7 @@ def train_face(u: np.ndarray, t: np.ndarray, seed: int = 0,
         off = np.ones(16, bool)
         off[list(features)] = False
         u[:, off] = 0.0
-    tr = Trainer(u, t, device)
+    tr = Trainer(u, t, device, head=head)
     tr.set_params(init_params(rng, least_squares_skip(u, t, features), features))
     hist = [(0, tr.gradient()[1])]
     done = 0
