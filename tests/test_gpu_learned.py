@@ -240,6 +240,43 @@ def test_replayed_ghosts_reproduce_thick_run(monkeypatch, kernel):
     assert np.array_equal(ft, ref)
 
 
+@pytest.mark.parametrize("overlap", ["1", "0"])
+def test_zero_ghosts_in_a_cpml_domain_equal_the_zero_boundary(monkeypatch, overlap):
+    """Learned faces in a domain that also has a CPML run in the frame
+    schedule (hooks beside the interior kernel, or serialized): replaying
+    all-zero ghost planes on every face must give exactly the run without
+    hooks, whose boundary is the reference's zero field."""
+    monkeypatch.setenv("WF_HOOK_OVERLAP", overlap)
+    _say(f"zero ghosts in a CPML domain, hook overlap {overlap}")
+    dims = GridDims(64, 48, 56)
+    dx, dt, _ = L.study_grid()
+    K = 30
+    from wavefront_amd.layout import PmlConfig, PmlContext, PmlProfile1D
+    n = (dims.nx, dims.ny, dims.nz)
+    ctx = PmlContext([PmlProfile1D.build(n[a], PmlConfig(thickness=16), dt, dx, 16)
+                      for a in range(3)], (2, 2, 2))
+    table = MaterialTable.vacuum_filled(dt, dx)
+    srcs = _point_sources((30, 20, 28))
+    with DeviceEngine(dims, None, table, ctx) as eng:
+        eng.set_sources(srcs, dt, K)
+        eng.step(K)
+        ref = eng.read_fields()
+    zeros = {}
+    for f in range(6):
+        a = f // 2
+        t1, t2 = (a + 1) % 3, (a + 2) % 3
+        zeros[L.FACES[f]] = np.zeros((K, 2, n[min(t1, t2)] * n[max(t1, t2)], L.STRIDE), np.float32)
+    from ctypes import POINTER, c_float
+    arr = (POINTER(c_float) * 6)(*[L.ptr(zeros[L.FACES[f]], c_float) for f in range(6)])
+    with DeviceEngine(dims, None, table, ctx) as eng:
+        L.check(L.lib().wf_replay_faces(eng.handle, 63, K, arr))
+        eng.set_sources(srcs, dt, K)
+        eng.step(K)
+        got = eng.read_fields()
+    assert np.abs(ref).max() > 0
+    assert np.array_equal(got, ref)
+
+
 def _train_x_faces(log):
     """Training data: 32 plane-wave runs along x (periodic y, z) with soft
     Gaussian / Ricker sheet sources at random positions and frequencies
