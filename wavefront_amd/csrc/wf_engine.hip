@@ -382,7 +382,7 @@ void plan_boxes(wf_ctx *c)
         if (hi[a] < lo[a]) hi[a] = lo[a];
     }
     const int zc = c->zchunk;
-    const int fr = fused_rows_per_tile();
+    const int fr = fused_rows_per_tile(false), frx = fused_rows_per_tile(true);
     Box in{lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], 64, 0, 0, 0, 0};
     const bool empty = hi[0] <= lo[0] || hi[1] <= lo[1] || hi[2] <= lo[2];
     if (empty) {
@@ -410,7 +410,7 @@ void plan_boxes(wf_ctx *c)
     c->frame = f;
     Box all{0, n[0], 0, n[1], 0, n[2], 64, 0, 0, 0, 0};
     all.gx = (n[0] + 255) / 256;
-    all.gy = (n[1] + fr - 1) / fr;
+    all.gy = (n[1] + frx - 1) / frx;  // swept by the CPML or ghost variant
     all.gz = (n[2] + zc - 1) / zc;
     c->all = all;
     BoxList w{};
@@ -436,6 +436,13 @@ int run_hooks(wf_ctx *c, bool high, const Ptrs &es, const Ptrs &hs, hipStream_t 
         if (c->hook_on[f]) on[n++] = &c->hook[f];
     if (n) WF_HIP(launch_face_hooks(c->g, es, hs, on, n, c->d_step, st));
     return WF_OK;
+}
+
+// The interior box never touches a face: its launch needs no ghost reads.
+StepLaunch plain(StepLaunch s)
+{
+    s.ghost = false;
+    return s;
 }
 
 // The hooks may run beside the interior kernel when every one of them reads
@@ -517,7 +524,7 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
         WF_HIP(c->lh(s, c->frame, c->side));
         if (int rc = run_hooks(c, false, c->p, c->alt, c->side)) return rc;
         WF_HIP(hipEventRecord(c->join, c->side));
-        WF_HIP(c->lf(s, c->interior, false, st));
+        WF_HIP(c->lf(plain(s), c->interior, false, st));
         WF_HIP(hipStreamWaitEvent(st, c->join, 0));
         WF_HIP(c->le(s, c->frame, st));
         std::swap(c->p, c->alt);
@@ -532,11 +539,11 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
             WF_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
             WF_HIP(c->lh(s, c->frame, c->side));
             WF_HIP(hipEventRecord(c->join, c->side));
-            WF_HIP(c->lf(s, c->interior, false, st));
+            WF_HIP(c->lf(plain(s), c->interior, false, st));
             WF_HIP(hipStreamWaitEvent(st, c->join, 0));
         } else {
             if (ev) WF_HIP(hipEventRecord(ev[0], st));
-            WF_HIP(c->lf(s, c->interior, false, st));
+            WF_HIP(c->lf(plain(s), c->interior, false, st));
             if (ev) WF_HIP(hipEventRecord(ev[1], st));
             if (ev) WF_HIP(hipEventRecord(ev[2], st));
             WF_HIP(c->lh(s, c->frame, st));

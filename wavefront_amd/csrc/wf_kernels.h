@@ -163,9 +163,10 @@ hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 // the high faces comes from the learned ghost planes (with pml_cells false);
 // the low faces' boundary E is left for the E kernel to redo.
 hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st);
-// Tile shapes of the two kernels (owned rows per fused tile, rows per split
-// workgroup of lpr = 64).
-int fused_rows_per_tile();
+// Tile shapes of the two kernels (owned rows per fused tile -- of the plain
+// variant, or of the CPML / ghost variants -- and rows per split workgroup
+// of lpr = 64).
+int fused_rows_per_tile(bool pml_or_ghost);
 int split_rows_per_wg();
 
 // One scene of a batch: its geometry and state (updated in place by the

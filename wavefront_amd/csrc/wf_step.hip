@@ -442,10 +442,18 @@ __global__ __launch_bounds__(64 * kSY) void k_e4_batch(const BatchScene *__restr
 // H(z-1) ride in registers; H at y-1 crosses waves through a double-buffered
 // LDS plane behind an LDS-only barrier, so stores are never waited for.
 // ===========================================================================
+// Waves per workgroup (owned rows + the halo row).  Plain variant: 12 -- at
+// 164 VGPRs three waves fit per SIMD, so one 12-wave workgroup fills a CU
+// and the halo row is 1/12 of the work and of the reads (8 waves, 1/8,
+// measured 1-2% slower per launch at 1024^3 on the same allocation).  The
+// CPML and ghost variants need more registers and keep 8.
 #ifndef WF_FUSED_FY
-#define WF_FUSED_FY 8
+#define WF_FUSED_FY 12
 #endif
 constexpr int kFY = WF_FUSED_FY;
+constexpr int kFYx = 8;
+template <bool PML, bool GHOST>
+constexpr int fused_waves() { return (PML || GHOST) ? kFYx : kFY; }
 
 // CPML of one derivative of a cell (quad) in the PML shell on axis `ax`:
 // graded -> the one-tap recursion with psi read from `pin` and, when
@@ -475,13 +483,15 @@ __device__ __forceinline__ float pml_d1(float raw, bool gr, float pb, float pa, 
 // kernel runs, so their boundary cells' E is redone afterwards by the E
 // kernel (see the engine's learned-face schedule).
 template <bool PML, bool GHOST>
-__global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box box, int zchunk)
+__global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo g, Ptrs a, Ptrs b,
+                                                                         Box box, int zchunk)
 {
-    __shared__ float2 s_ce[256];       // (ca, cb)
-    __shared__ float2 s_ch[256];       // (da, db)
-    __shared__ f4 s_h[2][2][kFY][64];  // [buf][hz, hx][wave][lane]
+    constexpr int FY = fused_waves<PML, GHOST>();
+    __shared__ float2 s_ce[256];      // (ca, cb)
+    __shared__ float2 s_ch[256];      // (da, db)
+    __shared__ f4 s_h[2][2][FY][64];  // [buf][hz, hx][wave][lane]
     const int lane = threadIdx.x, w = threadIdx.y;
-    for (int i = w * 64 + lane; i < 256; i += 64 * kFY) {
+    for (int i = w * 64 + lane; i < 256; i += 64 * FY) {
         const float4 c = a.coef[i];
         s_ce[i] = make_float2(c.x, c.y);
         s_ch[i] = make_float2(c.z, c.w);
@@ -491,7 +501,7 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
     const int t = xcd_tile(blockIdx.x, box.gx * box.gy * box.gz);
     const int tx = t % box.gx, ty = (t / box.gx) % box.gy, tz = t / (box.gx * box.gy);
     const int x = box.x0 + tx * 256 + lane * 4;
-    const int yr = box.y0 + ty * (kFY - 1) - 1 + w;
+    const int yr = box.y0 + ty * (FY - 1) - 1 + w;
     int y = yr;
     bool live = x < box.x1 && yr < box.y1;
     if (yr < 0) {
@@ -790,7 +800,7 @@ __global__ __launch_bounds__(64 * kFY) void k_fused4(Geo g, Ptrs a, Ptrs b, Box 
 }  // namespace
 
 #ifndef WF_STEP_FMA
-int fused_rows_per_tile() { return kFY - 1; }
+int fused_rows_per_tile(bool pml_or_ghost) { return (pml_or_ghost ? kFYx : kFY) - 1; }
 int split_rows_per_wg() { return kSY; }
 #endif
 
@@ -849,10 +859,10 @@ hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hip
     if (n <= 0) return hipSuccess;
     if (pml_cells && s.ghost) return hipErrorInvalidValue;  // not a supported combination
     if (pml_cells)
-        hipLaunchKernelGGL((k_fused4<true, false>), dim3((unsigned)n), dim3(64, kFY), 0, st, s.g,
+        hipLaunchKernelGGL((k_fused4<true, false>), dim3((unsigned)n), dim3(64, kFYx), 0, st, s.g,
                            s.cur, s.alt, box, s.zchunk);
     else if (s.ghost)
-        hipLaunchKernelGGL((k_fused4<false, true>), dim3((unsigned)n), dim3(64, kFY), 0, st, s.g,
+        hipLaunchKernelGGL((k_fused4<false, true>), dim3((unsigned)n), dim3(64, kFYx), 0, st, s.g,
                            s.cur, s.alt, box, s.zchunk);
     else
         hipLaunchKernelGGL((k_fused4<false, false>), dim3((unsigned)n), dim3(64, kFY), 0, st, s.g,
