@@ -193,6 +193,17 @@ uint64_t wf_step_count(const wf_ctx *ctx);
 uint64_t wf_field_bytes(const wf_ctx *ctx);
 int wf_read_fields(wf_ctx *ctx, void *dst);
 int wf_write_fields(wf_ctx *ctx, const void *src);
+/* Asynchronous snapshot (the reference's serialize_snapshot into a buffer
+ * that is written while the next steps run, src/engine.rs:622-640,
+ * 785-793): packs the current state on the device, in stream order after the
+ * steps queued so far, and copies it to dst (wf_field_bytes bytes; pinned
+ * memory, e.g. hipHostMalloc, lets the copy overlap the following steps)
+ * on a side stream.  dst must not be read or reused before wf_wait.  A
+ * second call waits on the device for the first copy to finish. */
+int wf_snapshot_async(wf_ctx *ctx, void *dst);
+/* Waits for the pending wf_snapshot_async copy (the reference's
+ * wait-before-reuse of a snapshot buffer). */
+int wf_wait(wf_ctx *ctx);
 
 /* The whole timestep loop with snapshot egress; replaces engine::run
  * (src/engine.rs:653-819).  Runs num_steps steps; after the step with loop

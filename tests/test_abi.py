@@ -84,6 +84,13 @@ def test_run_argument_errors():
     assert lib.wf_step(None, 1) == _lib.WF_ERR_INVALID
 
 
+def test_snapshot_async_argument_errors():
+    lib = _lib.lib()
+    assert lib.wf_snapshot_async(None, None) == _lib.WF_ERR_INVALID
+    assert "null" in _lib.last_error()
+    assert lib.wf_wait(None) == _lib.WF_ERR_INVALID
+
+
 def test_batch_argument_errors():
     from ctypes import byref, c_void_p
     lib = _lib.lib()

@@ -233,6 +233,33 @@ def test_write_read_roundtrip():
     eng.close()
 
 
+def test_snapshot_async_holds_the_state_at_the_call():
+    """wf_snapshot_async: the buffer receives the state after the steps
+    queued before the call, even though more steps are queued before
+    wf_wait (a second snapshot reuses the device buffer only after the first
+    copy has left it); both equal wf_read_fields at those steps."""
+    w = workloads.plain_64()
+    _say("snapshot async")
+    eng = DeviceEngine(w.dims, w.ids, w.table, w.pml())
+    eng.set_sources(w.sources, w.dt, 60)
+    eng.step(20)
+    ref20 = eng.read_fields()
+    a = np.empty_like(ref20)
+    b = np.empty_like(ref20)
+    eng.snapshot_async(a)
+    eng.step(17)
+    eng.snapshot_async(b)
+    eng.step(5)
+    eng.wait()
+    assert_bitwise(a, ref20, "snapshot at 20")
+    ref42 = eng.read_fields()
+    assert not np.array_equal(ref42, ref20)
+    sim = oracle_sim_for(w)
+    sim.run(37, w.dt, threads=8)
+    assert_bitwise(b, sim.read_fields(), "snapshot at 37 vs oracle")
+    eng.close()
+
+
 def test_wave_arrives_at_c():
     """src/engine.rs:841-932: Ricker 30 GHz at the centre of a 64^3 vacuum
     grid, no PML; |Ez| at +20 cells first exceeds 5e-4 within 20% of 20*dx/c."""

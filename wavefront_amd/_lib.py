@@ -81,6 +81,8 @@ _SIGNATURES = {
     "wf_step": (c_int, [c_void_p, c_uint64]),
     "wf_sync": (c_int, [c_void_p]),
     "wf_set_fp_contract": (c_int, [c_void_p, c_int]),
+    "wf_snapshot_async": (c_int, [c_void_p, c_void_p]),
+    "wf_wait": (c_int, [c_void_p]),
     "wf_step_count": (c_uint64, [c_void_p]),
     "wf_field_bytes": (c_uint64, [c_void_p]),
     "wf_read_fields": (c_int, [c_void_p, c_void_p]),

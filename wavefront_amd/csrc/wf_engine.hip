@@ -257,6 +257,12 @@ struct wf_ctx {
     float *d_stage = nullptr;
     uint64_t stage_bytes = 0;
 
+    // wf_snapshot_async: a device copy of the packed state and its copy-out
+    float *d_snap = nullptr;
+    hipStream_t snap_stream = nullptr;
+    hipEvent_t snap_packed = nullptr, snap_done = nullptr;
+    bool snap_pending = false;
+
     // learned boundary: face hooks (predict or record) and their buffers,
     // per face x_lo, x_hi, y_lo, y_hi, z_lo, z_hi
     bool hook_on[6] = {false, false, false, false, false, false};
@@ -880,6 +886,11 @@ void wf_destroy(wf_ctx *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->snap_stream) (void)hipStreamSynchronize(c->snap_stream);
+    dfree(c->d_snap);
+    if (c->snap_packed) (void)hipEventDestroy(c->snap_packed);
+    if (c->snap_done) (void)hipEventDestroy(c->snap_done);
+    if (c->snap_stream) (void)hipStreamDestroy(c->snap_stream);
     free_sources(c);
     free_probes(c);
     dfree(c->d_fields);
@@ -1054,6 +1065,41 @@ int wf_read_fields(wf_ctx *c, void *dst)
                               hipMemcpyDeviceToHost, c->stream));
         WF_HIP(hipStreamSynchronize(c->stream));
     }
+    return WF_OK;
+}
+
+int wf_snapshot_async(wf_ctx *c, void *dst)
+{
+    clear_error();
+    if (!c || !dst) return fail(WF_ERR_INVALID, "wf_snapshot_async: null pointer");
+    WF_HIP(hipSetDevice(c->device));
+    const uint64_t bytes = (uint64_t)c->g.nx * c->g.ny * c->g.nz * 24;
+    if (!c->d_snap) {
+        if (int rc = dmalloc(&c->d_snap, bytes)) return rc;
+        if (hipStreamCreateWithFlags(&c->snap_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->snap_packed, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming) != hipSuccess)
+            return fail(WF_ERR_HIP, "wf_snapshot_async: stream or event creation failed");
+    }
+    // the previous snapshot must have left the device buffer before it is packed again
+    if (c->snap_pending) WF_HIP(hipStreamWaitEvent(c->stream, c->snap_done, 0));
+    if (int rc = pack_snapshot(c, c->d_snap)) return rc;
+    WF_HIP(hipEventRecord(c->snap_packed, c->stream));
+    WF_HIP(hipStreamWaitEvent(c->snap_stream, c->snap_packed, 0));
+    WF_HIP(hipMemcpyAsync(dst, c->d_snap, bytes, hipMemcpyDeviceToHost, c->snap_stream));
+    WF_HIP(hipEventRecord(c->snap_done, c->snap_stream));
+    c->snap_pending = true;
+    return WF_OK;
+}
+
+int wf_wait(wf_ctx *c)
+{
+    clear_error();
+    if (!c) return fail(WF_ERR_INVALID, "wf_wait: null context");
+    if (!c->snap_pending) return WF_OK;
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipEventSynchronize(c->snap_done));
+    c->snap_pending = false;
     return WF_OK;
 }
 

@@ -174,6 +174,18 @@ class DeviceEngine:
         check(lib().wf_read_fields(self._h, out.ctypes.data_as(c_void_p)))
         return out
 
+    def snapshot_async(self, buf: np.ndarray) -> None:
+        """Starts copying the current state (snapshot byte format) into
+        `buf` (wf_field_bytes bytes, C-contiguous); wait() before reading it."""
+        if not buf.flags.c_contiguous or buf.nbytes != int(lib().wf_field_bytes(self._h)):
+            raise ValueError("snapshot buffer must be C-contiguous and wf_field_bytes long")
+        check(lib().wf_snapshot_async(self._h, buf.ctypes.data_as(c_void_p)))
+        self._snap_buf = buf  # keep it alive until wait()
+
+    def wait(self) -> None:
+        check(lib().wf_wait(self._h))
+        self._snap_buf = None
+
     def write_fields(self, fields: np.ndarray) -> None:
         f = np.ascontiguousarray(fields, dtype=np.float32)
         if f.size != self.dims.voxel_count() * 6:
