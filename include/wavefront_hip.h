@@ -6,7 +6,7 @@
  * Source::inject (src/source.rs:79-95) per source + Probe::accumulate
  * (src/probe.rs:126-138) per probe, plus the snapshot serializer
  * (src/engine.rs:622-640).  The reference has no FFI of its own; these are the
- * entry points a Rust `-sys` crate binds (see INTEGRATION.md).
+ * entry points a Rust `-sys` crate binds.
  *
  * Conventions: extern "C" only; no exceptions cross the boundary; every
  * function returning int returns WF_OK (0) or a negative WF_ERR_* code and
