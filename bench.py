@@ -334,6 +334,9 @@ def main():
     log(f"scene {w.name}: {w.notes['materials']} materials, voxelized in "
         f"{time.perf_counter() - t0:.1f}s")
     eng = DeviceEngine(w.dims, w.ids, w.table, w.pml())
+    placement = eng.placement()
+    log(f"field placements tried (ms per step): {placement['ms_per_step']}, kept "
+        f"{placement['kept']}")
     eng.set_sources(w.sources, w.dt, total)
     eng.set_probes(w.probes, w.dt, total)
     log("engine ready; warmup")
@@ -408,6 +411,7 @@ def main():
                         f"sinusoid Ez source, 1 probe",
             "cells": cells,
             "kernel_variant": variant,
+            "field_placement": placement,
             "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
             "kernel_bytes": kbytes,
             "bytes_per_step_algorithmic": step_bytes,

@@ -369,6 +369,16 @@ int wf_kernel_bytes(const wf_ctx *ctx, uint64_t *fused_bytes, uint64_t *h_bytes,
  * over the whole domain, in place); chosen at wf_create from the WF_KERNEL
  * environment variable (WF_ZCHUNK sets the z planes per workgroup). */
 const char *wf_kernel_variant(const wf_ctx *ctx);
+/* Field placements timed by wf_create: the step time (ms) of each candidate
+ * allocation of the field buffers, in the order tried (up to `cap` written),
+ * their number in *n (0 when no probe ran) and the index of the one kept in
+ * *kept.  No counterpart in the reference: on MI355X one allocation of the
+ * field arrays runs every step kernel at one of two speeds (about 10% apart
+ * at 1024^3), depending on where the driver placed its pages, so for grids
+ * whose fields take >= 8 GiB per state wf_create times a few steps on four
+ * allocations (two held at a time) and keeps the fastest (WF_PLACEMENT=n
+ * tries n, 1 = off, at most 6; WF_PLACEMENT_MIN_MIB moves the threshold). */
+int wf_placement(const wf_ctx *ctx, double *ms, int cap, int *n, int *kept);
 /* Copies `bytes` (read + write = 2*bytes of HBM traffic) iters times with a
  * plain 16-byte-per-lane copy kernel on `device`; writes the mean ms per
  * copy.  No counterpart in the reference: the measured HBM ceiling and the

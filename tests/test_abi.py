@@ -91,6 +91,14 @@ def test_snapshot_async_argument_errors():
     assert lib.wf_wait(None) == _lib.WF_ERR_INVALID
 
 
+def test_placement_argument_errors():
+    from ctypes import byref, c_int
+    lib = _lib.lib()
+    n, kept = c_int(), c_int()
+    assert lib.wf_placement(None, None, 0, byref(n), byref(kept)) == _lib.WF_ERR_INVALID
+    assert "null" in _lib.last_error()
+
+
 def test_batch_argument_errors():
     from ctypes import byref, c_void_p
     lib = _lib.lib()

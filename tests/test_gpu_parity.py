@@ -141,6 +141,44 @@ def test_zchunk_invariance(monkeypatch):
     assert_bitwise(outs[0], outs[2], "zchunk 1 vs 96")
 
 
+def test_field_placement_probe_leaves_the_initial_state(monkeypatch):
+    """The field-placement probe of wf_create, forced on a small grid with
+    three candidates: each is timed, the fastest is kept, and the engine then
+    starts from the initial state (zero fields and psi, no negative zeros,
+    step 0), so 40 steps of plain_64 with its sources and probes stay
+    bit-identical to the CPU engine."""
+    monkeypatch.setenv("WF_PLACEMENT", "3")
+    monkeypatch.setenv("WF_PLACEMENT_MIN_MIB", "0")
+    w = workloads.plain_64()
+    steps = 40
+    _say(f"placement probe plain_64 steps={steps}")
+    eng = DeviceEngine(w.dims, w.ids, w.table, w.pml())
+    pl = eng.placement()
+    assert len(pl["ms_per_step"]) == 3 and all(t > 0 for t in pl["ms_per_step"])
+    assert pl["ms_per_step"][pl["kept"]] == min(pl["ms_per_step"])
+    f0, p0 = eng.read_fields(), eng.read_psi()
+    assert not f0.any() and not np.signbit(f0).any()
+    assert not p0.any() and not np.signbit(p0).any()
+    eng.set_sources(w.sources, w.dt, steps)
+    eng.set_probes(w.probes, w.dt, steps)
+    eng.step(steps)
+    eng.sync()
+    sim = oracle_sim_for(w)
+    sim.run(steps, w.dt, threads=4)
+    assert_bitwise(eng.read_fields(), sim.read_fields(), "fields")
+    assert_bitwise(eng.read_psi(), sim.read_psi(), "psi")
+    sc, ss, n = eng.probe_sums()
+    oc, os_, on = sim.probe_sums()
+    assert_bitwise(sc, oc, "sum_cos")
+    assert_bitwise(ss, os_, "sum_sin")
+    assert list(n) == list(on) == [steps] * 6
+    eng.close()
+    monkeypatch.setenv("WF_PLACEMENT", "1")
+    eng = DeviceEngine(w.dims, w.ids, w.table, w.pml())
+    assert eng.placement() == {"ms_per_step": [], "kept": 0}
+    eng.close()
+
+
 def test_sources_order_soft_and_hard():
     """Several sources on one voxel: applied in list order (src/engine.rs:770)."""
     from wavefront_amd.source import FieldComponent as SC, Source, Waveform

@@ -124,6 +124,7 @@ _SIGNATURES = {
     "wf_kernel_bytes": (c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64),
                                 POINTER(c_uint64)]),
     "wf_kernel_variant": (c_char_p, [c_void_p]),
+    "wf_placement": (c_int, [c_void_p, POINTER(c_double), c_int, POINTER(c_int), POINTER(c_int)]),
     "wf_hbm_copy": (c_int, [c_uint64, c_int, c_int, POINTER(c_double)]),
 }
 

@@ -230,7 +230,12 @@ struct wf_ctx {
     float4 *d_coef = nullptr;
     float *d_prof = nullptr;
     float *d_psi = nullptr;
+    size_t psi_bytes = 0;       // each of d_psi, d_psi2
     uint64_t nshell_blocks = 0;
+
+    // field placements timed at wf_create (ms per step) and the one kept
+    double place_ms[6] = {};
+    int place_n = 0, place_kept = 0;
     uint64_t *d_shell_list = nullptr;
 
     // sources
@@ -592,6 +597,111 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
     return WF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Field placement.  On MI355X the same kernels over the same 24 GiB field
+// arrays run in one of two speeds, fixed for the life of an allocation
+// (1024^3 with CPML 24: about 14.0 or 12.7 ms per step, every kernel moving
+// with it): it follows where the driver puts the pages physically, not the
+// virtual layout.  For large grids wf_create therefore times a few steps on
+// the zeroed state, allocates a second set of field buffers while holding
+// the first (which lands elsewhere), times it, keeps the faster set and
+// frees the other, for each further candidate.  The state is zeroed again
+// and the step counter reset afterwards.
+// ---------------------------------------------------------------------------
+constexpr int kPlaceMax = 6;
+
+// Milliseconds per step over `timed` steps after `warm` untimed ones (both
+// even, so the A/B states end where they started).  No sources, probes or
+// hooks exist yet; the fields stay zero.
+int probe_step_ms(wf_ctx *c, int warm, int timed, double *ms)
+{
+    hipEvent_t e0, e1;
+    WF_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(WF_ERR_HIP, "placement probe: event creation failed");
+    }
+    int rc = WF_OK;
+    for (int s = 0; s < warm && rc == WF_OK; ++s) rc = launch_step(c);
+    if (rc == WF_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(WF_ERR_HIP, "placement probe: event");
+    for (int s = 0; s < timed && rc == WF_OK; ++s) rc = launch_step(c);
+    float t = 0.f;
+    if (rc == WF_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+        rc = fail(WF_ERR_HIP, "placement probe: timing failed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / timed;
+    return rc;
+}
+
+void point_fields(wf_ctx *c, uint64_t N)
+{
+    for (int i = 0; i < 6; ++i) {
+        c->p.f[i] = c->d_fields + (uint64_t)i * N;
+        c->alt.f[i] = c->d_fields2 ? c->d_fields2 + (uint64_t)i * N : nullptr;
+    }
+}
+
+// Tries up to `tries` placements of the field buffers and keeps the fastest.
+// A candidate that cannot be allocated ends the search (the current set
+// stays); HIP errors while timing are returned.
+int choose_placement(wf_ctx *c, int tries, uint64_t N)
+{
+    const size_t bytes = 6 * N * sizeof(float);
+    c->place_n = 0;
+    c->place_kept = 0;
+    double best = 0.0;
+    if (int rc = probe_step_ms(c, 2, 4, &best)) return rc;
+    c->place_ms[c->place_n++] = best;
+    for (int t = 1; t < tries && t < kPlaceMax; ++t) {
+        float *f1 = nullptr, *f2 = nullptr;
+        if (hipMalloc(reinterpret_cast<void **>(&f1), bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            break;
+        }
+        if (c->d_fields2 && hipMalloc(reinterpret_cast<void **>(&f2), bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipFree(f1);
+            break;
+        }
+        float *o1 = c->d_fields, *o2 = c->d_fields2;
+        c->d_fields = f1;
+        c->d_fields2 = f2;
+        point_fields(c, N);
+        int rc = WF_OK;
+        if (hipMemsetAsync(f1, 0, bytes, c->stream) != hipSuccess ||
+            (f2 && hipMemsetAsync(f2, 0, bytes, c->stream) != hipSuccess))
+            rc = fail(WF_ERR_HIP, "placement probe: hipMemset failed");
+        double ms = 0.0;
+        if (rc == WF_OK) rc = probe_step_ms(c, 2, 4, &ms);
+        if (rc == WF_OK) c->place_ms[c->place_n++] = ms;
+        if (rc == WF_OK && ms < best) {
+            best = ms;
+            c->place_kept = c->place_n - 1;
+            (void)hipFree(o1);
+            if (o2) (void)hipFree(o2);
+        } else {
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipFree(f1);
+            if (f2) (void)hipFree(f2);
+            c->d_fields = o1;
+            c->d_fields2 = o2;
+            point_fields(c, N);
+        }
+        if (rc) return rc;
+    }
+    // Back to the initial state: zero fields and psi, step 0.
+    WF_HIP(hipMemsetAsync(c->d_fields, 0, bytes, c->stream));
+    if (c->d_fields2) WF_HIP(hipMemsetAsync(c->d_fields2, 0, bytes, c->stream));
+    if (c->d_psi) WF_HIP(hipMemsetAsync(c->d_psi, 0, c->psi_bytes, c->stream));
+    if (c->d_psi2) WF_HIP(hipMemsetAsync(c->d_psi2, 0, c->psi_bytes, c->stream));
+    WF_HIP(hipMemsetAsync(c->d_step, 0, sizeof(uint64_t), c->stream));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    c->step = 0;
+    return WF_OK;
+}
+
 constexpr int kGraphSteps = 8;
 
 // Captures kGraphSteps consecutive steps (an even number, so the fused
@@ -810,6 +920,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     }
     uint64_t slab[3] = {2ull * g.T[0] * ny * nz, 2ull * g.T[1] * nx * nz, 2ull * g.T[2] * nx * ny};
     const uint64_t psi_total = 4 * (slab[0] + slab[1] + slab[2]);
+    c->psi_bytes = psi_total * sizeof(float);
     if (psi_total) {
         if (int rc = dmalloc(&c->d_psi, psi_total * sizeof(float))) return bail(rc);
         if (hipMemsetAsync(c->d_psi, 0, psi_total * sizeof(float), c->stream) != hipSuccess)
@@ -877,7 +988,30 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
             c->lf = fma::launch_fused;
         }
     plan_boxes(c);
+    // Field placement (see choose_placement): by default four candidates
+    // (at most two allocated at a time) when one state's fields take at least
+    // 8 GiB.  WF_PLACEMENT=n tries n (1 = no probe, at most 6);
+    // WF_PLACEMENT_MIN_MIB moves the size threshold.
+    {
+        int tries = 4;
+        uint64_t min_bytes = 8ull << 30;
+        if (const char *pl = std::getenv("WF_PLACEMENT")) tries = std::atoi(pl);
+        if (const char *mb = std::getenv("WF_PLACEMENT_MIN_MIB"))
+            min_bytes = (uint64_t)std::max(0L, std::atol(mb)) << 20;
+        if (tries > 1 && 6 * N * sizeof(float) >= min_bytes)
+            if (int rc = choose_placement(c, std::min(tries, kPlaceMax), N)) return bail(rc);
+    }
     *out = c;
+    return WF_OK;
+}
+
+int wf_placement(const wf_ctx *c, double *ms, int cap, int *n, int *kept)
+{
+    clear_error();
+    if (!c || !n || !kept || (cap > 0 && !ms)) return fail(WF_ERR_INVALID, "wf_placement: null pointer");
+    *n = c->place_n;
+    *kept = c->place_kept;
+    for (int i = 0; i < c->place_n && i < cap; ++i) ms[i] = c->place_ms[i];
     return WF_OK;
 }
 

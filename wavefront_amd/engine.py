@@ -167,6 +167,14 @@ class DeviceEngine:
     def kernel_variant(self) -> str:
         return lib().wf_kernel_variant(self._h).decode()
 
+    def placement(self) -> dict:
+        """Field placements timed at creation: ms per step of each candidate
+        allocation (empty when no probe ran) and the index of the one kept."""
+        ms = (c_double * 6)()
+        n, kept = c_int(), c_int()
+        check(lib().wf_placement(self._h, ms, 6, byref(n), byref(kept)))
+        return {"ms_per_step": [round(ms[i], 4) for i in range(n.value)], "kept": kept.value}
+
     # -- state transfer --------------------------------------------------------
     def read_fields(self) -> np.ndarray:
         """Field state as the reference snapshot layout (blocks, 6, 512) f32."""
