@@ -333,10 +333,12 @@ def main():
     w = workloads.random_1024(steps=total, n=args.n, pml=args.pml)
     log(f"scene {w.name}: {w.notes['materials']} materials, voxelized in "
         f"{time.perf_counter() - t0:.1f}s")
+    t_create = time.perf_counter()
     eng = DeviceEngine(w.dims, w.ids, w.table, w.pml())
     placement = eng.placement()
+    placement["create_s"] = round(time.perf_counter() - t_create, 3)
     log(f"field placements tried (ms per step): {placement['ms_per_step']}, kept "
-        f"{placement['kept']}")
+        f"{placement['kept']}; engine created in {placement['create_s']} s")
     eng.set_sources(w.sources, w.dt, total)
     eng.set_probes(w.probes, w.dt, total)
     log("engine ready; warmup")

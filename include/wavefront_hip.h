@@ -369,15 +369,18 @@ int wf_kernel_bytes(const wf_ctx *ctx, uint64_t *fused_bytes, uint64_t *h_bytes,
  * over the whole domain, in place); chosen at wf_create from the WF_KERNEL
  * environment variable (WF_ZCHUNK sets the z planes per workgroup). */
 const char *wf_kernel_variant(const wf_ctx *ctx);
-/* Field placements timed by wf_create: the step time (ms) of each candidate
- * allocation of the field buffers, in the order tried (up to `cap` written),
- * their number in *n (0 when no probe ran) and the index of the one kept in
- * *kept.  No counterpart in the reference: on MI355X one allocation of the
- * field arrays runs every step kernel at one of two speeds (about 10% apart
- * at 1024^3), depending on where the driver placed its pages, so for grids
- * whose fields take >= 8 GiB per state wf_create times a few steps on four
- * allocations (two held at a time) and keeps the fastest (WF_PLACEMENT=n
- * tries n, 1 = off, at most 6; WF_PLACEMENT_MIN_MIB moves the threshold). */
+/* Field placement chosen by wf_create: the step time (ms) of each
+ * arrangement it timed, in order (up to `cap` written; the initial buffers
+ * first, then each extra state buffer in place of state A and, in the fused
+ * variant, of state B), their number in *n (0 when no probe ran) and the
+ * index of the one kept in *kept.  No counterpart in the reference: on
+ * MI355X the step kernels over one allocation of a 24 GiB state buffer run
+ * up to ~12% faster or slower than over another, depending on where the
+ * driver placed its pages, so for grids whose fields take >= 8 GiB per state
+ * wf_create tries eight extra state buffers (one held at a time, four
+ * steps timed per arrangement: about 1 s at 1024^3) and keeps the fastest
+ * arrangement (WF_PLACEMENT=n tries n, 0 = off, at most 8;
+ * WF_PLACEMENT_MIN_MIB moves the threshold). */
 int wf_placement(const wf_ctx *ctx, double *ms, int cap, int *n, int *kept);
 /* Copies `bytes` (read + write = 2*bytes of HBM traffic) iters times with a
  * plain 16-byte-per-lane copy kernel on `device`; writes the mean ms per

@@ -234,7 +234,7 @@ struct wf_ctx {
     uint64_t nshell_blocks = 0;
 
     // field placements timed at wf_create (ms per step) and the one kept
-    double place_ms[6] = {};
+    double place_ms[17] = {};
     int place_n = 0, place_kept = 0;
     uint64_t *d_shell_list = nullptr;
 
@@ -598,17 +598,20 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
 }
 
 // ---------------------------------------------------------------------------
-// Field placement.  On MI355X the same kernels over the same 24 GiB field
-// arrays run in one of two speeds, fixed for the life of an allocation
-// (1024^3 with CPML 24: about 14.0 or 12.7 ms per step, every kernel moving
-// with it): it follows where the driver puts the pages physically, not the
-// virtual layout.  For large grids wf_create therefore times a few steps on
-// the zeroed state, allocates a second set of field buffers while holding
-// the first (which lands elsewhere), times it, keeps the faster set and
-// frees the other, for each further candidate.  The state is zeroed again
-// and the step counter reset afterwards.
+// Field placement.  On MI355X the same kernels over the same 24 GiB state
+// buffer run at speeds up to ~12% apart depending on the allocation, fixed
+// for its life (1024^3 with CPML 24: the fused kernel reading one buffer
+// and writing the other takes 8.2 to 9.6 ms, and A->B differs from B->A):
+// it follows where the driver puts the pages physically, not the virtual
+// layout.  For large grids wf_create therefore times a few steps on the
+// zeroed state, then, for each further candidate, allocates one more state
+// buffer while holding the current ones (so it lands elsewhere), times it in
+// place of each state in turn, keeps the fastest arrangement and frees the
+// buffer left over.  The state is zeroed again and the step counter reset
+// afterwards.
 // ---------------------------------------------------------------------------
-constexpr int kPlaceMax = 6;
+constexpr int kPlaceMax = 8;       // extra state buffers tried
+constexpr int kPlaceProbes = 17;   // step timings recorded (1 + 2 per extra buffer at most)
 
 // Milliseconds per step over `timed` steps after `warm` untimed ones (both
 // even, so the A/B states end where they started).  No sources, probes or
@@ -643,51 +646,51 @@ void point_fields(wf_ctx *c, uint64_t N)
     }
 }
 
-// Tries up to `tries` placements of the field buffers and keeps the fastest.
-// A candidate that cannot be allocated ends the search (the current set
-// stays); HIP errors while timing are returned.
-int choose_placement(wf_ctx *c, int tries, uint64_t N)
+// Tries `extra` more state buffers (at most kPlaceMax) and keeps the fastest
+// arrangement.  A buffer that cannot be allocated ends the search (the
+// current arrangement stays); HIP errors are returned.
+int choose_placement(wf_ctx *c, int extra, uint64_t N)
 {
     const size_t bytes = 6 * N * sizeof(float);
+    float **slot[2] = {&c->d_fields, &c->d_fields2};
+    const int nslot = c->d_fields2 ? 2 : 1;
     c->place_n = 0;
     c->place_kept = 0;
     double best = 0.0;
-    if (int rc = probe_step_ms(c, 2, 4, &best)) return rc;
+    if (int rc = probe_step_ms(c, 2, 2, &best)) return rc;
     c->place_ms[c->place_n++] = best;
-    for (int t = 1; t < tries && t < kPlaceMax; ++t) {
-        float *f1 = nullptr, *f2 = nullptr;
-        if (hipMalloc(reinterpret_cast<void **>(&f1), bytes) != hipSuccess) {
+    for (int t = 0; t < extra && t < kPlaceMax && c->place_n + nslot <= kPlaceProbes; ++t) {
+        float *x = nullptr;
+        if (hipMalloc(reinterpret_cast<void **>(&x), bytes) != hipSuccess) {
             (void)hipGetLastError();
             break;
         }
-        if (c->d_fields2 && hipMalloc(reinterpret_cast<void **>(&f2), bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            (void)hipFree(f1);
-            break;
-        }
-        float *o1 = c->d_fields, *o2 = c->d_fields2;
-        c->d_fields = f1;
-        c->d_fields2 = f2;
-        point_fields(c, N);
-        int rc = WF_OK;
-        if (hipMemsetAsync(f1, 0, bytes, c->stream) != hipSuccess ||
-            (f2 && hipMemsetAsync(f2, 0, bytes, c->stream) != hipSuccess))
+        int rc = WF_OK, win = -1;
+        if (hipMemsetAsync(x, 0, bytes, c->stream) != hipSuccess)
             rc = fail(WF_ERR_HIP, "placement probe: hipMemset failed");
-        double ms = 0.0;
-        if (rc == WF_OK) rc = probe_step_ms(c, 2, 4, &ms);
-        if (rc == WF_OK) c->place_ms[c->place_n++] = ms;
-        if (rc == WF_OK && ms < best) {
-            best = ms;
-            c->place_kept = c->place_n - 1;
-            (void)hipFree(o1);
-            if (o2) (void)hipFree(o2);
-        } else {
-            (void)hipStreamSynchronize(c->stream);
-            (void)hipFree(f1);
-            if (f2) (void)hipFree(f2);
-            c->d_fields = o1;
-            c->d_fields2 = o2;
+        for (int k = 0; k < nslot && rc == WF_OK; ++k) {
+            float *held = *slot[k];
+            *slot[k] = x;
             point_fields(c, N);
+            double ms = 0.0;
+            rc = probe_step_ms(c, 2, 2, &ms);
+            *slot[k] = held;
+            point_fields(c, N);
+            if (rc) break;
+            c->place_ms[c->place_n++] = ms;
+            if (ms < best) {
+                best = ms;
+                win = k;
+                c->place_kept = c->place_n - 1;
+            }
+        }
+        (void)hipStreamSynchronize(c->stream);
+        if (win >= 0) {
+            (void)hipFree(*slot[win]);
+            *slot[win] = x;
+            point_fields(c, N);
+        } else {
+            (void)hipFree(x);
         }
         if (rc) return rc;
     }
@@ -988,17 +991,18 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
             c->lf = fma::launch_fused;
         }
     plan_boxes(c);
-    // Field placement (see choose_placement): by default four candidates
-    // (at most two allocated at a time) when one state's fields take at least
-    // 8 GiB.  WF_PLACEMENT=n tries n (1 = no probe, at most 6);
-    // WF_PLACEMENT_MIN_MIB moves the size threshold.
+    // Field placement (see choose_placement): by default eight extra state
+    // buffers tried, one held at a time beside the current ones, when one
+    // state's fields take at least 8 GiB (4 steps per arrangement timed, 17
+    // arrangements: about 1 s at 1024^3).  WF_PLACEMENT=n tries n (0 = no
+    // probe, at most 8); WF_PLACEMENT_MIN_MIB moves the size threshold.
     {
-        int tries = 4;
+        int tries = 8;
         uint64_t min_bytes = 8ull << 30;
         if (const char *pl = std::getenv("WF_PLACEMENT")) tries = std::atoi(pl);
         if (const char *mb = std::getenv("WF_PLACEMENT_MIN_MIB"))
             min_bytes = (uint64_t)std::max(0L, std::atol(mb)) << 20;
-        if (tries > 1 && 6 * N * sizeof(float) >= min_bytes)
+        if (tries > 0 && 6 * N * sizeof(float) >= min_bytes)
             if (int rc = choose_placement(c, std::min(tries, kPlaceMax), N)) return bail(rc);
     }
     *out = c;

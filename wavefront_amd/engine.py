@@ -168,11 +168,12 @@ class DeviceEngine:
         return lib().wf_kernel_variant(self._h).decode()
 
     def placement(self) -> dict:
-        """Field placements timed at creation: ms per step of each candidate
-        allocation (empty when no probe ran) and the index of the one kept."""
-        ms = (c_double * 6)()
+        """Field placement chosen at creation: ms per step of each arrangement
+        timed (initial buffers, then each extra state buffer as state A and as
+        state B; empty when no probe ran) and the index of the one kept."""
+        ms = (c_double * 17)()
         n, kept = c_int(), c_int()
-        check(lib().wf_placement(self._h, ms, 6, byref(n), byref(kept)))
+        check(lib().wf_placement(self._h, ms, 17, byref(n), byref(kept)))
         return {"ms_per_step": [round(ms[i], 4) for i in range(n.value)], "kept": kept.value}
 
     # -- state transfer --------------------------------------------------------
