@@ -128,17 +128,20 @@ def test_random_fields_plain(periodic):
 
 
 def test_zchunk_invariance(monkeypatch):
-    """The z-chunking of the launch grid must not change any value."""
+    """The z-chunking of the launch grids (interior and frame) must not change
+    any value."""
     d = GridDims(64, 32, 96)
     outs = []
-    for zc in ("1", "7", "96"):
+    for zc, fz in (("1", "5"), ("7", "1"), ("96", "96")):
         monkeypatch.setenv("WF_ZCHUNK", zc)
-        _say(f"zchunk={zc}")
+        monkeypatch.setenv("WF_FRAME_ZCHUNK", fz)
+        _say(f"zchunk={zc} frame zchunk={fz}")
         eng, sim = _random_case(d, (1, 1, 2), 0, steps=3, seed=5)
         outs.append(eng.read_fields())
+        assert_bitwise(outs[-1], sim.read_fields(), f"zchunk {zc}/{fz} vs the CPU engine")
         eng.close()
-    assert_bitwise(outs[0], outs[1], "zchunk 1 vs 7")
-    assert_bitwise(outs[0], outs[2], "zchunk 1 vs 96")
+    assert_bitwise(outs[0], outs[1], "zchunk 1/5 vs 7/1")
+    assert_bitwise(outs[0], outs[2], "zchunk 1/5 vs 96/96")
 
 
 def test_field_placement_probe_leaves_the_initial_state(monkeypatch):

@@ -610,7 +610,7 @@ __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo
                 P.ex_py = ld4(a.ghost[3][1] + (size_t)z * nx + xc);
             }
             P.hx_o = ld4(ahx + zo + row);
-            P.hy_o = ld4(ahy + zo + row);
+            if (w >= 1) P.hy_o = ld4(ahy + zo + row);  // the halo row needs hx, hz only
             P.hz_o = ld4(ahz + zo + row);
             P.ids = *reinterpret_cast<const uint32_t *>(a.id + zo + row);
             if (edge && xq >= 0) {
