@@ -211,8 +211,10 @@ struct wf_ctx {
     Ptrs alt{};   // second state for the fused kernel (written, then swapped)
     bool pml = false;
     bool fused = true;
-    int zchunk = 32;    // planes per workgroup: fused interior, split variant, point boxes
+    int zchunk = 32;    // planes per workgroup: split variant, low-face fix-up boxes
     int fzchunk = 1;    // planes per workgroup of the frame kernels (fused variant)
+    bool zchunk_env = false;  // WF_ZCHUNK given: the fused kernel takes it too
+    int cus = 256;      // compute units of the device
     Box interior{};     // fused variant: cells swept by the single-pass kernel
     BoxList frame{};    // fused variant: the rest (holds the whole CPML shell)
     Box all{};          // fused variant with a CPML: the whole domain, one pass
@@ -371,6 +373,30 @@ void add_box(BoxList &bl, const Box &b)
 // cells, the cells from T+1 (T+4 on x, to keep quads aligned) to n-T, so
 // that neither its cells nor its one-cell halos (x-1, y-1, z-1) are in the
 // shell; the rest of the domain becomes up to six frame boxes.
+// Number of z slices of a box for the fused kernel, which runs one
+// workgroup per CU at a time (12 waves at 164 VGPRs, or 8 at up to 232):
+// the launch takes about ceil(tiles * slices / CUs) rounds of (planes per
+// slice + the plane before it), so pick the slice count that minimises that
+// product (ties: more slices).  At 1024^3 (356 tiles, 975 planes, 256 CUs)
+// that is 5 slices of 195 planes, 8.09 ms per launch against 8.17 ms for
+// 31 slices of <= 32 planes (whose last round is a tenth full); at 312^3
+// with a 32-cell CPML 11 slices, 0.179 against 0.187 ms.
+int fused_slices(long tiles, int planes, int cus)
+{
+    if (planes <= 0 || tiles <= 0) return 1;
+    long best_cost = -1;
+    int best = 1;
+    for (int gz = 1; gz <= planes; ++gz) {
+        const long rounds = (tiles * gz + cus - 1) / cus;
+        const long cost = rounds * ((planes + gz - 1) / gz + 1);
+        if (best_cost < 0 || cost <= best_cost) {
+            best_cost = cost;
+            best = gz;
+        }
+    }
+    return best;
+}
+
 void plan_boxes(wf_ctx *c)
 {
     const Geo &g = c->g;
@@ -395,6 +421,9 @@ void plan_boxes(wf_ctx *c)
     }
     const int zc = c->zchunk, fz = c->fzchunk;
     const int fr = fused_rows_per_tile(false), frx = fused_rows_per_tile(true);
+    auto slices = [&](long tiles, int planes) {
+        return c->zchunk_env ? (planes + zc - 1) / zc : fused_slices(tiles, planes, c->cus);
+    };
     Box in{lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], 64, 0, 0, 0, 0};
     const bool empty = hi[0] <= lo[0] || hi[1] <= lo[1] || hi[2] <= lo[2];
     if (empty) {
@@ -405,7 +434,7 @@ void plan_boxes(wf_ctx *c)
     } else {
         in.gx = (hi[0] - lo[0] + 255) / 256;
         in.gy = (hi[1] - lo[1] + fr - 1) / fr;
-        in.gz = (hi[2] - lo[2] + zc - 1) / zc;
+        in.gz = slices((long)in.gx * in.gy, hi[2] - lo[2]);
     }
     c->interior = in;
     BoxList f{};
@@ -423,7 +452,7 @@ void plan_boxes(wf_ctx *c)
     Box all{0, n[0], 0, n[1], 0, n[2], 64, 0, 0, 0, 0};
     all.gx = (n[0] + 255) / 256;
     all.gy = (n[1] + frx - 1) / frx;  // swept by the CPML or ghost variant
-    all.gz = (n[2] + zc - 1) / zc;
+    all.gz = slices((long)all.gx * all.gy, n[2]);
     c->all = all;
     BoxList w{};
     add_box(w, split_box(0, n[0], 0, n[1], 0, n[2], zc));
@@ -985,7 +1014,15 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     }
     if (const char *zc = std::getenv("WF_ZCHUNK")) {
         const int v = std::atoi(zc);
-        if (v > 0) c->zchunk = v;
+        if (v > 0) {
+            c->zchunk = v;
+            c->zchunk_env = true;
+        }
+    }
+    {
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
+            c->cus = cu;
     }
     // The frame kernels (thin slabs around the interior) run fastest with
     // one plane per workgroup: at 1024^3 with CPML 24 the frame's H + E take

@@ -157,11 +157,13 @@ hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 // E kernel over the boxes: reads H(n+1/2) from alt and E^n / psi from cur;
 // writes E(n+1) and the E-phase psi to alt.
 hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
-// Single-pass kernel over a box, cur -> alt.  pml_cells: the box may hold
-// CPML shell cells (the kernel then updates their psi too); false selects
-// the plain variant for a box wholly outside the shell.  s.ghost: E beyond
-// the high faces comes from the learned ghost planes (with pml_cells false);
-// the low faces' boundary E is left for the E kernel to redo.
+// Single-pass kernel over a box, cur -> alt; tile (tx, ty, tz) marches the
+// tz-th of box.gz even slices of the box's planes (s.zchunk is not used).
+// pml_cells: the box may hold CPML shell cells (the kernel then updates
+// their psi too); false selects the plain variant for a box wholly outside
+// the shell.  s.ghost: E beyond the high faces comes from the learned ghost
+// planes (with pml_cells false); the low faces' boundary E is left for the
+// E kernel to redo.
 hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st);
 // Tile shapes of the two kernels (owned rows per fused tile -- of the plain
 // variant, or of the CPML / ghost variants -- and rows per split workgroup

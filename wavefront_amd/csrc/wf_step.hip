@@ -484,7 +484,7 @@ __device__ __forceinline__ float pml_d1(float raw, bool gr, float pb, float pa, 
 // kernel (see the engine's learned-face schedule).
 template <bool PML, bool GHOST>
 __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo g, Ptrs a, Ptrs b,
-                                                                         Box box, int zchunk)
+                                                                         Box box)
 {
     constexpr int FY = fused_waves<PML, GHOST>();
     __shared__ float2 s_ce[256];      // (ca, cb)
@@ -565,7 +565,9 @@ __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo
     const float *__restrict__ ahy = a.f[4];
     const float *__restrict__ ahz = a.f[5];
 
-    const int z0 = box.z0 + tz * zchunk, z1 = min(z0 + zchunk, box.z1);
+    // the box's planes split evenly into gz chunks (sizes differ by <= 1)
+    const long nzb = box.z1 - box.z0;
+    const int z0 = box.z0 + (int)(tz * nzb / box.gz), z1 = box.z0 + (int)((tz + 1) * nzb / box.gz);
     const int zpre = minus1(z0, g.nz, g.periodic & 4u);
     const int it0 = zpre >= 0 ? -1 : 0;
     const int nit = z1 - z0;
@@ -860,13 +862,13 @@ hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hip
     if (pml_cells && s.ghost) return hipErrorInvalidValue;  // not a supported combination
     if (pml_cells)
         hipLaunchKernelGGL((k_fused4<true, false>), dim3((unsigned)n), dim3(64, kFYx), 0, st, s.g,
-                           s.cur, s.alt, box, s.zchunk);
+                           s.cur, s.alt, box);
     else if (s.ghost)
         hipLaunchKernelGGL((k_fused4<false, true>), dim3((unsigned)n), dim3(64, kFYx), 0, st, s.g,
-                           s.cur, s.alt, box, s.zchunk);
+                           s.cur, s.alt, box);
     else
         hipLaunchKernelGGL((k_fused4<false, false>), dim3((unsigned)n), dim3(64, kFY), 0, st, s.g,
-                           s.cur, s.alt, box, s.zchunk);
+                           s.cur, s.alt, box);
     return hipGetLastError();
 }
 
