@@ -288,7 +288,7 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=10)
     ap.add_argument("--pml", type=int, default=20,
                     help="requested CPML thickness in voxels (the workload uses 20)")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-gib", type=float, default=2.0,
