@@ -207,6 +207,43 @@ def test_sources_order_soft_and_hard():
     eng.close()
 
 
+def test_many_sources_with_repeated_voxels():
+    """3000 sources (sheets, scattered points and voxels named several times
+    in the list, soft and hard mixed, the three E components) applied in
+    parallel per voxel: identical to the oracle's list-order loop."""
+    from wavefront_amd.source import FieldComponent as SC, Source, Waveform
+    rng = np.random.default_rng(21)
+    d = GridDims(32, 24, 40)
+    dt, dx = 1.5e-12, 1e-3
+    comps = (SC.Ex, SC.Ey, SC.Ez)
+    forms = (Waveform.sinusoid(3e10), Waveform.ricker_wavelet(2e10, 5e-11),
+             Waveform.gaussian_pulse(2e-11, 1e-11))
+    srcs = []
+    for y in range(d.ny):  # an Ez sheet at z = 9
+        for x in range(d.nx):
+            srcs.append(Source(x, y, 9, SC.Ez, 0.5, forms[0]))
+    pts = [(int(rng.integers(d.nx)), int(rng.integers(d.ny)), int(rng.integers(d.nz)))
+           for _ in range(300)]
+    while len(srcs) < 3000:
+        x, y, z = pts[int(rng.integers(len(pts)))] if rng.random() < 0.7 else (
+            int(rng.integers(d.nx)), 5, int(rng.integers(d.nz)))
+        srcs.append(Source(x, y, z, comps[int(rng.integers(3))], float(rng.uniform(-2, 2)),
+                           forms[int(rng.integers(3))], hard=bool(rng.random() < 0.2)))
+    order = rng.permutation(len(srcs))  # interleave the sheet with the rest
+    srcs = [srcs[i] for i in order]
+    _say(f"many_sources n={len(srcs)}")
+    eng = DeviceEngine(d, None, None, None, dt=dt, dx=dx)
+    eng.set_sources(srcs, dt, 10)
+    eng.step(10)
+    sim = wfo.OracleSim(d.nx, d.ny, d.nz, table=wfo.vacuum_table(dt, dx))
+    for s in srcs:
+        sim.add_source(s.x, s.y, s.z, int(s.component), s.amplitude, s.waveform.kind,
+                       s.waveform.p0, s.waveform.p1, hard=s.hard)
+    sim.run(10, dt)
+    assert_bitwise(eng.read_fields(), sim.read_fields(), "fields")
+    eng.close()
+
+
 def test_scene_spheres_128():
     """The two-sphere scene scaled to 128^3 (lossy sphere, CPML 16, Ricker)."""
     w = workloads.spheres_512(steps=60, n=128)

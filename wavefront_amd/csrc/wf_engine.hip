@@ -43,30 +43,41 @@ __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src,
 }
 
 // ---------------------------------------------------------------------------
-// Sources (src/source.rs:79-95): applied in list order by one lane.
+// Sources (src/source.rs:79-95).  The reference applies them in list order;
+// only sources on the same field value can see each other, so the table is
+// kept grouped by (component, voxel), each group in list order, and one lane
+// applies one group: the same result as the sequential loop.
 // ---------------------------------------------------------------------------
 // `row` is the device step counter minus the step at which the table was
 // set, so a captured graph of these launches stays valid for any step.
+__device__ __forceinline__ void apply_source(const Ptrs &p, const DevSource &q,
+                                             const float *__restrict__ v)
+{
+    float *f = p.f[q.comp] + q.off;
+    if (q.hard)
+        *f = v[q.col];
+    else
+        *f = *f + v[q.col];
+}
+
 __device__ __forceinline__ void apply_sources(const Ptrs &p, const DevSource *__restrict__ src,
                                               int n, const float *__restrict__ values,
                                               uint64_t row)
 {
     const float *v = values + row * (uint64_t)n;
-    for (int s = 0; s < n; ++s) {
-        float *f = p.f[src[s].comp] + src[s].off;
-        if (src[s].hard)
-            *f = v[s];
-        else
-            *f = *f + v[s];
-    }
+    for (int s = 0; s < n; ++s) apply_source(p, src[s], v);
 }
 
+// One lane per group [grp[i], grp[i+1]) of the grouped table.
 __global__ void k_sources(Ptrs p, const DevSource *__restrict__ src, int n,
+                          const uint32_t *__restrict__ grp, int ngrp,
                           const float *__restrict__ values, const uint64_t *__restrict__ step,
                           uint64_t base)
 {
-    if (threadIdx.x != 0) return;
-    apply_sources(p, src, n, values, *step - base);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ngrp) return;
+    const float *v = values + (*step - base) * (uint64_t)n;
+    for (uint32_t s = grp[i]; s < grp[i + 1]; ++s) apply_source(p, src[s], v);
 }
 
 // ---------------------------------------------------------------------------
@@ -243,7 +254,9 @@ struct wf_ctx {
 
     // sources
     uint32_t nsrc = 0;
-    DevSource *d_src = nullptr;
+    DevSource *d_src = nullptr;     // grouped by (component, voxel), list order within a group
+    uint32_t *d_src_grp = nullptr;  // group starts, ngrp + 1 entries
+    uint32_t nsrc_grp = 0;
     float *d_srcval = nullptr;
     uint64_t src_base = 0, src_steps = 0;
 
@@ -323,6 +336,8 @@ void free_sources(wf_ctx *c)
 {
     drop_graph_exec(c);
     dfree(c->d_src);
+    dfree(c->d_src_grp);
+    c->nsrc_grp = 0;
     dfree(c->d_srcval);
     c->nsrc = 0;
     c->src_steps = 0;
@@ -619,7 +634,8 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
         if (ev) WF_HIP(hipEventRecord(ev[5], st));
     }
     if (c->nsrc) {
-        hipLaunchKernelGGL(k_sources, dim3(1), dim3(64), 0, st, c->p, c->d_src, (int)c->nsrc,
+        hipLaunchKernelGGL(k_sources, dim3((c->nsrc_grp + 255) / 256), dim3(256), 0, st, c->p,
+                           c->d_src, (int)c->nsrc, c->d_src_grp, (int)c->nsrc_grp,
                            c->d_srcval, c->d_step, c->src_base);
     }
     if (c->nprobe) {
@@ -1120,6 +1136,7 @@ int wf_set_sources(wf_ctx *c, uint32_t n, const wf_source_site *sites, uint64_t 
     free_sources(c);
     if (n == 0) return WF_OK;
     if (!sites || (!values && nsteps)) return fail(WF_ERR_INVALID, "wf_set_sources: null pointer");
+    if (n > 0x7fffffffu) return fail(WF_ERR_INVALID, "wf_set_sources: more than 2^31 - 1 sources");
     std::vector<DevSource> ds(n);
     for (uint32_t s = 0; s < n; ++s) {
         const wf_source_site &q = sites[s];
@@ -1130,10 +1147,23 @@ int wf_set_sources(wf_ctx *c, uint32_t n, const wf_source_site *sites, uint64_t 
         ds[s].off = ((uint64_t)q.z * c->g.ny + q.y) * c->g.nx + q.x;
         ds[s].comp = q.component;
         ds[s].hard = q.hard ? 1 : 0;
+        ds[s].col = s;
+        ds[s].pad = 0;
     }
+    // group by (component, voxel), keeping list order inside a group
+    std::stable_sort(ds.begin(), ds.end(), [](const DevSource &a, const DevSource &b) {
+        return a.comp != b.comp ? a.comp < b.comp : a.off < b.off;
+    });
+    std::vector<uint32_t> grp;
+    for (uint32_t s = 0; s < n; ++s)
+        if (s == 0 || ds[s].comp != ds[s - 1].comp || ds[s].off != ds[s - 1].off) grp.push_back(s);
+    grp.push_back(n);
     if (int rc = dmalloc(&c->d_src, n * sizeof(DevSource))) return rc;
+    if (int rc = dmalloc(&c->d_src_grp, grp.size() * sizeof(uint32_t))) return rc;
     if (int rc = dmalloc(&c->d_srcval, nsteps * n * sizeof(float))) return rc;
     WF_HIP(hipMemcpy(c->d_src, ds.data(), n * sizeof(DevSource), hipMemcpyHostToDevice));
+    WF_HIP(hipMemcpy(c->d_src_grp, grp.data(), grp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    c->nsrc_grp = (uint32_t)(grp.size() - 1);
     if (nsteps) WF_HIP(hipMemcpy(c->d_srcval, values, nsteps * n * sizeof(float), hipMemcpyHostToDevice));
     c->nsrc = n;
     c->src_base = c->step;

@@ -45,6 +45,8 @@ struct DevSource {
     uint64_t off;  // voxel linear index
     int comp;      // 0..2
     int hard;
+    uint32_t col;  // the source's position in the caller's list: its column of values
+    uint32_t pad;
 };
 
 struct DevProbe {

@@ -415,15 +415,19 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float *__restrict__ u,
     }
 }
 
-// Sums the wave rows in a fixed order: grad[p] for p < kMlpParams, the
-// squared error at grad[kMlpParams].
-__global__ void k_mlp_reduce(const float *__restrict__ partials, int rows, float *__restrict__ grad)
+// Sums the wave rows in a fixed order, in two stages so that enough lanes
+// run: stage one sums chunk blockIdx.y of `per` consecutive rows into row
+// blockIdx.y of `out`; stage two (gridDim.y = 1, per = rows) sums those.
+// Row kMlpParams of a column is the squared error.
+__global__ void k_mlp_reduce(const float *__restrict__ partials, int rows, int per,
+                             float *__restrict__ out)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p > kMlpParams) return;
+    const int r0 = blockIdx.y * per, r1 = min(r0 + per, rows);
     float s = 0.f;
-    for (int r = 0; r < rows; ++r) s = s + partials[(size_t)r * (kMlpParams + 1) + p];
-    grad[p] = s;
+    for (int r = r0; r < r1; ++r) s = s + partials[(size_t)r * (kMlpParams + 1) + p];
+    out[(size_t)blockIdx.y * (kMlpParams + 1) + p] = s;
 }
 
 // Adam; the linear skip (wlin) moves at lr_skip instead of lr, and with
@@ -718,6 +722,7 @@ struct wf_mlp_trainer {
     uint64_t n = 0;
     float *d_u = nullptr, *d_t = nullptr;
     float *d_p = nullptr, *d_m = nullptr, *d_v = nullptr, *d_grad = nullptr, *d_part = nullptr;
+    float *d_part2 = nullptr;  // per-chunk sums of d_part's rows
     MlpPacked *d_w = nullptr;
     int blocks = 0;
     int head = kHeadDirect;
@@ -727,6 +732,7 @@ struct wf_mlp_trainer {
 namespace {
 
 constexpr int kTrainBlocks = 256;  // 1024 waves
+constexpr int kReduceChunks = 32;  // first reduction stage: 32 rows per chunk
 
 template <typename T>
 int dalloc(T **p, size_t bytes)
@@ -743,8 +749,12 @@ int trainer_grad(wf_mlp_trainer *t)
 {
     hipLaunchKernelGGL(k_mlp_train, dim3(t->blocks), dim3(256), 0, t->stream, t->d_u, t->d_t,
                        (int)t->n, 1.0f / (float)t->n, t->d_w, t->d_part);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, t->stream,
-                       t->d_part, t->blocks * 4, t->d_grad);
+    const int rows = t->blocks * 4, per = (rows + kReduceChunks - 1) / kReduceChunks;
+    const unsigned cols = (kMlpParams + 1 + 255) / 256;
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(cols, kReduceChunks), dim3(256), 0, t->stream, t->d_part,
+                       rows, per, t->d_part2);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(cols, 1), dim3(256), 0, t->stream, t->d_part2,
+                       kReduceChunks, kReduceChunks, t->d_grad);
     WF_HIP(hipGetLastError());
     return WF_OK;
 }
@@ -760,7 +770,7 @@ void wf_mlp_trainer_destroy(wf_mlp_trainer *t)
     if (!t) return;
     (void)hipSetDevice(t->device);
     if (t->stream) (void)hipStreamSynchronize(t->stream);
-    for (float *p : {t->d_u, t->d_t, t->d_p, t->d_m, t->d_v, t->d_grad, t->d_part})
+    for (float *p : {t->d_u, t->d_t, t->d_p, t->d_m, t->d_v, t->d_grad, t->d_part, t->d_part2})
         if (p) (void)hipFree(p);
     if (t->d_w) (void)hipFree(t->d_w);
     if (t->stream) (void)hipStreamDestroy(t->stream);
@@ -792,6 +802,7 @@ int wf_mlp_trainer_create(int device, uint64_t n, const float *inputs, const flo
     if (!rc) rc = dalloc(&t->d_v, kMlpParams * sizeof(float));
     if (!rc) rc = dalloc(&t->d_grad, (kMlpParams + 1) * sizeof(float));
     if (!rc) rc = dalloc(&t->d_part, (size_t)t->blocks * 4 * (kMlpParams + 1) * sizeof(float));
+    if (!rc) rc = dalloc(&t->d_part2, (size_t)kReduceChunks * (kMlpParams + 1) * sizeof(float));
     if (!rc) rc = dalloc(&t->d_w, sizeof(MlpPacked));
     if (!rc && (hipMemcpy(t->d_u, inputs, n * kMlpIn * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(t->d_t, targets, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
