@@ -378,9 +378,9 @@ const char *wf_kernel_variant(const wf_ctx *ctx);
  * up to ~12% faster or slower than over another, depending on where the
  * driver placed its pages, so for grids whose fields take >= 8 GiB per state
  * wf_create tries eight extra state buffers (one held at a time, four
- * steps timed per arrangement: about 1 s at 1024^3) and keeps the fastest
- * arrangement (WF_PLACEMENT=n tries n, 0 = off, at most 8;
- * WF_PLACEMENT_MIN_MIB moves the threshold). */
+ * steps timed per arrangement: about 1 s at 1024^3; no new buffer after
+ * 4 s) and keeps the fastest arrangement (WF_PLACEMENT=n tries n, 0 = off,
+ * at most 8; WF_PLACEMENT_MIN_MIB moves the threshold). */
 int wf_placement(const wf_ctx *ctx, double *ms, int cap, int *n, int *kept);
 /* Copies `bytes` (read + write = 2*bytes of HBM traffic) iters times with a
  * plain 16-byte-per-lane copy kernel on `device`; writes the mean ms per

@@ -700,10 +700,15 @@ void point_fields(wf_ctx *c, uint64_t N)
 }
 
 // Tries `extra` more state buffers (at most kPlaceMax) and keeps the fastest
-// arrangement.  A buffer that cannot be allocated ends the search (the
-// current arrangement stays); HIP errors are returned.
-int choose_placement(wf_ctx *c, int extra, uint64_t N)
+// arrangement.  A buffer that cannot be allocated, or a search that has run
+// for budget_ms, ends it (the current arrangement stays); HIP errors are
+// returned.
+int choose_placement(wf_ctx *c, int extra, uint64_t N, double budget_ms)
 {
+    const auto t_start = std::chrono::steady_clock::now();
+    auto elapsed_ms = [&] {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    };
     const size_t bytes = 6 * N * sizeof(float);
     float **slot[2] = {&c->d_fields, &c->d_fields2};
     const int nslot = c->d_fields2 ? 2 : 1;
@@ -712,7 +717,8 @@ int choose_placement(wf_ctx *c, int extra, uint64_t N)
     double best = 0.0;
     if (int rc = probe_step_ms(c, 2, 2, &best)) return rc;
     c->place_ms[c->place_n++] = best;
-    for (int t = 0; t < extra && t < kPlaceMax && c->place_n + nslot <= kPlaceProbes; ++t) {
+    for (int t = 0; t < extra && t < kPlaceMax && c->place_n + nslot <= kPlaceProbes && elapsed_ms() < budget_ms;
+         ++t) {
         float *x = nullptr;
         if (hipMalloc(reinterpret_cast<void **>(&x), bytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -1065,8 +1071,9 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     // Field placement (see choose_placement): by default eight extra state
     // buffers tried, one held at a time beside the current ones, when one
     // state's fields take at least 8 GiB (4 steps per arrangement timed, 17
-    // arrangements: about 1 s at 1024^3).  WF_PLACEMENT=n tries n (0 = no
-    // probe, at most 8); WF_PLACEMENT_MIN_MIB moves the size threshold.
+    // arrangements: about 1 s at 1024^3), no new buffer after 4 s.
+    // WF_PLACEMENT=n tries n (0 = no probe, at most 8); WF_PLACEMENT_MIN_MIB
+    // moves the size threshold.
     {
         int tries = 8;
         uint64_t min_bytes = 8ull << 30;
@@ -1074,7 +1081,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
         if (const char *mb = std::getenv("WF_PLACEMENT_MIN_MIB"))
             min_bytes = (uint64_t)std::max(0L, std::atol(mb)) << 20;
         if (tries > 0 && 6 * N * sizeof(float) >= min_bytes)
-            if (int rc = choose_placement(c, std::min(tries, kPlaceMax), N)) return bail(rc);
+            if (int rc = choose_placement(c, std::min(tries, kPlaceMax), N, 4000.0)) return bail(rc);
     }
     *out = c;
     return WF_OK;
