@@ -1,13 +1,16 @@
-"""The 1024^3 random-fill workload at its full size (1 073 741 824 cells, CPML 24,
-200 materials, sinusoid source, probe): two steps from random fields on the
-GPU, in the default fused variant and in the in-place split variant, against
-the CPU oracle -- fields, psi and probe sums bit for bit.  This covers, at the
-real shape, what the small parity cases cannot: the interior box with its
-z slices, the frame boxes, the chosen field placement and 64-bit indexing past
-2^31 elements.
+"""Workloads at their full size against the CPU oracle, bit for bit.
 
-Host memory: about 90 GB (initial fields, the oracle's state and result, one
-read-back at a time); the oracle runs on up to 16 threads."""
+* The 1024^3 random fill (1 073 741 824 cells, CPML 24, 200 materials,
+  sinusoid source, probe): two steps from random fields on the GPU, in the
+  default fused variant and in the in-place split variant -- fields, psi and
+  probe sums.  This covers, at the real shape, what the small parity cases
+  cannot: the interior box with its z slices, the frame boxes, the chosen
+  field placement and 64-bit indexing past 2^31 elements.
+* The 512^3 two-sphere run through the snapshot stream (wf_run): every
+  snapshot written, the final state and the probe sums.
+
+Host memory: about 90 GB at 1024^3 (initial fields, the oracle's state and
+result, one read-back at a time); the oracle runs on up to 16 threads."""
 import os
 
 import numpy as np
@@ -83,3 +86,43 @@ def test_random_1024_full_size_bit_exact(oracle_run, variant, monkeypatch):
     assert list(n) == list(on) == [STEPS] * len(w.probes)
     # the steps changed the fields everywhere, not just near the source
     assert np.count_nonzero(of.reshape(-1)[::4099] != f0.reshape(-1)[::4099]) > 0.9 * (of.size // 4099)
+
+
+def test_spheres_512_full_size_snapshots(tmp_path):
+    """The 512^3 two-sphere workload at its full size (134 M cells, CPML 16,
+    Ricker source, 32 probes) from random fields through the snapshot stream:
+    30 steps with a snapshot every 10 (three 3.22 GB states written by
+    wf_run), each snapshot and the final state bit for bit against the
+    oracle's state after the same step, and the probe sums."""
+    steps, every = 30, 10
+    w = workloads.spheres_512(steps=steps)
+    _say(f"spheres_512 full size: {steps} steps, a snapshot every {every}")
+    f0 = _random_fields(w.dims)
+    per = w.dims.voxel_count() * 24
+    path = tmp_path / "spheres_512.bin"
+    with DeviceEngine(w.dims, w.ids, w.table, w.pml()) as eng:
+        eng.write_fields(f0)
+        eng.set_sources(w.sources, w.dt, steps)
+        eng.set_probes(w.probes, w.dt, steps)
+        eng.run(steps, every, str(path))
+        final = eng.read_fields()
+        sc, ss, n = eng.probe_sums()
+    assert os.path.getsize(path) == 3 * per
+    sim = oracle_sim_for(w)
+    sim.write_fields(f0)
+    threads = min(os.cpu_count() or 1, 16)
+    done = 0
+    with open(path, "rb") as fh:
+        for k in range(0, steps, every):  # snapshot after loop index k = after k + 1 steps
+            sim.run(k + 1 - done, w.dt, first_step=done, threads=threads)
+            done = k + 1
+            snap = np.frombuffer(fh.read(per), dtype=np.float32)
+            _equal(snap, sim.read_fields().reshape(-1), f"snapshot after step {k + 1}")
+    sim.run(steps - done, w.dt, first_step=done, threads=threads)
+    _equal(final, sim.read_fields(), "final state")
+    oc, os_, on = sim.probe_sums()
+    sim.close()
+    _equal(sc, oc, "sum_cos")
+    _equal(ss, os_, "sum_sin")
+    assert list(n) == list(on) == [steps] * len(w.probes)
+    os.remove(path)
