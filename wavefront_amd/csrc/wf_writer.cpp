@@ -49,7 +49,21 @@ int SnapshotWriter::open(const char *path, uint64_t snapshot_bytes, int device,
             return fail(WF_ERR_HIP, "snapshot writer: event creation failed");
     }
     stop_ = false;
+    io_stop_ = false;
     error_ = 0;
+    pipelined_ = false;
+    if (direct_) {
+        void *a = nullptr, *b = nullptr;
+        if (::posix_memalign(&a, kAlign, chunk_) == 0 && ::posix_memalign(&b, kAlign, chunk_) == 0) {
+            stage_[0] = a;
+            stage_[1] = b;
+            stage_free_[0] = stage_free_[1] = true;
+            pipelined_ = true;
+            io_thread_ = std::thread([this] { io_run(); });
+        } else {
+            std::free(a);  // buffered path below: copy and write in turn
+        }
+    }
     thread_ = std::thread([this] { run(); });
     return WF_OK;
 }
@@ -185,8 +199,28 @@ void SnapshotWriter::run()
             }
             const uint64_t off = k * chunk_;
             const uint64_t len = snap_ - off < chunk_ ? snap_ - off : chunk_;
-            rc = write_all(pinned_[b], len, job.offset + off);
-            if (rc) msg = wf_last_error();
+            if (pipelined_) {
+                // hand the chunk to the I/O thread through a free staging buffer
+                int j = -1;
+                {
+                    std::unique_lock<std::mutex> lk(mu_);
+                    cv_.wait(lk, [&] { return stage_free_[0] || stage_free_[1] || error_; });
+                    if (error_) {
+                        rc = error_;
+                        msg = error_msg_;
+                        break;
+                    }
+                    j = stage_free_[0] ? 0 : 1;
+                    stage_free_[j] = false;
+                }
+                std::memcpy(stage_[j], pinned_[b], len);
+                std::unique_lock<std::mutex> lk(mu_);
+                writes_.push_back(WriteJob{j, len, job.offset + off});
+                cv_.notify_all();
+            } else {
+                rc = write_all(pinned_[b], len, job.offset + off);
+                if (rc) msg = wf_last_error();
+            }
         }
         if (rc == WF_OK) (void)hipStreamSynchronize(st);
         {
@@ -211,6 +245,37 @@ void SnapshotWriter::run()
     }
 }
 
+// Writes the staged chunks in the order they were handed over; after an
+// error the rest are dropped (the file is already incomplete).
+void SnapshotWriter::io_run()
+{
+    for (;;) {
+        WriteJob wj;
+        bool skip;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [&] { return io_stop_ || !writes_.empty(); });
+            if (writes_.empty()) return;
+            wj = writes_.front();
+            writes_.pop_front();
+            skip = error_ != 0;
+        }
+        int rc = WF_OK;
+        std::string msg;
+        if (!skip) {
+            rc = write_all(stage_[wj.buf], wj.len, wj.offset);
+            if (rc) msg = wf_last_error();
+        }
+        std::unique_lock<std::mutex> lk(mu_);
+        stage_free_[wj.buf] = true;
+        if (rc && !error_) {
+            error_ = rc;
+            error_msg_ = msg;
+        }
+        cv_.notify_all();
+    }
+}
+
 int SnapshotWriter::finish()
 {
     if (!opened_) return WF_OK;
@@ -221,6 +286,17 @@ int SnapshotWriter::finish()
         cv_.notify_all();
     }
     if (thread_.joinable()) thread_.join();
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        io_stop_ = true;
+        cv_.notify_all();
+    }
+    if (io_thread_.joinable()) io_thread_.join();
+    for (auto &b : stage_) {
+        std::free(b);
+        b = nullptr;
+    }
+    pipelined_ = false;
     for (auto &st : stream_) {
         if (!st) continue;
         (void)hipStreamSynchronize(st);

@@ -10,6 +10,12 @@
 // releases the device buffer.  The timestep loop only blocks when it wants
 // to pack into a device buffer whose previous snapshot is still being
 // written.
+//
+// O_DIRECT cannot write from pinned HIP memory on this platform (the kernel
+// cannot pin driver-mapped pages: EFAULT), so with O_DIRECT each chunk is
+// copied into one of two page-aligned host buffers and written by a second
+// thread: the copy of chunk i+1 overlaps the write of chunk i, and the file
+// is written at the disk's speed instead of at copy + write.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -58,6 +64,7 @@ private:
         uint64_t offset;
     };
     void run();
+    void io_run();
     int write_all(const void *buf, uint64_t len, uint64_t offset);
 
     bool opened_ = false;
@@ -70,6 +77,17 @@ private:
     int nbuf_ = 0;
     void *pinned_[kMaxBuf] = {};
     void *bounce_ = nullptr;  // page-aligned staging when O_DIRECT refuses pinned memory
+    // the O_DIRECT path: two page-aligned buffers handed to the I/O thread
+    struct WriteJob {
+        int buf;
+        uint64_t len, offset;
+    };
+    bool pipelined_ = false;
+    void *stage_[2] = {nullptr, nullptr};
+    bool stage_free_[2] = {true, true};
+    std::deque<WriteJob> writes_;
+    bool io_stop_ = false;
+    std::thread io_thread_;
     hipEvent_t copied_[kMaxBuf] = {};
     std::thread thread_;
     std::mutex mu_;
