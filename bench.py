@@ -160,6 +160,17 @@ def run_batch_96(args):
         e.close()
     log(f"solo, one scene after another: {solo_ms / 1e3:.3f}s")
     mcells = cells * steps / wall / 1e6
+    # algorithmic bytes per step: the two-pass 74 B per cell plus 8 B (read +
+    # write) per non-identity psi term, 4 terms per graded cell per axis
+    from wavefront_amd.layout import effective_pml_thickness_blocks
+    step_bytes = 0
+    for w in ws:
+        n = (w.dims.nx, w.dims.ny, w.dims.nz)
+        t = 8 * effective_pml_thickness_blocks(w.dims, w.pml_config.thickness) if w.pml_config else 0
+        graded = sum(2 * t * w.cells() // n[a] for a in range(3))
+        step_bytes += 74 * w.cells() + 8 * 4 * graded
+    ms_step = ms_rest / (steps - taken)
+    achieved = step_bytes / (ms_step / 1e3) / 1e9
     out = {
         "metric": "Mcell-updates/s, 96 batched scenes, FP32, 1 MI355X",
         "value": round(mcells, 1),
@@ -184,7 +195,11 @@ def run_batch_96(args):
             "batched_ms_per_step_after_tuning": round(ms_rest / (steps - taken), 4),
             "solo_sequential_seconds": round(solo_ms / 1e3, 3),
             "solo_sequential_mcells": round(cells * steps / (solo_ms / 1e3) / 1e6, 1),
+            "bytes_per_step_algorithmic": step_bytes,
         },
+        "roofline": {"bound": "hbm", "kernel": "k_h4_batch + k_e4_batch (one step after tuning)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
     }
     print(json.dumps(out), flush=True)
 
