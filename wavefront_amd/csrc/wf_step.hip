@@ -438,9 +438,10 @@ __global__ __launch_bounds__(64 * kSY) void k_e4_batch(const BatchScene *__restr
 // arithmetic as that row's owner, so the same bits) for the E update of the
 // owned rows, and never stores.  H at x-1 comes from the previous lane; lane
 // 0 of every owned row also recomputes the one halo cell x0-1.  The sweep
-// marches z over a chunk, starting one plane early for H(z0-1); E(z+1) and
-// H(z-1) ride in registers; H at y-1 crosses waves through a double-buffered
-// LDS plane behind an LDS-only barrier, so stores are never waited for.
+// marches z over its slice of the box's planes, starting one plane early for
+// H(z0-1); E(z+1) and H(z-1) ride in registers; H at y-1 crosses waves
+// through a double-buffered LDS plane behind an LDS-only barrier, so stores
+// are never waited for.
 // ===========================================================================
 // Waves per workgroup (owned rows + the halo row).  Plain variant: 12 -- at
 // 164 VGPRs three waves fit per SIMD, so one 12-wave workgroup fills a CU
