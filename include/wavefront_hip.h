@@ -301,6 +301,22 @@ int wf_mlp_train(wf_mlp_trainer *t, uint32_t iters, float lr, float lr_skip, int
                  double *loss);
 /* Network output for n scaled input rows. */
 int wf_mlp_predict(wf_mlp_trainer *t, uint64_t n, const float *inputs, float *outputs);
+/* Allocation-free wf_mlp_predict.  No counterpart in the reference; for
+ * callers that cannot allocate device memory (tight evaluation loops,
+ * harnesses that forbid hipMalloc).  Evaluates the network `params`
+ * (wf_mlp_param_count() floats) with output head `head` on n scaled input
+ * rows (inputs[n][16] -> outputs[n], host memory) on HIP device `device`,
+ * with the same kernels and the same results as wf_mlp_predict, staging
+ * parameters, packed network, inputs and outputs in a static device array
+ * that each device holds until the process ends: no hipMalloc, hipFree or
+ * stream creation.  n at most wf_mlp_predict_noalloc_max() (65536), else
+ * WF_ERR_INVALID before any device work.  Returns after the outputs are
+ * written.  All calls on one device share its array: calls that overlap on
+ * one device (from several threads) race and must be serialized by the
+ * caller.  Work is queued on the device's null stream. */
+int wf_mlp_predict_noalloc(int device, const float *params, int head, uint64_t n,
+                           const float *inputs, float *outputs);
+uint64_t wf_mlp_predict_noalloc_max(void);
 
 /* Predicted faces: the faces in face_mask get ghost planes predicted by the
  * networks params[f * wf_mlp_param_count() ...] with output head `head` each

@@ -106,6 +106,9 @@ _SIGNATURES = {
     "wf_mlp_gradient": (c_int, [c_void_p, POINTER(c_float), POINTER(c_double)]),
     "wf_mlp_train": (c_int, [c_void_p, c_uint32, c_float, c_float, c_int, POINTER(c_double)]),
     "wf_mlp_predict": (c_int, [c_void_p, c_uint64, POINTER(c_float), POINTER(c_float)]),
+    "wf_mlp_predict_noalloc": (c_int, [c_int, POINTER(c_float), c_int, c_uint64, POINTER(c_float),
+                                       POINTER(c_float)]),
+    "wf_mlp_predict_noalloc_max": (c_uint64, []),
     "wf_set_learned_faces": (c_int, [c_void_p, c_uint32, POINTER(c_float), c_int]),
     "wf_mlp_set_head": (c_int, [c_void_p, c_int]),
     "wf_clear_faces": (c_int, [c_void_p]),

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "wf_internal.h"
@@ -932,6 +933,103 @@ int wf_mlp_predict(wf_mlp_trainer *t, uint64_t n, const float *inputs, float *ou
     (void)hipFree(du);
     (void)hipFree(dy);
     return rc;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Allocation-free evaluation of a face network
+// ===========================================================================
+// wf_mlp_predict_noalloc stages the parameters, the packed network, the
+// inputs and the outputs in one static device array instead of device
+// memory of its own: the same k_pack_mlp and k_mlp_predict as wf_mlp_predict,
+// for callers that may not allocate.  The array's layout, in bytes:
+//   [0, kScrNet)            FP32 parameters (kMlpParams floats)
+//   [kScrNet, kScrIn)       MlpPacked network
+//   [kScrIn, kScrOut)       inputs, kNoAllocMaxSamples x 16 floats
+//   [kScrOut, kScratchBytes) outputs, kNoAllocMaxSamples floats
+// Each device holds its own copy of the array until the process ends.
+namespace {
+
+constexpr uint64_t kNoAllocMaxSamples = 65536;
+constexpr size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t kScrNet = round256(kMlpParams * sizeof(float));
+constexpr size_t kScrIn = kScrNet + round256(sizeof(MlpPacked));
+constexpr size_t kScrOut = kScrIn + kNoAllocMaxSamples * kMlpIn * sizeof(float);
+constexpr size_t kScratchBytes = kScrOut + kNoAllocMaxSamples * sizeof(float);
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kScrMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kScrMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), looked up once per
+// device.  In host code the array's name is only a stand-in: kernels and
+// copies get this address.
+int scratch_on(int device, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(WF_ERR_HIP, "wf_mlp_predict_noalloc: hipGetSymbolAddress(scratch) on device %d failed: %s",
+                        device, hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return WF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t wf_mlp_predict_noalloc_max(void) { return kNoAllocMaxSamples; }
+
+int wf_mlp_predict_noalloc(int device, const float *params, int head, uint64_t n,
+                           const float *inputs, float *outputs)
+{
+    clear_error();
+    if (!params || (n && (!inputs || !outputs)))
+        return fail(WF_ERR_INVALID, "wf_mlp_predict_noalloc: null pointer");
+    if (head != kHeadDirect && head != kHeadOneWay && head != kHeadDelta)
+        return fail(WF_ERR_INVALID, "wf_mlp_predict_noalloc: unknown head %d", head);
+    if (n > kNoAllocMaxSamples)
+        return fail(WF_ERR_INVALID,
+                    "wf_mlp_predict_noalloc: %llu samples exceed the static scratch's maximum of %llu",
+                    (unsigned long long)n, (unsigned long long)kNoAllocMaxSamples);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev ||
+        device >= kScrMaxDevices)
+        return fail(WF_ERR_HIP, "no HIP device %d available", device);
+    WF_HIP(hipSetDevice(device));
+    unsigned char *base = nullptr;
+    if (int rc = scratch_on(device, &base)) return rc;
+    float *d_p = reinterpret_cast<float *>(base);
+    MlpPacked *d_w = reinterpret_cast<MlpPacked *>(base + kScrNet);
+    float *d_u = reinterpret_cast<float *>(base + kScrIn);
+    float *d_y = reinterpret_cast<float *>(base + kScrOut);
+    // every byte the kernels read is written first in this call: parameters,
+    // packed network (all fields k_mlp_predict loads) and the n input rows
+    WF_HIP(hipMemcpyAsync(d_p, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, nullptr));
+    hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, nullptr, d_p, head, d_w);
+    WF_HIP(hipGetLastError());
+    if (n) {
+        WF_HIP(hipMemcpyAsync(d_u, inputs, n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, nullptr));
+        const uint64_t groups = (n + 15) / 16;
+        const unsigned blocks = (unsigned)std::min<uint64_t>((groups + 3) / 4, 1024);
+        hipLaunchKernelGGL(k_mlp_predict, dim3(blocks), dim3(256), 0, nullptr, d_u, (int)n, d_w, d_y);
+        WF_HIP(hipGetLastError());
+        WF_HIP(hipMemcpyAsync(outputs, d_y, n * sizeof(float), hipMemcpyDeviceToHost, nullptr));
+    }
+    WF_HIP(hipStreamSynchronize(nullptr));
+    return WF_OK;
 }
 
 }  // extern "C"
