@@ -74,7 +74,7 @@ size_t wfo_effective_pml_thickness_blocks(size_t nx, size_t ny, size_t nz,
 {
     if (requested_voxels == 0) return 0;
     size_t bx = nx / BD, by = ny / BD, bz = nz / BD;
-    size_t req = (requested_voxels + BD - 1) / BD;
+    size_t req = requested_voxels / BD + (requested_voxels % BD != 0); /* usize::div_ceil */
     if (req < 1) req = 1;
     size_t mn = bx < by ? bx : by;
     mn = mn < bz ? mn : bz;
@@ -676,6 +676,11 @@ static void accumulate_probes(wfo_sim *s, float t, float *trace_row)
     }
 }
 
+/* Probe::accumulate(&grid, t) for every probe, on the fields as they are
+ * (src/probe.rs:126-138): the probes' known-answer tests drive a voxel by
+ * writing the fields and call this once per step. */
+void wfo_sim_accumulate_probes(wfo_sim *s, float t) { accumulate_probes(s, t, NULL); }
+
 /* ===================================================================== */
 /* SPMD runner: Z-slabs (src/engine.rs:666-669) with a barrier per phase   */
 /* ===================================================================== */
@@ -829,19 +834,25 @@ void wfo_voxelize_demo_sphere(size_t nx, size_t ny, size_t nz, uint8_t *ids)
             }
 }
 
-/* Conservative voxel range [lo, hi) for |v + 0.5 - c| <= r; the predicate
- * itself decides membership, the range only skips voxels that cannot pass. */
-static void sphere_range(float c, float r, size_t n, size_t *lo, size_t *hi)
+/* Voxels [lo, hi) along one axis whose own term can pass the sphere
+ * predicate, fl(d * d) <= r2 with d = fl(fl(v + 0.5) - c): the predicate of
+ * src/scene.rs:197-209 compares a rounded sum of three non-negative rounded
+ * squares, which is never below any of them, so a voxel failing on one axis
+ * fails whatever the others are.  Every v of the axis is tested, so the box
+ * holds every voxel the reference's full scan sets, for any centre and
+ * radius. */
+static void sphere_range(float c, float r2, size_t n, size_t *lo, size_t *hi)
 {
-    double ar = fabs((double)r);
-    if (!isfinite(c) || !isfinite(ar)) {
-        *lo = 0;
-        *hi = n;
-        return;
+    size_t a = n, b = n;
+    for (size_t v = 0; v < n; v++) {
+        float d = (float)v + 0.5f - c;
+        if (d * d <= r2) {
+            if (a == n) a = v;
+            b = v + 1;
+        }
     }
-    double l = floor((double)c - ar) - 2.0, h = ceil((double)c + ar) + 2.0;
-    *lo = l <= 0.0 ? 0 : (l >= (double)n ? n : (size_t)l);
-    *hi = h <= 0.0 ? 0 : (h >= (double)n ? n : (size_t)h);
+    *lo = a == n ? 0 : a;
+    *hi = a == n ? 0 : b;
 }
 
 typedef struct {
@@ -946,9 +957,9 @@ int wfo_scene_voxelize(const char *text, size_t nx, size_t ny, size_t nz, float 
             float cx = pr->v[3], cy = pr->v[4], cz = pr->v[5], r = pr->v[6];
             float r2 = r * r;
             size_t x0, x1, y0, y1, z0, z1;
-            sphere_range(cx, r, nx, &x0, &x1);
-            sphere_range(cy, r, ny, &y0, &y1);
-            sphere_range(cz, r, nz, &z0, &z1);
+            sphere_range(cx, r2, nx, &x0, &x1);
+            sphere_range(cy, r2, ny, &y0, &y1);
+            sphere_range(cz, r2, nz, &z0, &z1);
             for (size_t z = z0; z < z1; z++)
                 for (size_t y = y0; y < y1; y++)
                     for (size_t x = x0; x < x1; x++) {

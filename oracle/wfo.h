@@ -101,6 +101,10 @@ void wfo_sim_run(wfo_sim *s, size_t first_step, size_t nsteps, float dt, int thr
 /* The timestep without sources/probes (step_all_slabs). */
 void wfo_sim_step_fields(wfo_sim *s, int threads);
 
+/* Probe::accumulate(&grid, t) of every probe on the current fields
+ * (src/probe.rs:126-138), outside the step loop. */
+void wfo_sim_accumulate_probes(wfo_sim *s, float t);
+
 /* Snapshot bytes in the reference format (src/engine.rs:622-640). */
 size_t wfo_sim_bytes(const wfo_sim *s);
 void wfo_sim_read_aosoa(const wfo_sim *s, void *dst);

@@ -39,6 +39,7 @@ _SIG = {
     "wfo_sim_write_aosoa": (None, [c_void_p, c_void_p]),
     "wfo_sim_shell_block_count": (c_size_t, [c_void_p]),
     "wfo_sim_read_psi": (None, [c_void_p, POINTER(c_float)]),
+    "wfo_sim_accumulate_probes": (None, [c_void_p, c_float]),
     "wfo_sim_probe_count": (c_size_t, [c_void_p]),
     "wfo_sim_probe_nfreq": (c_size_t, [c_void_p, c_size_t]),
     "wfo_sim_probe_sums": (None, [c_void_p, c_size_t, POINTER(c_double), POINTER(c_double),
@@ -145,6 +146,10 @@ class OracleSim:
         lib().wfo_sim_run(self.h, first_step, nsteps, dt, threads,
                           None if tr is None else _p(tr, c_float))
         return None if tr is None else tr[:, :len(self.probe_nfreq)]
+
+    def accumulate_probes(self, t):
+        """Probe::accumulate(&grid, t) of every probe on the current fields."""
+        lib().wfo_sim_accumulate_probes(self.h, t)
 
     def step_fields(self, threads=1):
         lib().wfo_sim_step_fields(self.h, threads)

@@ -146,10 +146,12 @@ def test_pml_profiles_match_oracle():
             for x, y in zip((a.b, a.a, a.inv_kappa), b):
                 assert x.view(np.uint32).tolist() == y.view(np.uint32).tolist()
     for dims in [(64, 64, 64), (1024, 1024, 1024), (16, 32, 48), (32, 40, 192)]:
-        for req in (0, 1, 8, 10, 16, 20, 1000):
+        for req in (0, 1, 8, 10, 16, 20, 1000, 2**32 - 8, 2**32 - 1):
             d = GridDims(*dims)
             assert effective_pml_thickness_blocks(d, req) == \
                 W.lib().wfo_effective_pml_thickness_blocks(*dims, req)
+    # the CLI clamps --pml-thickness to UINT32_MAX: div_ceil must not wrap to 1 block
+    assert effective_pml_thickness_blocks(GridDims(64, 64, 64), 2**32 - 1) == 4
 
 
 def test_waveforms_and_source_values_match_oracle():
@@ -182,6 +184,59 @@ def test_scene_voxelizer_matches_oracle():
     assert np.array_equal(table.q16, otable)
     with pytest.raises(_lib.WavefrontError):
         Scene.parse("sphere 1 2\n").voxelize(grid, table, 1e-12, 1e-3)
+
+
+def _full_scan_spheres(text, nx, ny, nz):
+    """The reference's sphere rasterizer taken literally (src/scene.rs:197-209):
+    every voxel of the grid, the f32 predicate (x + 0.5 - c) per axis, squared,
+    summed left to right and compared with r*r, each operation rounded to f32
+    (numpy float32 arithmetic, no contraction).  Spheres only; id k for the
+    k-th distinct material, later primitives overwrite earlier ones."""
+    f = np.float32
+    ids = np.zeros((nz, ny, nx), np.uint8)
+    mats = {}
+    for line in text.splitlines():
+        tok = line.split()
+        if not tok or tok[0].startswith("#"):
+            continue
+        assert tok[0] == "sphere"
+        v = [f(t) for t in tok[1:]]
+        key = tuple(np.array(v[:3], f).view(np.uint32).tolist())
+        mid = mats.setdefault(key, len(mats) + 1)
+        c, r = v[3:6], v[6]
+        d = [((np.arange(n, dtype=f) + f(0.5)) - c[a]).astype(f) for a, n in
+             enumerate((nx, ny, nz))]
+        sq = [(x * x).astype(f) for x in d]
+        s = ((sq[0][None, None, :] + sq[1][None, :, None]).astype(f) + sq[2][:, None, None]).astype(f)
+        ids[s <= f(r * r)] = mid
+    return ids.reshape(-1)
+
+
+def test_sphere_rasterizer_equals_full_scan():
+    """Product and oracle against the literal full scan, on spheres whose f32
+    rounding at |c| spans many cells: a half-space written as a huge sphere
+    (centre 1e9, radius 999999900: the full scan sets x = 96..125, where a
+    range of c +- (|r| + 2) cells would start at 126), its mirror image, tiny
+    and zero radii, centres outside the grid and an ordinary sphere."""
+    scenes = [
+        "sphere 4 1 0 1e9 8 8 999999900\n",
+        "sphere 2.5 1 0.01 -999999900 8 8 999999972\n",
+        "sphere 3 1 0 8 1e8 8 99999960\nsphere 5 1 0 40 8 8 6.5\n",
+        "sphere 2 1 0 60.5 3.5 12.25 0\nsphere 2 1 0 61 4 12 0.4\n",
+        "sphere 7 1 0 -30 -30 -30 60\nsphere 1.5 1 0 3e7 -3e7 8 4.2426408e7\n",
+    ]
+    nx, ny, nz = 128, 16, 24
+    for text in scenes:
+        exp = _full_scan_spheres(text, nx, ny, nz)
+        grid = MaterialGrid.create(GridDims(nx, ny, nz))
+        table = MaterialTable.vacuum_filled(1.9e-12, 1e-3)
+        Scene.parse(text).voxelize(grid, table, 1.9e-12, 1e-3)
+        ids, _, _ = W.scene_voxelize(text, nx, ny, nz, 1.9e-12, 1e-3)
+        assert np.array_equal(exp, grid.ids), text
+        assert np.array_equal(exp, ids), text
+    # the huge sphere of the first scene: x = 96..125 inside on the centre row
+    row = _full_scan_spheres(scenes[0], nx, ny, nz).reshape(nz, ny, nx)[8, 8]
+    assert np.flatnonzero(row).tolist() == list(range(96, 128))
 
 
 def test_demo_sphere_matches_oracle():

@@ -295,34 +295,53 @@ def test_soft_source_adds():
 
 
 def test_probe_recovers_sinusoid_and_start_gate():
-    """src/probe.rs:188-260 via wfo_probe_spectrum on sums built the same way."""
-    lib = W.lib()
-    f, A, ph, dt = np.float32(1e9), 3.5, 0.7, np.float32(1e-12)
-    om = np.float32(2.0) * np.float32(math.pi) * f
-    sc = np.zeros(2)
-    ss = np.zeros(2)
+    """src/probe.rs:188-260 through the oracle's own Probe::accumulate: the
+    probe voxel (8,8,8) is driven with A sin(omega t + phi) in f32 (A = 3.5,
+    phi = 0.7, 1 GHz, dt = 1 ps, 4000 steps) by writing the fields, the
+    oracle accumulates once per step, and its spectrum recovers the amplitude
+    and phase within 5% (0.05 rad) and a near-zero amplitude at the undriven
+    3 GHz.  Then the start gate: a huge value before start is never recorded.
+    Plants: a dropped sin term (phase error pi/2), the wrong sign of the
+    phase (error 1.4 rad) or accumulating the f32 frequency twice would each
+    fail one of these bounds."""
+    import ctypes
+    f32 = np.float32
+    freq, A, ph, dt = f32(1e9), f32(3.5), 0.7, f32(1e-12)
+    sim = W.OracleSim(16, 16, 16, table=W.vacuum_table(1e-12, 1e-3))
+    sim.add_probe(8, 8, 8, 2, [freq, f32(3) * freq], 0.0)
+    fields = np.zeros((8, 6, 512), np.float32)
+    blk, loc = (1 * 2 + 1) * 2 + 1, 0  # voxel (8,8,8): block (1,1,1), local (0,0,0)
+    omega = f32(2.0) * f32(math.pi) * freq
     n = 4000
     for step in range(n):
-        t = (np.float32(step) + np.float32(1.0)) * dt
-        v = float(np.float32(A) * np.float32(math.sin(float(om * t + np.float32(ph)))))
-        for i, fr in enumerate([f, np.float32(3) * f]):
-            phase = 2.0 * math.pi * float(fr) * float(t)
-            sc[i] += v * math.cos(phase)
-            ss[i] += v * math.sin(phase)
+        t = (f32(step) + f32(1.0)) * dt
+        fields[blk, 2, loc] = A * f32(math.sin(float(omega * t + f32(ph))))
+        sim.write_fields(fields)
+        sim.accumulate_probes(t)
+    sc, ss, cnt = sim.probe_sums()
+    assert cnt.tolist() == [n]
     amp = np.zeros(2)
     pha = np.zeros(2)
-    import ctypes
     P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-    lib.wfo_probe_spectrum(2, P(sc), P(ss), n, P(amp), P(pha))
-    assert abs(amp[0] - A) < 0.05 * A
+    W.lib().wfo_probe_spectrum(2, P(sc), P(ss), n, P(amp), P(pha))
+    assert abs(amp[0] - A) < 0.05 * A, amp
     err = (pha[0] - ph + math.pi) % (2 * math.pi) - math.pi
-    assert abs(err) < 0.05
-    assert amp[1] < 0.05 * A
+    assert abs(err) < 0.05, pha
+    assert amp[1] < 0.05 * A, amp
+    sim.close()
+    # start gate (src/probe.rs:244-259), again through the oracle's accumulate
     sim = W.OracleSim(16, 16, 16, table=W.vacuum_table(1e-12, 1e-3))
     sim.add_probe(0, 0, 0, 0, [1e9], 5e-9)
-    sim.run(2, 1e-9)
+    fields = np.zeros((8, 6, 512), np.float32)
+    fields[0, 0, 0] = 1.0e6
+    sim.write_fields(fields)
+    sim.accumulate_probes(f32(1.0e-9))
+    sim.accumulate_probes(f32(4.9e-9))
     c, s, cnt = sim.probe_sums()
-    assert cnt.tolist() == [0] and c[0] == 0
+    assert cnt.tolist() == [0] and c[0] == 0 and s[0] == 0
+    sim.accumulate_probes(f32(5.0e-9))  # t == start records
+    c, s, cnt = sim.probe_sums()
+    assert cnt.tolist() == [1] and c[0] != 0
 
 
 def test_scene_voxelize_reference_example(wfo):

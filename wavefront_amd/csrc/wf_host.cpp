@@ -100,7 +100,8 @@ uint32_t wf_effective_pml_thickness_blocks(uint32_t nx, uint32_t ny, uint32_t nz
 {
     if (requested_voxels == 0) return 0;
     const uint32_t nbx = nx / WF_BLOCK_DIM, nby = ny / WF_BLOCK_DIM, nbz = nz / WF_BLOCK_DIM;
-    uint32_t req = (requested_voxels + WF_BLOCK_DIM - 1) / WF_BLOCK_DIM;
+    // div_ceil without the overflow of (requested + 7) near UINT32_MAX
+    uint32_t req = requested_voxels / WF_BLOCK_DIM + (requested_voxels % WF_BLOCK_DIM != 0);
     if (req < 1) req = 1;
     uint32_t lim = nbx < nby ? nbx : nby;
     lim = (lim < nbz ? lim : nbz) / 2;
@@ -243,21 +244,28 @@ struct Primitive {
     float v[9];  // eps_r, mu_r, sigma, then centre+radius or lo+hi
 };
 
-// Voxel range [lo, hi) outside which |v + 0.5 - c| exceeds |r| by more than
-// rounding can hide; the sphere predicate itself decides membership, so the
-// range only skips voxels that cannot pass (src/scene.rs:197-209 tests all).
-void sphere_range(float c, float r, uint32_t n, uint32_t *lo, uint32_t *hi)
+// Voxels [lo, hi) along one axis whose own term can pass the sphere
+// predicate, fl(d * d) <= r2 with d = fl(fl(v + 0.5) - c).  The predicate
+// (src/scene.rs:197-209) compares a rounded sum of three non-negative rounded
+// squares with r2, and such a sum is never below any of its terms, so a voxel
+// whose term on one axis exceeds r2 fails whatever the other two are.  d is
+// monotone in v, so the passing voxels form one interval, found here by
+// testing every v of the axis: O(n) per axis instead of O(n^3), and the box
+// [lo, hi)^3 holds every voxel the reference's full scan sets, for any
+// centre and radius (including spheres far larger than the grid, whose f32
+// rounding at |c| spans many cells).
+void sphere_range(float c, float r2, uint32_t n, uint32_t *lo, uint32_t *hi)
 {
-    const double ar = std::fabs(static_cast<double>(r));
-    if (!std::isfinite(c) || !std::isfinite(ar)) {
-        *lo = 0;
-        *hi = n;
-        return;
+    uint32_t a = n, b = n;
+    for (uint32_t v = 0; v < n; ++v) {
+        const float d = static_cast<float>(v) + 0.5f - c;
+        if (d * d <= r2) {
+            if (a == n) a = v;
+            b = v + 1;
+        }
     }
-    const double l = std::floor(static_cast<double>(c) - ar) - 2.0;
-    const double h = std::ceil(static_cast<double>(c) + ar) + 2.0;
-    *lo = l <= 0.0 ? 0 : (l >= n ? n : static_cast<uint32_t>(l));
-    *hi = h <= 0.0 ? 0 : (h >= n ? n : static_cast<uint32_t>(h));
+    *lo = a == n ? 0 : a;
+    *hi = a == n ? 0 : b;
 }
 
 int parse_scene(const char *text, std::vector<Primitive> &out)
@@ -337,9 +345,9 @@ extern "C" int wf_scene_voxelize(const char *text, uint32_t nx, uint32_t ny, uin
             const float cx = pr.v[3], cy = pr.v[4], cz = pr.v[5], r = pr.v[6];
             const float r2 = r * r;
             uint32_t x0, x1, y0, y1, z0, z1;
-            sphere_range(cx, r, nx, &x0, &x1);
-            sphere_range(cy, r, ny, &y0, &y1);
-            sphere_range(cz, r, nz, &z0, &z1);
+            sphere_range(cx, r2, nx, &x0, &x1);
+            sphere_range(cy, r2, ny, &y0, &y1);
+            sphere_range(cz, r2, nz, &z0, &z1);
             for (uint32_t z = z0; z < z1; ++z)
                 for (uint32_t y = y0; y < y1; ++y) {
                     uint8_t *row = ids + (static_cast<size_t>(z) * ny + y) * nx;
