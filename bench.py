@@ -228,105 +228,93 @@ def run_batch_96(args):
 
 
 def run_learned(args):
-    """The learned thin boundary: training data from 64-cell-CPML plane-wave
-    runs (both polarizations, Gaussian / Ricker, 10-45 GHz), per-face
-    networks trained on the GPU (the x faces four times, the run with the
-    lowest reflection at held-out frequencies kept), then the reference's two-run reflection
-    measurement (30 GHz, 20 points per wavelength, Courant 0.4) for the
-    learned 4-cell layer, its linear skip alone, the same layer without the
-    network (zero boundary) and 32-cell CPML; then the same networks and the
-    3D-stable configuration (a linear predictor fitted to 3D data) in a 3D
-    vacuum cube (Ricker point source, 26 probes) after 520 and 1500 steps,
-    and the step throughput of each boundary around a 248^3 interior (256^3
-    with the learned layer)."""
+    """The learned thin boundary at its 256^3 workload.  Per-face MLPs
+    (16-64-64-1, BF16 on MFMA) trained on the GPU on samples of 64-cell-CPML
+    plane-wave runs (32 per axis, Gaussian / Ricker, 10-45 GHz), with the
+    passive limits on; beside them the linear 3D predictor fitted to 3D
+    point-source runs.  Reported: the reference's two-run reflection at
+    normal incidence (30 GHz, 20 points per wavelength, Courant 0.4) for the
+    MLP layer, its linear skip alone, the zero boundary and 8-/32-cell CPML;
+    the reflected energy of a 3D Ricker point source over 3000 steps in a
+    248^3 interior (256^3 with the learned layer) against a 64-cell CPML run;
+    the field energy after 300 and 3000 steps with and without the limits;
+    and the step throughput of the interior for each boundary."""
     import numpy as np
 
     from wavefront_amd import learned as L
-    from wavefront_amd.engine import PERIODIC_Y, PERIODIC_Z
     from wavefront_amd.layout import GridDims
 
     dx, dt, _ = L.study_grid()
     t0 = time.perf_counter()
-    S = L.collect_3d(GridDims(96, 96, 96), runs=0, steps=0, every=1, per_face=200000, seed=11,
-                     dt=dt, dx=dx, plane_runs=32)
+    S1 = L.collect_3d(GridDims(96, 96, 96), runs=0, steps=0, every=1, per_face=200000, seed=11,
+                      dt=dt, dx=dx, plane_runs=32)
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
-    nets = L.train_all(S, dt, dx, log=log)
+    nets = L.train_all(S1, dt, dx)
     t_train = time.perf_counter() - t0
-    # x faces: four training runs, one kept by its reflection at held-out
-    # frequencies (20, 25, 35, 40 GHz); every candidate's 30 GHz reflection
-    # is reported beside the kept one's
     t0 = time.perf_counter()
-    nets, cands = L.train_x_selected(S, nets, log=log)
-    t_select = time.perf_counter() - t0
+    S3 = L.collect_3d(GridDims(128, 128, 128), runs=8, steps=700, every=10, per_face=300000,
+                      seed=23, dt=dt, dx=dx, plane_runs=24)
+    lin3 = L.linear_all(S3, dt, dx)
+    t_data3 = time.perf_counter() - t0
+    log(f"data {t_data:.1f}s, training {t_train:.1f}s, 3D data + linear fit {t_data3:.1f}s")
     r_learned, _, ref = L.reflection("learned", nets)
-    r_cands = [{"seeds": c[0], "validation": c[1],
-                "test": L.reflection("learned", c[2], reference=ref)[0]} for c in cands]
-    log("candidates (seeds, validation R, 30 GHz R): "
-        + "; ".join(f"{c['seeds']} {c['validation']:.2e} {c['test']:.2e}" for c in r_cands))
     lin = L.FaceNets(nets.params.copy(), dt, dx)
     lin.params[:, 5248:5312] = 0.0
     r_lin, _, _ = L.reflection("learned", lin, reference=ref)
+    r_3lin, _, _ = L.reflection("learned", lin3, reference=ref)
     r_zero, _, _ = L.reflection("zero", reference=ref)
     r_c8, _, _ = L.reflection("cpml", cpml_voxels=8, reference=ref)
     r_c32, _, _ = L.reflection("cpml", cpml_voxels=32, reference=ref)
-    log(f"R learned {r_learned:.3e}, linear {r_lin:.3e}, no network {r_zero:.3e}, "
-        f"CPML-8 {r_c8:.3e}, CPML-32 {r_c32:.3e}")
-    # 3D: the normal-incidence networks, and the 3D-stable configuration
-    # (linear skip on P and history, fitted to 3D point-source and
-    # plane-wave data of the 64-cell-CPML twin)
-    t0 = time.perf_counter()
-    S3 = L.collect_3d(GridDims(96, 96, 96), runs=6, steps=600, every=10, per_face=300000,
-                      seed=23, dt=dt, dx=dx, plane_runs=24)
-    nets3 = L.linear_all(S3, dt, dx)
-    t_data3 = time.perf_counter() - t0
-    r_3lin, _, _ = L.reflection("learned", nets3, reference=ref)
-    r3 = {}
-    for steps in (520, 1500):
-        # a run that blew up is reported as None (non-finite)
-        fin = lambda v: v if np.isfinite(v) else None
-        r3[steps] = {k: fin(v) for k, v in L.reflection_3d(nets, steps=steps).items()}
-        r3[steps]["learned_3d_linear"] = fin(
-            L.reflection_3d(nets3, steps=steps, kinds=("learned", "cpml64"))["learned"])
-        log(f"3D, {steps} steps: " + ", ".join(f"{k} {v}" for k, v in r3[steps].items()))
-    log(f"3D-stable configuration at normal incidence: R {r_3lin:.3e}")
+    log(f"normal incidence: R learned {r_learned:.3e}, its skip {r_lin:.3e}, linear 3D {r_3lin:.3e}, "
+        f"no network {r_zero:.3e}, CPML-8 {r_c8:.3e}, CPML-32 {r_c32:.3e}")
+    fin = lambda v: v if np.isfinite(v) else None
+    r3 = {k: fin(v) for k, v in L.reflection_3d(nets, interior=248, steps=3000,
+                                                 kinds=("learned", "cpml8", "cpml32")).items()}
+    r3["linear_3d"] = fin(L.reflection_3d(lin3, interior=248, steps=3000,
+                                          kinds=("learned", "cpml64"))["learned"])
+    log("3D reflected energy, 3000 steps: " + ", ".join(f"{k} {v}" for k, v in r3.items()))
+    e_lim = L.energy_history(GridDims(256, 256, 256), nets, dt, dx)
+    e_free = L.energy_history(GridDims(256, 256, 256), nets, dt, dx, passive=False)
+    log(f"energy after 300 / 3000 steps: limited {e_lim}, without limits {e_free}")
     tp = {}
-    for kind in ("learned", "zero", "cpml32"):
-        mc, ms, cells = L.throughput(kind, nets, interior=248, steps=args.steps if args.steps < 300 else 100)
+    for kind, nt in (("learned", nets), ("learned_linear", lin3), ("zero", nets), ("cpml32", nets)):
+        mc, ms, cells = L.throughput("learned" if kind.startswith("learned") else kind, nt,
+                                     interior=248, steps=100)
         tp[kind] = {"interior_mcells": round(mc, 1), "ms_per_step": round(ms, 4), "cells": cells}
         log(f"throughput {kind}: {mc:.0f} interior Mcell/s, {ms:.3f} ms/step")
     out = {
         "metric": "reflection coefficient of the learned 4-cell layer (two-run method, 30 GHz, "
-                  "20 ppw, Courant 0.4), with 32-cell CPML and throughput beside it",
+                  "20 ppw, Courant 0.4), with 32-cell CPML, 3D reflected energy and throughput "
+                  "beside it",
         "value": r_learned,
         "unit": "reflection coefficient",
         "n_gpus": 1,
         "steps": None,
         "warmup": 0,
-        "ms_per_step": None,
+        "ms_per_step": tp["learned"]["ms_per_step"],
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 fields, bf16 network on MFMA",
         "data": "synthetic",
         "config": {
-            "workload": "learned thin boundary: 4-cell layer of plain cells ending in predicted ghost planes, per-face "
-                        "MLP 16-64-64-1 (hardtanh, odd) + linear skip, trained on 32 plane-wave runs "
-                        "per axis from the 64-cell-CPML twin",
-            "reflection": {"learned": r_learned, "linear_skip_only": r_lin,
-                           "no_network": r_zero, "cpml8": r_c8, "cpml32": r_c32,
-                           "learned_3d_linear": r_3lin},
-            "x_face_candidates": r_cands,
-            "selection": "lowest mean two-run reflection at 20, 25, 35, 40 GHz (held out; "
-                         "value is the kept candidate at 30 GHz)",
-            "seconds_selection": round(t_select, 2),
-            "reflection_3d_energy": {str(k): v for k, v in r3.items()},
-            "learned_3d_linear": "linear skip on P(depth 0..3) and history, least squares on "
-                                 "6 3D point-source runs + 24 plane-wave runs per axis",
-            "seconds_data_3d": round(t_data3, 2),
+            "workload": "learned thin boundary, 256^3: 4-cell layer ending in predicted ghost "
+                        "planes, per-face MLP 16-64-64-1 (hardtanh, odd) + linear skip trained on "
+                        "32 plane-wave runs per axis of the 64-cell-CPML twin, passive limits "
+                        f"(retain {L.RETAIN}, gamma {L.GAMMA})",
+            "reflection_normal_incidence": {"learned": r_learned, "linear_skip_only": r_lin,
+                                            "linear_3d": r_3lin, "no_network": r_zero,
+                                            "cpml8": r_c8, "cpml32": r_c32},
+            "reflection_3d_energy_3000_steps": r3,
+            "energy_300_3000_limited": e_lim,
+            "energy_300_3000_unlimited": [fin(v) for v in e_free],
             "throughput_interior_248": tp,
+            "throughput_ratio_learned_vs_cpml32": round(tp["learned"]["interior_mcells"] /
+                                                        tp["cpml32"]["interior_mcells"], 3),
             "seconds_data": round(t_data, 2),
             "seconds_training": round(t_train, 2),
+            "seconds_data_3d": round(t_data3, 2),
         },
     }
     print(json.dumps(out), flush=True)

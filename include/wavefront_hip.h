@@ -325,6 +325,32 @@ uint64_t wf_mlp_predict_noalloc_max(void);
  * boundary.  A face's axis must not be periodic.  Clears the faces' history.
  * face_mask 0 removes nothing; wf_clear_faces removes every hook. */
 int wf_set_learned_faces(wf_ctx *ctx, uint32_t face_mask, const float *params, int head);
+/* Limits on the predicted ghost values (defaults: passive = 1, retain = 0.5,
+ * gamma = 8).  passive != 0: the face keeps an energy credit C, the discrete
+ * Yee energy that has left the domain through its ghosts so far (per ghost
+ * and step sigma g (Q^a + Q^b) / 2, g the ghost value, Q the boundary field
+ * its curl term updates, before and after that update, sigma = +1 / -1 for
+ * polarization 0 / 1), and A, the energy absorbed in steps of net outflow;
+ * every ghost value may draw an equal share of C - retain A in a step, a
+ * prediction beyond it moves to the nearest value within it, and ghosts
+ * read by one boundary cell on an edge of two limited faces are limited as
+ * a pair.  The face then never returns more than (1 - retain) of the energy
+ * it has absorbed, so the fields stay bounded whatever the network
+ * predicts (the energy of the staggered Yee scheme is not positive for a
+ * ghost that grows on its own; gamma > 0 bounds |ghost| by gamma times the
+ * largest |P| of its 4-cell stencil, which rules that out).  retain in
+ * [0, 1).  Applies to the predicted faces now set and later ones. */
+int wf_set_face_limits(wf_ctx *ctx, int passive, float retain, float gamma);
+/* The same limits on replayed faces (wf_replay_faces; default off): the
+ * recorded ghosts of a thick-CPML run take the place of the predictions, a
+ * check of the limits against the exact boundary. */
+int wf_set_replay_limits(wf_ctx *ctx, int passive, float retain, float gamma);
+/* State of a predicted (or limited replayed) face: 2 (polarizations) x
+ * positions x 8 floats: g(n-1), g(n-2), P0(n-1), P0(n-2), the last step's
+ * flux through the ghost, steps limited by the energy bound, steps limited by
+ * the magnitude bound, sum of squared corrections.  Positions run fast axis
+ * first (the ghost plane's order). */
+int wf_read_face_state(wf_ctx *ctx, uint32_t face, float *dst);
 int wf_clear_faces(wf_ctx *ctx);
 
 /* Recording faces (training data from a thick-CPML run): a virtual face of

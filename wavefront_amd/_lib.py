@@ -110,6 +110,9 @@ _SIGNATURES = {
                                        POINTER(c_float)]),
     "wf_mlp_predict_noalloc_max": (c_uint64, []),
     "wf_set_learned_faces": (c_int, [c_void_p, c_uint32, POINTER(c_float), c_int]),
+    "wf_set_face_limits": (c_int, [c_void_p, c_int, c_float, c_float]),
+    "wf_read_face_state": (c_int, [c_void_p, c_uint32, POINTER(c_float)]),
+    "wf_set_replay_limits": (c_int, [c_void_p, c_int, c_float, c_float]),
     "wf_mlp_set_head": (c_int, [c_void_p, c_int]),
     "wf_clear_faces": (c_int, [c_void_p]),
     "wf_record_face_samples": (c_int, [c_void_p, c_uint32, POINTER(FaceRegion), c_uint64,

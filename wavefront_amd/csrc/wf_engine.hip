@@ -288,6 +288,10 @@ struct wf_ctx {
     // per face x_lo, x_hi, y_lo, y_hi, z_lo, z_hi
     bool hook_on[6] = {false, false, false, false, false, false};
     FaceHook hook[6] = {};
+    int face_passive = 1;      // predicted faces: passive limit on (wf_set_face_limits)
+    float *d_pool[6] = {};     // limited faces: energy credit and flux partials
+    float face_retain = 0.5f;  // share of the absorbed energy a limited face keeps
+    float face_gamma = 8.0f;   // and the magnitude bound's factor (0 = none)
     float *d_ghost[6] = {};     // [2][nslow][nfast]
     float *d_mem[6] = {};       // [2][npos][4]
     float *d_samples[6] = {};   // [rows][2][npos][kSampleStride]
@@ -488,8 +492,14 @@ int run_hooks(wf_ctx *c, bool high, const Ptrs &es, const Ptrs &hs, hipStream_t 
 {
     const FaceHook *on[3];
     int n = 0;
+    unsigned predicted = 0;
+    for (int f = 0; f < 6; ++f)
+        if (c->hook_on[f] && c->hook[f].mode != 1) predicted |= 1u << f;
     for (int f = high ? 1 : 0; f < 6; f += 2)
-        if (c->hook_on[f]) on[n++] = &c->hook[f];
+        if (c->hook_on[f]) {
+            c->hook[f].predicted = predicted;
+            on[n++] = &c->hook[f];
+        }
     if (n) WF_HIP(launch_face_hooks(c->g, es, hs, on, n, c->d_step, st));
     return WF_OK;
 }
@@ -1650,6 +1660,7 @@ void clear_face(wf_ctx *c, int f)
     dfree(c->d_mem[f]);
     dfree(c->d_samples[f]);
     dfree(c->d_net[f]);
+    dfree(c->d_pool[f]);
     c->hook_on[f] = false;
     c->hook[f] = FaceHook{};
     for (int q = 0; q < 2; ++q) c->p.ghost[f][q] = c->alt.ghost[f][q] = nullptr;
@@ -1703,6 +1714,80 @@ int wf_clear_faces(wf_ctx *c)
     return WF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Energy credit (zeroed) and flux partials of face f when its ghosts are
+// limited; none otherwise.
+int set_pool(wf_ctx *c, int f, int passive)
+{
+    if (!passive) {
+        c->hook[f].credit = c->hook[f].parts = nullptr;
+        return WF_OK;
+    }
+    const size_t bytes = (16 + (size_t)kPartsFloats) * sizeof(float);
+    if (!c->d_pool[f])
+        if (int rc = dmalloc(&c->d_pool[f], bytes)) return rc;
+    if (hipMemset(c->d_pool[f], 0, bytes) != hipSuccess) return fail(WF_ERR_HIP, "credit clear failed");
+    c->hook[f].credit = c->d_pool[f];
+    c->hook[f].parts = c->d_pool[f] + 16;
+    return WF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int wf_set_face_limits(wf_ctx *c, int passive, float retain, float gamma)
+{
+    clear_error();
+    if (!c) return fail(WF_ERR_INVALID, "wf_set_face_limits: null context");
+    if (!(gamma >= 0.f) || !std::isfinite(gamma))
+        return fail(WF_ERR_INVALID, "wf_set_face_limits: gamma must be finite and >= 0");
+    if (!(retain >= 0.f && retain < 1.f))
+        return fail(WF_ERR_INVALID, "wf_set_face_limits: retain must lie in [0, 1)");
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    drop_graph_exec(c);
+    c->face_passive = passive != 0;
+    c->face_gamma = gamma;
+    c->face_retain = retain;
+    for (int f = 0; f < 6; ++f)
+        if (c->hook_on[f] && c->hook[f].mode == 0) {
+            if (int rc = set_pool(c, f, c->face_passive)) return rc;
+            c->hook[f].passive = c->face_passive;
+            c->hook[f].gamma = c->face_gamma;
+            c->hook[f].retain = c->face_retain;
+        }
+    return WF_OK;
+}
+
+int wf_set_replay_limits(wf_ctx *c, int passive, float retain, float gamma)
+{
+    clear_error();
+    if (!c) return fail(WF_ERR_INVALID, "wf_set_replay_limits: null context");
+    if (!(gamma >= 0.f) || !std::isfinite(gamma))
+        return fail(WF_ERR_INVALID, "wf_set_replay_limits: gamma must be finite and >= 0");
+    if (!(retain >= 0.f && retain < 1.f))
+        return fail(WF_ERR_INVALID, "wf_set_replay_limits: retain must lie in [0, 1)");
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    drop_graph_exec(c);
+    for (int f = 0; f < 6; ++f)
+        if (c->hook_on[f] && c->hook[f].mode == 2) {
+            if (passive && !c->d_mem[f]) {
+                const uint64_t bytes = 2ull * c->hook[f].nfast * c->hook[f].nslow * kMemStride * sizeof(float);
+                if (int rc = dmalloc(&c->d_mem[f], bytes)) return rc;
+                WF_HIP(hipMemset(c->d_mem[f], 0, bytes));
+                c->hook[f].mem = c->d_mem[f];
+            }
+            if (int rc = set_pool(c, f, passive != 0)) return rc;
+            c->hook[f].passive = passive != 0;
+            c->hook[f].gamma = gamma;
+            c->hook[f].retain = retain;
+        }
+    return WF_OK;
+}
+
 int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params, int head)
 {
     clear_error();
@@ -1734,11 +1819,11 @@ int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params, int
         const FaceGeom fg = face_geom(c, f);
         const uint64_t npos = (uint64_t)fg.nf * fg.ns;
         rc = dmalloc(&c->d_ghost[f], 2 * npos * sizeof(float));
-        if (!rc) rc = dmalloc(&c->d_mem[f], 2 * npos * 4 * sizeof(float));
+        if (!rc) rc = dmalloc(&c->d_mem[f], 2 * npos * kMemStride * sizeof(float));
         if (!rc) rc = dmalloc(&c->d_net[f], sizeof(MlpPacked));
         if (rc) break;
         if (hipMemset(c->d_ghost[f], 0, 2 * npos * sizeof(float)) != hipSuccess ||
-            hipMemset(c->d_mem[f], 0, 2 * npos * 4 * sizeof(float)) != hipSuccess ||
+            hipMemset(c->d_mem[f], 0, 2 * npos * kMemStride * sizeof(float)) != hipSuccess ||
             hipMemcpy(d_par, params + (size_t)f * kMlpParams, kMlpParams * sizeof(float),
                       hipMemcpyHostToDevice) != hipSuccess ||
             pack_mlp(d_par, head, c->d_net[f], c->stream) != hipSuccess ||
@@ -1759,6 +1844,10 @@ int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params, int
         h.ns = fg.ns;
         h.Ta = c->g.T[fg.a];
         h.mode = 0;
+        if ((rc = set_pool(c, f, c->face_passive))) break;
+        h.passive = c->face_passive;
+        h.gamma = c->face_gamma;
+        h.retain = c->face_retain;
         h.ghost[0] = c->d_ghost[f];
         h.ghost[1] = c->d_ghost[f] + npos;
         h.mem = c->d_mem[f];
@@ -1800,10 +1889,10 @@ int wf_record_face_samples(wf_ctx *c, uint32_t face_mask, const wf_face_region *
         const wf_face_region &r = regions[f];
         const uint64_t npos = (uint64_t)r.nfast * r.nslow;
         const uint64_t nsamp = rows * 2 * npos * kSampleStride;
-        rc = dmalloc(&c->d_mem[f], 2 * npos * 4 * sizeof(float));
+        rc = dmalloc(&c->d_mem[f], 2 * npos * kMemStride * sizeof(float));
         if (!rc) rc = dmalloc(&c->d_samples[f], nsamp * sizeof(float));
         if (rc) break;
-        if (hipMemset(c->d_mem[f], 0, 2 * npos * 4 * sizeof(float)) != hipSuccess ||
+        if (hipMemset(c->d_mem[f], 0, 2 * npos * kMemStride * sizeof(float)) != hipSuccess ||
             hipMemset(c->d_samples[f], 0, nsamp * sizeof(float)) != hipSuccess) {
             rc = fail(WF_ERR_HIP, "wf_record_face_samples: clear failed");
             break;
@@ -1885,6 +1974,20 @@ int wf_replay_faces(wf_ctx *c, uint32_t face_mask, uint64_t rows, const float *c
     }
     update_ghost_reads(c);
     return rc;
+}
+
+int wf_read_face_state(wf_ctx *c, uint32_t face, float *dst)
+{
+    clear_error();
+    if (!c || !dst) return fail(WF_ERR_INVALID, "wf_read_face_state: null pointer");
+    if (face >= 6 || !c->hook_on[face] || c->hook[face].mode == 1 || !c->d_mem[face])
+        return fail(WF_ERR_STATE, "face %u has no predicted ghost plane", face);
+    const FaceHook &h = c->hook[face];
+    WF_HIP(hipSetDevice(c->device));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    WF_HIP(hipMemcpy(dst, c->d_mem[face], 2ull * h.nfast * h.nslow * kMemStride * sizeof(float),
+                     hipMemcpyDeviceToHost));
+    return WF_OK;
 }
 
 uint64_t wf_face_sample_count(const wf_ctx *c, uint32_t face)

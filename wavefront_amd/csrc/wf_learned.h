@@ -37,6 +37,13 @@ constexpr int kOffWL = kOffB3 + 1;
 constexpr int kMlpParams = kOffWL + kMlpIn;  // 5329
 // A recorded sample: 16 scaled inputs, target / s, target, s, 1 float padding.
 constexpr int kSampleStride = 20;
+// Floats per (polarization, position) of a face's memory.
+constexpr int kMemStride = 8;
+// Flux partials per face and step: one per hook workgroup (at most
+// kFluxParts), then kEdgeParts per edge family the face belongs to (two).
+constexpr int kFluxParts = 1024;
+constexpr int kEdgeParts = 64;
+constexpr int kPartsFloats = kFluxParts + 2 * kEdgeParts;
 
 typedef short bfv4 __attribute__((ext_vector_type(4)));  // 4 x bf16 (raw bits)
 typedef float fv4 __attribute__((ext_vector_type(4)));
@@ -81,8 +88,18 @@ struct FaceHook {
     int na, nf, ns;     // axis lengths: normal, fast, slow
     int Ta;             // graded CPML depth on the normal axis
     int mode;           // 0 predict, 1 record, 2 replay recorded ghosts
+    int passive;        // modes 0, 2: limit the ghosts so the face stays passive
+    unsigned predicted; // faces (bit f) whose ghost planes are predicted or replayed
+    float *credit;      // passive: the face's energy credit (one float)
+    float *parts;       // passive: per-step flux partials, kFluxParts from the hook
+                        // blocks, then 2 x kEdgeParts from the edge blocks
+    float gamma;        // passive: |ghost| <= gamma max_d |P(d)| (0 = no bound)
+    float retain;       // passive: net outflow >= retain x the absorbed energy (0..1)
     float *ghost[2];    // mode 0: ghost planes [nslow][nfast], polarization 0 / 1
-    float *mem;         // [2][npos][4]: g(n-1), g(n-2), P0(n-1), P0(n-2)
+    float *mem;         // [2][npos][kMemStride]: g(n-1), g(n-2), P0(n-1), P0(n-2),
+                        // the last step's flux through the ghost, steps limited
+                        // by the energy bound and by the magnitude bound, and the
+                        // sum of squared corrections
     float *samples;     // modes 1, 2: [rows][2][npos][kSampleStride]
     uint64_t emit_base, emit_every, emit_rows;
     const MlpPacked *net;

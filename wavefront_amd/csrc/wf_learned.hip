@@ -265,6 +265,18 @@ __device__ __forceinline__ float sum16(float v)
     return v;
 }
 
+// Sum over the 64 lanes by a fixed butterfly (every lane gets the total).
+__device__ __forceinline__ float wave_sum(float v)
+{
+    v = v + __shfl_xor(v, 1);
+    v = v + __shfl_xor(v, 2);
+    v = v + __shfl_xor(v, 4);
+    v = v + __shfl_xor(v, 8);
+    v = v + __shfl_xor(v, 16);
+    v = v + __shfl_xor(v, 32);
+    return v;
+}
+
 __device__ __forceinline__ fv4 sum16v(fv4 v) { return fv4{sum16(v.x), sum16(v.y), sum16(v.z), sum16(v.w)}; }
 
 // Mean-squared-error gradient of the whole sample set: every wave loops
@@ -508,6 +520,125 @@ __device__ __forceinline__ const float *psi_ptr(const Ptrs &p, int a, int t)
     }
 }
 
+// Passive limit on a predicted ghost value (scaled units: every field value
+// divided by the sample's input scale s).
+//
+// The Yee scheme conserves two staggered energies, W^n = sum eps E^n.E^n / 2 +
+// mu H^{n-1/2}.H^{n+1/2} / 2 and V^{n+1/2} = sum mu H^{n+1/2}.H^{n+1/2} / 2 +
+// eps E^n.E^{n+1} / 2 (cell volumes and material coefficients folded in);
+// with ghost planes they change only by the flux through them.  Each ghost
+// value is paired with the boundary field its curl term updates (Q(0)), and
+// with sigma = +1 for polarization 0 and -1 for polarization 1 the energy
+// leaving through it in one step is
+//   low face (ghost H_g^{n+1/2}, boundary E, W-form):
+//       f = sigma H_g (Q(0)^n + Q(0)^{n+1}) / 2,
+//       Q(0)^{n+1} = ca Q(0)^n + cb (R - sigma H_g);
+//   high face (ghost E_g^n, boundary H, V-form):
+//       f = sigma E_g (Q(0)^{n-1/2} + Q(0)^{n+1/2}) / 2,
+//       Q(0)^{n+1/2} = da Q(0)^{n-1/2} - db (R + sigma E_g),
+// R being the boundary cell's curl without the ghost term.  Both are
+// f(g) = sigma B g - q g^2 / 2 with B = ((1 + ca) Q(0) + cb R) / 2, q = cb
+// (low) or B = ((1 + da) Q(0) - db R) / 2, q = db (high): a concave quadratic
+// in the new ghost value alone.  (A constraint on the time average of two
+// successive ghost values, the W-form of the high face, would admit a ghost
+// that alternates in sign every step, which no flux sees; summed over steps
+// the V-form flux equals the W-form flux up to one instantaneous term.)
+//
+// The face keeps one energy credit C, the energy that has left through it so
+// far (exact ghosts of a thick-CPML run never take it below zero, but a single
+// position can: reactive energy leaves at one position and returns at
+// another).  Each of the face's N = 2 x positions ghost values may draw C / N:
+// a prediction with f(g) < -C / N is moved to the nearest value with
+// f(g) = -C / N, inside the interval between the quadratic's roots, which
+// always contains 0.  The face's total flux in the step is then at least -C,
+// so C stays non-negative and the face never returns more energy than it has
+// absorbed, whatever the network predicts.  Second, gamma > 0 bounds
+// |ghost| <= gamma max_d |P(d)|.  The boundary cell on an edge shared with
+// another limited face reads both faces' ghosts; such values get only the
+// magnitude bound here and are limited in pairs by k_edge_limit.  Exact for
+// faces without a CPML on the boundary cell (the learned domains have none).
+// Returns the limited prediction on lanes 0..15, its flux (raw units) in
+// `flux`, and the statistics: steps limited by the energy bound and by the
+// magnitude bound, and the sum of squared corrections.
+__device__ __forceinline__ float limit_ghost(const Geo &g, const Ptrs &es, const Ptrs &hs,
+                                             const FaceHook &h, int pol, int pos, bool live,
+                                             int lane, fv4 u, float s, float rs,
+                                             const float *mem, float pred, float &flux,
+                                             float *mem_stats)
+{
+    const int sl = lane & 15;
+    const int a = h.axis, t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+    // every lane takes part in the shuffle; lanes 0..15 hold P(0..3) in u
+    const float q0 = __shfl(u.x, sl + 16);  // Q(0)
+    const float pm = fmaxf(fmaxf(fabsf(u.x), fabsf(u.y)), fmaxf(fabsf(u.z), fabsf(u.w)));
+    flux = 0.f;
+    mem_stats[0] = mem[5];
+    mem_stats[1] = mem[6];
+    mem_stats[2] = mem[7];
+    if (lane >= 16 || !live) return pred;
+    const float sg = pol ? -1.0f : 1.0f;
+    // this value's share of the credit, in scaled units (inf for a tiny s:
+    // no bound; 0 stays 0)
+    const float c0 = fmaxf(h.credit[0] - h.retain * h.credit[1], 0.f) / (float)(2 * h.nfast * h.nslow);
+    const float cs = c0 > 0.f ? (c0 * rs) * rs : 0.f;
+    const float inf = __builtin_inff();
+    const int f = h.fast0 + pos % h.nfast, sc = h.slow0 + pos / h.nfast;
+    const size_t cell = face_cell(g, a, h.b0, f, sc);
+    // the normal component beside the boundary cell along the tangential
+    // axis t of this polarization's curl: one cell back (low face, H) or
+    // forward (high face, E)
+    const int t = pol ? t2 : t1;
+    const int ct = t == (t1 < t2 ? t1 : t2) ? f : sc;
+    const int nt = t == 0 ? g.nx : (t == 1 ? g.ny : g.nz);
+    const size_t st = t == 0 ? 1 : (t == 1 ? (size_t)g.nx : (size_t)g.nx * g.ny);
+    const bool wrap = g.periodic >> t & 1u;
+    if (!wrap && ct == (h.hi ? nt - 1 : 0) && (h.predicted >> (2 * t + h.hi) & 1u)) {
+        // on an edge with another limited face: k_edge_limit takes the pair
+        const float gl = h.gamma > 0.f ? fminf(fmaxf(pred, -h.gamma * pm), h.gamma * pm) : pred;
+        mem_stats[1] += gl != pred ? 1.f : 0.f;
+        return gl;
+    }
+    const float4 cf = es.coef[es.id[cell]];
+    float B, cq;
+    if (h.hi) {
+        const float *Ea = field_ptr(es, a);
+        const float e0 = Ea[cell];
+        const float e1 = ct + 1 < nt ? Ea[cell + st] : (wrap ? Ea[cell - (size_t)(nt - 1) * st] : 0.f);
+        const float d = (e1 - e0) * rs;
+        const float R = pol ? d + u.x : -u.x - d;  // u.x = P(0) = E_t1 / E_t2 at the boundary
+        B = 0.5f * ((1.0f + cf.z) * q0 - cf.w * R);
+        cq = cf.w;
+    } else {
+        const float *Ha = field_ptr(hs, 3 + a);
+        const float h0 = Ha[cell];
+        const float hm = ct > 0 ? Ha[cell - st] : (wrap ? Ha[cell + (size_t)(nt - 1) * st] : 0.f);
+        const float d = (h0 - hm) * rs;
+        const float R = pol ? d - u.x : u.x - d;  // u.x = P(0) = H_t1 / H_t2 at the boundary
+        B = 0.5f * ((1.0f + cf.x) * q0 + cf.y * R);
+        cq = cf.y;
+    }
+    // f(g) = sg B g - cq g^2 / 2 >= -cs  <=>  g between the roots
+    float lo = -inf, hi = inf;
+    if (cq > 0.f) {
+        const float disc = sqrtf(B * B + 2.0f * cq * cs);
+        lo = (sg * B - disc) / cq;
+        hi = (sg * B + disc) / cq;
+    }
+    float glo = lo, ghi = hi;
+    if (h.gamma > 0.f) {
+        glo = fmaxf(lo, -h.gamma * pm);
+        ghi = fminf(hi, h.gamma * pm);
+    }
+    const float gl = glo <= ghi ? fminf(fmaxf(pred, glo), ghi) : 0.f;
+    flux = (sg * B * gl - 0.5f * cq * gl * gl) * (s * s);
+    const bool by_energy = pred < lo || pred > hi, by_mag = !by_energy && gl != pred;
+    mem_stats[0] += by_energy ? 1.f : 0.f;
+    mem_stats[1] += by_mag ? 1.f : 0.f;
+    const float dg = (gl - pred) * s;
+    mem_stats[2] += dg * dg;
+    return gl;
+}
+
 // Inputs per sample (face position, polarization): P = the component whose
 // ghost value the boundary cell's update reads, Q = its partner in the
 // curl, d = depth from the boundary cell inward:
@@ -556,7 +687,7 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
         if (k % h.emit_every == 0 && k / h.emit_every < h.emit_rows) row = (long)(k / h.emit_every);
     }
 
-    if (h.mode == 2) {
+    if (h.mode == 2 && !h.passive) {
         // replay: ghost values recorded by a thick-CPML run, step by step
         const uint64_t k = *step - h.emit_base;
         for (int grp = wave; grp < 2 * gpp; grp += nwaves) {
@@ -598,13 +729,14 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
                 x[j] = gr ? Ps[face_psi(g, a, Ta, na, n, f, sc)] : 0.f;
             }
         } else {
-            x = *reinterpret_cast<const fv4 *>(h.mem + ((size_t)pol * npos + pos) * 4);
+            x = *reinterpret_cast<const fv4 *>(h.mem + ((size_t)pol * npos + pos) * kMemStride);
         }
         return x;
     };
 
     // Each wave takes groups wave, wave + nwaves, ...; the next group's
     // inputs are loaded before the current one is evaluated.
+    float wflux = 0.f;  // this wave's flux through its ghosts in this step
     fv4 xn = wave < 2 * gpp ? gather(wave) : fv4{0.f, 0.f, 0.f, 0.f};
     for (int grp = wave; grp < 2 * gpp; grp += nwaves) {
         const fv4 x = xn;
@@ -612,7 +744,7 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
         const int pol = grp / gpp;
         const int pos = (grp - pol * gpp) * 16 + sl;
         const bool live = pos < npos;
-        float *mem = h.mem + ((size_t)pol * npos + (live ? pos : 0)) * 4;
+        float *mem = h.mem + ((size_t)pol * npos + (live ? pos : 0)) * kMemStride;
         // scale: the largest magnitude of the 16 inputs (no squares, so no
         // underflow for the tiny fields ahead of a wave)
         float s = fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w)));
@@ -626,14 +758,28 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
         const fv4 u = x * rs;
         // history on lanes 0..15: their own P(0) and the memory lanes' values
         const float g1 = __shfl(x.x, sl + 48), p1 = __shfl(x.z, sl + 48);
-        float gnew;
+        float gnew, mem_stats[3] = {0.f, 0.f, 0.f}, flux = 0.f;
         if (h.mode == 0) {
             Acts t;
             const fv4 ui = mlp_in(r.head, u, sl);
             float sk;
             const float m = mlp_forward(r, to_bf4(ui), skip_in(r.head, u, ui), t, sk);
-            const float pred =
+            float pred =
                 head_out(r.head, m, sk, u.x, __shfl(u.x, sl + 48), __shfl(u.z, sl + 48));
+            if (h.passive) pred = limit_ghost(g, es, hs, h, pol, pos, live, lane, u, s, rs, mem, pred, flux,
+                                               mem_stats);
+            gnew = s * pred;
+            if (lane < 16 && live) (pol == 0 ? h.ghost[0] : h.ghost[1])[pos] = gnew;
+        } else if (h.mode == 2) {
+            // replay through the limits: the recorded ghost in place of the
+            // network's prediction (a check that the exact ghosts of a
+            // thick-CPML run pass the limits untouched)
+            const uint64_t k = *step - h.emit_base;
+            float pred = 0.f;
+            if (live && k < h.emit_rows)
+                pred = h.samples[((k * 2 + pol) * npos + pos) * kSampleStride + 17] * rs;
+            pred = limit_ghost(g, es, hs, h, pol, pos, live, lane, u, s, rs, mem, pred, flux,
+                               mem_stats);
             gnew = s * pred;
             if (lane < 16 && live) (pol == 0 ? h.ghost[0] : h.ghost[1])[pos] = gnew;
         } else {
@@ -652,7 +798,146 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
                 }
             }
         }
-        if (lane < 16 && live) *reinterpret_cast<fv4 *>(mem) = fv4{gnew, g1, x.x, p1};
+        wflux += sum16(flux);  // the 16 samples' flux, summed in lane order
+        if (lane < 16 && live) {
+            *reinterpret_cast<fv4 *>(mem) = fv4{gnew, g1, x.x, p1};
+            if (h.mode != 1) *reinterpret_cast<fv4 *>(mem + 4) = fv4{flux, mem_stats[0], mem_stats[1], mem_stats[2]};
+        }
+    }
+    if (h.passive && h.mode != 1) {
+        // the workgroup's flux partial, its four waves summed in order
+        __shared__ float red[4];
+        if (lane == 0) red[threadIdx.x >> 6] = wflux;
+        __syncthreads();
+        if (threadIdx.x == 0) h.parts[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+    }
+}
+
+// Edges shared by two limited faces of one side.  The boundary cell on such
+// an edge (its component along the edge, c) reads one ghost of each face:
+// face A on axis a = k (polarization 0) and face B on axis b = k + 1
+// (polarization 1), c = k + 2.  Its update holds both as v = g_A - g_B (low
+// faces: E_c^{n+1} = ca E_c + cb (R0 - v), R0 = H_b - H_a at the cell; high
+// faces: H_c^{n+1/2} = da H_c - db (R0 + v), R0 = E_a - E_b), so the energy
+// leaving through the pair is f(v) = B v - q v^2 / 2 with B and q as for one
+// ghost in limit_ghost, and only v matters: v is bounded by the pair's share
+// of the two credits (C_A / N_A + C_B / N_B), the two predictions move by
+// equal and opposite amounts (the smallest change that reaches the bound),
+// and half of f goes to each face's flux.  One thread per edge cell,
+// blockIdx.y = k; each workgroup writes its flux partial for face A (slot
+// kFluxParts + blockIdx.x) and face B (slot kFluxParts + kEdgeParts +
+// blockIdx.x).  Launched only for families whose two faces are limited.
+__global__ __launch_bounds__(256) void k_edge_limit(Geo g, Ptrs es, Ptrs hs, FaceHook hx,
+                                                    FaceHook hy, FaceHook hz, int hi)
+{
+    const int k = blockIdx.y;
+    const int a = k, b = (k + 1) % 3, c = (k + 2) % 3;
+    const FaceHook &A = a == 0 ? hx : (a == 1 ? hy : hz);
+    const FaceHook &B = b == 0 ? hx : (b == 1 ? hy : hz);
+    if (A.mode < 0 || A.mode == 1 || B.mode < 0 || B.mode == 1 || !A.passive || !B.passive) return;
+    const int n[3] = {g.nx, g.ny, g.nz};
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float fr = 0.f;
+    if (i < n[c]) {
+        int xyz[3];
+        xyz[a] = A.b0;
+        xyz[b] = B.b0;
+        xyz[c] = i;
+        const size_t cell = ((size_t)xyz[2] * g.ny + xyz[1]) * g.nx + xyz[0];
+        auto pos_on = [&](const FaceHook &F, int ax) {
+            const int u1 = (ax + 1) % 3, u2 = (ax + 2) % 3;
+            const int fa = u1 < u2 ? u1 : u2, sa = u1 < u2 ? u2 : u1;
+            return (xyz[fa] - F.fast0) + F.nfast * (xyz[sa] - F.slow0);
+        };
+        const int pA = pos_on(A, a), pB = pos_on(B, b);
+        float *mA = A.mem + (size_t)pA * kMemStride;                         // polarization 0
+        float *mB = B.mem + ((size_t)B.nfast * B.nslow + pB) * kMemStride;  // polarization 1
+        const float gA = A.ghost[0][pA], gB = B.ghost[1][pB];
+        const float4 cf = es.coef[es.id[cell]];
+        float q, f1, f2, uc, uq;
+        if (hi) {
+            q = field_ptr(hs, 3 + c)[cell];
+            f1 = field_ptr(es, a)[cell];
+            f2 = field_ptr(es, b)[cell];
+            uc = cf.z;
+            uq = cf.w;
+        } else {
+            q = field_ptr(es, c)[cell];
+            f1 = field_ptr(hs, 3 + b)[cell];
+            f2 = field_ptr(hs, 3 + a)[cell];
+            uc = cf.x;
+            uq = cf.y;
+        }
+        float s = fmaxf(fmaxf(fabsf(gA), fabsf(gB)), fmaxf(fabsf(q), fmaxf(fabsf(f1), fabsf(f2))));
+        s = fmaxf(s, 0x1p-100f);
+        const float rs = 1.0f / s;
+        const float r0 = (f1 - f2) * rs;
+        const float Bv = hi ? 0.5f * ((1.0f + uc) * (q * rs) - uq * r0)
+                            : 0.5f * ((1.0f + uc) * (q * rs) + uq * r0);
+        const float c0 = fmaxf(A.credit[0] - A.retain * A.credit[1], 0.f) / (float)(2 * A.nfast * A.nslow) +
+                         fmaxf(B.credit[0] - B.retain * B.credit[1], 0.f) / (float)(2 * B.nfast * B.nslow);
+        const float cs = c0 > 0.f ? (c0 * rs) * rs : 0.f;
+        const float v = (gA - gB) * rs;
+        float vt = v;
+        if (uq > 0.f) {
+            const float disc = sqrtf(Bv * Bv + 2.0f * uq * cs);
+            vt = fminf(fmaxf(v, (Bv - disc) / uq), (Bv + disc) / uq);
+        }
+        const float d = 0.5f * (vt - v) * s;
+        const float nA = gA + d, nB = gB - d;
+        fr = (Bv * vt - 0.5f * uq * vt * vt) * (s * s);
+        A.ghost[0][pA] = nA;
+        B.ghost[1][pB] = nB;
+        mA[0] = nA;
+        mB[0] = nB;
+        mA[4] = 0.5f * fr;
+        mB[4] = 0.5f * fr;
+        if (vt != v) {
+            mA[5] += 1.f;
+            mB[5] += 1.f;
+            mA[7] += d * d;
+            mB[7] += d * d;
+        }
+    }
+    // the workgroup's flux, summed in a fixed order (waves, then lanes)
+    __shared__ float red[4];
+    const float w = wave_sum(fr);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float t = ((red[0] + red[1]) + red[2]) + red[3];
+        A.parts[kFluxParts + blockIdx.x] = 0.5f * t;
+        B.parts[kFluxParts + kEdgeParts + blockIdx.x] = 0.5f * t;
+    }
+}
+
+// The faces' credits after a step: C = max(C + F, 0) and A = A + max(F, 0),
+// F = the step's flux (its partials summed in a fixed order: lanes stride
+// over them, then a fixed butterfly, so the result does not depend on
+// timing).  The limits allow a step's flux down to -(C - retain A), so
+// C - retain A stays >= 0: at most (1 - retain) of the absorbed energy can
+// come back.
+// One wave per face (blockIdx.x picks the hook); nparts hook partials, and
+// the edge partials of the families the face took part in (na as A, nb as B
+// workgroups; 0 = none).
+struct CreditJob {
+    float *credit, *parts;
+    int na, nb;
+};
+__global__ __launch_bounds__(64) void k_credit_reduce(CreditJob j0, CreditJob j1, CreditJob j2,
+                                                       int nparts)
+{
+    const CreditJob j = blockIdx.x == 0 ? j0 : (blockIdx.x == 1 ? j1 : j2);
+    if (!j.credit) return;
+    const int lane = threadIdx.x;
+    float t = 0.f;
+    for (int i = lane; i < nparts; i += 64) t += j.parts[i];
+    for (int i = lane; i < j.na; i += 64) t += j.parts[kFluxParts + i];
+    for (int i = lane; i < j.nb; i += 64) t += j.parts[kFluxParts + kEdgeParts + i];
+    t = wave_sum(t);
+    if (lane == 0) {
+        j.credit[0] = fmaxf(j.credit[0] + t, 0.f);  // net energy out so far
+        j.credit[1] = j.credit[1] + fmaxf(t, 0.f);  // energy absorbed in steps of net outflow
     }
 }
 
@@ -689,7 +974,7 @@ hipError_t launch_face_hooks(const Geo &g, const Ptrs &esrc, const Ptrs &hsrc,
     if (!cus[dev] &&
         hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return hipErrorInvalidDevice;
-    const int cap = std::max(1, 3 * cus[dev] / n);
+    const int cap = std::min(kFluxParts, std::max(1, 3 * cus[dev] / n));
     int blocks = 1;
     for (int k = 0; k < n; ++k) {
         const int npos = hooks[k]->nfast * hooks[k]->nslow;
@@ -699,6 +984,32 @@ hipError_t launch_face_hooks(const Geo &g, const Ptrs &esrc, const Ptrs &hsrc,
     const FaceHook &h0 = *hooks[0], &h1 = *hooks[n > 1 ? 1 : 0], &h2 = *hooks[n > 2 ? 2 : 0];
     hipLaunchKernelGGL(k_face_hooks, dim3(blocks, n), dim3(256), 0, st, g, esrc, hsrc, h0, h1,
                        h2, step);
+    // limited faces: the edges shared by two of them, then the credits
+    FaceHook ax[3];
+    bool lim[3] = {false, false, false};
+    for (int k = 0; k < 3; ++k) ax[k].mode = -1;
+    for (int k = 0; k < n; ++k) {
+        const FaceHook &h = *hooks[k];
+        ax[h.axis] = h;
+        lim[h.axis] = h.mode != 1 && h.passive;
+    }
+    if (!lim[0] && !lim[1] && !lim[2]) return hipGetLastError();
+    const int nd[3] = {g.nx, g.ny, g.nz};
+    int eb[3] = {0, 0, 0};  // edge workgroups of family k (faces on axes k and k + 1)
+    for (int k = 0; k < 3; ++k)
+        if (lim[k] && lim[(k + 1) % 3]) eb[k] = (nd[(k + 2) % 3] + 255) / 256;
+    const int emax = std::max(eb[0], std::max(eb[1], eb[2]));
+    if (emax > kEdgeParts) return hipErrorInvalidValue;
+    if (emax)
+        hipLaunchKernelGGL(k_edge_limit, dim3(emax, 3), dim3(256), 0, st, g, esrc, hsrc, ax[0],
+                           ax[1], ax[2], hooks[0]->hi);
+    CreditJob job[3] = {};
+    for (int k = 0; k < n; ++k) {
+        const FaceHook &h = *hooks[k];
+        if (!lim[h.axis]) continue;
+        job[k] = CreditJob{h.credit, h.parts, eb[h.axis], eb[(h.axis + 2) % 3]};
+    }
+    hipLaunchKernelGGL(k_credit_reduce, dim3(n), dim3(64), 0, st, job[0], job[1], job[2], blocks);
     return hipGetLastError();
 }
 
