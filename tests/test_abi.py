@@ -239,6 +239,22 @@ def test_sphere_rasterizer_equals_full_scan():
     assert np.flatnonzero(row).tolist() == list(range(96, 128))
 
 
+def test_source_columns_share_equal_sources():
+    """Sources with the same amplitude and waveform share one value column,
+    whose values equal each source's own amplitude * waveform(t)."""
+    from wavefront_amd.source import source_columns
+    g = Waveform.gaussian_pulse(4e-11, 1e-11)
+    r = Waveform.ricker_wavelet(3e10, 3.3333333e-11)
+    srcs = ([Source(x, y, 3, FieldComponent.Ez, 1.0, g) for x in range(8) for y in range(8)] +
+            [Source(1, 2, 3, FieldComponent.Ex, 0.5, r), Source(2, 2, 3, FieldComponent.Ex, 0.5, r),
+             Source(4, 4, 4, FieldComponent.Ey, 0.25, g)])
+    cols, vals = source_columns(srcs, 1.5e-12, 7, 300)
+    assert vals.shape == (300, 3)
+    assert cols.tolist() == [0] * 64 + [1, 1, 2]
+    per = source_values(srcs, 1.5e-12, 7, 300)
+    assert np.array_equal(vals[:, cols].view(np.uint32), per.view(np.uint32))
+
+
 def test_demo_sphere_matches_oracle():
     import ctypes
     g = MaterialGrid.create(GridDims(32, 24, 40))

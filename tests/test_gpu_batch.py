@@ -46,6 +46,53 @@ def test_batch_matches_oracle_with_tuning():
         eng.close()
 
 
+def test_batch_96_full_run_matches_oracle():
+    """The whole batched workload: all 96 scenes, 500 steps in one batch, the
+    schedule autotuned on the first steps as the bench does, then every
+    scene's fields, psi and probe sums bit for bit against its own CPU run
+    (the oracle's solo engine::run, src/engine.rs:1008-1099 decomposition
+    invariance; 71 G cell updates, the scenes spread over 16 host threads,
+    largest first)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    steps = 500
+    ws = workloads.batch_scenes(steps=steps)
+    print(f"[gpu-case] batch of 96 vs oracle, {steps} steps, "
+          f"{sum(w.cells() for w in ws)} cells", flush=True)
+    engs = [_engine(w, steps) for w in ws]
+    with DeviceBatch(engs) as b:
+        _, _, _, taken = b.tune(4)
+        b.step(steps - taken)
+        b.sync()
+    got = []
+    for eng in engs:
+        assert eng.step_count() == steps
+        got.append((eng.read_fields(), eng.read_psi(), eng.probe_sums()))
+        eng.close()
+    print("[gpu-case] batch of 96: GPU done, oracle runs", flush=True)
+
+    def oracle(i):
+        sim = oracle_sim_for(ws[i])
+        sim.run(steps, ws[i].dt, threads=1)
+        out = (sim.read_fields(), sim.read_psi(), sim.probe_sums())
+        sim.close()
+        return out
+
+    order = sorted(range(len(ws)), key=lambda i: -ws[i].cells())
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
+        results = dict(zip(order, pool.map(oracle, order)))
+    for i, w in enumerate(ws):
+        of, op, (oc, os_, on) = results[i]
+        gf, gp, (gc, gs, gn) = got[i]
+        assert_bitwise(gf, of, f"{w.name} fields")
+        assert_bitwise(gp, op, f"{w.name} psi")
+        assert_bitwise(gc, oc, f"{w.name} sum_cos")
+        assert_bitwise(gs, os_, f"{w.name} sum_sin")
+        assert list(gn) == list(on) == [steps] * len(w.probes)
+        assert np.abs(of).max() > 0
+
+
 def test_batch_96_matches_solo_engines():
     """All 96 scenes, 12 batched steps, against each scene stepped alone."""
     ws = workloads.batch_scenes(steps=12)

@@ -88,6 +88,64 @@ def test_random_1024_full_size_bit_exact(oracle_run, variant, monkeypatch):
     assert np.count_nonzero(of.reshape(-1)[::4099] != f0.reshape(-1)[::4099]) > 0.9 * (of.size // 4099)
 
 
+def _rel_l2(a, b, chunk=1 << 26):
+    """||a - b|| / ||b|| in float64, chunked to bound the temporary memory."""
+    va, vb = a.reshape(-1), b.reshape(-1)
+    num = den = 0.0
+    for i in range(0, va.size, chunk):
+        x = va[i:i + chunk].astype(np.float64)
+        y = vb[i:i + chunk].astype(np.float64)
+        num += float(((x - y) ** 2).sum())
+        den += float((y ** 2).sum())
+    return (num / den) ** 0.5
+
+
+def test_random_1024_full_size_contracted(oracle_run):
+    """The contracted kernel set (a*b+c as fused multiply-adds) at the full
+    1024^3 size, two steps from random fields: relative L2 field error
+    against the CPU engine <= 1e-6, and not bit-identical (the contracted
+    kernels ran)."""
+    w, f0, (of, op, _) = oracle_run
+    _say(f"random_1024 full size: GPU contracted, {STEPS} steps")
+    with DeviceEngine(w.dims, w.ids, w.table, w.pml()) as eng:
+        eng.set_fp_contract(True)
+        eng.write_fields(f0)
+        eng.set_sources(w.sources, w.dt, STEPS)
+        eng.set_probes(w.probes, w.dt, STEPS)
+        eng.step(STEPS)
+        eng.sync()
+        got = eng.read_fields()
+    rel = _rel_l2(got, of)
+    print(f"contracted 1024^3: relative L2 error {rel:.3e}")
+    assert rel <= 1e-6
+    assert not np.array_equal(got.reshape(-1)[::4097], of.reshape(-1)[::4097])
+
+
+def test_spheres_512_full_size_contracted():
+    """The contracted kernel set on the 512^3 two-sphere workload, 200 steps of
+    the Ricker source from zero fields (the pulse well past the spheres):
+    relative L2 field error against the CPU engine <= 1e-6."""
+    steps = 200
+    w = workloads.spheres_512(steps=steps)
+    _say(f"spheres_512 full size: GPU contracted, {steps} steps")
+    with DeviceEngine(w.dims, w.ids, w.table, w.pml()) as eng:
+        eng.set_fp_contract(True)
+        eng.set_sources(w.sources, w.dt, steps)
+        eng.set_probes(w.probes, w.dt, steps)
+        eng.step(steps)
+        eng.sync()
+        got = eng.read_fields()
+    sim = oracle_sim_for(w)
+    sim.run(steps, w.dt, threads=min(os.cpu_count() or 1, 16))
+    ref = sim.read_fields()
+    sim.close()
+    rel = _rel_l2(got, ref)
+    print(f"contracted 512^3 spheres, {steps} steps: relative L2 error {rel:.3e}")
+    assert np.abs(ref).max() > 0
+    assert rel <= 1e-6
+    assert not np.array_equal(got, ref)
+
+
 def test_spheres_512_full_size_snapshots(tmp_path):
     """The 512^3 two-sphere workload at its full size (134 M cells, CPML 16,
     Ricker source, 32 probes) from random fields through the snapshot stream:

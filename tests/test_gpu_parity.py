@@ -415,3 +415,30 @@ def test_contracted_kernels_within_tolerance(case):
     print(f"contracted {case}: relative L2 error {rel:.3e}")
     assert rel <= 1e-6
     assert not np.array_equal(got, ref)
+
+
+def test_sheet_source_shares_one_value_column():
+    """A 1024 x 1024 sheet of identical soft Gaussian Ez sources (1 048 576
+    sources) with a 10 000-step value table: one shared column, 40 KB of
+    values on the device (one column per source would be 42 GB).  20 steps
+    against the oracle's per-source loop, bit for bit."""
+    from wavefront_amd.source import FieldComponent as SC
+    from wavefront_amd.source import Source, Waveform, source_columns
+    w = workloads.Workload(name="sheet_1024", dims=GridDims(1024, 1024, 8), ids=None,
+                           table=MaterialTable.vacuum_filled(1.9065752e-12, 1e-3),
+                           pml_config=None, dt=1.9065752e-12, dx=1e-3, num_steps=20,
+                           sources=[Source(x, y, 4, SC.Ez, 1.0,
+                                           Waveform.gaussian_pulse(4.0e-11, 1.0e-11))
+                                    for y in range(1024) for x in range(1024)],
+                           probes=[], scene_text="")
+    cols, vals = source_columns(w.sources, w.dt, 0, 10000)
+    assert vals.shape == (10000, 1) and not cols.any()
+    _say("sheet source 1024^2, 20 of 10000 steps")
+    with DeviceEngine(w.dims, None, w.table, None) as eng:
+        eng.set_sources(w.sources, w.dt, 10000)
+        eng.step(20)
+        got = eng.read_fields()
+    sim = oracle_sim_for(w)
+    sim.run(20, w.dt, threads=8)
+    assert_bitwise(got, sim.read_fields(), "sheet fields")
+    assert np.abs(got).max() > 0

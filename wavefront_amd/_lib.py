@@ -76,6 +76,8 @@ _SIGNATURES = {
                           POINTER(c_uint32), POINTER(c_float), c_uint32, c_int, POINTER(c_void_p)]),
     "wf_destroy": (None, [c_void_p]),
     "wf_set_sources": (c_int, [c_void_p, c_uint32, POINTER(SourceSite), c_uint64, POINTER(c_float)]),
+    "wf_set_source_columns": (c_int, [c_void_p, c_uint32, POINTER(SourceSite), POINTER(c_uint32),
+                                      c_uint32, c_uint64, POINTER(c_float)]),
     "wf_set_probes": (c_int, [c_void_p, c_uint32, POINTER(ProbeSite), c_uint64, POINTER(c_double),
                               POINTER(c_double), POINTER(c_uint8), c_int]),
     "wf_step": (c_int, [c_void_p, c_uint64]),

@@ -61,22 +61,23 @@ __device__ __forceinline__ void apply_source(const Ptrs &p, const DevSource &q,
 }
 
 __device__ __forceinline__ void apply_sources(const Ptrs &p, const DevSource *__restrict__ src,
-                                              int n, const float *__restrict__ values,
+                                              int n, int ncol, const float *__restrict__ values,
                                               uint64_t row)
 {
-    const float *v = values + row * (uint64_t)n;
+    const float *v = values + row * (uint64_t)ncol;
     for (int s = 0; s < n; ++s) apply_source(p, src[s], v);
 }
 
-// One lane per group [grp[i], grp[i+1]) of the grouped table.
-__global__ void k_sources(Ptrs p, const DevSource *__restrict__ src, int n,
+// One lane per group [grp[i], grp[i+1]) of the grouped table; a step's
+// values are one row of ncol columns.
+__global__ void k_sources(Ptrs p, const DevSource *__restrict__ src, int ncol,
                           const uint32_t *__restrict__ grp, int ngrp,
                           const float *__restrict__ values, const uint64_t *__restrict__ step,
                           uint64_t base)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ngrp) return;
-    const float *v = values + (*step - base) * (uint64_t)n;
+    const float *v = values + (*step - base) * (uint64_t)ncol;
     for (uint32_t s = grp[i]; s < grp[i + 1]; ++s) apply_source(p, src[s], v);
 }
 
@@ -122,7 +123,7 @@ __global__ void k_sources_batch(const BatchScene *__restrict__ sc)
 {
     const BatchScene &d = sc[blockIdx.x];
     if (threadIdx.x != 0 || d.nsrc == 0) return;
-    apply_sources(d.p, d.src, d.nsrc, d.srcval, *d.step - d.src_base);
+    apply_sources(d.p, d.src, d.nsrc, d.ncol, d.srcval, *d.step - d.src_base);
 }
 
 __global__ void k_probes_batch(const BatchScene *__restrict__ sc)
@@ -254,6 +255,7 @@ struct wf_ctx {
 
     // sources
     uint32_t nsrc = 0;
+    uint32_t nsrc_col = 0;  // source value columns (a row per step)
     DevSource *d_src = nullptr;     // grouped by (component, voxel), list order within a group
     uint32_t *d_src_grp = nullptr;  // group starts, ngrp + 1 entries
     uint32_t nsrc_grp = 0;
@@ -344,6 +346,7 @@ void free_sources(wf_ctx *c)
     c->nsrc_grp = 0;
     dfree(c->d_srcval);
     c->nsrc = 0;
+    c->nsrc_col = 0;
     c->src_steps = 0;
 }
 
@@ -645,7 +648,7 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
     }
     if (c->nsrc) {
         hipLaunchKernelGGL(k_sources, dim3((c->nsrc_grp + 255) / 256), dim3(256), 0, st, c->p,
-                           c->d_src, (int)c->nsrc, c->d_src_grp, (int)c->nsrc_grp,
+                           c->d_src, (int)c->nsrc_col, c->d_src_grp, (int)c->nsrc_grp,
                            c->d_srcval, c->d_step, c->src_base);
     }
     if (c->nprobe) {
@@ -1143,17 +1146,20 @@ void wf_destroy(wf_ctx *c)
     delete c;
 }
 
-int wf_set_sources(wf_ctx *c, uint32_t n, const wf_source_site *sites, uint64_t nsteps,
-                   const float *values)
+int wf_set_source_columns(wf_ctx *c, uint32_t n, const wf_source_site *sites,
+                          const uint32_t *cols, uint32_t ncol, uint64_t nsteps,
+                          const float *values)
 {
     clear_error();
-    if (!c) return fail(WF_ERR_INVALID, "wf_set_sources: null context");
+    if (!c) return fail(WF_ERR_INVALID, "wf_set_source_columns: null context");
     WF_HIP(hipSetDevice(c->device));
     WF_HIP(hipStreamSynchronize(c->stream));
     free_sources(c);
     if (n == 0) return WF_OK;
-    if (!sites || (!values && nsteps)) return fail(WF_ERR_INVALID, "wf_set_sources: null pointer");
+    if (!sites || (!values && nsteps && ncol)) return fail(WF_ERR_INVALID, "wf_set_sources: null pointer");
     if (n > 0x7fffffffu) return fail(WF_ERR_INVALID, "wf_set_sources: more than 2^31 - 1 sources");
+    if (ncol == 0) return fail(WF_ERR_INVALID, "wf_set_source_columns: no value columns");
+    if (nsteps > (~0ull >> 2) / ncol) return fail(WF_ERR_INVALID, "wf_set_sources: value table too large");
     std::vector<DevSource> ds(n);
     for (uint32_t s = 0; s < n; ++s) {
         const wf_source_site &q = sites[s];
@@ -1161,10 +1167,13 @@ int wf_set_sources(wf_ctx *c, uint32_t n, const wf_source_site *sites, uint64_t 
             return fail(WF_ERR_INVALID, "source %u at (%u, %u, %u) is outside the domain", s, q.x, q.y, q.z);
         if (q.component < 0 || q.component > 2)
             return fail(WF_ERR_INVALID, "source %u: component %d is not Ex/Ey/Ez", s, q.component);
+        const uint32_t col = cols ? cols[s] : s;
+        if (col >= ncol)
+            return fail(WF_ERR_INVALID, "source %u: column %u of %u", s, col, ncol);
         ds[s].off = ((uint64_t)q.z * c->g.ny + q.y) * c->g.nx + q.x;
         ds[s].comp = q.component;
         ds[s].hard = q.hard ? 1 : 0;
-        ds[s].col = s;
+        ds[s].col = col;
         ds[s].pad = 0;
     }
     // group by (component, voxel), keeping list order inside a group
@@ -1177,15 +1186,22 @@ int wf_set_sources(wf_ctx *c, uint32_t n, const wf_source_site *sites, uint64_t 
     grp.push_back(n);
     if (int rc = dmalloc(&c->d_src, n * sizeof(DevSource))) return rc;
     if (int rc = dmalloc(&c->d_src_grp, grp.size() * sizeof(uint32_t))) return rc;
-    if (int rc = dmalloc(&c->d_srcval, nsteps * n * sizeof(float))) return rc;
+    if (int rc = dmalloc(&c->d_srcval, std::max<uint64_t>(nsteps * ncol, 1) * sizeof(float))) return rc;
     WF_HIP(hipMemcpy(c->d_src, ds.data(), n * sizeof(DevSource), hipMemcpyHostToDevice));
     WF_HIP(hipMemcpy(c->d_src_grp, grp.data(), grp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     c->nsrc_grp = (uint32_t)(grp.size() - 1);
-    if (nsteps) WF_HIP(hipMemcpy(c->d_srcval, values, nsteps * n * sizeof(float), hipMemcpyHostToDevice));
+    if (nsteps) WF_HIP(hipMemcpy(c->d_srcval, values, nsteps * ncol * sizeof(float), hipMemcpyHostToDevice));
     c->nsrc = n;
+    c->nsrc_col = ncol;
     c->src_base = c->step;
     c->src_steps = nsteps;
     return WF_OK;
+}
+
+int wf_set_sources(wf_ctx *c, uint32_t n, const wf_source_site *sites, uint64_t nsteps,
+                   const float *values)
+{
+    return wf_set_source_columns(c, n, sites, nullptr, n, nsteps, values);
 }
 
 int wf_set_probes(wf_ctx *c, uint32_t n, const wf_probe_site *sites, uint64_t nsteps,
@@ -2078,6 +2094,7 @@ int batch_upload(wf_batch *b)
         d.srcval = c->d_srcval;
         d.src_base = c->src_base;
         d.nsrc = (int)c->nsrc;
+        d.ncol = (int)c->nsrc_col;
         d.probe = c->d_probe;
         d.cos_tab = c->d_cos;
         d.sin_tab = c->d_sin;

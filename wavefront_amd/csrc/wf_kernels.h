@@ -45,7 +45,7 @@ struct DevSource {
     uint64_t off;  // voxel linear index
     int comp;      // 0..2
     int hard;
-    uint32_t col;  // the source's position in the caller's list: its column of values
+    uint32_t col;  // the source's column of values (values[row * ncol + col])
     uint32_t pad;
 };
 
@@ -191,7 +191,8 @@ struct BatchScene {
     float *trace;
     uint64_t probe_base;
     uint64_t *step;  // the scene's device step counter
-    int nsrc, nprobe, nfreq, pad;
+    int nsrc, nprobe, nfreq;
+    int ncol;  // source value columns: values[row * ncol + src.col]
 };
 
 // H / E kernels over every scene of a batch in one launch: tile t (after

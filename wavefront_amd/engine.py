@@ -20,7 +20,7 @@ from ._lib import ProbeSite, SourceSite, WavefrontError, check, lib, ptr
 from .layout import (CoeffGrid, GridDims, MaterialTable, PmlContext, VOXELS_PER_BLOCK,
                      field_grid_zeroed)
 from .probe import Probe, probe_tables
-from .source import Source, source_values
+from .source import Source, source_columns, source_values
 
 PERIODIC_X, PERIODIC_Y, PERIODIC_Z = 1, 2, 4
 
@@ -96,13 +96,19 @@ class DeviceEngine:
 
     # -- tables --------------------------------------------------------------
     def set_sources(self, sources: list[Source], dt: float, nsteps: int) -> None:
-        """Precomputes amplitude*waveform(t) for the next nsteps steps."""
+        """Precomputes amplitude*waveform(t) for the next nsteps steps, one
+        value column per distinct (amplitude, waveform)."""
         first = self.step_count()
-        vals = source_values(sources, dt, first, nsteps)
         n = len(sources)
-        sites = (SourceSite * max(n, 1))(*[s.site() for s in sources])
-        check(lib().wf_set_sources(self._h, n, sites, nsteps,
-                                   ptr(np.ascontiguousarray(vals), c_float) if n else None))
+        if n == 0:
+            check(lib().wf_set_sources(self._h, 0, None, nsteps, None))
+            self.nsources = 0
+            return
+        cols, vals = source_columns(sources, dt, first, nsteps)
+        vals = np.ascontiguousarray(vals, np.float32)
+        sites = (SourceSite * n)(*[s.site() for s in sources])
+        check(lib().wf_set_source_columns(self._h, n, sites, ptr(cols, c_uint32), vals.shape[1],
+                                          nsteps, ptr(vals, c_float)))
         self.nsources = n
 
     def set_source_values(self, sites: list[SourceSite], values: np.ndarray) -> None:

@@ -82,6 +82,25 @@ def source_values(sources, dt: float, first_step: int, nsteps: int) -> np.ndarra
     return out
 
 
+def source_columns(sources, dt: float, first_step: int, nsteps: int):
+    """Sources grouped by (amplitude, waveform): (cols[s], values[k, col]),
+    one column per distinct pair, evaluated once (amplitude * waveform(t),
+    t = (first_step+k+1)*dt); sources with equal parameters give equal
+    values, so a sheet source needs one column whatever its size."""
+    key_col: dict = {}
+    cols = np.zeros(len(sources), np.uint32)
+    reps = []
+    for i, s in enumerate(sources):
+        w = s.waveform
+        key = (np.float32(s.amplitude).tobytes(), int(w.kind), np.float32(w.p0).tobytes(),
+               np.float32(w.p1).tobytes())
+        if key not in key_col:
+            key_col[key] = len(reps)
+            reps.append(s)
+        cols[i] = key_col[key]
+    return cols, source_values(reps, dt, first_step, nsteps)
+
+
 def gaussian_from_freq(freq_hz: float) -> Waveform:
     """src/main.rs:443-449: spread = 0.5/f, t0 = 4*spread (f32)."""
     spread = np.float32(0.5) / np.float32(freq_hz)
