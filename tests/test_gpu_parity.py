@@ -244,6 +244,57 @@ def test_many_sources_with_repeated_voxels():
     eng.close()
 
 
+def test_sources_on_interior_tile_boundaries():
+    """The single-pass kernel's tiles meet at inner boundaries where an edge
+    lane reads (and recomputes) its neighbour tile's cells.  A 552 x 40 x 40
+    domain with CPML 8 (interior box x in [12, 544): three 256-wide tiles,
+    inner boundaries at x = 268 and 524), soft and hard sources of every E
+    component on the cells at x = 267, 268, 523, 524, random fields written
+    before the first step, 20 steps (graph replays included): fields, psi and
+    probes against the oracle bit for bit."""
+    from wavefront_amd.source import FieldComponent as SC, Source, Waveform
+    from wavefront_amd.probe import FieldComponent as PC, Probe
+    rng = np.random.default_rng(5)
+    d = GridDims(552, 40, 40)
+    dt, dx = 1.9065752e-12, 1e-3
+    forms = (Waveform.sinusoid(3e10), Waveform.ricker_wavelet(2e10, 5e-11),
+             Waveform.gaussian_pulse(2e-11, 1e-11))
+    srcs = []
+    for x in (267, 268, 523, 524):
+        for k in range(12):
+            srcs.append(Source(x, int(rng.integers(0, 40)), int(rng.integers(0, 40)),
+                               (SC.Ex, SC.Ey, SC.Ez)[k % 3], float(rng.uniform(-2, 2)),
+                               forms[k % 3], hard=(k % 4 == 3)))
+    probes = [Probe(x, 20, 20, PC.Ez, [2e10], 0.0) for x in (267, 268, 300)]
+    cfg = PmlConfig(thickness=8)
+    prof = [PmlProfile1D.build(n, cfg, dt, dx, 8) for n in (d.nx, d.ny, d.nz)]
+    pml = PmlContext(prof, (1, 1, 1))
+    f0 = rng.standard_normal((d.block_count(), 6, 512)).astype(np.float32)
+    _say("sources on tile boundaries 552x40x40")
+    eng = DeviceEngine(d, None, MaterialTable.vacuum_filled(dt, dx), pml)
+    eng.write_fields(f0)
+    eng.set_sources(srcs, dt, 20)
+    eng.set_probes(probes, dt, 20)
+    eng.step(3)
+    eng.step(17)
+    shell, profs = oracle_pml(d, cfg, dt, dx)
+    sim = wfo.OracleSim(d.nx, d.ny, d.nz, table=wfo.vacuum_table(dt, dx), shell_blocks=shell,
+                        profiles=profs)
+    for s_ in srcs:
+        sim.add_source(s_.x, s_.y, s_.z, int(s_.component), s_.amplitude, s_.waveform.kind,
+                       s_.waveform.p0, s_.waveform.p1, hard=s_.hard)
+    for p_ in probes:
+        sim.add_probe(p_.x, p_.y, p_.z, int(p_.component), p_.frequencies_hz, 0.0)
+    sim.write_fields(f0)
+    sim.run(20, dt, threads=4)
+    assert_bitwise(eng.read_fields(), sim.read_fields(), "fields")
+    assert_bitwise(eng.read_psi(), sim.read_psi(), "psi")
+    sc, ss, n = eng.probe_sums()
+    oc, os_, on = sim.probe_sums()
+    assert_bitwise(sc, oc, "sum_cos")
+    eng.close()
+
+
 def test_scene_spheres_128():
     """The two-sphere scene scaled to 128^3 (lossy sphere, CPML 16, Ricker)."""
     w = workloads.spheres_512(steps=60, n=128)
