@@ -251,7 +251,8 @@ int wf_batch_sync(wf_batch *batch);
 /* nsteps steps between two events on the batch stream; *total_ms = their time. */
 int wf_batch_time_steps(wf_batch *batch, uint64_t nsteps, double *total_ms);
 /* Autotuner: runs steps_per_candidate real steps (the first untimed) under
- * each candidate tiling and schedule (z-chunk, lanes-per-row cap, grid: 0 =
+ * each candidate tiling and schedule (z-chunk, lanes-per-row cap -- a power
+ * of two up to 64, or 0 for rows fitted to each scene's width --, grid: 0 =
  * one workgroup per tile, n = n persistent workgroups striding over the
  * tiles), keeps the fastest and reports it; *steps_taken = steps advanced
  * (results do not depend on the tiling).  WF_BATCH_ZCHUNK sets the initial

@@ -19,23 +19,26 @@ def _engine(w, steps):
 
 
 def test_batch_matches_oracle_with_tuning():
-    """The first six scenes of the 96-scene set plus two hand-picked shapes,
-    36 steps: 9 tuner candidates x 3 real steps each, then 9 more."""
-    ws = workloads.batch_scenes(nscenes=6, steps=36)
-    ws += workloads.batch_scenes(nscenes=2, steps=36, seed=11,
-                                 edges=[(32, 48, 40), (200, 32, 64)], max_edge=200)
-    print("[gpu-case] batch of 8 vs oracle, 36 steps: "
+    """The first six scenes of the 96-scene set plus three hand-picked shapes
+    (rows of 10 quads and of 66 quads, split in two fitted segments of 33),
+    48 steps: 13 tuner candidates x 3 real steps each, then 9 more."""
+    ws = workloads.batch_scenes(nscenes=6, steps=48)
+    ws += workloads.batch_scenes(nscenes=3, steps=48, seed=11,
+                                 edges=[(32, 48, 40), (200, 32, 64), (40, 24, 32)], max_edge=264)
+    ws[-1] = workloads.batch_scenes(nscenes=1, steps=48, seed=12, edges=[(264, 16, 24)],
+                                    max_edge=264)[0]
+    print("[gpu-case] batch of 9 vs oracle, 48 steps: "
           + ", ".join(w.name for w in ws), flush=True)
-    engs = [_engine(w, 36) for w in ws]
+    engs = [_engine(w, 48) for w in ws]
     with DeviceBatch(engs) as b:
         z, lpr, grid, taken = b.tune(3)
-        assert taken == 27 and z >= 1 and lpr in (16, 64) and grid in (0, 2048, 8192)
-        b.step(36 - taken)
+        assert taken == 39 and z >= 1 and lpr in (0, 16, 64) and grid in (0, 2048, 8192)
+        b.step(48 - taken)
         b.sync()
     for w, eng in zip(ws, engs):
-        assert eng.step_count() == 36
+        assert eng.step_count() == 48
         sim = oracle_sim_for(w)
-        sim.run(36, w.dt, threads=8)
+        sim.run(48, w.dt, threads=8)
         assert_bitwise(eng.read_fields(), sim.read_fields(), f"{w.name} fields")
         assert_bitwise(eng.read_psi(), sim.read_psi(), f"{w.name} psi")
         sc, ss, n = eng.probe_sums()
@@ -105,7 +108,9 @@ def test_batch_96_matches_solo_engines():
         b.set_tiling(16, 16)
         b.step(4)
         b.set_tiling(8, 64, 1024)  # persistent: 1024 workgroups stride over the tiles
-        b.step(4)
+        b.step(2)
+        b.set_tiling(4, 0)  # rows fitted to each scene's width
+        b.step(2)
         b.sync()
     for w, eng in zip(ws, engs):
         solo = _engine(w, 12)

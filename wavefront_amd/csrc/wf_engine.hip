@@ -367,13 +367,21 @@ void free_probes(wf_ctx *c)
 
 // Tiles a box for the split kernels: lanes per row = the smallest power of
 // two covering the box's quads (at most lpr_cap <= 64), kSY waves of 64/lpr
-// rows each.
+// rows each.  lpr_cap = 0 fits the rows instead: a row of q quads takes
+// ceil(q / ceil(q / 64)) lanes (q itself up to 64), floor(64 / lpr) rows per
+// wave, so narrow rows leave few lanes idle (40 cells: 10 lanes, 6 rows
+// and 4 idle lanes per wave instead of 16-lane rows with 6 idle each).
 Box split_box(int x0, int x1, int y0, int y1, int z0, int z1, int zchunk, int lpr_cap = 64)
 {
     Box b{x0, x1, y0, y1, z0, z1, 64, 0, 0, 0, 0};
     const int quads = (x1 - x0 + 3) / 4;
     int lpr = 1;
-    while (lpr < quads && lpr < lpr_cap) lpr *= 2;
+    if (lpr_cap <= 0) {
+        const int segs = (std::max(quads, 1) + 63) / 64;
+        lpr = (std::max(quads, 1) + segs - 1) / segs;
+    } else {
+        while (lpr < quads && lpr < lpr_cap) lpr *= 2;
+    }
     b.lpr = lpr;
     const int rows = split_rows_per_wg() * (64 / lpr);
     b.gx = (quads + lpr - 1) / lpr;
@@ -2250,9 +2258,12 @@ int wf_batch_tune(wf_batch *b, uint32_t steps_per_candidate, int *zchunk, int *l
     // small scenes; a 16-lane cap avoids idle lanes on rows of 33-48 quads;
     // a persistent grid (8 or 32 workgroups per CU) trades launch width for
     // fewer, longer-lived workgroups
+    // lanes-per-row cap 0: rows fitted to the scene's width (split_box)
     static const int cand[][3] = {{4, 64, 0},    {8, 64, 0},    {16, 64, 0},
                                   {32, 64, 0},   {8, 16, 0},    {16, 16, 0},
-                                  {8, 64, 2048}, {16, 64, 2048}, {8, 64, 8192}};
+                                  {8, 64, 2048}, {16, 64, 2048}, {8, 64, 8192},
+                                  {2, 0, 0},     {4, 0, 0},     {8, 0, 0},
+                                  {4, 0, 2048}};
     const int ncand = (int)(sizeof cand / sizeof cand[0]);
     if (int rc = batch_prepare(b, (uint64_t)ncand * steps_per_candidate)) return rc;
     hipEvent_t e0, e1;
@@ -2298,7 +2309,7 @@ int wf_batch_tune(wf_batch *b, uint32_t steps_per_candidate, int *zchunk, int *l
 int wf_batch_set_tiling(wf_batch *b, int zchunk, int lanes_per_row, int grid)
 {
     if (!b) return fail(WF_ERR_INVALID, "wf_batch_set_tiling: null batch");
-    if (zchunk < 1 || lanes_per_row < 1 || lanes_per_row > 64 ||
+    if (zchunk < 1 || lanes_per_row < 0 || lanes_per_row > 64 ||
         (lanes_per_row & (lanes_per_row - 1)) || grid < 0)
         return fail(WF_ERR_INVALID,
                     "wf_batch_set_tiling: zchunk >= 1, lanes per row a power of two <= 64, grid >= 0");

@@ -59,8 +59,9 @@ __device__ __forceinline__ const Box &find_box(const BoxList &bl, int i, int &lo
 // Split kernels: update_h_field[_pml] over the boxes, then
 // update_e_field[_pml] over the same boxes.
 //
-// A wave is 64/lpr rows of lpr lanes; lane (r, l) owns the quad
-// x = x0 + 4*(lpr*tx + l) of row y = y0 + (kSY*rows_per_wave)*ty + w*rpw + r.
+// A wave is rpw = 64/lpr (rounded down) rows of lpr lanes, lpr any width up
+// to 64 (lanes past rpw*lpr idle); lane (r, l) owns the quad
+// x = x0 + 4*(lpr*tx + l) of row y = y0 + (kSY*rpw)*ty + w*rpw + r.
 // Each thread marches z over a chunk with E(z+1) (H kernel) or H(z-1)
 // (E kernel) carried in registers.  Sources and destinations are separate so
 // the same kernels serve the in-place split variant (src == dst) and the
@@ -84,7 +85,7 @@ __device__ __forceinline__ SplitGeo split_geo(const Box &b, int local, int zchun
     s.y = b.y0 + (kSY * rpw) * ty + w * rpw + r;
     s.z0 = b.z0 + tz * zchunk;
     s.z1 = min(s.z0 + zchunk, b.z1);
-    s.live = s.x < b.x1 && s.y < b.y1;
+    s.live = s.x < b.x1 && s.y < b.y1 && r < rpw;
     s.edge = hphase ? (l == b.lpr - 1 || s.x + 4 >= b.x1) : (l == 0 || s.x == b.x0);
     return s;
 }
@@ -102,7 +103,6 @@ __device__ __forceinline__ void h4_tile(const Geo &g, const Ptrs &src, const Ptr
     }
     __syncthreads();
 
-    const int lpr = b.lpr;
     const SplitGeo s = split_geo(b, local, zchunk, true);
     const bool live = s.live;
     const int x = live ? s.x : 0, y = live ? s.y : 0;
@@ -154,7 +154,8 @@ __device__ __forceinline__ void h4_tile(const Geo &g, const Ptrs &src, const Ptr
                 ex_py = ld4(src.ghost[3][1] + (size_t)z * nx + x);
             }
         }
-        float sez = __shfl_down(ez_c.x, 1, lpr), sey = __shfl_down(ey_c.x, 1, lpr);
+        // the next lane's cell; a row's last lane loads its own (edge)
+        float sez = __shfl_down(ez_c.x, 1, 64), sey = __shfl_down(ey_c.x, 1, 64);
         if (s.edge) {
             sez = 0.f;
             sey = 0.f;
@@ -272,7 +273,6 @@ __device__ __forceinline__ void e4_tile(const Geo &g, const Ptrs &hsrc, const Pt
     }
     __syncthreads();
 
-    const int lpr = b.lpr;
     const SplitGeo s = split_geo(b, local, zchunk, false);
     const bool live = s.live;
     const int x = live ? s.x : 0, y = live ? s.y : 0;
@@ -324,7 +324,7 @@ __device__ __forceinline__ void e4_tile(const Geo &g, const Ptrs &hsrc, const Pt
                 hx_my = ld4(esrc.ghost[2][1] + (size_t)z * nx + x);
             }
         }
-        float shz = __shfl_up(hz_c.w, 1, lpr), shy = __shfl_up(hy_c.w, 1, lpr);
+        float shz = __shfl_up(hz_c.w, 1, 64), shy = __shfl_up(hy_c.w, 1, 64);
         if (s.edge) {
             shz = 0.f;
             shy = 0.f;
