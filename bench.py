@@ -194,6 +194,15 @@ def run_batch_96(args):
         step_bytes += 74 * w.cells() + 8 * 4 * graded
     ms_step = ms_rest / (steps - taken)
     achieved = step_bytes / (ms_step / 1e3) / 1e9
+    # HBM bytes per step of the two batched kernels, measured with rocprofv3
+    # --pmc (FETCH_SIZE, WRITE_SIZE, scaled by a plain copy kernel)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "batch_96_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f).get("step", {})
+        if t.get("algorithmic_bytes") == step_bytes:
+            traffic = t["traffic_bytes"]
     out = {
         "metric": "Mcell-updates/s, 96 batched scenes, FP32, 1 MI355X",
         "value": round(mcells, 1),
@@ -222,7 +231,7 @@ def run_batch_96(args):
         },
         "roofline": {"bound": "hbm", "kernel": "k_h4_batch + k_e4_batch (one step after tuning)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
     print(json.dumps(out), flush=True)
 
