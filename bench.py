@@ -237,17 +237,24 @@ def run_batch_96(args):
 
 
 def run_learned(args):
-    """The learned thin boundary at its 256^3 workload.  Per-face MLPs
-    (16-64-64-1, BF16 on MFMA) trained on the GPU on samples of 64-cell-CPML
-    plane-wave runs (32 per axis, Gaussian / Ricker, 10-45 GHz), with the
-    passive limits on; beside them the linear 3D predictor fitted to 3D
-    point-source runs.  Reported: the reference's two-run reflection at
-    normal incidence (30 GHz, 20 points per wavelength, Courant 0.4) for the
-    MLP layer, its linear skip alone, the zero boundary and 8-/32-cell CPML;
-    the reflected energy of a 3D Ricker point source over 3000 steps in a
-    248^3 interior (256^3 with the learned layer) against a 64-cell CPML run;
-    the field energy after 300 and 3000 steps with and without the limits;
-    and the step throughput of the interior for each boundary."""
+    """The learned thin boundary at its 256^3 workload.  The shipped layer:
+    per face the second-order Higdon update (the Higdon head: a fixed
+    one-way factor at 45 degrees, memory damped by 0.95, times an adaptive
+    one) whose adaptive factor's coefficient is the output of a 16-64-64-1
+    MLP (BF16 weights on MFMA, FP32 accumulation) trained on the GPU on
+    samples of 64-cell-CPML plane-wave runs (32 per axis, Gaussian / Ricker,
+    10-45 GHz) and early-stopped on closed-loop validation (other
+    frequencies, another geometry).  Beside it: the same layer with the
+    network's output held at its start value (the method without the
+    network's adaptation), the direct-head MLP trained on the same data with
+    the passive limits, the linear 3D predictor fitted to 3D point-source
+    runs, the zero boundary, 8- and 32-cell CPML.  Reported: the reference's
+    two-run reflection at normal incidence (30 GHz, 20 points per
+    wavelength, Courant 0.4); the reflected energy of a 3D Ricker point
+    source over 3000 steps in a 248^3 interior (256^3 with the 4-cell layer)
+    against a 64-cell CPML run, overall and over the last 500 steps; the
+    field energy of the 256^3 domain after 300, 3000 and 6000 steps; and the
+    step throughput of the interior for each boundary."""
     import numpy as np
 
     from wavefront_amd import learned as L
@@ -259,35 +266,45 @@ def run_learned(args):
                       dt=dt, dx=dx, plane_runs=32)
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
-    nets = L.train_all(S1, dt, dx)
+    nets, table = L.train_higdon(S1, dt, dx, log=log)
     t_train = time.perf_counter() - t0
+    base = L.FaceNets(L.higdon_base(dt, dx), dt, dx, L.HEAD_HIGDON, False)
+    direct = L.train_all(S1, dt, dx)
     t0 = time.perf_counter()
     S3 = L.collect_3d(GridDims(128, 128, 128), runs=8, steps=700, every=10, per_face=300000,
                       seed=23, dt=dt, dx=dx, plane_runs=24)
     lin3 = L.linear_all(S3, dt, dx)
     t_data3 = time.perf_counter() - t0
-    log(f"data {t_data:.1f}s, training {t_train:.1f}s, 3D data + linear fit {t_data3:.1f}s")
-    r_learned, _, ref = L.reflection("learned", nets)
-    lin = L.FaceNets(nets.params.copy(), dt, dx)
-    lin.params[:, 5248:5312] = 0.0
-    r_lin, _, _ = L.reflection("learned", lin, reference=ref)
-    r_3lin, _, _ = L.reflection("learned", lin3, reference=ref)
-    r_zero, _, _ = L.reflection("zero", reference=ref)
-    r_c8, _, _ = L.reflection("cpml", cpml_voxels=8, reference=ref)
-    r_c32, _, _ = L.reflection("cpml", cpml_voxels=32, reference=ref)
-    log(f"normal incidence: R learned {r_learned:.3e}, its skip {r_lin:.3e}, linear 3D {r_3lin:.3e}, "
-        f"no network {r_zero:.3e}, CPML-8 {r_c8:.3e}, CPML-32 {r_c32:.3e}")
+    log(f"data {t_data:.1f}s, training with validation {t_train:.1f}s, 3D data + linear fit "
+        f"{t_data3:.1f}s")
+    r = {}
+    r["learned"], _, ref = L.reflection("learned", nets)
+    r["network_held"] = L.reflection("learned", base, reference=ref)[0]
+    r["direct_mlp"] = L.reflection("learned", direct, reference=ref)[0]
+    r["linear_3d"] = L.reflection("learned", lin3, reference=ref)[0]
+    r["no_network"] = L.reflection("zero", reference=ref)[0]
+    r["cpml8"] = L.reflection("cpml", cpml_voxels=8, reference=ref)[0]
+    r["cpml32"] = L.reflection("cpml", cpml_voxels=32, reference=ref)[0]
+    log("normal incidence: " + ", ".join(f"{k} {v:.3e}" for k, v in r.items()))
     fin = lambda v: v if np.isfinite(v) else None
-    r3 = {k: fin(v) for k, v in L.reflection_3d(nets, interior=248, steps=3000,
-                                                 kinds=("learned", "cpml8", "cpml32")).items()}
-    r3["linear_3d"] = fin(L.reflection_3d(lin3, interior=248, steps=3000,
-                                          kinds=("learned", "cpml64"))["learned"])
-    log("3D reflected energy, 3000 steps: " + ", ".join(f"{k} {v}" for k, v in r3.items()))
-    e_lim = L.energy_history(GridDims(256, 256, 256), nets, dt, dx)
-    e_free = L.energy_history(GridDims(256, 256, 256), nets, dt, dx, passive=False)
-    log(f"energy after 300 / 3000 steps: limited {e_lim}, without limits {e_free}")
+    refs = {}
+    r3 = {}
+    for name, nt, kinds in (("learned", nets, ("learned", "cpml8", "cpml32")),
+                            ("network_held", base, ("learned",)),
+                            ("direct_mlp", direct, ("learned",)),
+                            ("linear_3d", lin3, ("learned",))):
+        got = L.reflection_3d(nt, interior=248, steps=3000, kinds=kinds, late=500, refs=refs)
+        for k, (tot, late) in got.items():
+            r3[name if k == "learned" else k] = {"all": fin(tot), "last_500": fin(late)}
+    log("3D reflected energy, 3000 steps: " + ", ".join(
+        f"{k} {v['all']} (last 500 steps {v['last_500']})" for k, v in r3.items()))
+    dims = GridDims(256, 256, 256)
+    e_free = L.energy_history(dims, nets, dt, dx, steps=(300, 3000, 6000))
+    e_lim = L.energy_history(dims, nets, dt, dx, passive=True, steps=(300, 3000, 6000))
+    log(f"field energy after 300 / 3000 / 6000 steps: {e_free} (passive limits on: {e_lim})")
     tp = {}
-    for kind, nt in (("learned", nets), ("learned_linear", lin3), ("zero", nets), ("cpml32", nets)):
+    for kind, nt in (("learned", nets), ("learned_direct_mlp", direct), ("zero", nets),
+                     ("cpml32", nets)):
         mc, ms, cells = L.throughput("learned" if kind.startswith("learned") else kind, nt,
                                      interior=248, steps=100)
         tp[kind] = {"interior_mcells": round(mc, 1), "ms_per_step": round(ms, 4), "cells": cells}
@@ -296,7 +313,7 @@ def run_learned(args):
         "metric": "reflection coefficient of the learned 4-cell layer (two-run method, 30 GHz, "
                   "20 ppw, Courant 0.4), with 32-cell CPML, 3D reflected energy and throughput "
                   "beside it",
-        "value": r_learned,
+        "value": r["learned"],
         "unit": "reflection coefficient",
         "n_gpus": 1,
         "steps": None,
@@ -309,20 +326,22 @@ def run_learned(args):
         "data": "synthetic",
         "config": {
             "workload": "learned thin boundary, 256^3: 4-cell layer ending in predicted ghost "
-                        "planes, per-face MLP 16-64-64-1 (hardtanh, odd) + linear skip trained on "
-                        "32 plane-wave runs per axis of the 64-cell-CPML twin, passive limits "
-                        f"(retain {L.RETAIN}, gamma {L.GAMMA})",
-            "reflection_normal_incidence": {"learned": r_learned, "linear_skip_only": r_lin,
-                                            "linear_3d": r_3lin, "no_network": r_zero,
-                                            "cpml8": r_c8, "cpml32": r_c32},
-            "reflection_3d_energy_3000_steps": r3,
-            "energy_300_3000_limited": e_lim,
-            "energy_300_3000_unlimited": [fin(v) for v in e_free],
+                        "planes; per face the second-order Higdon update, fixed factor 45 deg "
+                        f"(memory damping {L.HIGDON_DAMP}) x adaptive factor from a 16-64-64-1 "
+                        "MLP trained on 32 plane-wave runs per axis of the 64-cell-CPML twin, "
+                        "early-stopped on closed-loop validation",
+            "training_candidates": [
+                {"iterations": it, "beta": b, "validation_R": v1, "validation_3d": v3,
+                 "validation_3d_last_500": lt} for it, b, v1, v3, lt in table],
+            "reflection_normal_incidence": r,
+            "reflection_3d_energy_3000_steps_248": r3,
+            "energy_256_after_300_3000_6000": [fin(v) for v in e_free],
+            "energy_256_after_300_3000_6000_passive_limits": [fin(v) for v in e_lim],
             "throughput_interior_248": tp,
             "throughput_ratio_learned_vs_cpml32": round(tp["learned"]["interior_mcells"] /
                                                         tp["cpml32"]["interior_mcells"], 3),
             "seconds_data": round(t_data, 2),
-            "seconds_training": round(t_train, 2),
+            "seconds_training_and_validation": round(t_train, 2),
             "seconds_data_3d": round(t_data3, 2),
         },
     }

@@ -300,6 +300,20 @@ int wf_mlp_set_params(wf_mlp_trainer *t, const float *params);
  * sign-normalized inputs) times the boundary field's step, ghost/s =
  * skip(u) + m (P0(n) - P0(n-1)): the network adds nothing at zero frequency. */
 #define WF_HEAD_DELTA 2
+/* WF_HEAD_HIGDON -- the second-order Higdon update, a product of two
+ * first-order one-way factors (1 + k g T) - X (g T + k) (T: one step back,
+ * X: one cell inward, g <= 1 damps the factor's memory): a fixed one with
+ * k = wlin[0], g = wlin[1] and an adaptive one with k = tanh(y), y the
+ * network's output on sign-normalized inputs, g = wlin[2] (the other skip
+ * weights are not used); P at depth 1 and its history (inputs 1, 10, 11)
+ * enter besides P0 and the ghost's history. */
+#define WF_HEAD_HIGDON 4
+/* WF_HEAD_DIFF -- ghost/s = P0(n) + skip(d) + m(d), network and skip on
+ * differences of the inputs that vanish for a field constant in space and
+ * time (depth-to-depth steps of P and Q, the Laplacians and psi as they are,
+ * g(n-1) - P0, g(n-2) - g(n-1), P0(n-1) - P0, P0(n-2) - P0(n-1)): a static
+ * uniform field is held exactly whatever the weights. */
+#define WF_HEAD_DIFF 3
 int wf_mlp_set_head(wf_mlp_trainer *t, int head);
 int wf_mlp_get_params(wf_mlp_trainer *t, float *params);
 /* Gradient of the loss at the current parameters (the network as it runs:

@@ -8,8 +8,8 @@ check them:
   forward_f64     the network as a real function: f64 arithmetic, the
                   weights as the GPU uses them (rounded to bf16), no rounding
                   of activations;
-  forward_bf16    the same with the GPU's rounding points emulated (bf16
-                  inputs and hidden activations, f32 accumulation);
+  forward_bf16    the same with the GPU's rounding points emulated (inputs
+                  and hidden activations as bf16 pairs, f32 accumulation);
 
 The hidden activation is hardtanh, clip(z, -1, 1), with derivative 1 inside
 (-1, 1) and 0 where it clips.
@@ -18,7 +18,10 @@ The hidden activation is hardtanh, clip(z, -1, 1), with derivative 1 inside
                   through the bf16 rounding of the weights), checked against
                   central finite differences in the tests;
   *_one_way_f64   the same for the one-way output head;
-  *_delta_f64     the same for the delta head.
+  *_delta_f64     the same for the delta head;
+  *_higdon_f64    the same for the Higdon head (higdon_terms: its product
+                  of one-way factors expanded independently);
+  predict_diff_f64  the difference head.
 
 Parameter layout: W1[16][64] b1[64] W2[64][64] b2[64] W3[64] b3 wlin[16]
 (W[in][out]).
@@ -80,20 +83,30 @@ def forward_f64(p, u):
     return a2 @ w["W3"] + w["b3"] + u @ w["wlin"], (a1, a2)
 
 
+def bf16_pair(x):
+    """x as the sum of two bfloat16 values, hi + lo (hi = x rounded, lo =
+    the remainder rounded), returned as float64."""
+    x = np.asarray(x, np.float32)
+    hi = bf16(x)
+    lo = bf16(x - hi)
+    return hi.astype(np.float64) + lo.astype(np.float64)
+
+
 def forward_bf16(p, u):
-    """The GPU's rounding points: bf16 inputs and activations into each
-    matrix product, f32 accumulation starting from the bias (products of
-    bf16 values are exact in f32), f32 activation, f32 linear skip."""
+    """The GPU's rounding points: inputs and activations into each matrix
+    product as bf16 pairs (hi + lo) against the bf16 weights, f32
+    accumulation starting from the bias (products of bf16 values are exact
+    in f32), f32 activation, f32 linear skip."""
     w = _gpu_weights(p)
     u32 = np.asarray(u, np.float32)
-    ub = bf16(u32).astype(np.float64)
+    ub = bf16_pair(u32)
     b1 = w["b1"].astype(np.float32).astype(np.float64)
     b2 = w["b2"].astype(np.float32).astype(np.float64)
     z1 = (ub @ w["W1"] + b1).astype(np.float32)
     a1 = np.clip(z1, np.float32(-1), np.float32(1))
-    z2 = (bf16(a1).astype(np.float64) @ w["W2"] + b2).astype(np.float32)
+    z2 = (bf16_pair(a1) @ w["W2"] + b2).astype(np.float32)
     a2 = np.clip(z2, np.float32(-1), np.float32(1))
-    z3 = (bf16(a2).astype(np.float64) @ w["W3"]).astype(np.float32)
+    z3 = (bf16_pair(a2) @ w["W3"]).astype(np.float32)
     wl = w["wlin"].astype(np.float32)
     lin = (u32 * wl).astype(np.float32)
     parts = [lin[:, 4 * q:4 * q + 4].sum(axis=1, dtype=np.float32) for q in range(4)]
@@ -245,3 +258,110 @@ def _loss_exact(p, u, t):
     a2 = act(a1 @ w["W2"] + w["b2"])
     y = a2 @ w["W3"] + w["b3"][0] + u @ w["wlin"]
     return float(np.mean((y - np.asarray(t, np.float64)) ** 2))
+
+
+def higdon_terms(kf, gf, ga, u):
+    """(A, B) of the Higdon head, ghost = A + ka B, from the product of the
+    two one-way factors as polynomials in X (one cell inward) and T (one
+    step back): c(X, T) = 1 + k g T - X (g T + k) each; the product's
+    coefficients p[x][t] give ghost = -sum p[x][t] W_x^{n-t} over
+    (x, t) != (0, 0), with W_0 the ghost, W_1 = P0, W_2 = P1 (inputs: 0 P0,
+    1 P1, 10, 11 P1(n-1), P1(n-2), 12, 13 g(n-1), g(n-2), 14, 15 P0(n-1),
+    P0(n-2)).  A is the part at ka = 0, B the coefficient of ka."""
+    u = np.asarray(u, np.float64)
+    where = {(0, 1): 12, (0, 2): 13, (1, 0): 0, (1, 1): 14, (1, 2): 15, (2, 0): 1, (2, 1): 10,
+             (2, 2): 11}
+
+    def factor(k, g):
+        return np.array([[1.0, k * g], [-k, -g]])
+
+    def ghost(ka):
+        a, b = factor(kf, gf), factor(ka, ga)
+        p = np.zeros((3, 3))
+        for i in range(2):
+            for j in range(2):
+                p[i:i + 2, j:j + 2] += a[i, j] * b
+        return -sum(p[x, t] * u[:, i] for (x, t), i in where.items()) / p[0, 0]
+
+    A = ghost(0.0)
+    return A, ghost(1.0) - A
+
+
+def _higdon_fixed(p):
+    wl = split(np.asarray(p, np.float32))["wlin"].astype(np.float64)
+    return wl[0], wl[1], wl[2], wl[3]
+
+
+def higdon_gate(u, c):
+    """1 where the tangential Laplacian u8 is 0, falling to 0 at
+    |u8| = c max|u0..u3| (c <= 0: 1)."""
+    u = np.asarray(u, np.float64)
+    if not c > 0:
+        return np.ones(len(u))
+    pm = np.abs(u[:, 0:4]).max(axis=1)
+    return np.maximum(1.0 - np.abs(u[:, 8]) / (c * pm + 1e-30), 0.0)
+
+
+def predict_higdon_f64(p, u, rounded=True):
+    """Higdon head: A + tanh(b3 + gate (y - b3)) B, y = the MLP part on the
+    sign-normalized inputs; kf, gf, ga, c = wlin[0..3]."""
+    u = np.asarray(u, np.float64)
+    kf, gf, ga, c = _higdon_fixed(p)
+    w = _weights(p, rounded)
+    y, _ = _mlp_f64(p, u * one_way_sign(u)[:, None], rounded)
+    A, B = higdon_terms(kf, gf, ga, u)
+    return A + np.tanh(w["b3"] + higdon_gate(u, c) * (y - w["b3"])) * B
+
+
+def loss_higdon_f64(p, u, t, rounded=True):
+    return float(np.mean((predict_higdon_f64(p, u, rounded) - np.asarray(t, np.float64)) ** 2))
+
+
+def grad_higdon_f64(p, u, t, rounded=True):
+    """d loss / d p for the Higdon head: b3 from the error times
+    (1 - ka^2) B, the network's other parameters from the same times the
+    gate; the skip weights (the fixed factor) get none."""
+    u = np.asarray(u, np.float64)
+    n = len(u)
+    kf, gf, ga, c = _higdon_fixed(p)
+    w = _weights(p, rounded)
+    v = u * one_way_sign(u)[:, None]
+    y, (a1, a2) = _mlp_f64(p, v, rounded)
+    A, B = higdon_terms(kf, gf, ga, u)
+    gt = higdon_gate(u, c)
+    ka = np.tanh(w["b3"] + gt * (y - w["b3"]))
+    e = 2.0 * (A + ka * B - np.asarray(t, np.float64)) / n
+    gb = e * (1.0 - ka * ka) * B
+    gm = gb * gt
+    out = {"wlin": np.zeros(IN), "b3": np.array([gb.sum()]), "W3": a2.T @ gm}
+    dz2 = np.outer(gm, w["W3"]) * dact(a2)
+    out["W2"] = a1.T @ dz2
+    out["b2"] = dz2.sum(0)
+    dz1 = (dz2 @ w["W2"].T) * dact(a1)
+    out["W1"] = v.T @ dz1
+    out["b1"] = dz1.sum(0)
+    return join(out)
+
+
+def diff_features(u):
+    """The difference head's inputs: P and Q depth steps, the Laplacians,
+    the P1 and ghost / P0 history as differences (3 and 7 zero)."""
+    u = np.asarray(u, np.float64)
+    d = np.zeros_like(u)
+    for b in (0, 4):
+        d[:, b:b + 3] = u[:, b:b + 3] - u[:, b + 1:b + 4]
+    d[:, 8:10] = u[:, 8:10]
+    d[:, 10] = u[:, 10] - u[:, 1]
+    d[:, 11] = u[:, 11] - u[:, 10]
+    d[:, 12] = u[:, 12] - u[:, 0]
+    d[:, 13] = u[:, 13] - u[:, 12]
+    d[:, 14] = u[:, 14] - u[:, 0]
+    d[:, 15] = u[:, 15] - u[:, 14]
+    return d
+
+
+def predict_diff_f64(p, u):
+    """Difference head: u0 + (network + skip) on the differences."""
+    u = np.asarray(u, np.float64)
+    y, _ = forward_f64(p, diff_features(u))
+    return u[:, 0] + y

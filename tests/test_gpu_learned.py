@@ -101,15 +101,16 @@ def _point_sources(c):
 
 
 def test_recorded_inputs_match_fields():
-    """The hook's inputs (fields at depths 0..3, CPML psi at depths 0, 1, history) and
-    true ghost values at the last step of a thick run, against the same
-    quantities taken from the fields and psi read back."""
+    """The hook's inputs (fields at depths 0..3, their tangential Laplacians at
+    depths 0, 1, the history of P at depths 0, 1 and of the ghost) and true
+    ghost values at the last step of a thick run, against the same
+    quantities taken from the fields read back and the earlier records."""
     _say("face hook record vs fields")
     thin = GridDims(24, 16, 32)
     dx, dt, _ = L.study_grid()
     big = L.thick_dims(thin)
     K = 44
-    # virtual faces LAYER cells inside the CPML, where psi is live
+    # virtual faces LAYER cells inside the CPML (the hooks read fields only)
     off = L.THICK - L.LAYER
     srcs = [Source(s.x + L.OFF, s.y + L.OFF, s.z + L.OFF, s.component, s.amplitude, s.waveform)
             for s in _point_sources((18, 10, 26))]
@@ -128,7 +129,6 @@ def test_recorded_inputs_match_fields():
         L.check(L.lib().wf_record_face_samples(eng.handle, 63, regions, 1, K))
         eng.step(K - 1)
         F1 = aosoa_to_xyz(eng.read_fields(), big)  # (6, z, y, x)
-        psi1 = eng.read_psi()  # (shell blocks, 12, 512), PmlAux order
         eng.step(1)
         F2 = aosoa_to_xyz(eng.read_fields(), big)
         recs = {}
@@ -145,30 +145,8 @@ def test_recorded_inputs_match_fields():
         c[a], c[tf], c[ts] = n, fc, sc
         return F[comp, c[2], c[1], c[0]]
 
-    # shell blocks in the order the psi read-back lists them (z, y, x loops)
-    nbk = (big.nx // 8, big.ny // 8, big.nz // 8)
-    sb = ctx.shell_blocks
-    order = {}
-    for bz in range(nbk[2]):
-        for by in range(nbk[1]):
-            for bx in range(nbk[0]):
-                bc = (bx, by, bz)
-                if any(sb[q] > 0 and (bc[q] < sb[q] or bc[q] >= nbk[q] - sb[q]) for q in range(3)):
-                    order[bc] = len(order)
-    # (normal axis, term) -> PmlAux component: terms 0, 1 of the H phase, 2, 3
-    # of the E phase, for the two tangential derivatives of each axis
-    aux = {(0, 0): 0, (0, 1): 1, (1, 0): 2, (1, 1): 3, (2, 0): 4, (2, 1): 5,
-           (0, 2): 6, (0, 3): 7, (1, 3): 8, (1, 2): 9, (2, 2): 10, (2, 3): 11}
-
-    def psi_at(a, term, n, fc, sc):
-        c = [0, 0, 0]
-        t1, t2 = (a + 1) % 3, (a + 2) % 3
-        c[a], c[min(t1, t2)], c[max(t1, t2)] = n, fc, sc
-        blk = order[(c[0] // 8, c[1] // 8, c[2] // 8)]
-        return psi1[blk, aux[(a, term)], ((c[2] % 8) * 8 + c[1] % 8) * 8 + c[0] % 8]
-
     rng = np.random.default_rng(7)
-    checked = psi_seen = 0
+    checked = lap_seen = 0
     for f in range(6):
         a, hi = f // 2, f % 2
         t1, t2 = (a + 1) % 3, (a + 2) % 3
@@ -180,11 +158,9 @@ def test_recorded_inputs_match_fields():
             if hi:
                 P, Q = (t1 if pol == 0 else t2), 3 + (t2 if pol == 0 else t1)
                 FP, FQ = F1, F1
-                term = pol if a == 1 else 1 - pol
             else:
                 P, Q = 3 + (t1 if pol == 0 else t2), (t2 if pol == 0 else t1)
                 FP, FQ = F2, F1
-                term = 2 + pol if a == 1 else 3 - pol
             for pos in rng.choice(r.nfast * r.nslow, 40, replace=False):
                 fi, si = pos % r.nfast, pos // r.nfast
                 fc, sc = r.fast0 + fi, r.slow0 + si
@@ -192,21 +168,32 @@ def test_recorded_inputs_match_fields():
                 for j in range(4):
                     x[j] = at(FP, P, a, r.b0 + d * j, fc, sc)
                     x[4 + j] = at(FQ, Q, a, r.b0 + d * j, fc, sc)
-                x[10], x[11] = (psi_at(a, term, r.b0 + d * j, fc, sc) for j in range(2))
+                for j in range(2):
+                    v = at(FP, P, a, r.b0 + d * j, fc, sc)
+                    if 0 < fi < r.nfast - 1:
+                        x[8 + j] += (at(FP, P, a, r.b0 + d * j, fc - 1, sc) +
+                                     at(FP, P, a, r.b0 + d * j, fc + 1, sc) - 2 * v)
+                    if 0 < si < r.nslow - 1:
+                        x[8 + j] += (at(FP, P, a, r.b0 + d * j, fc, sc - 1) +
+                                     at(FP, P, a, r.b0 + d * j, fc, sc + 1) - 2 * v)
                 row = rec[K - 1, pol, pos]
                 prev1, prev2 = rec[K - 2, pol, pos], rec[K - 3, pol, pos]
+                x[10], x[11] = prev1[1] * prev1[18], prev2[1] * prev2[18]
                 x[12], x[13] = prev1[17], prev2[17]
                 x[14], x[15] = prev1[0] * prev1[18], prev2[0] * prev2[18]
                 s = row[18]
                 got = row[:16].astype(np.float64) * s
-                psi_seen += bool(np.any(x[10:12] != 0))
+                lap_seen += bool(np.any(x[8:10] != 0))
                 for i in range(16):
-                    assert abs(got[i] - x[i]) <= 1e-5 * max(abs(x[i]), 1e-3 * s) + 1e-30, (f, pol, pos, i)
+                    # the Laplacians are sums of three FP32 terms: within a
+                    # few ulp of the largest
+                    tol = 1e-6 * s if i in (8, 9) else 1e-5 * max(abs(x[i]), 1e-3 * s)
+                    assert abs(got[i] - x[i]) <= tol + 1e-30, (f, pol, pos, i)
                 ghost = at(FP, P, a, r.b0 - d, fc, sc)
                 assert row[17] == np.float32(ghost)
                 checked += 1
     assert checked == 6 * 2 * 40
-    assert psi_seen > checked // 2  # the virtual faces lie in the graded layer
+    assert lap_seen > checked // 2
 
 
 @pytest.mark.parametrize("kernel", ["fused", "split"])
@@ -394,6 +381,51 @@ def test_delta_head_gradient():
     rel = np.linalg.norm(g - g64) / np.linalg.norm(g64)
     print(f"delta gradient rel L2 {rel:.3e}")
     assert rel < 3e-2
+
+
+def test_higdon_and_difference_heads_match_fp64():
+    """The Higdon head (fixed factor and gate in wlin[0..3], adaptive factor
+    from the network) and the difference head on the GPU against their FP64
+    restatements: predictions within the bf16-weight tolerance of the other
+    heads, the Higdon gradient within 3e-2 (relative L2), no gradient on the
+    fixed factor."""
+    _say("higdon and difference heads")
+    from tests.test_learned_cpu import _higdon_params
+    rng = np.random.default_rng(10)
+    p = _higdon_params(rng).astype(np.float32)
+    u = rng.normal(0, 1, (3000, 16)).astype(np.float32)
+    u[:1500, 8] *= 1e-3
+    t = rng.normal(0, 1, 3000).astype(np.float32)
+    tr = L.Trainer(u, t, head=L.HEAD_HIGDON)
+    tr.set_params(p)
+    g, loss = tr.gradient()
+    y = tr.predict(u)
+    tr.close()
+    y64 = mlp_ref.predict_higdon_f64(p, u)
+    assert np.max(np.abs(y - y64)) < 3e-2 * np.max(np.abs(y64))
+    g64 = mlp_ref.grad_higdon_f64(p, u, t)
+    assert abs(loss - mlp_ref.loss_higdon_f64(p, u, t)) < 2e-2 * loss
+    rel = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+    print(f"higdon gradient rel L2 {rel:.3e}")
+    assert rel < 3e-2
+    assert not np.any(g[L.param_count() - 16:])
+    # with the network's output weights zero the head is the fixed product
+    p0 = p.copy()
+    p0[L.param_count() - 81:L.param_count() - 17] = 0.0
+    tr = L.Trainer(u, t, head=L.HEAD_HIGDON)
+    tr.set_params(p0)
+    y0 = tr.predict(u)
+    tr.close()
+    A, B = mlp_ref.higdon_terms(*[float(v) for v in p0[L.param_count() - 16:L.param_count() - 13]], u)
+    want = A + np.tanh(float(p0[L.param_count() - 17])) * B
+    assert np.max(np.abs(y0 - want)) < 1e-5 * np.max(np.abs(want))
+    pd = _params(rng).astype(np.float32)
+    tr = L.Trainer(u, t, head=L.HEAD_DIFF)
+    tr.set_params(pd)
+    yd = tr.predict(u)
+    tr.close()
+    yd64 = mlp_ref.predict_diff_f64(pd, u)
+    assert np.max(np.abs(yd - yd64)) < 3e-2 * np.max(np.abs(yd64))
 
 
 def test_3d_linear_configuration_is_stable():

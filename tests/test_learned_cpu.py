@@ -102,3 +102,67 @@ def test_thin_domain_lies_beyond_the_twin_cpml():
     for p in range(L.THICK):
         assert thick.get(p) != (1.0, 0.0, 1.0) and thick.get(nb - 1 - p) != (1.0, 0.0, 1.0)
     assert all(thick.get(L.OFF + i) == (1.0, 0.0, 1.0) for i in range(n))
+
+
+def _higdon_params(rng, kf=-0.2, gf=0.95, ga=1.0, c=0.01):
+    p = _params(rng)
+    d = {k: v.copy() for k, v in mlp_ref.split(p).items()}
+    d["b3"] = np.array([np.arctanh(-0.4)])
+    d["wlin"] = np.zeros(16)
+    d["wlin"][:4] = (kf, gf, ga, c)
+    return mlp_ref.join(d)
+
+
+def test_higdon_head_gradient_matches_finite_differences():
+    """The Higdon head's FP64 gradient (unrounded weights, gate on) against
+    central differences of its loss over the network's weights and biases
+    (the fixed factor in wlin is held, so it is left out)."""
+    rng = np.random.default_rng(6)
+    p = _higdon_params(rng)
+    u = rng.normal(0, 1, (40, 16))
+    u[:20, 8] *= 1e-3  # half the samples inside the gate's ramp
+    t = rng.normal(0, 1, 40)
+    g = mlp_ref.grad_higdon_f64(p, u, t, rounded=False)
+    idx = np.concatenate([rng.choice(mlp_ref.P - 16, 60, replace=False), [mlp_ref.P - 17]])
+    h = 1e-6
+    fd = []
+    for i in idx:
+        q1, q2 = p.copy(), p.copy()
+        q1[i] += h
+        q2[i] -= h
+        fd.append((mlp_ref.loss_higdon_f64(q1, u, t, False) -
+                   mlp_ref.loss_higdon_f64(q2, u, t, False)) / (2 * h))
+    np.testing.assert_allclose(g[idx], fd, rtol=2e-5, atol=1e-9)
+    assert not np.any(g[mlp_ref.P - 16:])
+
+
+def test_higdon_terms_hold_a_static_field_and_solve_the_factor_product():
+    """The Higdon head's expansion: a field constant in space and time is
+    held exactly when one factor is undamped (an undamped factor annihilates
+    constants), whatever the coefficients; and on random data the ghost it
+    gives makes the two one-way factors, applied one after the other to the
+    space-time stencil, vanish."""
+    rng = np.random.default_rng(7)
+    for kf, gf, ka in ((-0.3, 0.95, -0.4), (0.2, 0.9, 0.5), (-0.7, 1.0, 0.1)):
+        u = np.full((3, 16), 2.5)
+        A, B = mlp_ref.higdon_terms(kf, gf, 1.0, u)
+        np.testing.assert_allclose(A + ka * B, 2.5, rtol=0, atol=1e-12)
+    # factor by factor: apply F(kf, gf) to the history, then D(ka, ga)
+    kf, gf, ka, ga = -0.35, 0.93, -0.1, 0.97
+    W = rng.normal(0, 1, (3, 3))  # W[x][t]: x = 0 ghost, 1 P0, 2 P1; t steps back
+
+    def apply(k, g, V):
+        # (1 + k g T) - X (g T + k) on V[x][t] -> rows x = 0..1, cols t = 0..1
+        out = np.zeros((V.shape[0] - 1, V.shape[1] - 1))
+        for x in range(out.shape[0]):
+            for t in range(out.shape[1]):
+                out[x, t] = (V[x, t] + k * g * V[x, t + 1] - g * V[x + 1, t + 1]
+                             - k * V[x + 1, t])
+        return out
+
+    u = np.zeros((1, 16))
+    u[0, [0, 1, 10, 11, 12, 13, 14, 15]] = (W[1, 0], W[2, 0], W[2, 1], W[2, 2], W[0, 1], W[0, 2],
+                                            W[1, 1], W[1, 2])
+    A, B = mlp_ref.higdon_terms(kf, gf, ga, u)
+    W[0, 0] = (A + ka * B)[0]
+    assert abs(apply(ka, ga, apply(kf, gf, W))[0, 0]) < 1e-12

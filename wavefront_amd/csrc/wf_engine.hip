@@ -1816,7 +1816,7 @@ int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params, int
 {
     clear_error();
     if (!c) return fail(WF_ERR_INVALID, "wf_set_learned_faces: null context");
-    if (head != kHeadDirect && head != kHeadOneWay && head != kHeadDelta)
+    if (!valid_head(head))
         return fail(WF_ERR_INVALID, "wf_set_learned_faces: unknown head %d", head);
     if (face_mask >= 64u) return fail(WF_ERR_INVALID, "wf_set_learned_faces: face mask has bits above 5");
     if (face_mask && !params) return fail(WF_ERR_INVALID, "wf_set_learned_faces: null parameters");
@@ -1866,7 +1866,6 @@ int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params, int
         h.na = fg.na;
         h.nf = fg.nf;
         h.ns = fg.ns;
-        h.Ta = c->g.T[fg.a];
         h.mode = 0;
         if ((rc = set_pool(c, f, c->face_passive))) break;
         h.passive = c->face_passive;
@@ -1932,7 +1931,6 @@ int wf_record_face_samples(wf_ctx *c, uint32_t face_mask, const wf_face_region *
         h.na = fg.na;
         h.nf = fg.nf;
         h.ns = fg.ns;
-        h.Ta = c->g.T[fg.a];
         h.mode = 1;
         h.mem = c->d_mem[f];
         h.samples = c->d_samples[f];
@@ -1985,7 +1983,6 @@ int wf_replay_faces(wf_ctx *c, uint32_t face_mask, uint64_t rows, const float *c
         h.nfast = h.nf = fg.nf;
         h.nslow = h.ns = fg.ns;
         h.na = fg.na;
-        h.Ta = c->g.T[fg.a];
         h.mode = 2;
         h.ghost[0] = c->d_ghost[f];
         h.ghost[1] = c->d_ghost[f] + npos;
@@ -2007,10 +2004,16 @@ int wf_read_face_state(wf_ctx *c, uint32_t face, float *dst)
     if (face >= 6 || !c->hook_on[face] || c->hook[face].mode == 1 || !c->d_mem[face])
         return fail(WF_ERR_STATE, "face %u has no predicted ghost plane", face);
     const FaceHook &h = c->hook[face];
+    const size_t n = 2ull * h.nfast * h.nslow;
+    std::vector<float> m(n * kMemStride);
     WF_HIP(hipSetDevice(c->device));
     WF_HIP(hipStreamSynchronize(c->stream));
-    WF_HIP(hipMemcpy(dst, c->d_mem[face], 2ull * h.nfast * h.nslow * kMemStride * sizeof(float),
-                     hipMemcpyDeviceToHost));
+    WF_HIP(hipMemcpy(m.data(), c->d_mem[face], m.size() * sizeof(float), hipMemcpyDeviceToHost));
+    // g(n-1), g(n-2), P0(n-1), P0(n-2), then the four statistics
+    for (size_t i = 0; i < n; ++i) {
+        std::memcpy(dst + 8 * i, &m[i * kMemStride], 4 * sizeof(float));
+        std::memcpy(dst + 8 * i + 4, &m[i * kMemStride + kMemStats], 4 * sizeof(float));
+    }
     return WF_OK;
 }
 

@@ -37,8 +37,13 @@ constexpr int kOffWL = kOffB3 + 1;
 constexpr int kMlpParams = kOffWL + kMlpIn;  // 5329
 // A recorded sample: 16 scaled inputs, target / s, target, s, 1 float padding.
 constexpr int kSampleStride = 20;
-// Floats per (polarization, position) of a face's memory.
-constexpr int kMemStride = 8;
+// Floats per (polarization, position) of a face's memory: the history
+// g(n-1), g(n-2), P0(n-1), P0(n-2), P1(n-1), P1(n-2), two unused, then from
+// kMemStats the last step's flux through the ghost, the steps limited by the
+// energy bound and by the magnitude bound, and the sum of squared
+// corrections.
+constexpr int kMemStride = 12;
+constexpr int kMemStats = 8;
 // Flux partials per face and step: one per hook workgroup (at most
 // kFluxParts), then kEdgeParts per edge family the face belongs to (two).
 constexpr int kFluxParts = 1024;
@@ -58,7 +63,7 @@ struct MlpPacked {
     float w3r[kMlpHid];  // W3 rounded to bf16 (the values the forward uses)
     float wlin[kMlpIn];
     float b3;
-    int head;  // output head (kHeadDirect / kHeadOneWay)
+    int head;  // output head (kHead*)
     float pad[2];
 };
 
@@ -73,9 +78,31 @@ struct MlpPacked {
 // u14), with the MLP on sign-normalized inputs (even, so the product is odd
 // in the fields).  A static field gets the skip alone: the network cannot
 // add a response at zero frequency.
+// Difference: ghost / s = P0(n) + skip(d) + m(d), where network and skip
+// see differences that vanish for a field constant in space and time:
+//   d0..2 = P(0)-P(1), P(1)-P(2), P(2)-P(3);  d4..6 = the same of Q;
+//   d8, d9 = the Laplacians;  d10, d11 = P1(n-1)-P1, P1(n-2)-P1(n-1);
+//   d12..15 = g(n-1)-P0, g(n-2)-g(n-1), P0(n-1)-P0, P0(n-2)-P0(n-1)
+// (d3 = d7 = 0).  A static uniform field is held exactly (ghost = P0)
+// whatever the weights, so rounding them cannot move the recursion's gain
+// at zero frequency above one.
 constexpr int kHeadDirect = 0;
 constexpr int kHeadOneWay = 1;
 constexpr int kHeadDelta = 2;
+// Higdon: the second-order Higdon update, the product of two first-order
+// one-way factors B(k, g) = (1 + k g T) - X (g T + k) (T one step back in
+// time, X one cell inward, g <= 1 a damping of the factor's memory): a
+// fixed factor B(kf, gf) and an adaptive one B(ka, ga) with ka = tanh(y),
+// y the network's output on sign-normalized inputs (even, so the ghost is
+// odd in the fields); kf, gf, ga = wlin[0..2], the other skip weights
+// unused.  Solved for the ghost, ghost / s = A + ka B with
+//   A = kf (u0 - gf u12 - ga u10) + (gf + ga) u14 + gf ga (kf u15 - u11)
+//   B = u0 - ga u12 - kf u1 + kf (ga + gf) u14 + gf (ga (u15 - kf u13) - u10)
+// (u0 = P0, u1 = P1, u10, u11 = P1(n-1), P1(n-2), u12, u13 = g(n-1), g(n-2),
+// u14, u15 = P0(n-1), P0(n-2)).
+constexpr int kHeadDiff = 3;
+constexpr int kHeadHigdon = 4;
+constexpr bool valid_head(int h) { return h >= kHeadDirect && h <= kHeadHigdon; }
 
 // One face hook: a face of the domain whose ghost plane is predicted
 // (mode 0), or a virtual face inside a thick-CPML run whose stencils and true
@@ -86,7 +113,6 @@ struct FaceHook {
     int fast0, nfast;   // range on the lower-numbered tangential axis
     int slow0, nslow;   // range on the higher-numbered tangential axis
     int na, nf, ns;     // axis lengths: normal, fast, slow
-    int Ta;             // graded CPML depth on the normal axis
     int mode;           // 0 predict, 1 record, 2 replay recorded ghosts
     int passive;        // modes 0, 2: limit the ghosts so the face stays passive
     unsigned predicted; // faces (bit f) whose ghost planes are predicted or replayed
@@ -96,10 +122,7 @@ struct FaceHook {
     float gamma;        // passive: |ghost| <= gamma max_d |P(d)| (0 = no bound)
     float retain;       // passive: net outflow >= retain x the absorbed energy (0..1)
     float *ghost[2];    // mode 0: ghost planes [nslow][nfast], polarization 0 / 1
-    float *mem;         // [2][npos][kMemStride]: g(n-1), g(n-2), P0(n-1), P0(n-2),
-                        // the last step's flux through the ghost, steps limited
-                        // by the energy bound and by the magnitude bound, and the
-                        // sum of squared corrections
+    float *mem;         // [2][npos][kMemStride]: history and statistics (kMemStats)
     float *samples;     // modes 1, 2: [rows][2][npos][kSampleStride]
     uint64_t emit_base, emit_every, emit_rows;
     const MlpPacked *net;

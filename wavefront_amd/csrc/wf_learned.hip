@@ -73,11 +73,21 @@ __device__ __forceinline__ fv4 act4(fv4 z) { return fv4{act_f(z.x), act_f(z.y), 
 __device__ __forceinline__ float dact_f(float a) { return fabsf(a) < 1.0f ? 1.0f : 0.0f; }
 __device__ __forceinline__ fv4 dact4(fv4 a) { return fv4{dact_f(a.x), dact_f(a.y), dact_f(a.z), dact_f(a.w)}; }
 
+// The Higdon head's fixed factor and damping, and the width of its gate:
+// wlin[0..3] = kf, gf, ga, c.
+struct HigdonFix {
+    float kf, gf, ga, c;
+    float beta;  // wlin[4]: time smoothing of the adaptive coefficient (0 or 1: none)
+    float b3;    // the network's output bias
+};
+
+
 // The network in registers: A operands and the per-lane biases.
 struct MlpRegs {
     bfv4 A1[4], A2[4][4], A3[4];
     fv4 b1[4], b2[4];
     fv4 wl;
+    HigdonFix hf;  // wlin[0..2]: the Higdon head's fixed factor
     float b3;
     int head;
 };
@@ -97,6 +107,7 @@ __device__ __forceinline__ void load_regs(const MlpPacked *__restrict__ w, int l
         for (int k = 0; k < 4; ++k) r.A2[o][k] = w->a2[o][k][lane];
     }
     r.wl = fv4{w->wlin[rb], w->wlin[rb + 1], w->wlin[rb + 2], w->wlin[rb + 3]};
+    r.hf = HigdonFix{w->wlin[0], w->wlin[1], w->wlin[2], w->wlin[3], w->wlin[4], w->b3};
     r.b3 = w->b3;
     r.head = w->head;
 }
@@ -109,47 +120,120 @@ __device__ __forceinline__ float input_sign(fv4 u, int sl)
     return (u0 > 0.f || (u0 == 0.f && u4 >= 0.f)) ? 1.0f : -1.0f;
 }
 
+// The scaled history and stencil values a head combines, on lanes 0..15
+// (sample = lane): P0, P1, P1(n-1), P1(n-2), g(n-1), g(n-2), P0(n-1),
+// P0(n-2).  Every lane takes part in the shuffles.
+struct HeadIn {
+    float u0, u1, u8, u10, u11, u12, u13, u14, u15;
+    float pm;  // max |P(0..3)|
+};
+__device__ __forceinline__ HeadIn head_in(fv4 u, int sl)
+{
+    HeadIn h;
+    h.u0 = __shfl(u.x, sl);
+    h.u1 = __shfl(u.y, sl);
+    h.pm = __shfl(fmaxf(fmaxf(fabsf(u.x), fabsf(u.y)), fmaxf(fabsf(u.z), fabsf(u.w))), sl);
+    h.u8 = __shfl(u.x, sl + 32);
+    h.u10 = __shfl(u.z, sl + 32);
+    h.u11 = __shfl(u.w, sl + 32);
+    h.u12 = __shfl(u.x, sl + 48);
+    h.u13 = __shfl(u.y, sl + 48);
+    h.u14 = __shfl(u.z, sl + 48);
+    h.u15 = __shfl(u.w, sl + 48);
+    return h;
+}
+
+// The Higdon head's gate on the network: 1 where the stencil is a plane
+// wave at normal incidence (tangential Laplacian u8 = 0), falling to 0 at
+// |u8| = c max|P| (c <= 0: always 1).
+__device__ __forceinline__ float higdon_gate(const HeadIn &h, const HigdonFix &c)
+{
+    if (!(c.c > 0.f)) return 1.0f;
+    return fmaxf(1.0f - fabsf(h.u8) / (c.c * h.pm + 1e-30f), 0.0f);
+}
+
+// The Higdon head's terms: ghost / s = A + ka B (the product's expansion
+// in the data u, with the history h).
+__device__ __forceinline__ void higdon_ab(const HeadIn &h, const HigdonFix &c, float &A, float &B)
+{
+    const float kf = c.kf, gf = c.gf, ga = c.ga;
+    A = kf * ((h.u0 - gf * h.u12) - ga * h.u10) + (gf + ga) * h.u14 +
+        gf * ga * (kf * h.u15 - h.u11);
+    B = ((h.u0 - ga * h.u12) - kf * h.u1) + (kf * (ga + gf)) * h.u14 +
+        gf * (ga * (h.u15 - kf * h.u13) - h.u10);
+}
+
 // Prediction in scaled units from the network's MLP output m and linear
-// skip p (lanes 0..15): u0, u12, u14 are the sample's scaled P0(n), g(n-1),
-// P0(n-1).
-__device__ __forceinline__ float head_out(int head, float m, float p, float u0, float u12,
-                                          float u14)
+// skip p (lanes 0..15); c: the Higdon head's fixed factor.
+__device__ __forceinline__ float head_out(int head, float m, float p, const HeadIn &h,
+                                          const HigdonFix &c)
 {
     if (head == kHeadDirect) return m + p;
-    if (head == kHeadOneWay) return u14 + tanh_f(m + p) * (u0 - u12);
-    return p + m * (u0 - u14);  // kHeadDelta
+    if (head == kHeadOneWay) return h.u14 + tanh_f(m + p) * (h.u0 - h.u12);
+    if (head == kHeadDiff) return h.u0 + (m + p);
+    if (head == kHeadHigdon) {
+        float A, B;
+        higdon_ab(h, c, A, B);
+        return A + tanh_f(c.b3 + higdon_gate(h, c) * (m - c.b3)) * B;
+    }
+    return p + m * (h.u0 - h.u14);  // kHeadDelta
 }
 
 struct Acts {
     fv4 a1[4], a2[4];
-    bfv4 a1b[4], a2b[4];
+    bfv4 a1b[4], a2b[4];  // the activations' leading bf16 parts
 };
 
-// Forward pass for the wave's 16 samples.  ub: this lane's 4 MLP inputs in
-// bf16, uf: its 4 linear-skip inputs (features 4(l/16)..+3 of sample l%16).
-// Returns the MLP output and sets p to the skip, valid on lanes 0..15
-// (sample = lane).
-__device__ __forceinline__ float mlp_forward(const MlpRegs &r, bfv4 ub, fv4 uf, Acts &t, float &p)
+__device__ __forceinline__ fv4 from_bf4(bfv4 b)
+{
+    return fv4{bf16_f((unsigned short)b.x), bf16_f((unsigned short)b.y),
+               bf16_f((unsigned short)b.z), bf16_f((unsigned short)b.w)};
+}
+
+// v as a pair of bf16 values, hi + lo: hi = v rounded, lo = the remainder
+// rounded (|v - hi - lo| <= 2^-17 |v|).
+__device__ __forceinline__ void split_bf4(fv4 v, bfv4 &hi, bfv4 &lo)
+{
+    hi = to_bf4(v);
+    lo = to_bf4(v - from_bf4(hi));
+}
+
+// Forward pass for the wave's 16 samples.  ui: this lane's 4 MLP inputs,
+// uf: its 4 linear-skip inputs (features 4(l/16)..+3 of sample l%16).
+// Inputs and hidden activations enter each product as bf16 pairs (two
+// MFMAs against the same bf16 weights), so the network is a function of its
+// inputs to about FP32 precision: a single bf16 rounding would move its
+// output by up to 2^-9 of the terms in steps that do not vary smoothly with
+// the fields.  Returns the MLP output and sets p to the skip, valid on lanes
+// 0..15 (sample = lane); ub gets the inputs' leading bf16 parts.
+__device__ __forceinline__ float mlp_forward(const MlpRegs &r, fv4 ui, fv4 uf, Acts &t, float &p,
+                                             bfv4 &ub)
 {
     const fv4 zero = {0.f, 0.f, 0.f, 0.f};
+    bfv4 ul, a1l[4], a2l[4];
+    split_bf4(ui, ub, ul);
     // biases enter as the accumulator of the first product of each layer
 #pragma unroll
     for (int o = 0; o < 4; ++o) {
-        const fv4 z = mfma(r.A1[o], ub, r.b1[o]);
+        const fv4 z = mfma(r.A1[o], ul, mfma(r.A1[o], ub, r.b1[o]));
         t.a1[o] = act4(z);
-        t.a1b[o] = to_bf4(t.a1[o]);
+        split_bf4(t.a1[o], t.a1b[o], a1l[o]);
     }
 #pragma unroll
     for (int o = 0; o < 4; ++o) {
         fv4 z = r.b2[o];
 #pragma unroll
         for (int k = 0; k < 4; ++k) z = mfma(r.A2[o][k], t.a1b[k], z);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) z = mfma(r.A2[o][k], a1l[k], z);
         t.a2[o] = act4(z);
-        t.a2b[o] = to_bf4(t.a2[o]);
+        split_bf4(t.a2[o], t.a2b[o], a2l[o]);
     }
     fv4 z3 = zero;
 #pragma unroll
     for (int k = 0; k < 4; ++k) z3 = mfma(r.A3[k], t.a2b[k], z3);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) z3 = mfma(r.A3[k], a2l[k], z3);
     // linear skip, summed over the four lanes that hold one sample
     p = r.wl.x * uf.x + r.wl.y * uf.y + r.wl.z * uf.z + r.wl.w * uf.w;
     p = p + __shfl_xor(p, 16);
@@ -157,14 +241,32 @@ __device__ __forceinline__ float mlp_forward(const MlpRegs &r, bfv4 ub, fv4 uf, 
     return z3.x + r.b3;
 }
 
+// The difference head's inputs (kHeadDiff): this lane's four of sample sl,
+// lane group q4 = inputs 4 q4 .. 4 q4 + 3.  Every lane takes part in the
+// shuffle.
+__device__ __forceinline__ fv4 diff_in(fv4 u, int sl)
+{
+    const float p0 = __shfl(u.x, sl), p1 = __shfl(u.y, sl);
+    const int q4 = (threadIdx.x & 63) >> 4;
+    if (q4 < 2) return fv4{u.x - u.y, u.y - u.z, u.z - u.w, 0.f};
+    if (q4 == 2) return fv4{u.x, u.y, u.z - p1, u.w - u.z};
+    return fv4{u.x - p0, u.y - u.x, u.z - p0, u.w - u.z};
+}
+
 // MLP input of a sample: the scaled inputs, sign-normalized for the heads
-// whose network must be even (one-way, delta); skip input: sign-normalized
-// for the one-way head only (its skip is inside the tanh).
+// whose network must be even (one-way, delta), their differences for the
+// difference head; skip input: sign-normalized for the one-way head (its
+// skip is inside the tanh), the differences for the difference head.
 __device__ __forceinline__ fv4 mlp_in(int head, fv4 u, int sl)
 {
-    return head == kHeadDirect ? u : u * input_sign(u, sl);
+    if (head == kHeadDirect) return u;
+    if (head == kHeadDiff) return diff_in(u, sl);
+    return u * input_sign(u, sl);
 }
-__device__ __forceinline__ fv4 skip_in(int head, fv4 u, fv4 ui) { return head == kHeadOneWay ? ui : u; }
+__device__ __forceinline__ fv4 skip_in(int head, fv4 u, fv4 ui)
+{
+    return head == kHeadOneWay || head == kHeadDiff ? ui : u;
+}
 
 // FP32 parameters -> per-lane MFMA operands.  One wave.
 __global__ void k_pack_mlp(const float *__restrict__ p, int head, MlpPacked *__restrict__ w)
@@ -231,8 +333,9 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ u
         const fv4 ui = mlp_in(r.head, uf, sl);
         Acts t;
         float sk;
-        const float m = mlp_forward(r, to_bf4(ui), skip_in(r.head, uf, ui), t, sk);
-        const float p = head_out(r.head, m, sk, uf.x, __shfl(uf.x, sl + 48), __shfl(uf.z, sl + 48));
+        bfv4 ub;
+        const float m = mlp_forward(r, ui, skip_in(r.head, uf, ui), t, sk, ub);
+        const float p = head_out(r.head, m, sk, head_in(uf, sl), r.hf);
         if (lane < 16 && s < n) out[s] = p;
     }
 }
@@ -327,24 +430,33 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float *__restrict__ u,
         }
         const fv4 ui = mlp_in(r.head, uf, sl);
         const fv4 us = skip_in(r.head, uf, ui);
-        const bfv4 ub = to_bf4(ui);
+        bfv4 ub;
         Acts a;
         float sk;
-        const float m = mlp_forward(r, ub, us, a, sk);
-        const float u12 = __shfl(uf.x, sl + 48), u14 = __shfl(uf.z, sl + 48);
-        const float pred = head_out(r.head, m, sk, uf.x, u12, u14);
+        const float m = mlp_forward(r, ui, us, a, sk, ub);
+        const HeadIn hv = head_in(uf, sl);
+        const float pred = head_out(r.head, m, sk, hv, r.hf);
         const float res = (lane < 16 && live) ? pred - t : 0.f;
         loss = loss + res * res;
         float g = 2.0f * res * inv_n;  // dL/dpred on lanes 0..15
         float gm = g, gp = g;           // dL/dm, dL/dp
+        float gb = -1.f;                // dL/db3 when it differs from dL/dm
         if (r.head == kHeadOneWay) {
             const float th = tanh_f(m + sk);
-            gm = gp = g * (1.0f - th * th) * (uf.x - u12);
+            gm = gp = g * (1.0f - th * th) * (hv.u0 - hv.u12);
         } else if (r.head == kHeadDelta) {
-            gm = g * (uf.x - u14);
+            gm = g * (hv.u0 - hv.u14);
+        } else if (r.head == kHeadHigdon) {
+            float A, B;
+            higdon_ab(hv, r.hf, A, B);
+            const float gt = higdon_gate(hv, r.hf);
+            const float th = tanh_f(r.b3 + gt * (m - r.b3));
+            gb = g * (1.0f - th * th) * B;
+            gm = gb * gt;
+            gp = 0.f;  // the skip is not used (wlin[0..3] hold the fixed factor)
         }
         const float gs = __shfl(gm, sl);  // broadcast to the sample's four lanes
-        if (lane < 16) gb3 = gb3 + gm;
+        if (lane < 16) gb3 = gb3 + (r.head == kHeadHigdon ? gb : gm);
         gwl = gwl + us * __shfl(gp, sl);
         // layer 3 (rank one): dW3 = a2 g, da2 = W3 g
         fv4 dz2[4];
@@ -477,16 +589,6 @@ __device__ __forceinline__ size_t face_cell(const Geo &g, int a, int n, int f, i
     return ((size_t)z * g.ny + y) * g.nx + x;
 }
 
-// Index into axis a's psi slab (layout in wf_kernels.h) for face
-// coordinates; T = graded depth of axis a, na = its length.
-__device__ __forceinline__ size_t face_psi(const Geo &g, int a, int T, int na, int n, int f, int s)
-{
-    const int gi = n < T ? n : n - (na - 2 * T);
-    if (a == 0) return ((size_t)s * g.ny + f) * (2 * T) + gi;   // (z, y)
-    if (a == 1) return ((size_t)s * (2 * T) + gi) * g.nx + f;  // (z, x)
-    return ((size_t)gi * g.ny + s) * g.nx + f;                  // (y, x)
-}
-
 // Pointer selection by a run-time index without indexing the by-value
 // kernel argument (which would spill it to scratch).
 __device__ __forceinline__ const float *field_ptr(const Ptrs &p, int c)
@@ -498,25 +600,6 @@ __device__ __forceinline__ const float *field_ptr(const Ptrs &p, int c)
     case 3: return p.f[3];
     case 4: return p.f[4];
     default: return p.f[5];
-    }
-}
-
-__device__ __forceinline__ const float *psi_ptr(const Ptrs &p, int a, int t)
-{
-    const int k = 4 * a + t;
-    switch (k) {
-    case 0: return p.psi[0][0];
-    case 1: return p.psi[0][1];
-    case 2: return p.psi[0][2];
-    case 3: return p.psi[0][3];
-    case 4: return p.psi[1][0];
-    case 5: return p.psi[1][1];
-    case 6: return p.psi[1][2];
-    case 7: return p.psi[1][3];
-    case 8: return p.psi[2][0];
-    case 9: return p.psi[2][1];
-    case 10: return p.psi[2][2];
-    default: return p.psi[2][3];
     }
 }
 
@@ -572,9 +655,9 @@ __device__ __forceinline__ float limit_ghost(const Geo &g, const Ptrs &es, const
     const float q0 = __shfl(u.x, sl + 16);  // Q(0)
     const float pm = fmaxf(fmaxf(fabsf(u.x), fabsf(u.y)), fmaxf(fabsf(u.z), fabsf(u.w)));
     flux = 0.f;
-    mem_stats[0] = mem[5];
-    mem_stats[1] = mem[6];
-    mem_stats[2] = mem[7];
+    mem_stats[0] = mem[kMemStats + 1];
+    mem_stats[1] = mem[kMemStats + 2];
+    mem_stats[2] = mem[kMemStats + 3];
     if (lane >= 16 || !live) return pred;
     const float sg = pol ? -1.0f : 1.0f;
     // this value's share of the credit, in scaled units (inf for a tiny s:
@@ -643,8 +726,10 @@ __device__ __forceinline__ float limit_ghost(const Geo &g, const Ptrs &es, const
 // ghost value the boundary cell's update reads, Q = its partner in the
 // curl, d = depth from the boundary cell inward:
 //   0..3   P(d)                     4..7  Q(d)
-//   8, 9   zero (unused)
-//   10, 11 CPML psi of P's normal derivative at d = 0, 1 (0 outside the grading)
+//   8, 9   tangential Laplacian of P at d = 0, 1: the sum over the two
+//          tangential axes of P(+1) + P(-1) - 2 P, an axis counted only
+//          where both neighbours lie on the face (or it wraps, periodic)
+//   10, 11 P(1)(n-1), P(1)(n-2)                    (history)
 //   12..15 g(n-1), g(n-2), P(0)(n-1), P(0)(n-2)   (history)
 // High faces (before the H phase): P = E_t1 / E_t2, Q = H_t2 / H_t1, ghost
 // E at b0+1.  Low faces (before the E phase): P = H_t1 / H_t2, Q = E_t2 /
@@ -660,24 +745,26 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
     const int a = h.axis, t1 = (a + 1) % 3, t2 = (a + 2) % 3;
     const int q4 = lane >> 4, sl = lane & 15;
     const int dir = h.hi ? -1 : 1;
-    const int na = h.na, Ta = h.Ta;
 
-    // per polarization: the P and Q fields and the psi slab, chosen once
-    // (the switches stay out of the loop)
-    // high faces: P = E_t1 / E_t2, Q = H_t2 / H_t1, H-phase psi terms;
-    // low faces: P = H_t1 / H_t2, Q = E_t2 / E_t1, E-phase psi terms
+    // per polarization: the P and Q fields, chosen once (the switches stay
+    // out of the loop); high faces: P = E_t1 / E_t2, Q = H_t2 / H_t1; low
+    // faces: P = H_t1 / H_t2, Q = E_t2 / E_t1
     const int P0 = (h.hi ? 0 : 3) + t1, P1 = (h.hi ? 0 : 3) + t2;
     const int Q0 = (h.hi ? 3 : 0) + t2, Q1 = (h.hi ? 3 : 0) + t1;
-    const int T0 = h.hi ? (a == 1 ? 0 : 1) : (a == 1 ? 2 : 3);
-    const int T1 = h.hi ? (a == 1 ? 1 : 0) : (a == 1 ? 3 : 2);
     const float *const Pf_0 = h.hi ? field_ptr(es, P0) : field_ptr(hs, P0);
     const float *const Pf_1 = h.hi ? field_ptr(es, P1) : field_ptr(hs, P1);
     const float *const Qf_0 = h.hi ? field_ptr(hs, Q0) : field_ptr(es, Q0);
     const float *const Qf_1 = h.hi ? field_ptr(hs, Q1) : field_ptr(es, Q1);
-    const float *const Ps_0 = psi_ptr(es, a, T0);
-    const float *const Ps_1 = psi_ptr(es, a, T1);
     // step of one cell inward along the normal, in the field arrays
     const long sa = (long)dir * (a == 0 ? 1l : (a == 1 ? (long)g.nx : (long)g.nx * g.ny));
+    // steps along the fast and slow tangential axes; the tangential
+    // Laplacian wraps where the face spans a periodic axis, and is taken
+    // along an axis only where both neighbours lie on the face otherwise
+    const int tf = t1 < t2 ? t1 : t2, ts = t1 < t2 ? t2 : t1;
+    const long sf = tf == 0 ? 1l : (long)g.nx;
+    const long ss = ts == 1 ? (long)g.nx : (long)g.nx * g.ny;
+    const bool wrap_f = (g.periodic >> tf & 1u) && h.nfast == h.nf && h.nfast > 2;
+    const bool wrap_s = (g.periodic >> ts & 1u) && h.nslow == h.ns && h.nslow > 2;
 
     MlpRegs r;
     if (h.mode == 0) load_regs(h.net, lane, r);
@@ -718,16 +805,29 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
 #pragma unroll
             for (int j = 0; j < 4; ++j) x[j] = F[c0 + j * sa];
         } else if (q4 == 2) {
-            // inputs 8..11: zero, zero and the psi of dP/dn at depths 0, 1
-            // (psi exists only in graded cells; a face of the domain has its
-            // first four graded)
-            const float *Ps = pol ? Ps_1 : Ps_0;
+            // inputs 8..11: the tangential Laplacian of P at depths 0, 1 and
+            // the history of P(1)
+            const float *Pp = pol ? Pf_1 : Pf_0;
+            const int fi = pos % h.nfast, si = pos / h.nfast;
+            const long c0 = (long)face_cell(g, a, h.b0, f, sc);
+            const long dfl = fi > 0 ? -sf : (long)(h.nfast - 1) * sf;
+            const long dfh = fi + 1 < h.nfast ? sf : -(long)(h.nfast - 1) * sf;
+            const bool fon = (fi > 0 && fi + 1 < h.nfast) || wrap_f;
+            const long dsl = si > 0 ? -ss : (long)(h.nslow - 1) * ss;
+            const long dsh = si + 1 < h.nslow ? ss : -(long)(h.nslow - 1) * ss;
+            const bool son = (si > 0 && si + 1 < h.nslow) || wrap_s;
 #pragma unroll
-            for (int j = 2; j < 4; ++j) {
-                const int n = h.b0 + dir * (j - 2);
-                const bool gr = Ta > 0 && (n < Ta || n >= na - Ta);
-                x[j] = gr ? Ps[face_psi(g, a, Ta, na, n, f, sc)] : 0.f;
+            for (int j = 0; j < 2; ++j) {
+                const long c = c0 + j * sa;
+                const float v2 = 2.0f * Pp[c];
+                const float lf = fon ? (Pp[c + dfl] + Pp[c + dfh]) - v2 : 0.f;
+                const float ls = son ? (Pp[c + dsl] + Pp[c + dsh]) - v2 : 0.f;
+                x[j] = lf + ls;
             }
+            const float2 q = *reinterpret_cast<const float2 *>(
+                h.mem + ((size_t)pol * npos + pos) * kMemStride + 4);
+            x[2] = q.x;
+            x[3] = q.y;
         } else {
             x = *reinterpret_cast<const fv4 *>(h.mem + ((size_t)pol * npos + pos) * kMemStride);
         }
@@ -758,14 +858,26 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
         const fv4 u = x * rs;
         // history on lanes 0..15: their own P(0) and the memory lanes' values
         const float g1 = __shfl(x.x, sl + 48), p1 = __shfl(x.z, sl + 48);
-        float gnew, mem_stats[3] = {0.f, 0.f, 0.f}, flux = 0.f;
+        const float q1 = __shfl(x.z, sl + 32);  // P1(n-1)
+        float gnew, mem_stats[3] = {0.f, 0.f, 0.f}, flux = 0.f, dk = 0.f;
         if (h.mode == 0) {
             Acts t;
             const fv4 ui = mlp_in(r.head, u, sl);
             float sk;
-            const float m = mlp_forward(r, to_bf4(ui), skip_in(r.head, u, ui), t, sk);
-            float pred =
-                head_out(r.head, m, sk, u.x, __shfl(u.x, sl + 48), __shfl(u.z, sl + 48));
+            bfv4 ub;
+            const float m = mlp_forward(r, ui, skip_in(r.head, u, ui), t, sk, ub);
+            const HeadIn hv = head_in(u, sl);
+            float pred = head_out(r.head, m, sk, hv, r.hf);
+            if (r.head == kHeadHigdon && r.hf.beta > 0.f && r.hf.beta < 1.f) {
+                // the adaptive coefficient's departure from its base
+                // tanh(b3), smoothed in time at this position
+                float A, B;
+                higdon_ab(hv, r.hf, A, B);
+                const float kb = tanh_f(r.hf.b3);
+                const float ki = tanh_f(r.hf.b3 + higdon_gate(hv, r.hf) * (m - r.hf.b3));
+                dk = (1.0f - r.hf.beta) * mem[6] + r.hf.beta * (ki - kb);
+                pred = A + (kb + dk) * B;
+            }
             if (h.passive) pred = limit_ghost(g, es, hs, h, pol, pos, live, lane, u, s, rs, mem, pred, flux,
                                                mem_stats);
             gnew = s * pred;
@@ -801,7 +913,11 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
         wflux += sum16(flux);  // the 16 samples' flux, summed in lane order
         if (lane < 16 && live) {
             *reinterpret_cast<fv4 *>(mem) = fv4{gnew, g1, x.x, p1};
-            if (h.mode != 1) *reinterpret_cast<fv4 *>(mem + 4) = fv4{flux, mem_stats[0], mem_stats[1], mem_stats[2]};
+            *reinterpret_cast<float2 *>(mem + 4) = float2{x.y, q1};
+            mem[6] = dk;
+            if (h.mode != 1)
+                *reinterpret_cast<fv4 *>(mem + kMemStats) =
+                    fv4{flux, mem_stats[0], mem_stats[1], mem_stats[2]};
         }
     }
     if (h.passive && h.mode != 1) {
@@ -890,13 +1006,13 @@ __global__ __launch_bounds__(256) void k_edge_limit(Geo g, Ptrs es, Ptrs hs, Fac
         B.ghost[1][pB] = nB;
         mA[0] = nA;
         mB[0] = nB;
-        mA[4] = 0.5f * fr;
-        mB[4] = 0.5f * fr;
+        mA[kMemStats] = 0.5f * fr;
+        mB[kMemStats] = 0.5f * fr;
         if (vt != v) {
-            mA[5] += 1.f;
-            mB[5] += 1.f;
-            mA[7] += d * d;
-            mB[7] += d * d;
+            mA[kMemStats + 1] += 1.f;
+            mB[kMemStats + 1] += 1.f;
+            mA[kMemStats + 3] += d * d;
+            mB[kMemStats + 3] += d * d;
         }
     }
     // the workgroup's flux, summed in a fixed order (waves, then lanes)
@@ -1154,7 +1270,7 @@ int wf_mlp_set_head(wf_mlp_trainer *t, int head)
 {
     clear_error();
     if (!t) return fail(WF_ERR_INVALID, "wf_mlp_set_head: null trainer");
-    if (head != kHeadDirect && head != kHeadOneWay && head != kHeadDelta)
+    if (!valid_head(head))
         return fail(WF_ERR_INVALID, "wf_mlp_set_head: unknown head %d", head);
     WF_HIP(hipSetDevice(t->device));
     t->head = head;
@@ -1309,7 +1425,7 @@ int wf_mlp_predict_noalloc(int device, const float *params, int head, uint64_t n
     clear_error();
     if (!params || (n && (!inputs || !outputs)))
         return fail(WF_ERR_INVALID, "wf_mlp_predict_noalloc: null pointer");
-    if (head != kHeadDirect && head != kHeadOneWay && head != kHeadDelta)
+    if (!valid_head(head))
         return fail(WF_ERR_INVALID, "wf_mlp_predict_noalloc: unknown head %d", head);
     if (n > kNoAllocMaxSamples)
         return fail(WF_ERR_INVALID,

@@ -5,9 +5,9 @@ The reference ends every face with a graded CPML backed by the zero field
 can instead end after a LAYER-cell layer of plain cells, beyond which the
 fields -- the "ghost" values the boundary cell's curl update reads, zero in
 the reference -- are predicted each half step by a small per-face network
-from the 4-cell-deep E/H/psi stencil of the layer and a two-step history
-(the C ABI's wf_set_learned_faces).  With no CPML anywhere, the whole domain
-runs in the single-pass kernel.
+from the 4-cell-deep E/H stencil of the layer and a two-step history (the C
+ABI's wf_set_learned_faces).  With no CPML anywhere, the whole domain runs
+in the single-pass kernel.
 
 Training data come from the engine itself: the same physics run with a
 THICK-cell CPML around the thin domain (its twin) records at the virtual
@@ -157,27 +157,33 @@ def _gauss(freq):
 # ---------------------------------------------------------------------------
 # Network and training
 # ---------------------------------------------------------------------------
-HEAD_DIRECT, HEAD_ONE_WAY, HEAD_DELTA = 0, 1, 2
+HEAD_DIRECT, HEAD_ONE_WAY, HEAD_DELTA, HEAD_DIFF, HEAD_HIGDON = 0, 1, 2, 3, 4
 
 
 @dataclass
 class FaceNets:
-    """Parameters of the six face networks, [6, param_count()] float32, and
-    their output head (HEAD_DIRECT: the scaled ghost value; HEAD_ONE_WAY: the
-    coefficient of a first-order one-way update)."""
+    """Parameters of the six face networks, [6, param_count()] float32, their
+    output head (HEAD_DIRECT: the scaled ghost value; HEAD_ONE_WAY: the
+    coefficient of a first-order one-way update; HEAD_DELTA, HEAD_DIFF,
+    HEAD_HIGDON: see wf_learned.h), and whether the layer runs with the
+    passive limits (None: the module's PASSIVE)."""
     params: np.ndarray
     dt: float
     dx: float
     head: int = HEAD_DIRECT
+    passive: bool | None = None
 
     def save(self, path: str) -> None:
-        np.savez(path, params=self.params, dt=self.dt, dx=self.dx, head=self.head)
+        np.savez(path, params=self.params, dt=self.dt, dx=self.dx, head=self.head,
+                 passive=-1 if self.passive is None else int(self.passive))
 
     @classmethod
     def load(cls, path: str) -> "FaceNets":
         z = np.load(path)
+        pv = int(z["passive"]) if "passive" in z else -1
         return cls(z["params"].astype(f32), float(z["dt"]), float(z["dx"]),
-                   int(z["head"]) if "head" in z else HEAD_DIRECT)
+                   int(z["head"]) if "head" in z else HEAD_DIRECT,
+                   None if pv < 0 else bool(pv))
 
 
 def init_params(rng: np.random.Generator, wlin=None, features=None) -> np.ndarray:
@@ -202,10 +208,33 @@ def init_params(rng: np.random.Generator, wlin=None, features=None) -> np.ndarra
     return p
 
 
-def least_squares_skip(u: np.ndarray, t: np.ndarray, features=None) -> np.ndarray:
+def diff_features(u: np.ndarray) -> np.ndarray:
+    """The difference head's inputs (WF_HEAD_DIFF) from scaled input rows:
+    P(d)-P(d+1) and Q(d)-Q(d+1) for d = 0..2, the Laplacians (8, 9) as they
+    are, P1(n-1)-P1, P1(n-2)-P1(n-1), g(n-1)-P0, g(n-2)-g(n-1),
+    P0(n-1)-P0, P0(n-2)-P0(n-1); 3 and 7 zero."""
+    u = np.asarray(u, f32)
+    d = np.zeros_like(u)
+    for b in (0, 4):
+        d[:, b:b + 3] = u[:, b:b + 3] - u[:, b + 1:b + 4]
+    d[:, 8:10] = u[:, 8:10]
+    d[:, 10] = u[:, 10] - u[:, 1]
+    d[:, 11] = u[:, 11] - u[:, 10]
+    d[:, 12] = u[:, 12] - u[:, 0]
+    d[:, 13] = u[:, 13] - u[:, 12]
+    d[:, 14] = u[:, 14] - u[:, 0]
+    d[:, 15] = u[:, 15] - u[:, 14]
+    return d
+
+
+def least_squares_skip(u: np.ndarray, t: np.ndarray, features=None,
+                       head: int = HEAD_DIRECT) -> np.ndarray:
     """Initial linear skip: the least-squares fit of the targets (FP64) over
-    the input columns `features` (default all; the others get 0)."""
+    the input columns `features` (default all; the others get 0).  For the
+    difference head, the fit of t - P0 over the difference inputs."""
     cols = list(range(16)) if features is None else list(features)
+    if head == HEAD_DIFF:
+        u, t = diff_features(u), t - u[:, 0]
     w = np.zeros(16, f32)
     sol, *_ = np.linalg.lstsq(u[:, cols].astype(np.float64), t.astype(np.float64), rcond=None)
     w[cols] = sol
@@ -266,19 +295,17 @@ class Trainer:
         return out
 
 
-# Inputs used by default: P and Q at depths 0..3, the normal-derivative psi
-# at depths 0 and 1, and the history (inputs 8, 9 are always zero).  Purely
-# per-column: no input couples neighbouring face positions.  (With the psi of
-# depths 2 and 3 added, the least-squares skip fits the 1D data as well but
-# is unstable in closed loop.)
-FEATURES = (0, 1, 2, 3, 4, 5, 6, 7, 10, 11, 12, 13, 14, 15)
+# Inputs of the direct head by default: P and Q at depths 0..3 and the
+# ghost / P0 history.  Purely per-column: no input couples neighbouring face
+# positions.
+FEATURES = (0, 1, 2, 3, 4, 5, 6, 7, 12, 13, 14, 15)
 
-# Inputs of the configuration that is stable in 3D: P at depths 0..3 and the
-# history, with the linear skip alone (least squares on 3D point-source and
-# plane-wave data).  Measured on MI355X in a 128^3 vacuum cube (1500 steps):
-# every variant that also takes the curl partner Q, and every one with the
-# MLP added, grows without bound (the MLP slowly, even scaled down to a
-# tenth); this one does not.
+# Inputs of the linear 3D predictor (the direct head's skip alone, least
+# squares on 3D point-source and plane-wave data): P at depths 0..3 and the
+# history.  Kept as the comparison for the Higdon layer; its coefficients
+# sit close to a growing mode at zero tangential wavenumber on the low
+# faces (rounded to bf16, it grows without bound), and every direct-head
+# MLP added to it grows in 3D.
 FEATURES_3D = (0, 1, 2, 3, 12, 13, 14, 15)
 
 
@@ -318,6 +345,55 @@ def train_face_one_way(u: np.ndarray, t: np.ndarray, dt: float, dx: float, seed:
     return p, hist
 
 
+def one_way_angle_coefficient(dt: float, dx: float, degrees: float) -> float:
+    """The one-way coefficient for a plane wave at `degrees` from the
+    normal (the factor's speed c / cos(angle))."""
+    s = 299792458.0 * dt / dx / math.cos(math.radians(degrees))
+    return (s - 1.0) / (s + 1.0)
+
+
+def higdon_params(kf: float, gf: float, ga: float, ka: float, rng=None, features=None,
+                  gate: float = 0.0):
+    """Higdon-head parameters: the fixed factor (kf, gf), the adaptive
+    factor's damping ga, the gate width (0 = no gate), and a network that
+    starts at ka (b3 = atanh ka, W3 = 0)."""
+    p = init_params(rng or np.random.default_rng(0), None, features)
+    p[param_count() - 17] = math.atanh(ka)
+    p[param_count() - 16:param_count() - 12] = (kf, gf, ga, gate)
+    return p
+
+
+def train_face_higdon(u: np.ndarray, t: np.ndarray, dt: float, dx: float, seed: int = 0,
+                      fixed=(45.0, 0.9), ga: float = 1.0, start: float = 0.0, gate: float = 0.0,
+                      schedule=((2000, 1e-3), (2000, 3e-4), (1000, 1e-4)), features=None,
+                      device: int = 0, log=None):
+    """The Higdon head: the fixed factor at angle fixed[0] with damping
+    fixed[1]; the network starts as the adaptive factor at angle `start`
+    (damping ga) and learns how its coefficient should vary with the local
+    stencil; Adam on the network, the fixed factor held."""
+    rng = np.random.default_rng(seed)
+    u = np.array(u, f32)
+    if features is not None:
+        off = np.ones(16, bool)
+        off[list(features)] = False
+        u[:, off] = 0.0
+    p = higdon_params(one_way_angle_coefficient(dt, dx, fixed[0]), fixed[1], ga,
+                      one_way_angle_coefficient(dt, dx, start), rng, features, gate)
+    tr = Trainer(u, t, device, head=HEAD_HIGDON)
+    tr.set_params(p)
+    hist = [(0, tr.gradient()[1])]
+    done = 0
+    for iters, lr in schedule:
+        loss = tr.train(iters, lr, 0.0)
+        done += iters
+        hist.append((done, loss))
+        if log:
+            log(f"  {done} iters (lr {lr:g}): loss {loss:.3e}")
+    p = tr.params()
+    tr.close()
+    return p, hist
+
+
 def train_face(u: np.ndarray, t: np.ndarray, seed: int = 0,
                schedule=((2000, 1e-4), (1000, 3e-5)), lr_skip: float = 1e-5,
                features=FEATURES, device: int = 0, log=None, head: int = HEAD_DIRECT):
@@ -333,7 +409,7 @@ def train_face(u: np.ndarray, t: np.ndarray, seed: int = 0,
         off[list(features)] = False
         u[:, off] = 0.0
     tr = Trainer(u, t, device, head=head)
-    tr.set_params(init_params(rng, least_squares_skip(u, t, features), features))
+    tr.set_params(init_params(rng, least_squares_skip(u, t, features, head), features))
     hist = [(0, tr.gradient()[1])]
     done = 0
     for iters, lr in schedule:
@@ -379,7 +455,9 @@ def thin_engine(dims: GridDims, nets: FaceNets | None, dt: float, dx: float, per
     eng = DeviceEngine(dims, ids, table, None, periodic=periodic, device=device)
     mask = face_mask(periodic)
     if mode == "learned":
-        check(lib().wf_set_face_limits(eng.handle, int(PASSIVE if passive is None else passive),
+        if passive is None:
+            passive = PASSIVE if nets.passive is None else nets.passive
+        check(lib().wf_set_face_limits(eng.handle, int(passive),
                                        float(RETAIN if retain is None else retain),
                                        float(GAMMA if gamma is None else gamma)))
         p = np.ascontiguousarray(nets.params, f32).reshape(-1)
@@ -502,14 +580,15 @@ def reflection(mode: str, nets: FaceNets | None = None, cpml_voxels: int = 0,
 # ---------------------------------------------------------------------------
 # Six-face training from 3D runs
 # ---------------------------------------------------------------------------
-def point_sources(rng, thin: GridDims, n: int, margin: int = 12) -> list[Source]:
-    """n Gaussian / Ricker point sources (10-45 GHz) at random interior
-    positions, random E component and amplitude."""
+def point_sources(rng, thin: GridDims, n: int, margin: int = 12,
+                  waves=("gaussian", "ricker")) -> list[Source]:
+    """n point sources (10-45 GHz; waveforms taken in turn from `waves`) at
+    random interior positions, random E component and amplitude."""
     out = []
     for k in range(n):
         freq = float(f32(rng.uniform(1.0e10, 4.5e10)))
-        wave = (Waveform.gaussian_pulse(*_gauss(freq)) if k % 2 == 0 else
-                Waveform.ricker_wavelet(freq, float(f32(1.0) / f32(freq))))
+        wave = (Waveform.gaussian_pulse(*_gauss(freq)) if waves[k % len(waves)] == "gaussian"
+                else Waveform.ricker_wavelet(freq, float(f32(1.0) / f32(freq))))
         c = [int(rng.integers(margin, m - margin)) for m in (thin.nx, thin.ny, thin.nz)]
         comp = [SC.Ex, SC.Ey, SC.Ez][int(rng.integers(0, 3))]
         out.append(Source(c[0], c[1], c[2], comp, float(rng.uniform(0.3, 1.0)), wave))
@@ -517,7 +596,8 @@ def point_sources(rng, thin: GridDims, n: int, margin: int = 12) -> list[Source]
 
 
 def collect_3d(thin: GridDims, runs: int, steps: int, every: int, per_face: int, seed: int,
-               dt: float, dx: float, plane_runs: int = 8, log=None, device: int = 0):
+               dt: float, dx: float, plane_runs: int = 8, log=None, device: int = 0,
+               waves=("gaussian", "ricker")):
     """Training samples for all six faces: `runs` thick-CPML runs of `thin`
     with random point sources (oblique incidence), plus `plane_runs`
     plane-wave runs along each axis (normal incidence), each face capped at
@@ -526,8 +606,8 @@ def collect_3d(thin: GridDims, runs: int, steps: int, every: int, per_face: int,
     rng = np.random.default_rng(seed)
     pools = {f: [] for f in FACES}
     for r in range(runs):
-        rec = record_run(thin, point_sources(rng, thin, 3), steps, dt, dx, every=every,
-                         device=device)
+        rec = record_run(thin, point_sources(rng, thin, 3, waves=waves), steps, dt, dx,
+                         every=every, device=device)
         for f in FACES:
             u, t = training_rows(rec[f])
             pools[f].append((u, t))
@@ -542,7 +622,7 @@ def collect_3d(thin: GridDims, runs: int, steps: int, every: int, per_face: int,
         for r in range(plane_runs):
             srcs = []
             for pol in (0, 1):
-                kind = ("gaussian", "ricker")[(r + pol) % 2]
+                kind = waves[(r + pol) % len(waves)]
                 freq = float(f32(rng.uniform(1.0e10, 4.5e10)))
                 srcs += plane_wave_sources(a, n, int(rng.integers(8, n - 8)), kind, freq, pol,
                                            float(rng.uniform(0.3, 1.0)))
@@ -560,14 +640,15 @@ def collect_3d(thin: GridDims, runs: int, steps: int, every: int, per_face: int,
     return out
 
 
-def linear_all(samples: dict, dt: float, dx: float, features=FEATURES_3D) -> FaceNets:
+def linear_all(samples: dict, dt: float, dx: float, features=FEATURES_3D,
+               head: int = HEAD_DIRECT) -> FaceNets:
     """Per face, the linear skip alone (least squares over `features`; MLP
-    weights zero): the learned boundary's 3D-stable configuration."""
+    weights zero)."""
     p = np.zeros((6, param_count()), f32)
     for i, f in enumerate(FACES):
         u, t = samples[f]
-        p[i, param_count() - 16:] = least_squares_skip(u, t, features)
-    return FaceNets(p, dt, dx)
+        p[i, param_count() - 16:] = least_squares_skip(u, t, features, head)
+    return FaceNets(p, dt, dx, head)
 
 
 # Held-out frequencies for choosing among training runs: the two-run
@@ -612,15 +693,111 @@ def train_x_selected(samples: dict, base: "FaceNets", seeds=(0, 10, 20, 30), log
     return best[2], cands
 
 
+# The Higdon head of the shipped layer: the fixed factor at 45 degrees with
+# its memory damped by HIGDON_DAMP, the adaptive factor undamped and
+# starting at normal incidence; the network sees P at depths 0..3, its
+# tangential Laplacians, the P1 history and the ghost / P0 history
+# (sign-normalized).  Measured on MI355X (30 GHz, 20 points per wavelength,
+# Courant 0.4): undamped, the product reflects 5.9e-5 at normal incidence
+# but the 3D error of a 128^3 cube grows after ~1000 steps (a periodic slab
+# with no edges does not grow: the growth is where two faces meet); damped
+# by 0.95, 8.9e-5 and 2.1e-4 over 3000 steps, the field energy flat from
+# 3000 to 15000 steps; 0.9 and 0.93 are as flat but reflect more.
+HIGDON_FIXED_ANGLE = 45.0
+HIGDON_DAMP = 0.95
+FEATURES_HIGDON = (0, 1, 2, 3, 8, 9, 10, 11, 12, 13, 14, 15)
+# Closed-loop validation for the Higdon head's training (never the test
+# geometry): the two-run reflection at VALIDATION_FREQS and the reflected
+# energy in a 96^3 vacuum cube with an off-centre Ricker source over 3000
+# steps, overall and over the last 500 steps.
+VALIDATION_3D = dict(interior=96, steps=3000, source=(40, 52, 45), late=500)
+
+
+def higdon_base(dt: float, dx: float, rng=None) -> np.ndarray:
+    """[6, param_count()] Higdon-head parameters with the network's output
+    fixed at the normal-incidence coefficient (W3 = 0): the method without
+    the network's adaptation."""
+    ang = lambda a: one_way_angle_coefficient(dt, dx, a)
+    return np.stack([higdon_params(ang(HIGDON_FIXED_ANGLE), HIGDON_DAMP, 1.0, ang(0.0),
+                                   np.random.default_rng(i) if rng is None else rng,
+                                   FEATURES_HIGDON) for i in range(6)])
+
+
+def validate_higdon(nets: FaceNets, refs1: dict, refs3: dict, device: int = 0):
+    """(validation reflection, 3D ratio, late 3D ratio) of a Higdon-head
+    layer."""
+    v1 = validation_reflection(nets, refs1, device=device)
+    v3, late = reflection_3d(nets, kinds=("learned",), refs=refs3, device=device,
+                             **VALIDATION_3D)["learned"]
+    return v1, v3, late
+
+
+def train_higdon(samples: dict, dt: float, dx: float, checkpoints=(50, 100, 200, 400),
+                 lr: float = 1e-4, betas=(0.02, 0.005), log=None, device: int = 0):
+    """The Higdon head's network trained on the GPU (Adam on the one-step
+    error of the recorded ghosts), with early stopping on closed-loop
+    validation: at each checkpoint and time smoothing beta of the adaptive
+    coefficient, the layer is run (validate_higdon); the candidate with the
+    lowest validation reflection among those whose 3D ratio is within 1.5x,
+    and late 3D ratio within 3x (or 1e-5), of the untrained layer's is kept,
+    or the untrained layer if none beats it.  Returns (FaceNets, table of
+    (iterations, beta, v1, v3, late))."""
+    base = higdon_base(dt, dx)
+    refs1, refs3 = {}, {}
+    b1, b3, bl = validate_higdon(FaceNets(base, dt, dx, HEAD_HIGDON, False), refs1, refs3,
+                                 device)
+    table = [(0, 1.0, b1, b3, bl)]
+    if log:
+        log(f"untrained: validation R {b1:.3e}, 3D {b3:.3e}, late {bl:.3e}")
+    trs = []
+    off = np.ones(16, bool)
+    off[list(FEATURES_HIGDON)] = False
+    for i, f in enumerate(FACES):
+        u, t = samples[f]
+        u = np.array(u, f32)
+        u[:, off] = 0.0
+        tr = Trainer(u, t, device, head=HEAD_HIGDON)
+        tr.set_params(base[i])
+        trs.append(tr)
+    best = (b1, base)
+    done = 0
+    try:
+        for it in checkpoints:
+            for tr in trs:
+                tr.train(it - done, lr, 0.0)
+            done = it
+            snap = np.stack([tr.params() for tr in trs])
+            for beta in betas:
+                p = snap.copy()
+                p[:, param_count() - 12] = beta
+                v1, v3, late = validate_higdon(FaceNets(p, dt, dx, HEAD_HIGDON, False), refs1,
+                                               refs3, device)
+                table.append((it, beta, v1, v3, late))
+                ok = v3 <= 1.5 * b3 and late <= max(3 * bl, 1e-5)
+                if log:
+                    log(f"{it} iterations, beta {beta}: validation R {v1:.3e}, 3D {v3:.3e}, "
+                        f"late {late:.3e}{'' if ok else ' (rejected)'}")
+                if ok and v1 < best[0]:
+                    best = (v1, p)
+    finally:
+        for tr in trs:
+            tr.close()
+    return FaceNets(best[1], dt, dx, HEAD_HIGDON, False), table
+
+
 def train_all(samples: dict, dt: float, dx: float, log=None, device: int = 0,
-              head: int = HEAD_DIRECT, features=FEATURES) -> FaceNets:
+              head: int = HEAD_DIRECT, features=FEATURES, higdon: dict | None = None) -> FaceNets:
     p = np.zeros((6, param_count()), f32)
     for i, f in enumerate(FACES):
         u, t = samples[f]
         if log:
             log(f"{f}: {len(t)} samples")
-        if head == HEAD_DIRECT:
-            p[i], _ = train_face(u, t, seed=i, features=features, device=device, log=log)
+        if head in (HEAD_DIRECT, HEAD_DIFF):
+            p[i], _ = train_face(u, t, seed=i, features=features, device=device, log=log,
+                                 head=head)
+        elif head == HEAD_HIGDON:
+            p[i], _ = train_face_higdon(u, t, dt, dx, seed=i, features=features, device=device,
+                                        log=log, **(higdon or {}))
         else:
             p[i], _ = train_face_one_way(u, t, dt, dx, seed=i, features=features, device=device,
                                          log=log)
@@ -665,31 +842,49 @@ def probe_points(interior: int, inset: int = 10):
 
 
 def reflection_3d(nets: FaceNets, interior: int = 128, steps: int = 520, freq: float = 3.0e10,
-                  kinds=BOUNDARIES, wave_kind: str = "ricker", device: int = 0) -> dict:
-    """A Ricker (or Gaussian) Ez point source at the centre of a vacuum cube; the Ez
-    trace at 26 probes near the faces, edges and corners, for each boundary
-    kind.  Returns per kind the reflected-to-incident energy ratio
-    sum (trace - trace_ref)^2 / sum trace_ref^2 over all probes, with the
-    64-cell CPML run as the reference (its own reflection is the floor)."""
+                  kinds=BOUNDARIES, wave_kind: str = "ricker", device: int = 0,
+                  source=None, late: int = 0, refs: dict | None = None) -> dict:
+    """A Ricker (or Gaussian) Ez point source at the centre of a vacuum cube
+    (or at `source`, interior coordinates); the Ez trace at 26 probes near
+    the faces, edges and corners, for each boundary kind.  Returns per kind
+    the reflected-to-incident energy ratio sum (trace - trace_ref)^2 /
+    sum trace_ref^2 over all probes, with the 64-cell CPML run as the
+    reference (its own reflection is the floor); with late > 0, per kind the
+    pair (that ratio, the same over the last `late` steps only: the energy a
+    boundary still returns after the wave has left).  `refs` caches the
+    reference traces (per interior, steps and source)."""
     from .probe import FieldComponent as PC
     from .probe import Probe
     dx, dt = nets.dx, nets.dt
     c = interior // 2
+    sx, sy, sz = source if source is not None else (c, c, c)
     wave = (Waveform.ricker_wavelet(freq, float(f32(1.0) / f32(freq))) if wave_kind == "ricker"
             else Waveform.gaussian_pulse(*_gauss(freq)))
     pts = probe_points(interior)
+    refs = {} if refs is None else refs
+    rkey = ("cpml64", interior, steps, (sx, sy, sz), freq, wave_kind)
     traces = {}
     for kind in dict.fromkeys(("cpml64",) + tuple(kinds)):
+        if kind == "cpml64" and rkey in refs:
+            traces[kind] = refs[rkey]
+            continue
         eng, m = _boundary_engine(kind, interior, nets, dt, dx, device)
         with eng:
-            eng.set_sources([Source(c + m, c + m, c + m, SC.Ez, 1.0, wave)], dt, steps)
+            eng.set_sources([Source(sx + m, sy + m, sz + m, SC.Ez, 1.0, wave)], dt, steps)
             eng.set_probes([Probe(x + m, y + m, z + m, PC.Ez, [], 0.0) for x, y, z in pts], dt,
                            steps, trace=True)
             eng.step(steps)
             traces[kind] = eng.read_trace(steps).astype(np.float64)
+    refs[rkey] = traces["cpml64"]
     ref = traces["cpml64"]
     e_ref = float((ref ** 2).sum())
-    return {k: float(((traces[k] - ref) ** 2).sum()) / e_ref for k in kinds if k != "cpml64"}
+    out = {}
+    for k in kinds:
+        if k == "cpml64":
+            continue
+        r = float(((traces[k] - ref) ** 2).sum()) / e_ref
+        out[k] = (r, float(((traces[k][-late:] - ref[-late:]) ** 2).sum()) / e_ref) if late else r
+    return out
 
 
 def throughput(kind: str, nets: FaceNets | None, interior: int = 248, steps: int = 100,
