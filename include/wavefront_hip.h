@@ -304,14 +304,19 @@ int wf_mlp_set_params(wf_mlp_trainer *t, const float *params);
  * first-order one-way factors (1 + k g T) - X (g T + k) (T: one step back,
  * X: one cell inward, g <= 1 damps the factor's memory): a fixed one with
  * k = wlin[0], g = wlin[1] and an adaptive one with k = tanh(y), y the
- * network's output on sign-normalized inputs, g = wlin[2] (the other skip
- * weights are not used); P at depth 1 and its history (inputs 1, 10, 11)
- * enter besides P0 and the ghost's history. */
+ * network's output on sign-normalized inputs, g = wlin[2]; 0 < wlin[3] < 1
+ * smooths the adaptive k in time at each face position (an exponential
+ * average with weight wlin[3] of tanh(y) - tanh(b3), added to tanh(b3));
+ * the other skip weights are not used.  P at depth 1 and its history
+ * (inputs 1, 10, 11) enter besides P0 and the ghost's history.  With the
+ * network's output weights zero this is the fixed second-order Higdon
+ * update. */
 #define WF_HEAD_HIGDON 4
 /* WF_HEAD_DIFF -- ghost/s = P0(n) + skip(d) + m(d), network and skip on
  * differences of the inputs that vanish for a field constant in space and
- * time (depth-to-depth steps of P and Q, the Laplacians and psi as they are,
- * g(n-1) - P0, g(n-2) - g(n-1), P0(n-1) - P0, P0(n-2) - P0(n-1)): a static
+ * time (depth-to-depth steps of P and Q, inputs 8, 9 as they are,
+ * P1(n-1) - P1, P1(n-2) - P1(n-1), g(n-1) - P0, g(n-2) - g(n-1),
+ * P0(n-1) - P0, P0(n-2) - P0(n-1)): a static
  * uniform field is held exactly whatever the weights. */
 #define WF_HEAD_DIFF 3
 int wf_mlp_set_head(wf_mlp_trainer *t, int head);

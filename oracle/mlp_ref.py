@@ -289,28 +289,17 @@ def higdon_terms(kf, gf, ga, u):
 
 def _higdon_fixed(p):
     wl = split(np.asarray(p, np.float32))["wlin"].astype(np.float64)
-    return wl[0], wl[1], wl[2], wl[3]
-
-
-def higdon_gate(u, c):
-    """1 where the tangential Laplacian u8 is 0, falling to 0 at
-    |u8| = c max|u0..u3| (c <= 0: 1)."""
-    u = np.asarray(u, np.float64)
-    if not c > 0:
-        return np.ones(len(u))
-    pm = np.abs(u[:, 0:4]).max(axis=1)
-    return np.maximum(1.0 - np.abs(u[:, 8]) / (c * pm + 1e-30), 0.0)
+    return wl[0], wl[1], wl[2]
 
 
 def predict_higdon_f64(p, u, rounded=True):
-    """Higdon head: A + tanh(b3 + gate (y - b3)) B, y = the MLP part on the
-    sign-normalized inputs; kf, gf, ga, c = wlin[0..3]."""
+    """Higdon head: A + tanh(y) B, y = the MLP part on the sign-normalized
+    inputs; kf, gf, ga = wlin[0..2]."""
     u = np.asarray(u, np.float64)
-    kf, gf, ga, c = _higdon_fixed(p)
-    w = _weights(p, rounded)
+    kf, gf, ga = _higdon_fixed(p)
     y, _ = _mlp_f64(p, u * one_way_sign(u)[:, None], rounded)
     A, B = higdon_terms(kf, gf, ga, u)
-    return A + np.tanh(w["b3"] + higdon_gate(u, c) * (y - w["b3"])) * B
+    return A + np.tanh(y) * B
 
 
 def loss_higdon_f64(p, u, t, rounded=True):
@@ -318,22 +307,20 @@ def loss_higdon_f64(p, u, t, rounded=True):
 
 
 def grad_higdon_f64(p, u, t, rounded=True):
-    """d loss / d p for the Higdon head: b3 from the error times
-    (1 - ka^2) B, the network's other parameters from the same times the
-    gate; the skip weights (the fixed factor) get none."""
+    """d loss / d p for the Higdon head: the network's parameters from the
+    error times (1 - ka^2) B; the skip weights (the fixed factor) get
+    none."""
     u = np.asarray(u, np.float64)
     n = len(u)
-    kf, gf, ga, c = _higdon_fixed(p)
+    kf, gf, ga = _higdon_fixed(p)
     w = _weights(p, rounded)
     v = u * one_way_sign(u)[:, None]
     y, (a1, a2) = _mlp_f64(p, v, rounded)
     A, B = higdon_terms(kf, gf, ga, u)
-    gt = higdon_gate(u, c)
-    ka = np.tanh(w["b3"] + gt * (y - w["b3"]))
+    ka = np.tanh(y)
     e = 2.0 * (A + ka * B - np.asarray(t, np.float64)) / n
-    gb = e * (1.0 - ka * ka) * B
-    gm = gb * gt
-    out = {"wlin": np.zeros(IN), "b3": np.array([gb.sum()]), "W3": a2.T @ gm}
+    gm = e * (1.0 - ka * ka) * B
+    out = {"wlin": np.zeros(IN), "b3": np.array([gm.sum()]), "W3": a2.T @ gm}
     dz2 = np.outer(gm, w["W3"]) * dact(a2)
     out["W2"] = a1.T @ dz2
     out["b2"] = dz2.sum(0)
@@ -344,8 +331,8 @@ def grad_higdon_f64(p, u, t, rounded=True):
 
 
 def diff_features(u):
-    """The difference head's inputs: P and Q depth steps, the Laplacians,
-    the P1 and ghost / P0 history as differences (3 and 7 zero)."""
+    """The difference head's inputs: P and Q depth steps, inputs 8, 9 as
+    they are, the P1 and ghost / P0 history as differences (3 and 7 zero)."""
     u = np.asarray(u, np.float64)
     d = np.zeros_like(u)
     for b in (0, 4):

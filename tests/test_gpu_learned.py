@@ -101,10 +101,10 @@ def _point_sources(c):
 
 
 def test_recorded_inputs_match_fields():
-    """The hook's inputs (fields at depths 0..3, their tangential Laplacians at
-    depths 0, 1, the history of P at depths 0, 1 and of the ghost) and true
-    ghost values at the last step of a thick run, against the same
-    quantities taken from the fields read back and the earlier records."""
+    """The hook's inputs (fields at depths 0..3, zeros, the history of P at
+    depths 0, 1 and of the ghost) and true ghost values at the last step of
+    a thick run, against the same quantities taken from the fields read back
+    and the earlier records."""
     _say("face hook record vs fields")
     thin = GridDims(24, 16, 32)
     dx, dt, _ = L.study_grid()
@@ -146,7 +146,7 @@ def test_recorded_inputs_match_fields():
         return F[comp, c[2], c[1], c[0]]
 
     rng = np.random.default_rng(7)
-    checked = lap_seen = 0
+    checked = hist_seen = 0
     for f in range(6):
         a, hi = f // 2, f % 2
         t1, t2 = (a + 1) % 3, (a + 2) % 3
@@ -168,14 +168,6 @@ def test_recorded_inputs_match_fields():
                 for j in range(4):
                     x[j] = at(FP, P, a, r.b0 + d * j, fc, sc)
                     x[4 + j] = at(FQ, Q, a, r.b0 + d * j, fc, sc)
-                for j in range(2):
-                    v = at(FP, P, a, r.b0 + d * j, fc, sc)
-                    if 0 < fi < r.nfast - 1:
-                        x[8 + j] += (at(FP, P, a, r.b0 + d * j, fc - 1, sc) +
-                                     at(FP, P, a, r.b0 + d * j, fc + 1, sc) - 2 * v)
-                    if 0 < si < r.nslow - 1:
-                        x[8 + j] += (at(FP, P, a, r.b0 + d * j, fc, sc - 1) +
-                                     at(FP, P, a, r.b0 + d * j, fc, sc + 1) - 2 * v)
                 row = rec[K - 1, pol, pos]
                 prev1, prev2 = rec[K - 2, pol, pos], rec[K - 3, pol, pos]
                 x[10], x[11] = prev1[1] * prev1[18], prev2[1] * prev2[18]
@@ -183,17 +175,15 @@ def test_recorded_inputs_match_fields():
                 x[14], x[15] = prev1[0] * prev1[18], prev2[0] * prev2[18]
                 s = row[18]
                 got = row[:16].astype(np.float64) * s
-                lap_seen += bool(np.any(x[8:10] != 0))
+                hist_seen += bool(np.any(x[10:12] != 0))
                 for i in range(16):
-                    # the Laplacians are sums of three FP32 terms: within a
-                    # few ulp of the largest
-                    tol = 1e-6 * s if i in (8, 9) else 1e-5 * max(abs(x[i]), 1e-3 * s)
-                    assert abs(got[i] - x[i]) <= tol + 1e-30, (f, pol, pos, i)
+                    assert abs(got[i] - x[i]) <= 1e-5 * max(abs(x[i]), 1e-3 * s) + 1e-30, \
+                        (f, pol, pos, i)
                 ghost = at(FP, P, a, r.b0 - d, fc, sc)
                 assert row[17] == np.float32(ghost)
                 checked += 1
     assert checked == 6 * 2 * 40
-    assert lap_seen > checked // 2
+    assert hist_seen > checked // 2
 
 
 @pytest.mark.parametrize("kernel", ["fused", "split"])
@@ -384,7 +374,7 @@ def test_delta_head_gradient():
 
 
 def test_higdon_and_difference_heads_match_fp64():
-    """The Higdon head (fixed factor and gate in wlin[0..3], adaptive factor
+    """The Higdon head (fixed factor in wlin[0..2], adaptive factor
     from the network) and the difference head on the GPU against their FP64
     restatements: predictions within the bf16-weight tolerance of the other
     heads, the Higdon gradient within 3e-2 (relative L2), no gradient on the
@@ -394,7 +384,6 @@ def test_higdon_and_difference_heads_match_fp64():
     rng = np.random.default_rng(10)
     p = _higdon_params(rng).astype(np.float32)
     u = rng.normal(0, 1, (3000, 16)).astype(np.float32)
-    u[:1500, 8] *= 1e-3
     t = rng.normal(0, 1, 3000).astype(np.float32)
     tr = L.Trainer(u, t, head=L.HEAD_HIGDON)
     tr.set_params(p)
@@ -416,7 +405,8 @@ def test_higdon_and_difference_heads_match_fp64():
     tr.set_params(p0)
     y0 = tr.predict(u)
     tr.close()
-    A, B = mlp_ref.higdon_terms(*[float(v) for v in p0[L.param_count() - 16:L.param_count() - 13]], u)
+    A, B = mlp_ref.higdon_terms(*[float(v) for v in p0[L.param_count() - 16:L.param_count() - 13]],
+                                u)
     want = A + np.tanh(float(p0[L.param_count() - 17])) * B
     assert np.max(np.abs(y0 - want)) < 1e-5 * np.max(np.abs(want))
     pd = _params(rng).astype(np.float32)

@@ -81,7 +81,7 @@ struct MlpPacked {
 // Difference: ghost / s = P0(n) + skip(d) + m(d), where network and skip
 // see differences that vanish for a field constant in space and time:
 //   d0..2 = P(0)-P(1), P(1)-P(2), P(2)-P(3);  d4..6 = the same of Q;
-//   d8, d9 = the Laplacians;  d10, d11 = P1(n-1)-P1, P1(n-2)-P1(n-1);
+//   d8, d9 = inputs 8, 9;  d10, d11 = P1(n-1)-P1, P1(n-2)-P1(n-1);
 //   d12..15 = g(n-1)-P0, g(n-2)-g(n-1), P0(n-1)-P0, P0(n-2)-P0(n-1)
 // (d3 = d7 = 0).  A static uniform field is held exactly (ghost = P0)
 // whatever the weights, so rounding them cannot move the recursion's gain
@@ -95,7 +95,11 @@ constexpr int kHeadDelta = 2;
 // fixed factor B(kf, gf) and an adaptive one B(ka, ga) with ka = tanh(y),
 // y the network's output on sign-normalized inputs (even, so the ghost is
 // odd in the fields); kf, gf, ga = wlin[0..2], the other skip weights
-// unused.  Solved for the ghost, ghost / s = A + ka B with
+// unused but wlin[3] = beta: in the face hooks, 0 < beta < 1 smooths the
+// adaptive coefficient in time per face position, ka(n) = kb + d(n),
+// d(n) = (1 - beta) d(n-1) + beta (tanh(y) - kb), kb = tanh(b3) (a
+// coefficient that follows the fields slowly cannot pump energy into the
+// grid at their frequencies).  Solved for the ghost, ghost / s = A + ka B with
 //   A = kf (u0 - gf u12 - ga u10) + (gf + ga) u14 + gf ga (kf u15 - u11)
 //   B = u0 - ga u12 - kf u1 + kf (ga + gf) u14 + gf (ga (u15 - kf u13) - u10)
 // (u0 = P0, u1 = P1, u10, u11 = P1(n-1), P1(n-2), u12, u13 = g(n-1), g(n-2),

@@ -73,12 +73,11 @@ __device__ __forceinline__ fv4 act4(fv4 z) { return fv4{act_f(z.x), act_f(z.y), 
 __device__ __forceinline__ float dact_f(float a) { return fabsf(a) < 1.0f ? 1.0f : 0.0f; }
 __device__ __forceinline__ fv4 dact4(fv4 a) { return fv4{dact_f(a.x), dact_f(a.y), dact_f(a.z), dact_f(a.w)}; }
 
-// The Higdon head's fixed factor and damping, and the width of its gate:
-// wlin[0..3] = kf, gf, ga, c.
+// The Higdon head's fixed factor and damping, and the time smoothing of
+// its adaptive coefficient: wlin[0..3] = kf, gf, ga, beta.
 struct HigdonFix {
-    float kf, gf, ga, c;
-    float beta;  // wlin[4]: time smoothing of the adaptive coefficient (0 or 1: none)
-    float b3;    // the network's output bias
+    float kf, gf, ga;
+    float beta;  // 0 or >= 1: none
 };
 
 
@@ -107,7 +106,7 @@ __device__ __forceinline__ void load_regs(const MlpPacked *__restrict__ w, int l
         for (int k = 0; k < 4; ++k) r.A2[o][k] = w->a2[o][k][lane];
     }
     r.wl = fv4{w->wlin[rb], w->wlin[rb + 1], w->wlin[rb + 2], w->wlin[rb + 3]};
-    r.hf = HigdonFix{w->wlin[0], w->wlin[1], w->wlin[2], w->wlin[3], w->wlin[4], w->b3};
+    r.hf = HigdonFix{w->wlin[0], w->wlin[1], w->wlin[2], w->wlin[3]};
     r.b3 = w->b3;
     r.head = w->head;
 }
@@ -124,16 +123,13 @@ __device__ __forceinline__ float input_sign(fv4 u, int sl)
 // (sample = lane): P0, P1, P1(n-1), P1(n-2), g(n-1), g(n-2), P0(n-1),
 // P0(n-2).  Every lane takes part in the shuffles.
 struct HeadIn {
-    float u0, u1, u8, u10, u11, u12, u13, u14, u15;
-    float pm;  // max |P(0..3)|
+    float u0, u1, u10, u11, u12, u13, u14, u15;
 };
 __device__ __forceinline__ HeadIn head_in(fv4 u, int sl)
 {
     HeadIn h;
     h.u0 = __shfl(u.x, sl);
     h.u1 = __shfl(u.y, sl);
-    h.pm = __shfl(fmaxf(fmaxf(fabsf(u.x), fabsf(u.y)), fmaxf(fabsf(u.z), fabsf(u.w))), sl);
-    h.u8 = __shfl(u.x, sl + 32);
     h.u10 = __shfl(u.z, sl + 32);
     h.u11 = __shfl(u.w, sl + 32);
     h.u12 = __shfl(u.x, sl + 48);
@@ -141,15 +137,6 @@ __device__ __forceinline__ HeadIn head_in(fv4 u, int sl)
     h.u14 = __shfl(u.z, sl + 48);
     h.u15 = __shfl(u.w, sl + 48);
     return h;
-}
-
-// The Higdon head's gate on the network: 1 where the stencil is a plane
-// wave at normal incidence (tangential Laplacian u8 = 0), falling to 0 at
-// |u8| = c max|P| (c <= 0: always 1).
-__device__ __forceinline__ float higdon_gate(const HeadIn &h, const HigdonFix &c)
-{
-    if (!(c.c > 0.f)) return 1.0f;
-    return fmaxf(1.0f - fabsf(h.u8) / (c.c * h.pm + 1e-30f), 0.0f);
 }
 
 // The Higdon head's terms: ghost / s = A + ka B (the product's expansion
@@ -174,7 +161,7 @@ __device__ __forceinline__ float head_out(int head, float m, float p, const Head
     if (head == kHeadHigdon) {
         float A, B;
         higdon_ab(h, c, A, B);
-        return A + tanh_f(c.b3 + higdon_gate(h, c) * (m - c.b3)) * B;
+        return A + tanh_f(m) * B;
     }
     return p + m * (h.u0 - h.u14);  // kHeadDelta
 }
@@ -200,22 +187,27 @@ __device__ __forceinline__ void split_bf4(fv4 v, bfv4 &hi, bfv4 &lo)
 
 // Forward pass for the wave's 16 samples.  ui: this lane's 4 MLP inputs,
 // uf: its 4 linear-skip inputs (features 4(l/16)..+3 of sample l%16).
-// Inputs and hidden activations enter each product as bf16 pairs (two
-// MFMAs against the same bf16 weights), so the network is a function of its
-// inputs to about FP32 precision: a single bf16 rounding would move its
-// output by up to 2^-9 of the terms in steps that do not vary smoothly with
-// the fields.  Returns the MLP output and sets p to the skip, valid on lanes
-// 0..15 (sample = lane); ub gets the inputs' leading bf16 parts.
+// pairs: inputs and hidden activations enter each product as bf16 pairs
+// (two MFMAs against the same bf16 weights), so the network is a function of
+// its inputs to about FP32 precision -- for the heads whose network output
+// is itself part of the ghost value (direct, delta, difference: one bf16
+// rounding moves it by up to 2^-9 of the terms, in steps that do not vary
+// smoothly with the fields); otherwise single bf16 values (the one-way and
+// Higdon heads, where the output sets a bounded coefficient).  Returns the
+// MLP output and sets p to the skip, valid on lanes 0..15 (sample = lane);
+// ub gets the inputs' leading bf16 parts.
 __device__ __forceinline__ float mlp_forward(const MlpRegs &r, fv4 ui, fv4 uf, Acts &t, float &p,
                                              bfv4 &ub)
 {
     const fv4 zero = {0.f, 0.f, 0.f, 0.f};
+    const bool pairs = r.head != kHeadOneWay && r.head != kHeadHigdon;
     bfv4 ul, a1l[4], a2l[4];
     split_bf4(ui, ub, ul);
     // biases enter as the accumulator of the first product of each layer
 #pragma unroll
     for (int o = 0; o < 4; ++o) {
-        const fv4 z = mfma(r.A1[o], ul, mfma(r.A1[o], ub, r.b1[o]));
+        fv4 z = mfma(r.A1[o], ub, r.b1[o]);
+        if (pairs) z = mfma(r.A1[o], ul, z);
         t.a1[o] = act4(z);
         split_bf4(t.a1[o], t.a1b[o], a1l[o]);
     }
@@ -224,16 +216,20 @@ __device__ __forceinline__ float mlp_forward(const MlpRegs &r, fv4 ui, fv4 uf, A
         fv4 z = r.b2[o];
 #pragma unroll
         for (int k = 0; k < 4; ++k) z = mfma(r.A2[o][k], t.a1b[k], z);
+        if (pairs) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) z = mfma(r.A2[o][k], a1l[k], z);
+            for (int k = 0; k < 4; ++k) z = mfma(r.A2[o][k], a1l[k], z);
+        }
         t.a2[o] = act4(z);
         split_bf4(t.a2[o], t.a2b[o], a2l[o]);
     }
     fv4 z3 = zero;
 #pragma unroll
     for (int k = 0; k < 4; ++k) z3 = mfma(r.A3[k], t.a2b[k], z3);
+    if (pairs) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) z3 = mfma(r.A3[k], a2l[k], z3);
+        for (int k = 0; k < 4; ++k) z3 = mfma(r.A3[k], a2l[k], z3);
+    }
     // linear skip, summed over the four lanes that hold one sample
     p = r.wl.x * uf.x + r.wl.y * uf.y + r.wl.z * uf.z + r.wl.w * uf.w;
     p = p + __shfl_xor(p, 16);
@@ -440,7 +436,6 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float *__restrict__ u,
         loss = loss + res * res;
         float g = 2.0f * res * inv_n;  // dL/dpred on lanes 0..15
         float gm = g, gp = g;           // dL/dm, dL/dp
-        float gb = -1.f;                // dL/db3 when it differs from dL/dm
         if (r.head == kHeadOneWay) {
             const float th = tanh_f(m + sk);
             gm = gp = g * (1.0f - th * th) * (hv.u0 - hv.u12);
@@ -449,14 +444,12 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float *__restrict__ u,
         } else if (r.head == kHeadHigdon) {
             float A, B;
             higdon_ab(hv, r.hf, A, B);
-            const float gt = higdon_gate(hv, r.hf);
-            const float th = tanh_f(r.b3 + gt * (m - r.b3));
-            gb = g * (1.0f - th * th) * B;
-            gm = gb * gt;
+            const float th = tanh_f(m);
+            gm = g * (1.0f - th * th) * B;
             gp = 0.f;  // the skip is not used (wlin[0..3] hold the fixed factor)
         }
         const float gs = __shfl(gm, sl);  // broadcast to the sample's four lanes
-        if (lane < 16) gb3 = gb3 + (r.head == kHeadHigdon ? gb : gm);
+        if (lane < 16) gb3 = gb3 + gm;
         gwl = gwl + us * __shfl(gp, sl);
         // layer 3 (rank one): dW3 = a2 g, da2 = W3 g
         fv4 dz2[4];
@@ -726,9 +719,7 @@ __device__ __forceinline__ float limit_ghost(const Geo &g, const Ptrs &es, const
 // ghost value the boundary cell's update reads, Q = its partner in the
 // curl, d = depth from the boundary cell inward:
 //   0..3   P(d)                     4..7  Q(d)
-//   8, 9   tangential Laplacian of P at d = 0, 1: the sum over the two
-//          tangential axes of P(+1) + P(-1) - 2 P, an axis counted only
-//          where both neighbours lie on the face (or it wraps, periodic)
+//   8, 9   zero (unused)
 //   10, 11 P(1)(n-1), P(1)(n-2)                    (history)
 //   12..15 g(n-1), g(n-2), P(0)(n-1), P(0)(n-2)   (history)
 // High faces (before the H phase): P = E_t1 / E_t2, Q = H_t2 / H_t1, ghost
@@ -757,14 +748,6 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
     const float *const Qf_1 = h.hi ? field_ptr(hs, Q1) : field_ptr(es, Q1);
     // step of one cell inward along the normal, in the field arrays
     const long sa = (long)dir * (a == 0 ? 1l : (a == 1 ? (long)g.nx : (long)g.nx * g.ny));
-    // steps along the fast and slow tangential axes; the tangential
-    // Laplacian wraps where the face spans a periodic axis, and is taken
-    // along an axis only where both neighbours lie on the face otherwise
-    const int tf = t1 < t2 ? t1 : t2, ts = t1 < t2 ? t2 : t1;
-    const long sf = tf == 0 ? 1l : (long)g.nx;
-    const long ss = ts == 1 ? (long)g.nx : (long)g.nx * g.ny;
-    const bool wrap_f = (g.periodic >> tf & 1u) && h.nfast == h.nf && h.nfast > 2;
-    const bool wrap_s = (g.periodic >> ts & 1u) && h.nslow == h.ns && h.nslow > 2;
 
     MlpRegs r;
     if (h.mode == 0) load_regs(h.net, lane, r);
@@ -805,25 +788,7 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
 #pragma unroll
             for (int j = 0; j < 4; ++j) x[j] = F[c0 + j * sa];
         } else if (q4 == 2) {
-            // inputs 8..11: the tangential Laplacian of P at depths 0, 1 and
-            // the history of P(1)
-            const float *Pp = pol ? Pf_1 : Pf_0;
-            const int fi = pos % h.nfast, si = pos / h.nfast;
-            const long c0 = (long)face_cell(g, a, h.b0, f, sc);
-            const long dfl = fi > 0 ? -sf : (long)(h.nfast - 1) * sf;
-            const long dfh = fi + 1 < h.nfast ? sf : -(long)(h.nfast - 1) * sf;
-            const bool fon = (fi > 0 && fi + 1 < h.nfast) || wrap_f;
-            const long dsl = si > 0 ? -ss : (long)(h.nslow - 1) * ss;
-            const long dsh = si + 1 < h.nslow ? ss : -(long)(h.nslow - 1) * ss;
-            const bool son = (si > 0 && si + 1 < h.nslow) || wrap_s;
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const long c = c0 + j * sa;
-                const float v2 = 2.0f * Pp[c];
-                const float lf = fon ? (Pp[c + dfl] + Pp[c + dfh]) - v2 : 0.f;
-                const float ls = son ? (Pp[c + dsl] + Pp[c + dsh]) - v2 : 0.f;
-                x[j] = lf + ls;
-            }
+            // inputs 8..11: zero, zero and the history of P(1)
             const float2 q = *reinterpret_cast<const float2 *>(
                 h.mem + ((size_t)pol * npos + pos) * kMemStride + 4);
             x[2] = q.x;
@@ -873,8 +838,8 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
                 // tanh(b3), smoothed in time at this position
                 float A, B;
                 higdon_ab(hv, r.hf, A, B);
-                const float kb = tanh_f(r.hf.b3);
-                const float ki = tanh_f(r.hf.b3 + higdon_gate(hv, r.hf) * (m - r.hf.b3));
+                const float kb = tanh_f(r.b3);
+                const float ki = tanh_f(m);
                 dk = (1.0f - r.hf.beta) * mem[6] + r.hf.beta * (ki - kb);
                 pred = A + (kb + dk) * B;
             }

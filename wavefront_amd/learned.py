@@ -210,8 +210,8 @@ def init_params(rng: np.random.Generator, wlin=None, features=None) -> np.ndarra
 
 def diff_features(u: np.ndarray) -> np.ndarray:
     """The difference head's inputs (WF_HEAD_DIFF) from scaled input rows:
-    P(d)-P(d+1) and Q(d)-Q(d+1) for d = 0..2, the Laplacians (8, 9) as they
-    are, P1(n-1)-P1, P1(n-2)-P1(n-1), g(n-1)-P0, g(n-2)-g(n-1),
+    P(d)-P(d+1) and Q(d)-Q(d+1) for d = 0..2, inputs 8, 9 as they are,
+    P1(n-1)-P1, P1(n-2)-P1(n-1), g(n-1)-P0, g(n-2)-g(n-1),
     P0(n-1)-P0, P0(n-2)-P0(n-1); 3 and 7 zero."""
     u = np.asarray(u, f32)
     d = np.zeros_like(u)
@@ -353,18 +353,18 @@ def one_way_angle_coefficient(dt: float, dx: float, degrees: float) -> float:
 
 
 def higdon_params(kf: float, gf: float, ga: float, ka: float, rng=None, features=None,
-                  gate: float = 0.0):
+                  beta: float = 0.0):
     """Higdon-head parameters: the fixed factor (kf, gf), the adaptive
-    factor's damping ga, the gate width (0 = no gate), and a network that
-    starts at ka (b3 = atanh ka, W3 = 0)."""
+    factor's damping ga and time smoothing beta (0: none), and a network
+    that starts at ka (b3 = atanh ka, W3 = 0)."""
     p = init_params(rng or np.random.default_rng(0), None, features)
     p[param_count() - 17] = math.atanh(ka)
-    p[param_count() - 16:param_count() - 12] = (kf, gf, ga, gate)
+    p[param_count() - 16:param_count() - 12] = (kf, gf, ga, beta)
     return p
 
 
 def train_face_higdon(u: np.ndarray, t: np.ndarray, dt: float, dx: float, seed: int = 0,
-                      fixed=(45.0, 0.9), ga: float = 1.0, start: float = 0.0, gate: float = 0.0,
+                      fixed=(45.0, 0.9), ga: float = 1.0, start: float = 0.0,
                       schedule=((2000, 1e-3), (2000, 3e-4), (1000, 1e-4)), features=None,
                       device: int = 0, log=None):
     """The Higdon head: the fixed factor at angle fixed[0] with damping
@@ -378,7 +378,7 @@ def train_face_higdon(u: np.ndarray, t: np.ndarray, dt: float, dx: float, seed: 
         off[list(features)] = False
         u[:, off] = 0.0
     p = higdon_params(one_way_angle_coefficient(dt, dx, fixed[0]), fixed[1], ga,
-                      one_way_angle_coefficient(dt, dx, start), rng, features, gate)
+                      one_way_angle_coefficient(dt, dx, start), rng, features)
     tr = Trainer(u, t, device, head=HEAD_HIGDON)
     tr.set_params(p)
     hist = [(0, tr.gradient()[1])]
@@ -695,9 +695,8 @@ def train_x_selected(samples: dict, base: "FaceNets", seeds=(0, 10, 20, 30), log
 
 # The Higdon head of the shipped layer: the fixed factor at 45 degrees with
 # its memory damped by HIGDON_DAMP, the adaptive factor undamped and
-# starting at normal incidence; the network sees P at depths 0..3, its
-# tangential Laplacians, the P1 history and the ghost / P0 history
-# (sign-normalized).  Measured on MI355X (30 GHz, 20 points per wavelength,
+# starting at normal incidence; the network sees P at depths 0..3, the P1
+# history and the ghost / P0 history (sign-normalized).  Measured on MI355X (30 GHz, 20 points per wavelength,
 # Courant 0.4): undamped, the product reflects 5.9e-5 at normal incidence
 # but the 3D error of a 128^3 cube grows after ~1000 steps (a periodic slab
 # with no edges does not grow: the growth is where two faces meet); damped
@@ -705,7 +704,7 @@ def train_x_selected(samples: dict, base: "FaceNets", seeds=(0, 10, 20, 30), log
 # 3000 to 15000 steps; 0.9 and 0.93 are as flat but reflect more.
 HIGDON_FIXED_ANGLE = 45.0
 HIGDON_DAMP = 0.95
-FEATURES_HIGDON = (0, 1, 2, 3, 8, 9, 10, 11, 12, 13, 14, 15)
+FEATURES_HIGDON = (0, 1, 2, 3, 10, 11, 12, 13, 14, 15)
 # Closed-loop validation for the Higdon head's training (never the test
 # geometry): the two-run reflection at VALIDATION_FREQS and the reflected
 # energy in a 96^3 vacuum cube with an off-centre Ricker source over 3000
@@ -769,7 +768,7 @@ def train_higdon(samples: dict, dt: float, dx: float, checkpoints=(50, 100, 200,
             snap = np.stack([tr.params() for tr in trs])
             for beta in betas:
                 p = snap.copy()
-                p[:, param_count() - 12] = beta
+                p[:, param_count() - 13] = beta
                 v1, v3, late = validate_higdon(FaceNets(p, dt, dx, HEAD_HIGDON, False), refs1,
                                                refs3, device)
                 table.append((it, beta, v1, v3, late))
