@@ -1867,6 +1867,7 @@ int wf_set_learned_faces(wf_ctx *c, uint32_t face_mask, const float *params, int
         h.nf = fg.nf;
         h.ns = fg.ns;
         h.mode = 0;
+        h.head = head;
         if ((rc = set_pool(c, f, c->face_passive))) break;
         h.passive = c->face_passive;
         h.gamma = c->face_gamma;

@@ -118,6 +118,7 @@ struct FaceHook {
     int slow0, nslow;   // range on the higher-numbered tangential axis
     int na, nf, ns;     // axis lengths: normal, fast, slow
     int mode;           // 0 predict, 1 record, 2 replay recorded ghosts
+    int head;           // mode 0: the network's output head (kHead*)
     int passive;        // modes 0, 2: limit the ghosts so the face stays passive
     unsigned predicted; // faces (bit f) whose ghost planes are predicted or replayed
     float *credit;      // passive: the face's energy credit (one float)

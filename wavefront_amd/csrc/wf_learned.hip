@@ -196,11 +196,12 @@ __device__ __forceinline__ void split_bf4(fv4 v, bfv4 &hi, bfv4 &lo)
 // Higdon heads, where the output sets a bounded coefficient).  Returns the
 // MLP output and sets p to the skip, valid on lanes 0..15 (sample = lane);
 // ub gets the inputs' leading bf16 parts.
+__device__ __forceinline__ bool head_pairs(int head) { return head != kHeadOneWay && head != kHeadHigdon; }
+
 __device__ __forceinline__ float mlp_forward(const MlpRegs &r, fv4 ui, fv4 uf, Acts &t, float &p,
-                                             bfv4 &ub)
+                                             bfv4 &ub, bool pairs)
 {
     const fv4 zero = {0.f, 0.f, 0.f, 0.f};
-    const bool pairs = r.head != kHeadOneWay && r.head != kHeadHigdon;
     bfv4 ul, a1l[4], a2l[4];
     split_bf4(ui, ub, ul);
     // biases enter as the accumulator of the first product of each layer
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ u
         Acts t;
         float sk;
         bfv4 ub;
-        const float m = mlp_forward(r, ui, skip_in(r.head, uf, ui), t, sk, ub);
+        const float m = mlp_forward(r, ui, skip_in(r.head, uf, ui), t, sk, ub, head_pairs(r.head));
         const float p = head_out(r.head, m, sk, head_in(uf, sl), r.hf);
         if (lane < 16 && s < n) out[s] = p;
     }
@@ -429,7 +430,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float *__restrict__ u,
         bfv4 ub;
         Acts a;
         float sk;
-        const float m = mlp_forward(r, ui, us, a, sk, ub);
+        const float m = mlp_forward(r, ui, us, a, sk, ub, head_pairs(r.head));
         const HeadIn hv = head_in(uf, sl);
         const float pred = head_out(r.head, m, sk, hv, r.hf);
         const float res = (lane < 16 && live) ? pred - t : 0.f;
@@ -725,6 +726,11 @@ __device__ __forceinline__ float limit_ghost(const Geo &g, const Ptrs &es, const
 // High faces (before the H phase): P = E_t1 / E_t2, Q = H_t2 / H_t1, ghost
 // E at b0+1.  Low faces (before the E phase): P = H_t1 / H_t2, Q = E_t2 /
 // E_t1, ghost H at b0-1.  t1 = (a+1)%3, t2 = (a+2)%3.
+// PAIRS: the network of the predicted faces takes bf16 pairs (its head);
+// LIMITS: some face of the launch is limited.  Both are the same for the
+// faces of one launch; each combination is its own kernel, so the register
+// budget of the others does not hold a lighter one back.
+template <bool PAIRS, bool LIMITS>
 __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Ptrs &hs,
                                           const FaceHook &h, const uint64_t *__restrict__ step)
 {
@@ -757,7 +763,7 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
         if (k % h.emit_every == 0 && k / h.emit_every < h.emit_rows) row = (long)(k / h.emit_every);
     }
 
-    if (h.mode == 2 && !h.passive) {
+    if (h.mode == 2 && !(LIMITS && h.passive)) {
         // replay: ghost values recorded by a thick-CPML run, step by step
         const uint64_t k = *step - h.emit_base;
         for (int grp = wave; grp < 2 * gpp; grp += nwaves) {
@@ -830,7 +836,7 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
             const fv4 ui = mlp_in(r.head, u, sl);
             float sk;
             bfv4 ub;
-            const float m = mlp_forward(r, ui, skip_in(r.head, u, ui), t, sk, ub);
+            const float m = mlp_forward(r, ui, skip_in(r.head, u, ui), t, sk, ub, PAIRS);
             const HeadIn hv = head_in(u, sl);
             float pred = head_out(r.head, m, sk, hv, r.hf);
             if (r.head == kHeadHigdon && r.hf.beta > 0.f && r.hf.beta < 1.f) {
@@ -843,7 +849,7 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
                 dk = (1.0f - r.hf.beta) * mem[6] + r.hf.beta * (ki - kb);
                 pred = A + (kb + dk) * B;
             }
-            if (h.passive) pred = limit_ghost(g, es, hs, h, pol, pos, live, lane, u, s, rs, mem, pred, flux,
+            if (LIMITS && h.passive) pred = limit_ghost(g, es, hs, h, pol, pos, live, lane, u, s, rs, mem, pred, flux,
                                                mem_stats);
             gnew = s * pred;
             if (lane < 16 && live) (pol == 0 ? h.ghost[0] : h.ghost[1])[pos] = gnew;
@@ -878,14 +884,13 @@ __device__ __forceinline__ void face_hook(const Geo &g, const Ptrs &es, const Pt
         wflux += sum16(flux);  // the 16 samples' flux, summed in lane order
         if (lane < 16 && live) {
             *reinterpret_cast<fv4 *>(mem) = fv4{gnew, g1, x.x, p1};
-            *reinterpret_cast<float2 *>(mem + 4) = float2{x.y, q1};
-            mem[6] = dk;
-            if (h.mode != 1)
+            *reinterpret_cast<fv4 *>(mem + 4) = fv4{x.y, q1, dk, 0.f};
+            if (LIMITS && h.passive && h.mode != 1)
                 *reinterpret_cast<fv4 *>(mem + kMemStats) =
                     fv4{flux, mem_stats[0], mem_stats[1], mem_stats[2]};
         }
     }
-    if (h.passive && h.mode != 1) {
+    if (LIMITS && h.passive && h.mode != 1) {
         // the workgroup's flux partial, its four waves summed in order
         __shared__ float red[4];
         if (lane == 0) red[threadIdx.x >> 6] = wflux;
@@ -1023,13 +1028,14 @@ __global__ __launch_bounds__(64) void k_credit_reduce(CreditJob j0, CreditJob j1
 }
 
 // Up to three faces of one side in one launch: blockIdx.y picks the face.
+template <bool PAIRS, bool LIMITS>
 __global__ __launch_bounds__(256) void k_face_hooks(Geo g, Ptrs es, Ptrs hs, FaceHook h0,
                                                     FaceHook h1, FaceHook h2,
                                                     const uint64_t *__restrict__ step)
 {
     const int k = blockIdx.y;
     const FaceHook h = k == 0 ? h0 : (k == 1 ? h1 : h2);
-    face_hook(g, es, hs, h, step);
+    face_hook<PAIRS, LIMITS>(g, es, hs, h, step);
 }
 
 }  // namespace
@@ -1063,8 +1069,21 @@ hipError_t launch_face_hooks(const Geo &g, const Ptrs &esrc, const Ptrs &hsrc,
         blocks = std::max(blocks, std::min((groups + 3) / 4, cap));
     }
     const FaceHook &h0 = *hooks[0], &h1 = *hooks[n > 1 ? 1 : 0], &h2 = *hooks[n > 2 ? 2 : 0];
-    hipLaunchKernelGGL(k_face_hooks, dim3(blocks, n), dim3(256), 0, st, g, esrc, hsrc, h0, h1,
-                       h2, step);
+    bool pairs = false, limits = false;
+    for (int k = 0; k < n; ++k) {
+        const FaceHook &h = *hooks[k];
+        pairs = pairs || (h.mode == 0 && h.head != kHeadOneWay && h.head != kHeadHigdon);
+        limits = limits || (h.mode != 1 && h.passive);
+    }
+    const dim3 grid(blocks, n), block(256);
+    if (pairs && limits)
+        hipLaunchKernelGGL((k_face_hooks<true, true>), grid, block, 0, st, g, esrc, hsrc, h0, h1, h2, step);
+    else if (pairs)
+        hipLaunchKernelGGL((k_face_hooks<true, false>), grid, block, 0, st, g, esrc, hsrc, h0, h1, h2, step);
+    else if (limits)
+        hipLaunchKernelGGL((k_face_hooks<false, true>), grid, block, 0, st, g, esrc, hsrc, h0, h1, h2, step);
+    else
+        hipLaunchKernelGGL((k_face_hooks<false, false>), grid, block, 0, st, g, esrc, hsrc, h0, h1, h2, step);
     // limited faces: the edges shared by two of them, then the credits
     FaceHook ax[3];
     bool lim[3] = {false, false, false};
