@@ -105,6 +105,11 @@ def _random_case(dims, shell, periodic, steps, seed, nmat=5, pml=True):
     ((40, 64, 48), (0, 1, 0), PERIODIC_X | PERIODIC_Z),
     ((16, 16, 24), (1, 1, 1), 0),
     ((296, 16, 16), (1, 0, 0), PERIODIC_Y | PERIODIC_Z),
+    # x at least 512 with an x CPML: the fused variant folds the x shell into
+    # two 256-cell edge columns of the single pass (no middle columns at 512)
+    ((512, 24, 32), (1, 1, 1), 0),
+    ((584, 32, 24), (2, 1, 1), 0),
+    ((528, 24, 40), (3, 0, 2), PERIODIC_Y),
 ])
 def test_random_fields_pml(dims, shell, periodic):
     """Random fields, random lossy materials, CPML on uneven per-axis shells:

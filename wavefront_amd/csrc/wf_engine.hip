@@ -229,6 +229,14 @@ struct wf_ctx {
     int cus = 256;      // compute units of the device
     Box interior{};     // fused variant: cells swept by the single-pass kernel
     BoxList frame{};    // fused variant: the rest (holds the whole CPML shell)
+    // fused variant, x axis folded into the single pass (plan_boxes): the
+    // interior's middle columns (plain kernel), its two edge columns with the
+    // x CPML cells (x-only CPML kernel), and the frame without x slabs; used
+    // while no face hook is set
+    bool xfold = false;
+    bool xfold_env = true;  // WF_XFOLD=0: never fold
+    Box xmid{}, xedges{};  // xedges: two tile columns, x from 0 and from nx - 256
+    BoxList framex{};
     Box all{};          // fused variant with a CPML: the whole domain, one pass
     BoxList lofix{};    // learned faces without a CPML: the low faces' boundary cells
     bool pml_fused = false;  // WF_PML_FUSED=1: single pass over the shell too (slower, see launch_step)
@@ -305,7 +313,7 @@ struct wf_ctx {
     bool contract = false;
     hipError_t (*lh)(const StepLaunch &, const BoxList &, hipStream_t) = launch_h;
     hipError_t (*le)(const StepLaunch &, const BoxList &, hipStream_t) = launch_e;
-    hipError_t (*lf)(const StepLaunch &, const Box &, bool, hipStream_t) = launch_fused;
+    hipError_t (*lf)(const StepLaunch &, const Box &, bool, hipStream_t, bool) = launch_fused;
 };
 
 namespace {
@@ -479,6 +487,36 @@ void plan_boxes(wf_ctx *c)
         add_box(f, split_box(hi[0], n[0], lo[1], hi[1], lo[2], hi[2], fz));
     }
     c->frame = f;
+    // x fold: the x CPML cells of the interior's rows join the single pass
+    // as two 256-cell edge columns (x-only CPML kernel, 8 waves), so that
+    // no frame box has narrow rows (the x slabs, 24 cells wide at 1024^3
+    // with T = 24, ran at about 3.5 TB/s against 5 for the y and z slabs).
+    // Needs the x shell (whole blocks) within one edge column, the middle
+    // columns' x-halo outside it, and no periodic x.
+    c->xfold = false;
+    const int W = 256, shx = g.sb[0] * 8;
+    if (c->xfold_env && !empty && g.T[0] > 0 && !(g.periodic & 1u) && shx < W &&
+        n[0] >= 2 * W) {
+        Box e{0, n[0], lo[1], hi[1], lo[2], hi[2], 64, 2, 0, 0, 0};
+        e.tstride = n[0] - W;
+        const int fxr = fused_rows_per_tile_x();
+        e.gy = (hi[1] - lo[1] + fxr - 1) / fxr;
+        e.gz = slices(2l * e.gy, hi[2] - lo[2]);
+        c->xedges = e;
+        Box mid{W, n[0] - W, lo[1], hi[1], lo[2], hi[2], 64, 0, 0, 0, 0};
+        mid.gx = (mid.x1 - mid.x0 + 255) / 256;
+        mid.gy = (hi[1] - lo[1] + fr - 1) / fr;
+        mid.gz = mid.gx > 0 ? slices((long)mid.gx * mid.gy, hi[2] - lo[2]) : 0;
+        if (mid.x1 <= mid.x0) mid.gx = mid.gy = mid.gz = 0;
+        c->xmid = mid;
+        BoxList fx{};
+        add_box(fx, split_box(0, n[0], 0, n[1], 0, lo[2], fz));
+        add_box(fx, split_box(0, n[0], 0, n[1], hi[2], n[2], fz));
+        add_box(fx, split_box(0, n[0], 0, lo[1], lo[2], hi[2], fz));
+        add_box(fx, split_box(0, n[0], hi[1], n[1], lo[2], hi[2], fz));
+        c->framex = fx;
+        c->xfold = true;
+    }
     Box all{0, n[0], 0, n[1], 0, n[2], 64, 0, 0, 0, 0};
     all.gx = (n[0] + 255) / 256;
     all.gy = (n[1] + frx - 1) / frx;  // swept by the CPML or ghost variant
@@ -560,6 +598,16 @@ bool single_pass(const wf_ctx *c) { return c->fused && c->pml && c->pml_fused &&
 // their ghost H.  No frame kernels.
 bool learned_single_pass(const wf_ctx *c) { return c->fused && !c->pml && c->ghost_reads; }
 
+// The folded x plan (plan_boxes) applies while no face hook is set: hooks
+// on a domain with a CPML touch frame cells only.
+bool folded(const wf_ctx *c)
+{
+    if (!c->xfold) return false;
+    for (int f = 0; f < 6; ++f)
+        if (c->hook_on[f]) return false;
+    return true;
+}
+
 int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
 {
     hipStream_t st = c->stream;
@@ -568,7 +616,7 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
         s.ghost = true;
         if (int rc = run_hooks(c, true, c->p, c->p, st)) return rc;
         if (ev) WF_HIP(hipEventRecord(ev[0], st));
-        WF_HIP(c->lf(s, c->all, false, st));
+        WF_HIP(c->lf(s, c->all, false, st, false));
         if (ev) {
             WF_HIP(hipEventRecord(ev[1], st));
             WF_HIP(hipEventRecord(ev[2], st));
@@ -586,7 +634,7 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
         StepLaunch s{c->g, c->p, c->alt, c->pml, c->zchunk, st};
         if (int rc = run_hooks(c, true, c->p, c->p, st)) return rc;
         if (ev) WF_HIP(hipEventRecord(ev[0], st));
-        WF_HIP(c->lf(s, c->all, true, st));
+        WF_HIP(c->lf(s, c->all, true, st, false));
         if (ev) {
             WF_HIP(hipEventRecord(ev[1], st));
             for (int k = 2; k < 6; ++k) WF_HIP(hipEventRecord(ev[k], st));
@@ -608,7 +656,7 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
         WF_HIP(c->lh(framed(s, c), c->frame, c->side));
         if (int rc = run_hooks(c, false, c->p, c->alt, c->side)) return rc;
         WF_HIP(hipEventRecord(c->join, c->side));
-        WF_HIP(c->lf(plain(s), c->interior, false, st));
+        WF_HIP(c->lf(plain(s), c->interior, false, st, false));
         WF_HIP(hipStreamWaitEvent(st, c->join, 0));
         WF_HIP(c->le(framed(s, c), c->frame, st));
         std::swap(c->p, c->alt);
@@ -616,26 +664,37 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
         StepLaunch s{c->g, c->p, c->alt, c->pml, c->zchunk, st};
         s.ghost = c->ghost_reads;
         if (int rc = run_hooks(c, true, c->p, c->p, st)) return rc;
-        const bool overlap =
-            c->overlap && !ev && c->frame.total > 0 && c->interior.gx > 0 && c->side;
+        const bool fold = folded(c);
+        const BoxList &frame = fold ? c->framex : c->frame;
+        // the single pass: the interior, or its middle and edge columns
+        auto interior = [&](hipStream_t q) -> int {
+            if (!fold) {
+                WF_HIP(c->lf(plain(s), c->interior, false, q, false));
+                return WF_OK;
+            }
+            if (c->xmid.gx > 0) WF_HIP(c->lf(plain(s), c->xmid, false, q, false));
+            WF_HIP(c->lf(plain(s), c->xedges, true, q, true));
+            return WF_OK;
+        };
+        const bool overlap = c->overlap && !ev && frame.total > 0 && c->interior.gx > 0 && c->side;
         if (overlap) {
             WF_HIP(hipEventRecord(c->fork, st));
             WF_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
-            WF_HIP(c->lh(framed(s, c), c->frame, c->side));
+            WF_HIP(c->lh(framed(s, c), frame, c->side));
             WF_HIP(hipEventRecord(c->join, c->side));
-            WF_HIP(c->lf(plain(s), c->interior, false, st));
+            if (int rc = interior(st)) return rc;
             WF_HIP(hipStreamWaitEvent(st, c->join, 0));
         } else {
             if (ev) WF_HIP(hipEventRecord(ev[0], st));
-            WF_HIP(c->lf(plain(s), c->interior, false, st));
+            if (int rc = interior(st)) return rc;
             if (ev) WF_HIP(hipEventRecord(ev[1], st));
             if (ev) WF_HIP(hipEventRecord(ev[2], st));
-            WF_HIP(c->lh(framed(s, c), c->frame, st));
+            WF_HIP(c->lh(framed(s, c), frame, st));
             if (ev) WF_HIP(hipEventRecord(ev[3], st));
         }
         if (int rc = run_hooks(c, false, c->p, c->alt, st)) return rc;
         if (ev) WF_HIP(hipEventRecord(ev[4], st));
-        WF_HIP(c->le(framed(s, c), c->frame, st));
+        WF_HIP(c->le(framed(s, c), frame, st));
         if (ev) WF_HIP(hipEventRecord(ev[5], st));
         std::swap(c->p, c->alt);
     } else {
@@ -1080,6 +1139,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     if (const char *ov = std::getenv("WF_OVERLAP")) c->overlap = std::atoi(ov) != 0;
     if (const char *ho = std::getenv("WF_HOOK_OVERLAP")) c->hook_overlap = std::atoi(ho) != 0;
     if (const char *pf = std::getenv("WF_PML_FUSED")) c->pml_fused = std::atoi(pf) != 0;
+    if (const char *xf = std::getenv("WF_XFOLD")) c->xfold_env = std::atoi(xf) != 0;
     if (const char *gr = std::getenv("WF_GRAPH")) c->use_graph = std::atoi(gr) != 0;
     if (const char *fc = std::getenv("WF_FP_CONTRACT"))
         if (std::atoi(fc) != 0) {
@@ -1620,17 +1680,29 @@ int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
         *e_bytes = 37ull * cells;
         return WF_OK;
     }
-    const BoxList &bl = c->fused ? c->frame : c->whole;
+    const bool fold = c->fused && folded(c);
+    const BoxList &bl = c->fused ? (fold ? c->framex : c->frame) : c->whole;
     uint64_t cells = 0;
     for (int k = 0; k < bl.n; ++k) {
         const Box &b = bl.b[k];
         cells += (uint64_t)(b.x1 - b.x0) * (b.y1 - b.y0) * (b.z1 - b.z0);
     }
     const Box &in = c->interior;
-    *fused_bytes = c->fused ? 49ull * (uint64_t)(in.x1 - in.x0) * (in.y1 - in.y0) * (in.z1 - in.z0)
-                            : 0;
-    *h_bytes = 37ull * cells + 8ull * terms;
-    *e_bytes = 37ull * cells + 8ull * terms;
+    uint64_t fb = 0, fterms = 0;
+    if (fold) {
+        // the middle and edge columns: 49 B per cell, and 16 B (read and
+        // write, both phases) per x psi term of the edges' graded cells
+        const Box &m = c->xmid, &e = c->xedges;
+        if (m.x1 > m.x0) fb += 49ull * (uint64_t)(m.x1 - m.x0) * (m.y1 - m.y0) * (m.z1 - m.z0);
+        fb += 49ull * 2 * 256 * (uint64_t)(e.y1 - e.y0) * (e.z1 - e.z0);
+        fterms = 2ull * 2ull * c->g.T[0] * (uint64_t)(in.y1 - in.y0) * (in.z1 - in.z0);
+        fb += 16ull * fterms;
+    } else if (c->fused) {
+        fb = 49ull * (uint64_t)(in.x1 - in.x0) * (in.y1 - in.y0) * (in.z1 - in.z0);
+    }
+    *fused_bytes = fb;
+    *h_bytes = 37ull * cells + 8ull * (terms - fterms);
+    *e_bytes = 37ull * cells + 8ull * (terms - fterms);
     return WF_OK;
 }
 

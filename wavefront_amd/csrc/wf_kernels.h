@@ -127,12 +127,15 @@ __device__ __forceinline__ void lds_barrier()
 
 // A box of cells [x0,x1) x [y0,y1) x [z0,z1) and its tiling: lpr lanes per
 // row (split kernels), gx*gy*gz tiles, the first of which has workgroup
-// index `first` in its launch.
+// index `first` in its launch.  Fused kernel: tile column tx starts at
+// x0 + tx * tstride (256: adjacent columns; larger: columns apart, cells
+// between them not in the box).
 struct Box {
     int x0, x1, y0, y1, z0, z1;
     int lpr;
     int gx, gy, gz;
     int first;
+    int tstride = 256;
 };
 
 constexpr int kMaxBoxes = 6;
@@ -163,14 +166,17 @@ hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 // tz-th of box.gz even slices of the box's planes (s.zchunk is not used).
 // pml_cells: the box may hold CPML shell cells (the kernel then updates
 // their psi too); false selects the plain variant for a box wholly outside
-// the shell.  s.ghost: E beyond the high faces comes from the learned ghost
+// the shell (x_only: on the x axis only, the box's y and z ranges lying
+// outside it).  s.ghost: E beyond the high faces comes from the learned ghost
 // planes (with pml_cells false); the low faces' boundary E is left for the
 // E kernel to redo.
-hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st);
+hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st,
+                        bool x_only = false);
 // Tile shapes of the two kernels (owned rows per fused tile -- of the plain
 // variant, or of the CPML / ghost variants -- and rows per split workgroup
 // of lpr = 64).
 int fused_rows_per_tile(bool pml_or_ghost);
+int fused_rows_per_tile_x();  // the x-only CPML variant
 int split_rows_per_wg();
 
 // One scene of a batch: its geometry and state (updated in place by the
@@ -207,7 +213,8 @@ hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, in
 namespace fma {
 hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
-hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st);
+hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st,
+                        bool x_only = false);
 hipError_t launch_batch_h(const BatchScene *scenes, const uint32_t *wg_scene, int total,
                           int zchunk, int grid, hipStream_t st);
 hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, int total,

@@ -453,8 +453,13 @@ __global__ __launch_bounds__(64 * kSY) void k_e4_batch(const BatchScene *__restr
 #endif
 constexpr int kFY = WF_FUSED_FY;
 constexpr int kFYx = 8;
-template <bool PML, bool GHOST>
-constexpr int fused_waves() { return (PML || GHOST) ? kFYx : kFY; }
+// PM: 0 no CPML cells, 1 CPML cells on any axis, 2 CPML cells on the x
+// axis only (a box whose y and z ranges lie outside the shell).
+#ifndef WF_FUSED_XPML_FY
+#define WF_FUSED_XPML_FY 8  // 198 VGPRs: 12 waves would spill
+#endif
+template <int PM, bool GHOST>
+constexpr int fused_waves() { return (PM == 1 || GHOST) ? kFYx : (PM == 2 ? WF_FUSED_XPML_FY : kFY); }
 
 // CPML of one derivative of a cell (quad) in the PML shell on axis `ax`:
 // graded -> the one-tap recursion with psi read from `pin` and, when
@@ -483,11 +488,13 @@ __device__ __forceinline__ float pml_d1(float raw, bool gr, float pb, float pa, 
 // boundary cells reads); the low faces' ghost H is not known yet when this
 // kernel runs, so their boundary cells' E is redone afterwards by the E
 // kernel (see the engine's learned-face schedule).
-template <bool PML, bool GHOST>
-__global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo g, Ptrs a, Ptrs b,
-                                                                         Box box)
+template <int PM, bool GHOST>
+__global__ __launch_bounds__((64 * fused_waves<PM, GHOST>())) void k_fused4(Geo g, Ptrs a, Ptrs b,
+                                                                        Box box)
 {
-    constexpr int FY = fused_waves<PML, GHOST>();
+    constexpr bool PML = PM != 0;   // the box may hold CPML shell cells
+    constexpr bool PYZ = PM == 1;   // ... on the y and z axes too
+    constexpr int FY = fused_waves<PM, GHOST>();
     __shared__ float2 s_ce[256];      // (ca, cb)
     __shared__ float2 s_ch[256];      // (da, db)
     __shared__ f4 s_h[2][2][FY][64];  // [buf][hz, hx][wave][lane]
@@ -501,7 +508,7 @@ __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo
 
     const int t = xcd_tile(blockIdx.x, box.gx * box.gy * box.gz);
     const int tx = t % box.gx, ty = (t / box.gx) % box.gy, tz = t / (box.gx * box.gy);
-    const int x = box.x0 + tx * 256 + lane * 4;
+    const int x = box.x0 + tx * box.tstride + lane * 4;
     const int yr = box.y0 + ty * (FY - 1) - 1 + w;
     int y = yr;
     bool live = x < box.x1 && yr < box.y1;
@@ -531,12 +538,14 @@ __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo
     float pby = 1.f, pay = 0.f, pky = 1.f, pbh = 1.f, pah = 0.f, pkh = 1.f;
     if (PML && live) {
         sh_x = in_shell_axis(g, 0, xc);
-        sh_y = in_shell_axis(g, 1, yc);
         gx = graded(g, 0, xc, g.nx);
-        gy = graded(g, 1, yc, g.ny);
         gxi = slab_index(g, 0, xc, g.nx);
-        gyi = slab_index(g, 1, yc, g.ny);
-        if (gx) {
+        if (PYZ) {
+            sh_y = in_shell_axis(g, 1, yc);
+            gy = graded(g, 1, yc, g.ny);
+            gyi = slab_index(g, 1, yc, g.ny);
+        }
+        if (gx && PYZ) {  // x-only boxes load them where they are used
             pbx = ld4(a.prof[0][0] + xc);
             pax = ld4(a.prof[0][1] + xc);
             pkx = ld4(a.prof[0][2] + xc);
@@ -546,7 +555,7 @@ __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo
             pay = a.prof[1][1][yc];
             pky = a.prof[1][2][yc];
         }
-        if (hlive) {
+        if (hlive && PYZ) {  // x-only boxes keep their x-halo cell outside the shell
             sh_h = in_shell_axis(g, 0, xh);
             gh = graded(g, 0, xh, g.nx);
             ghi = slab_index(g, 0, xh, g.nx);
@@ -584,6 +593,7 @@ __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo
         float sez, sey;                           // edge lane: E at x+4
         float hexn, heyn, hezn, hexpy, hhy, hhz;  // x-halo cell
         int hid;
+        f4 psx[PM == 2 ? 4 : 1];                  // x-only boxes: the quad's x psi terms
     };
     auto plane_z = [&](int it) { return it < 0 ? zpre : z0 + it; };
     auto load = [&](Plane &P, int it) {
@@ -595,6 +605,15 @@ __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo
         P.sez = P.sey = 0.f;
         P.hexn = P.heyn = P.hezn = P.hexpy = P.hhy = P.hhz = 0.f;
         P.hid = 0;
+        if (PM == 2) {
+            // the x psi terms of a graded quad, loaded a plane ahead like the
+            // fields (a quad is graded or not as a whole: T is whole blocks)
+            for (int t = 0; t < (PM == 2 ? 4 : 1); ++t) P.psx[t] = splat(0.f);
+            if (live && gx) {
+                const size_t jx = ((size_t)plane_z(it) * g.ny + yc) * (2 * (size_t)g.T[0]) + gxi;
+                for (int t = 0; t < (PM == 2 ? 4 : 1); ++t) P.psx[t] = ld4(a.psi[0][t] + jx);
+            }
+        }
         if (live) {
             if (zp >= 0) {
                 const size_t on = (size_t)zp * plane + row;
@@ -676,8 +695,10 @@ __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo
         size_t gzo = 0;
         float pbz = 1.f, paz = 0.f, pkz = 1.f;
         if (PML) {
-            sh_z = in_shell_axis(g, 2, z);
-            gz = graded(g, 2, z, g.nz);
+            if (PYZ) {
+                sh_z = in_shell_axis(g, 2, z);
+                gz = graded(g, 2, z, g.nz);
+            }
             if (gz) {
                 gzo = (size_t)slab_index(g, 2, z, g.nz) * plane;
                 pbz = a.prof[2][0][z];
@@ -714,8 +735,23 @@ __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo
                              b.psi[2][0], iz, store);
             d_ex_dz = pml_d4(d_ex_dz, gz, splat(pbz), splat(paz), splat(pkz), a.psi[2][1],
                              b.psi[2][1], iz, store);
-            d_ez_dx = pml_d4(d_ez_dx, gx, pbx, pax, pkx, a.psi[0][0], b.psi[0][0], ix, store);
-            d_ey_dx = pml_d4(d_ey_dx, gx, pbx, pax, pkx, a.psi[0][1], b.psi[0][1], ix, store);
+            if (PYZ) {
+                d_ez_dx = pml_d4(d_ez_dx, gx, pbx, pax, pkx, a.psi[0][0], b.psi[0][0], ix, store);
+                d_ey_dx = pml_d4(d_ey_dx, gx, pbx, pax, pkx, a.psi[0][1], b.psi[0][1], ix, store);
+            } else if (gx) {
+                const f4 qb = ld4(a.prof[0][0] + xc), qa = ld4(a.prof[0][1] + xc),
+                         qk = ld4(a.prof[0][2] + xc);
+                f4 p0 = cur.psx[0], p1 = cur.psx[PM == 2 ? 1 : 0];
+                d_ez_dx = cpml4(d_ez_dx, qb, qa, qk, p0);
+                d_ey_dx = cpml4(d_ey_dx, qb, qa, qk, p1);
+                if (store) {
+                    st4(b.psi[0][0] + ix, p0);
+                    st4(b.psi[0][1] + ix, p1);
+                }
+            } else {
+                d_ez_dx = d_ez_dx + 0.0f;
+                d_ey_dx = d_ey_dx + 0.0f;
+            }
         }
         f4 hxn = da * cur.hx_o - db * (d_ez_dy - d_ey_dz);
         f4 hyn = da * cur.hy_o - db * (d_ex_dz - d_ez_dx);
@@ -733,7 +769,7 @@ __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo
                 const float2 c = s_ch[cur.hid];
                 float h_ex_dz = cur.hexn - hex, h_ez_dx = ez_c.x - hez;
                 float h_ey_dx = ey_c.x - hey, h_ex_dy = cur.hexpy - hex;
-                if (PML && (sh_h || sh_y || sh_z)) {
+                if (PYZ && (sh_h || sh_y || sh_z)) {
                     const size_t jx = ((size_t)z * g.ny + yc) * Tx2 + ghi;
                     const size_t jy = ((size_t)z * Ty2 + gyi) * nx + xh;
                     h_ex_dz = pml_d1(h_ex_dz, gz, pbz, paz, pkz, a.psi[2][1], gzo + rowh);
@@ -774,10 +810,23 @@ __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo
                                      a.psi[2][2], b.psi[2][2], iz, true);
                     d_hx_dz = pml_d4(d_hx_dz, gz, splat(pbz), splat(paz), splat(pkz),
                                      a.psi[2][3], b.psi[2][3], iz, true);
-                    d_hz_dx = pml_d4(d_hz_dx, gx, pbx, pax, pkx, a.psi[0][2], b.psi[0][2], ix,
-                                     true);
-                    d_hy_dx = pml_d4(d_hy_dx, gx, pbx, pax, pkx, a.psi[0][3], b.psi[0][3], ix,
-                                     true);
+                    if (PYZ) {
+                        d_hz_dx = pml_d4(d_hz_dx, gx, pbx, pax, pkx, a.psi[0][2], b.psi[0][2], ix,
+                                         true);
+                        d_hy_dx = pml_d4(d_hy_dx, gx, pbx, pax, pkx, a.psi[0][3], b.psi[0][3], ix,
+                                         true);
+                    } else if (gx) {
+                        const f4 qb = ld4(a.prof[0][0] + xc), qa = ld4(a.prof[0][1] + xc),
+                                 qk = ld4(a.prof[0][2] + xc);
+                        f4 p2 = cur.psx[PM == 2 ? 2 : 0], p3 = cur.psx[PM == 2 ? 3 : 0];
+                        d_hz_dx = cpml4(d_hz_dx, qb, qa, qk, p2);
+                        d_hy_dx = cpml4(d_hy_dx, qb, qa, qk, p3);
+                        st4(b.psi[0][2] + ix, p2);
+                        st4(b.psi[0][3] + ix, p3);
+                    } else {
+                        d_hz_dx = d_hz_dx + 0.0f;
+                        d_hy_dx = d_hy_dx + 0.0f;
+                    }
                 }
                 const size_t o = zo + row;
                 st4(b.f[0] + o, ca * ex_c + cb * (d_hz_dy - d_hy_dz));
@@ -804,6 +853,7 @@ __global__ __launch_bounds__((64 * fused_waves<PML, GHOST>())) void k_fused4(Geo
 
 #ifndef WF_STEP_FMA
 int fused_rows_per_tile(bool pml_or_ghost) { return (pml_or_ghost ? kFYx : kFY) - 1; }
+int fused_rows_per_tile_x() { return WF_FUSED_XPML_FY - 1; }
 int split_rows_per_wg() { return kSY; }
 #endif
 
@@ -856,19 +906,23 @@ hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, in
     return hipGetLastError();
 }
 
-hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st)
+hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st,
+                        bool x_only)
 {
     const int n = box.gx * box.gy * box.gz;
     if (n <= 0) return hipSuccess;
     if (pml_cells && s.ghost) return hipErrorInvalidValue;  // not a supported combination
-    if (pml_cells)
-        hipLaunchKernelGGL((k_fused4<true, false>), dim3((unsigned)n), dim3(64, kFYx), 0, st, s.g,
+    if (pml_cells && x_only)
+        hipLaunchKernelGGL((k_fused4<2, false>), dim3((unsigned)n), dim3(64, WF_FUSED_XPML_FY), 0,
+                           st, s.g, s.cur, s.alt, box);
+    else if (pml_cells)
+        hipLaunchKernelGGL((k_fused4<1, false>), dim3((unsigned)n), dim3(64, kFYx), 0, st, s.g,
                            s.cur, s.alt, box);
     else if (s.ghost)
-        hipLaunchKernelGGL((k_fused4<false, true>), dim3((unsigned)n), dim3(64, kFYx), 0, st, s.g,
+        hipLaunchKernelGGL((k_fused4<0, true>), dim3((unsigned)n), dim3(64, kFYx), 0, st, s.g,
                            s.cur, s.alt, box);
     else
-        hipLaunchKernelGGL((k_fused4<false, false>), dim3((unsigned)n), dim3(64, kFY), 0, st, s.g,
+        hipLaunchKernelGGL((k_fused4<0, false>), dim3((unsigned)n), dim3(64, kFY), 0, st, s.g,
                            s.cur, s.alt, box);
     return hipGetLastError();
 }
