@@ -84,14 +84,18 @@ __global__ void k_sources(Ptrs p, const DevSource *__restrict__ src, int ncol,
 // ---------------------------------------------------------------------------
 // Probes (src/probe.rs:126-138): one lane per probe, frequencies in order.
 // ---------------------------------------------------------------------------
+// slot / inslot (two-step passes, first step): probes whose cell the
+// two-step kernel owns read the value it recorded.
 __device__ __forceinline__ void accumulate_probe(const Ptrs &p, const DevProbe &q, int i, int n,
                                                  int nf, const double *__restrict__ cos_tab,
                                                  const double *__restrict__ sin_tab,
                                                  const uint8_t *__restrict__ active,
                                                  uint64_t row, double *sums,
-                                                 unsigned long long *counts, float *trace)
+                                                 unsigned long long *counts, float *trace,
+                                                 const float *slot = nullptr,
+                                                 const uint8_t *inslot = nullptr)
 {
-    const float v32 = p.f[q.comp][q.off];
+    const float v32 = (slot && inslot[i]) ? slot[i] : p.f[q.comp][q.off];
     if (trace) trace[row * (uint64_t)n + i] = v32;
     if (!active[row * (uint64_t)n + i]) return;
     const double v = (double)v32;
@@ -109,12 +113,13 @@ __device__ __forceinline__ void accumulate_probe(const Ptrs &p, const DevProbe &
 __global__ void k_probes(Ptrs p, const DevProbe *__restrict__ pr, int n, int nf,
                          const double *__restrict__ cos_tab, const double *__restrict__ sin_tab,
                          const uint8_t *__restrict__ active, const uint64_t *__restrict__ step,
-                         uint64_t base, double *sums, unsigned long long *counts, float *trace)
+                         uint64_t base, double *sums, unsigned long long *counts, float *trace,
+                         const float *slot, const uint8_t *inslot)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     accumulate_probe(p, pr[i], i, n, nf, cos_tab, sin_tab, active, *step - base, sums, counts,
-                     trace);
+                     trace, slot, inslot);
 }
 
 // Batched point work, one workgroup per scene: sources, then (next launch)
@@ -241,6 +246,25 @@ struct wf_ctx {
     BoxList lofix{};    // learned faces without a CPML: the low faces' boundary cells
     bool pml_fused = false;  // WF_PML_FUSED=1: single pass over the shell too (slower, see launch_step)
     BoxList whole{};    // split variant: the whole domain
+    // two steps per pass (launch_pair): the two-step kernel's box; the frame
+    // of the pass's first step (H over the rest and all six boundary layers
+    // of the box, E over the rest and its three low layers), computed into a
+    // third state C, and of its second step (the rest, C -> B); the point
+    // tables of the kernel's first step
+    bool pair_env = true;   // WF_TWOSTEP=0: single steps only
+    bool pair_geo = false;  // plan_boxes: the geometry allows it
+    Box two{};
+    BoxList fr1h{}, fr1e{}, fr2{};
+    Ptrs third{};
+    float *d_fields3 = nullptr, *d_psi3 = nullptr;
+    struct PointTable {
+        uint32_t *bits = nullptr;
+        uint64_t *key = nullptr;
+        PointEnt *ent = nullptr;
+        int n = 0;
+    } src_pts, prb_pts;
+    float *d_pslot = nullptr;   // probe values of a pass's first step, per probe
+    uint8_t *d_pin = nullptr;   // 1: the probe's cell belongs to the two-step box
     hipStream_t side = nullptr;            // frame H kernel, overlapped
     hipEvent_t fork = nullptr, join = nullptr;
     bool overlap = false;                  // WF_OVERLAP=1
@@ -314,6 +338,7 @@ struct wf_ctx {
     hipError_t (*lh)(const StepLaunch &, const BoxList &, hipStream_t) = launch_h;
     hipError_t (*le)(const StepLaunch &, const BoxList &, hipStream_t) = launch_e;
     hipError_t (*lf)(const StepLaunch &, const Box &, bool, hipStream_t, bool) = launch_fused;
+    hipError_t (*lt)(const StepLaunch &, const Box &, const TwoStepPoints &, hipStream_t) = launch_twostep;
 };
 
 namespace {
@@ -346,6 +371,44 @@ void drop_graph_exec(wf_ctx *c)
     c->gexec = nullptr;
 }
 
+void free_points(wf_ctx::PointTable &t)
+{
+    dfree(t.bits);
+    dfree(t.key);
+    dfree(t.ent);
+    t.n = 0;
+}
+
+// Row table of points for the two-step kernel: entries sorted by row
+// (z*ny + y) keeping their given order within a row, and the bitmap of
+// rows holding any.
+int upload_points(const wf_ctx *c, std::vector<std::pair<uint64_t, PointEnt>> &v,
+                  wf_ctx::PointTable &t)
+{
+    if (v.empty()) return WF_OK;
+    std::stable_sort(v.begin(), v.end(),
+                     [](const std::pair<uint64_t, PointEnt> &a, const std::pair<uint64_t, PointEnt> &b) {
+                         return a.first < b.first;
+                     });
+    const uint64_t rows = (uint64_t)c->g.ny * c->g.nz;
+    std::vector<uint32_t> bits((rows + 31) / 32, 0u);
+    std::vector<uint64_t> key(v.size());
+    std::vector<PointEnt> ent(v.size());
+    for (size_t i = 0; i < v.size(); ++i) {
+        key[i] = v[i].first;
+        ent[i] = v[i].second;
+        bits[key[i] >> 5] |= 1u << (key[i] & 31);
+    }
+    if (int rc = dmalloc(&t.bits, bits.size() * sizeof(uint32_t))) return rc;
+    if (int rc = dmalloc(&t.key, key.size() * sizeof(uint64_t))) return rc;
+    if (int rc = dmalloc(&t.ent, ent.size() * sizeof(PointEnt))) return rc;
+    WF_HIP(hipMemcpy(t.bits, bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    WF_HIP(hipMemcpy(t.key, key.data(), key.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    WF_HIP(hipMemcpy(t.ent, ent.data(), ent.size() * sizeof(PointEnt), hipMemcpyHostToDevice));
+    t.n = (int)v.size();
+    return WF_OK;
+}
+
 void free_sources(wf_ctx *c)
 {
     drop_graph_exec(c);
@@ -353,6 +416,7 @@ void free_sources(wf_ctx *c)
     dfree(c->d_src_grp);
     c->nsrc_grp = 0;
     dfree(c->d_srcval);
+    free_points(c->src_pts);
     c->nsrc = 0;
     c->nsrc_col = 0;
     c->src_steps = 0;
@@ -368,6 +432,9 @@ void free_probes(wf_ctx *c)
     dfree(c->d_active);
     dfree(c->d_counts);
     dfree(c->d_trace);
+    free_points(c->prb_pts);
+    dfree(c->d_pslot);
+    dfree(c->d_pin);
     c->nprobe = 0;
     c->nfreq = 0;
     c->probe_steps = 0;
@@ -414,19 +481,20 @@ void add_box(BoxList &bl, const Box &b)
 // Number of z slices of a box for the fused kernel, which runs one
 // workgroup per CU at a time (12 waves at 164 VGPRs, or 8 at up to 232):
 // the launch takes about ceil(tiles * slices / CUs) rounds of (planes per
-// slice + the plane before it), so pick the slice count that minimises that
-// product (ties: more slices).  At 1024^3 (356 tiles, 975 planes, 256 CUs)
+// slice + the `extra` planes swept before and after it: 1 for the fused
+// kernel, 3 for the two-step kernel), so pick the slice count that
+// minimises that product (ties: more slices).  At 1024^3 (356 tiles, 975 planes, 256 CUs)
 // that is 5 slices of 195 planes, 8.09 ms per launch against 8.17 ms for
 // 31 slices of <= 32 planes (whose last round is a tenth full); at 312^3
 // with a 32-cell CPML 11 slices, 0.179 against 0.187 ms.
-int fused_slices(long tiles, int planes, int cus)
+int fused_slices(long tiles, int planes, int cus, int extra = 1)
 {
     if (planes <= 0 || tiles <= 0) return 1;
     long best_cost = -1;
     int best = 1;
     for (int gz = 1; gz <= planes; ++gz) {
         const long rounds = (tiles * gz + cus - 1) / cus;
-        const long cost = rounds * ((planes + gz - 1) / gz + 1);
+        const long cost = rounds * ((planes + gz - 1) / gz + extra);
         if (best_cost < 0 || cost <= best_cost) {
             best_cost = cost;
             best = gz;
@@ -517,6 +585,46 @@ void plan_boxes(wf_ctx *c)
         c->framex = fx;
         c->xfold = true;
     }
+    // Two steps per pass: the two-step kernel's box keeps two cells (x: one
+    // quad) below it and one above it on every axis outside the shell, and
+    // on y and z one more above it inside the domain (its upper margin reads
+    // E^n there); no periodic axis.  Its first step's frame covers the rest
+    // of the domain plus the box's boundary layers (H: all six, so that E
+    // just above the box finds H(n+1/2) below it; E: the three low ones,
+    // which H(n+3/2) just below the box reads), its second step's frame the
+    // rest alone.
+    c->pair_geo = false;
+    if (!g.periodic) {
+        int tl[3], th[3];
+        for (int a = 0; a < 3; ++a) {
+            tl[a] = g.T[a] + (a == 0 ? 4 : 2);
+            th[a] = a == 0 ? n[a] - g.T[a] - 4 : n[a] - std::max(g.T[a], 1) - 1;
+        }
+        if (th[0] - tl[0] >= 8 && th[1] - tl[1] >= 2 && th[2] - tl[2] >= 2) {
+            Box tb{tl[0], th[0], tl[1], th[1], tl[2], th[2], 64, 0, 0, 0, 0};
+            const int r2 = twostep_rows_per_tile();
+            tb.gx = ((th[0] - tl[0]) / 4 + kTwoQuads - 1) / kTwoQuads;
+            tb.gy = (th[1] - tl[1] + r2 - 1) / r2;
+            tb.gz = c->zchunk_env ? (th[2] - tl[2] + zc - 1) / zc
+                                  : fused_slices((long)tb.gx * tb.gy, th[2] - tl[2], c->cus, 3);
+            tb.gz = std::min(tb.gz, th[2] - tl[2]);
+            c->two = tb;
+            auto frame_around = [&](int x0, int x1, int y0, int y1, int z0, int z1) {
+                BoxList f{};
+                add_box(f, split_box(0, n[0], 0, n[1], 0, z0, fz));
+                add_box(f, split_box(0, n[0], 0, n[1], z1, n[2], fz));
+                add_box(f, split_box(0, n[0], 0, y0, z0, z1, fz));
+                add_box(f, split_box(0, n[0], y1, n[1], z0, z1, fz));
+                add_box(f, split_box(0, x0, y0, y1, z0, z1, fz));
+                add_box(f, split_box(x1, n[0], y0, y1, z0, z1, fz));
+                return f;
+            };
+            c->fr1h = frame_around(tl[0] + 4, th[0] - 4, tl[1] + 1, th[1] - 1, tl[2] + 1, th[2] - 1);
+            c->fr1e = frame_around(tl[0] + 4, th[0], tl[1] + 1, th[1], tl[2] + 1, th[2]);
+            c->fr2 = frame_around(tl[0], th[0], tl[1], th[1], tl[2], th[2]);
+            c->pair_geo = true;
+        }
+    }
     Box all{0, n[0], 0, n[1], 0, n[2], 64, 0, 0, 0, 0};
     all.gx = (n[0] + 255) / 256;
     all.gy = (n[1] + frx - 1) / frx;  // swept by the CPML or ghost variant
@@ -606,6 +714,27 @@ bool folded(const wf_ctx *c)
     for (int f = 0; f < 6; ++f)
         if (c->hook_on[f]) return false;
     return true;
+}
+
+// The end of a step (src/engine.rs:770-782): sources, probes, then the
+// device step counter.  slot: probes in the two-step kernel's box read the
+// values it recorded (the first step of a pass).
+int point_work(wf_ctx *c, const Ptrs &p, hipStream_t st, bool slot)
+{
+    if (c->nsrc) {
+        hipLaunchKernelGGL(k_sources, dim3((c->nsrc_grp + 255) / 256), dim3(256), 0, st, p,
+                           c->d_src, (int)c->nsrc_col, c->d_src_grp, (int)c->nsrc_grp,
+                           c->d_srcval, c->d_step, c->src_base);
+    }
+    if (c->nprobe) {
+        hipLaunchKernelGGL(k_probes, dim3((c->nprobe + 63) / 64), dim3(64), 0, st, p,
+                           c->d_probe, (int)c->nprobe, (int)c->nfreq, c->d_cos, c->d_sin,
+                           c->d_active, c->d_step, c->probe_base, c->d_sums, c->d_counts,
+                           c->d_trace, slot ? c->d_pslot : nullptr, slot ? c->d_pin : nullptr);
+    }
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, c->d_step);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
 }
 
 int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
@@ -713,20 +842,77 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
         WF_HIP(c->le(s, c->whole, st));
         if (ev) WF_HIP(hipEventRecord(ev[5], st));
     }
-    if (c->nsrc) {
-        hipLaunchKernelGGL(k_sources, dim3((c->nsrc_grp + 255) / 256), dim3(256), 0, st, c->p,
-                           c->d_src, (int)c->nsrc_col, c->d_src_grp, (int)c->nsrc_grp,
-                           c->d_srcval, c->d_step, c->src_base);
-    }
-    if (c->nprobe) {
-        hipLaunchKernelGGL(k_probes, dim3((c->nprobe + 63) / 64), dim3(64), 0, st, c->p,
-                           c->d_probe, (int)c->nprobe, (int)c->nfreq, c->d_cos, c->d_sin,
-                           c->d_active, c->d_step, c->probe_base, c->d_sums, c->d_counts,
-                           c->d_trace);
-    }
-    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, c->d_step);
-    WF_HIP(hipGetLastError());
+    if (int rc = point_work(c, c->p, st, false)) return rc;
     c->step += 1;
+    return WF_OK;
+}
+
+// Two steps per pass apply to the fused variant without face hooks or the
+// whole-domain CPML pass, when the geometry allows the two-step box.
+bool pair_active(const wf_ctx *c)
+{
+    if (!c->fused || !c->pair_env || !c->pair_geo || !c->d_fields3 || c->pml_fused) return false;
+    for (int f = 0; f < 6; ++f)
+        if (c->hook_on[f]) return false;
+    return true;
+}
+
+// Two loop iterations in one pass (src/engine.rs:747-782 twice): the
+// two-step kernel over its box, state A -> B, adds the first step's sources
+// to E(n+1) on the way and records its probes in the box; the frame's first
+// step A -> C over the rest and the box's boundary layers, then the first
+// step's sources and probes (probes in the box from the kernel's slots);
+// the frame's second step C -> B (its E reads the kernel's H(n+3/2) just
+// inside the box), then the second step's sources and probes on B.  With
+// ev (6 timing events): event pairs around the two-step kernel (0, 1), the
+// first step's frame (2, 3) and the second step's frame (4, 5).
+int launch_pair(wf_ctx *c, hipEvent_t *ev = nullptr)
+{
+    hipStream_t st = c->stream;
+    TwoStepPoints pts;
+    pts.src = PointRows{c->src_pts.bits, c->src_pts.key, c->src_pts.ent, c->src_pts.n};
+    pts.prb = PointRows{c->prb_pts.bits, c->prb_pts.key, c->prb_pts.ent, c->prb_pts.n};
+    pts.values = c->d_srcval;
+    pts.ncol = (int)c->nsrc_col;
+    pts.base = c->src_base;
+    pts.step = c->d_step;
+    pts.slot = c->d_pslot;
+    if (ev) WF_HIP(hipEventRecord(ev[0], st));
+    StepLaunch s{c->g, c->p, c->alt, c->pml, c->fzchunk, st};
+    WF_HIP(c->lt(s, c->two, pts, st));
+    if (ev) {
+        WF_HIP(hipEventRecord(ev[1], st));
+        WF_HIP(hipEventRecord(ev[2], st));
+    }
+    StepLaunch s1{c->g, c->p, c->third, c->pml, c->fzchunk, st};
+    WF_HIP(c->lh(s1, c->fr1h, st));
+    WF_HIP(c->le(s1, c->fr1e, st));
+    if (ev) WF_HIP(hipEventRecord(ev[3], st));
+    if (int rc = point_work(c, c->third, st, true)) return rc;
+    if (ev) WF_HIP(hipEventRecord(ev[4], st));
+    StepLaunch s2{c->g, c->third, c->alt, c->pml, c->fzchunk, st};
+    WF_HIP(c->lh(s2, c->fr2, st));
+    WF_HIP(c->le(s2, c->fr2, st));
+    if (ev) WF_HIP(hipEventRecord(ev[5], st));
+    if (int rc = point_work(c, c->alt, st, false)) return rc;
+    std::swap(c->p, c->alt);
+    c->step += 2;
+    return WF_OK;
+}
+
+// n loop iterations, two per pass where they apply.
+int launch_steps(wf_ctx *c, uint64_t n)
+{
+    const bool pairs = pair_active(c);
+    for (uint64_t s = 0; s < n;) {
+        if (pairs && n - s >= 2) {
+            if (int rc = launch_pair(c)) return rc;
+            s += 2;
+        } else {
+            if (int rc = launch_step(c)) return rc;
+            s += 1;
+        }
+    }
     return WF_OK;
 }
 
@@ -757,10 +943,9 @@ int probe_step_ms(wf_ctx *c, int warm, int timed, double *ms)
         (void)hipEventDestroy(e0);
         return fail(WF_ERR_HIP, "placement probe: event creation failed");
     }
-    int rc = WF_OK;
-    for (int s = 0; s < warm && rc == WF_OK; ++s) rc = launch_step(c);
+    int rc = launch_steps(c, (uint64_t)warm);
     if (rc == WF_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(WF_ERR_HIP, "placement probe: event");
-    for (int s = 0; s < timed && rc == WF_OK; ++s) rc = launch_step(c);
+    if (rc == WF_OK) rc = launch_steps(c, (uint64_t)timed);
     float t = 0.f;
     if (rc == WF_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                         hipEventElapsedTime(&t, e0, e1) != hipSuccess))
@@ -857,8 +1042,7 @@ int capture_graph(wf_ctx *c)
     const Ptrs p0 = c->p, a0 = c->alt;
     hipGraph_t graph = nullptr;
     WF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = WF_OK;
-    for (int k = 0; k < kGraphSteps && rc == WF_OK; ++k) rc = launch_step(c);
+    int rc = launch_steps(c, kGraphSteps);
     const hipError_t e = hipStreamEndCapture(c->stream, &graph);
     // capture only records: restore the host view of the state
     c->step = step0;
@@ -1147,8 +1331,31 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
             c->lh = fma::launch_h;
             c->le = fma::launch_e;
             c->lf = fma::launch_fused;
+            c->lt = fma::launch_twostep;
         }
+    if (const char *ts = std::getenv("WF_TWOSTEP")) c->pair_env = std::atoi(ts) != 0;
     plan_boxes(c);
+    // Third state for the frame of two-step passes (its first step's values).
+    if (c->fused && c->pair_env && c->pair_geo) {
+        if (int rc = dmalloc(&c->d_fields3, 6 * N * sizeof(float))) return bail(rc);
+        if (hipMemsetAsync(c->d_fields3, 0, 6 * N * sizeof(float), c->stream) != hipSuccess)
+            return bail(fail(WF_ERR_HIP, "hipMemset(fields) failed"));
+        if (psi_total) {
+            if (int rc = dmalloc(&c->d_psi3, psi_total * sizeof(float))) return bail(rc);
+            if (hipMemsetAsync(c->d_psi3, 0, psi_total * sizeof(float), c->stream) != hipSuccess)
+                return bail(fail(WF_ERR_HIP, "hipMemset(psi) failed"));
+        }
+        c->third = c->p;
+        for (int i = 0; i < 6; ++i) c->third.f[i] = c->d_fields3 + (uint64_t)i * N;
+        float *pp = c->d_psi3;
+        for (int a = 0; a < 3; ++a)
+            for (int t = 0; t < 4; ++t) {
+                c->third.psi[a][t] = slab[a] ? pp : nullptr;
+                pp += slab[a];
+            }
+        if (hipStreamSynchronize(c->stream) != hipSuccess)
+            return bail(fail(WF_ERR_HIP, "initialisation failed"));
+    }
     // Field placement (see choose_placement): by default eight extra state
     // buffers tried, one held at a time beside the current ones, when one
     // state's fields take at least 8 GiB (4 steps per arrangement timed, 17
@@ -1193,6 +1400,8 @@ void wf_destroy(wf_ctx *c)
     dfree(c->d_fields);
     dfree(c->d_fields2);
     dfree(c->d_psi2);
+    dfree(c->d_fields3);
+    dfree(c->d_psi3);
     dfree(c->d_step);
     dfree(c->d_id);
     dfree(c->d_coef);
@@ -1259,6 +1468,19 @@ int wf_set_source_columns(wf_ctx *c, uint32_t n, const wf_source_site *sites,
     WF_HIP(hipMemcpy(c->d_src_grp, grp.data(), grp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     c->nsrc_grp = (uint32_t)(grp.size() - 1);
     if (nsteps) WF_HIP(hipMemcpy(c->d_srcval, values, nsteps * ncol * sizeof(float), hipMemcpyHostToDevice));
+    if (c->d_fields3) {
+        // the two-step kernel's table: by row, list order kept per voxel
+        std::vector<std::pair<uint64_t, PointEnt>> v(n);
+        for (uint32_t s = 0; s < n; ++s) {
+            PointEnt e{};
+            e.x = (uint32_t)(ds[s].off % (uint64_t)c->g.nx);
+            e.comp = (uint16_t)ds[s].comp;
+            e.hard = (uint16_t)ds[s].hard;
+            e.col = ds[s].col;
+            v[s] = {ds[s].off / (uint64_t)c->g.nx, e};
+        }
+        if (int rc = upload_points(c, v, c->src_pts)) return rc;
+    }
     c->nsrc = n;
     c->nsrc_col = ncol;
     c->src_base = c->step;
@@ -1316,6 +1538,28 @@ int wf_set_probes(wf_ctx *c, uint32_t n, const wf_probe_site *sites, uint64_t ns
     if (nsteps) WF_HIP(hipMemcpy(c->d_active, active, nsteps * n, hipMemcpyHostToDevice));
     WF_HIP(hipMemset(c->d_sums, 0, 2ull * (nf ? nf : 1) * sizeof(double)));
     WF_HIP(hipMemset(c->d_counts, 0, n * sizeof(unsigned long long)));
+    if (c->d_fields3) {
+        // the two-step kernel records the probes of its box at a pass's
+        // first step into one slot per probe
+        std::vector<std::pair<uint64_t, PointEnt>> v(n);
+        std::vector<uint8_t> in(n);
+        const Box &t = c->two;
+        for (uint32_t p = 0; p < n; ++p) {
+            const wf_probe_site &q = sites[p];
+            PointEnt e{};
+            e.x = q.x;
+            e.comp = (uint16_t)q.component;
+            e.col = p;
+            v[p] = {(uint64_t)q.z * c->g.ny + q.y, e};
+            in[p] = (int)q.x >= t.x0 && (int)q.x < t.x1 && (int)q.y >= t.y0 && (int)q.y < t.y1 &&
+                    (int)q.z >= t.z0 && (int)q.z < t.z1;
+        }
+        if (int rc = upload_points(c, v, c->prb_pts)) return rc;
+        if (int rc = dmalloc(&c->d_pslot, n * sizeof(float))) return rc;
+        if (int rc = dmalloc(&c->d_pin, n)) return rc;
+        WF_HIP(hipMemset(c->d_pslot, 0, n * sizeof(float)));
+        WF_HIP(hipMemcpy(c->d_pin, in.data(), n, hipMemcpyHostToDevice));
+    }
     c->nprobe = n;
     c->nfreq = nf;
     c->probe_base = c->step;
@@ -1350,9 +1594,7 @@ int wf_step(wf_ctx *c, uint64_t nsteps)
             left -= kGraphSteps;
         }
     }
-    for (uint64_t s = 0; s < left; ++s)
-        if (int rc = launch_step(c)) return rc;
-    return WF_OK;
+    return launch_steps(c, left);
 }
 
 int wf_sync(wf_ctx *c)
@@ -1575,8 +1817,12 @@ int wf_time_kernels(wf_ctx *c, uint64_t nsteps, double *fused_ms, double *h_ms, 
     for (auto &e : ev) WF_HIP(hipEventCreate(&e));
     double acc[3] = {0.0, 0.0, 0.0};
     int rc = WF_OK;
-    for (uint64_t s = 0; s < nsteps && rc == WF_OK; ++s) {
-        rc = launch_step(c, ev);
+    // two-step passes: the two-step kernel per launch, and the frame of the
+    // pass's first and of its second step
+    const bool pairs = pair_active(c) && nsteps >= 2;
+    const uint64_t n = pairs ? nsteps / 2 : nsteps;
+    for (uint64_t s = 0; s < n && rc == WF_OK; ++s) {
+        rc = pairs ? launch_pair(c, ev) : launch_step(c, ev);
         if (rc) break;
         WF_HIP(hipEventSynchronize(ev[5]));
         for (int k = 0; k < 3; ++k) {
@@ -1586,7 +1832,7 @@ int wf_time_kernels(wf_ctx *c, uint64_t nsteps, double *fused_ms, double *h_ms, 
         }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
-    const double d = nsteps ? (double)nsteps : 1.0;
+    const double d = n ? (double)n : 1.0;
     // the split variant launches no fused kernel (its event pair is adjacent)
     *fused_ms = c->fused ? acc[0] / d : 0.0;
     *h_ms = acc[1] / d;
@@ -1648,7 +1894,7 @@ int wf_hbm_copy(uint64_t bytes, int device, int iters, double *ms_per_copy)
 
 const char *wf_kernel_variant(const wf_ctx *c)
 {
-    return !c ? "" : (c->fused ? "fused" : "split");
+    return !c ? "" : (c->fused ? (pair_active(c) ? "twostep" : "fused") : "split");
 }
 
 int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
@@ -1660,6 +1906,24 @@ int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
     // non-identity psi terms per phase: 2 per graded axis per cell
     const uint64_t terms = 2ull * (2ull * c->g.T[0] * ny * nz + 2ull * c->g.T[1] * nx * nz +
                                    2ull * c->g.T[2] * nx * ny);
+    auto box_cells = [](const BoxList &bl) {
+        uint64_t cells = 0;
+        for (int k = 0; k < bl.n; ++k) {
+            const Box &b = bl.b[k];
+            cells += (uint64_t)(b.x1 - b.x0) * (b.y1 - b.y0) * (b.z1 - b.z0);
+        }
+        return cells;
+    };
+    if (pair_active(c)) {
+        // two-step kernel: 49 B per cell of its box for two steps; the
+        // frame of a pass's first step (H over fr1h, E over fr1e) and of its
+        // second (fr2): 37 B per cell per kernel and 8 B per psi term
+        const Box &t = c->two;
+        *fused_bytes = 49ull * (uint64_t)(t.x1 - t.x0) * (t.y1 - t.y0) * (t.z1 - t.z0);
+        *h_bytes = 37ull * (box_cells(c->fr1h) + box_cells(c->fr1e)) + 16ull * terms;
+        *e_bytes = 37ull * 2 * box_cells(c->fr2) + 16ull * terms;
+        return WF_OK;
+    }
     if (single_pass(c)) {
         // one pass: 49 B per cell, and per psi term of each phase 4 B read
         // and 4 B written

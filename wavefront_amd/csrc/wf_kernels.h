@@ -57,6 +57,34 @@ struct DevProbe {
     int pad;
 };
 
+// Point work of the two-step kernel at its first step: sources (applied to
+// E(n+1) wherever the kernel computes it) and probes (the values of owned
+// cells recorded into one slot per probe).  Entries are sorted by row
+// (z*ny + y), list order kept within a row; a bitmap flags the rows that
+// hold any entry.
+struct PointEnt {
+    uint32_t x;
+    uint16_t comp;  // sources: 0..2; probes: 0..5
+    uint16_t hard;  // sources: '=' instead of '+='
+    uint32_t col;   // sources: value column; probes: probe index
+};
+
+struct PointRows {
+    const uint32_t *bits = nullptr;  // null: no entries
+    const uint64_t *key = nullptr;   // row of each entry, ascending
+    const PointEnt *ent = nullptr;
+    int n = 0;
+};
+
+struct TwoStepPoints {
+    PointRows src, prb;
+    const float *values = nullptr;  // source values, values[row * ncol + col]
+    int ncol = 0;
+    uint64_t base = 0;               // step of the table's first row
+    const uint64_t *step = nullptr;  // device step counter (the first step's loop index)
+    float *slot = nullptr;           // probe values of the first step
+};
+
 // Four consecutive x-cells: the unit a lane loads and stores (16 B).
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -172,6 +200,17 @@ hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 // E kernel to redo.
 hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st,
                         bool x_only = false);
+// Two-step kernel over a box: E(n+2), H(n+3/2) of the box's cells into
+// s.alt from s.cur (E^n, H^{n-1/2}), the sources of step n+1 applied and
+// its probes recorded on the way.  Every cell within two cells below and
+// one above the box on each axis (x: one quad below and above) must lie
+// outside the CPML shell and inside the domain, no axis periodic.  Tiles:
+// box.gx columns of kTwoQuads quads, box.gy row groups of
+// twostep_rows_per_tile() rows, box.gz even z slices.
+hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoints &pts,
+                          hipStream_t st);
+constexpr int kTwoQuads = 62;  // owned quads per tile row (lanes 1..62)
+int twostep_rows_per_tile();
 // Tile shapes of the two kernels (owned rows per fused tile -- of the plain
 // variant, or of the CPML / ghost variants -- and rows per split workgroup
 // of lpr = 64).
@@ -215,6 +254,8 @@ hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st,
                         bool x_only = false);
+hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoints &pts,
+                          hipStream_t st);
 hipError_t launch_batch_h(const BatchScene *scenes, const uint32_t *wg_scene, int total,
                           int zchunk, int grid, hipStream_t st);
 hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, int total,

@@ -849,13 +849,307 @@ __global__ __launch_bounds__((64 * fused_waves<PM, GHOST>())) void k_fused4(Geo 
     }
 }
 
+// ===========================================================================
+// Two-step kernel: two whole timesteps of a box in one sweep, state A (E^n,
+// H^{n-1/2}) -> state B (E^{n+2}, H^{n+3/2}), no CPML (every cell it
+// computes lies outside the PML shell).  Per owned cell 24 B read + 24 B
+// written + 1 B id for the two steps: half the single-pass kernel's bytes
+// per step.
+//
+// Workgroup = kF2 waves x 64 lanes; lane l holds the quad
+// x = x0 - 4 + 4 (kTwoQuads tx + l), wave w the row y = y0 + (kF2-3) ty - 2 + w.
+// Two steps reach two cells down and one up on each axis, so a tile also
+// computes a margin that it never stores: lanes 0 and 63 (one quad on each
+// side), waves 0 and 1 (two rows below), wave kF2-1 (one row above), two
+// planes below its z slice and one above.  The outermost margin values are
+// wrong (their neighbours lie outside the tile), and each step reads them
+// only for margin cells one cell further out:
+//   step 1 at plane p: H(n+1/2) on every wave, E(n+1) on waves >= 1;
+//   step 2 at plane p-1 (H(n+3/2) at p-1 reads E(n+1) at p): H(n+3/2) on
+//   waves 1..kF2-2, E(n+2) on waves 2..kF2-2, the rows the tile owns.
+// y-neighbours cross waves through LDS (H(n+1/2) at y-1, E(n+1) at y+1 one
+// plane back, H(n+3/2) at y-1; two LDS-only barriers per plane),
+// x-neighbours come from the adjacent lane, z-neighbours ride in registers.
+// The sources of step n+1 are added to E(n+1) wherever it is computed (rows
+// flagged in a bitmap, entries in list order), and the probes of step n+1
+// at owned cells are recorded into their slots.  The arithmetic of every
+// update is the single-pass kernel's, so the bits are the same.
+// ===========================================================================
+#ifndef WF_TWOSTEP_FY
+#define WF_TWOSTEP_FY 12
+#endif
+constexpr int kF2 = WF_TWOSTEP_FY;
+
+// First entry of row `key` in a point table (entries sorted by row).
+__device__ __forceinline__ int point_row_begin(const PointRows &t, uint64_t key)
+{
+    int lo = 0, hi = t.n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (t.key[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool point_row_flagged(const PointRows &t, uint64_t key)
+{
+    return t.bits && ((t.bits[key >> 5] >> (key & 31)) & 1u);
+}
+
+// Element k (0..3) of a quad, read or replaced without dynamic indexing.
+__device__ __forceinline__ float quad_get(const f4 &v, int k)
+{
+    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+
+// src/source.rs:79-95: the field value += the waveform value, or = it.
+__device__ __forceinline__ void quad_source(f4 &v, int k, float val, bool hard)
+{
+    const float nv = hard ? val : quad_get(v, k) + val;
+    v.x = k == 0 ? nv : v.x;
+    v.y = k == 1 ? nv : v.y;
+    v.z = k == 2 ? nv : v.z;
+    v.w = k == 3 ? nv : v.w;
+}
+
+__global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Box box,
+                                                       TwoStepPoints pts)
+{
+    __shared__ float2 s_ce[256];         // (ca, cb)
+    __shared__ float2 s_ch[256];         // (da, db)
+    __shared__ f4 s_h1[2][kF2][64];      // H(n+1/2) of plane p: [hz, hx][wave][lane]
+    __shared__ f4 s_e1[2][2][kF2][64];   // E(n+1): [plane parity][ez, ex][wave][lane]
+    __shared__ f4 s_h2[2][kF2][64];      // H(n+3/2) of plane p-1: [hz, hx][wave][lane]
+    const int lane = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    for (int i = w * 64 + lane; i < 256; i += 64 * kF2) {
+        const float4 c = a.coef[i];
+        s_ce[i] = make_float2(c.x, c.y);
+        s_ch[i] = make_float2(c.z, c.w);
+    }
+    __syncthreads();
+
+    const int t = xcd_tile(blockIdx.x, box.gx * box.gy * box.gz);
+    const int tx = t % box.gx, ty = (t / box.gx) % box.gy, tz = t / (box.gx * box.gy);
+    const int x = box.x0 - 4 + 4 * (kTwoQuads * tx + lane);
+    const int y = box.y0 + (kF2 - 3) * ty - 2 + w;
+    // x1 and y1 themselves hold the upper margin; beyond them lanes and
+    // rows are idle (no loads)
+    const bool live_row = y <= box.y1;
+    const bool live = live_row && x <= box.x1;
+    const bool own_row = live_row && y < box.y1 && w >= 2 && w <= kF2 - 2;
+    const bool own = own_row && live && lane >= 1 && lane <= kTwoQuads && x < box.x1;
+    const size_t nx = g.nx, plane = nx * g.ny;
+    const size_t row = (size_t)(live ? y : 0) * nx + (live ? x : 0);
+    const size_t rowp = row + nx;  // E^n at y+1
+
+    const long nzb = box.z1 - box.z0;
+    const int z0 = box.z0 + (int)(tz * nzb / box.gz), z1 = box.z0 + (int)((tz + 1) * nzb / box.gz);
+    const int nit = z1 - z0;
+
+    // step n+1's row of source values
+    const float *sv = nullptr;
+    if (pts.src.bits) sv = pts.values + (*pts.step - pts.base) * (uint64_t)pts.ncol;
+
+    const float *__restrict__ aex = a.f[0];
+    const float *__restrict__ aey = a.f[1];
+    const float *__restrict__ aez = a.f[2];
+    const float *__restrict__ ahx = a.f[3];
+    const float *__restrict__ ahy = a.f[4];
+    const float *__restrict__ ahz = a.f[5];
+
+    // State A at plane p: E at p+1 and at y+1, H, the material ids; loaded
+    // a plane ahead of its use.
+    struct Plane {
+        f4 ex_n, ey_n, ez_n, ez_py, ex_py, hx_o, hy_o, hz_o;
+        uint32_t ids;
+    };
+    auto load = [&](Plane &P, int p) {
+        P.ex_n = P.ey_n = P.ez_n = P.ez_py = P.ex_py = P.hx_o = P.hy_o = P.hz_o = splat(0.f);
+        P.ids = 0;
+        if (live) {
+            const size_t zo = (size_t)p * plane;
+            const size_t on = zo + plane + row;
+            P.ex_n = ld4(aex + on);
+            P.ey_n = ld4(aey + on);
+            P.ez_n = ld4(aez + on);
+            P.ez_py = ld4(aez + zo + rowp);
+            P.ex_py = ld4(aex + zo + rowp);
+            P.hx_o = ld4(ahx + zo + row);
+            if (w >= 1) P.hy_o = ld4(ahy + zo + row);  // row y0-2 needs H(n+1/2) hz, hx only
+            P.hz_o = ld4(ahz + zo + row);
+            P.ids = *reinterpret_cast<const uint32_t *>(a.id + zo + row);
+        }
+    };
+
+    f4 ex_c = splat(0.f), ey_c = splat(0.f), ez_c = splat(0.f);  // E^n at p
+    if (live) {
+        const size_t o = (size_t)(z0 - 2) * plane + row;
+        ex_c = ld4(aex + o);
+        ey_c = ld4(aey + o);
+        ez_c = ld4(aez + o);
+    }
+    f4 h1x_c = splat(0.f), h1y_c = splat(0.f), h1z_c = splat(0.f);  // H(n+1/2) at p-1
+    f4 e1x_c = splat(0.f), e1y_c = splat(0.f), e1z_c = splat(0.f);  // E(n+1) at p-1
+    f4 h2x_m = splat(0.f), h2y_m = splat(0.f);                      // H(n+3/2) at p-2
+    uint32_t ids_c = 0;                                             // ids at p-1
+    Plane cur;
+    load(cur, z0 - 2);
+
+    for (int it = -2; it <= nit; ++it) {
+        const int p = z0 + it;
+        Plane nxt;
+        if (it < nit) load(nxt, p + 1);
+
+        // ---- step 1: H(n+1/2) at plane p (update_h_field) ---------------
+        const float sez = __shfl_down(ez_c.x, 1, 64), sey = __shfl_down(ey_c.x, 1, 64);
+        const f4 ez_px = f4{ez_c.y, ez_c.z, ez_c.w, sez};
+        const f4 ey_px = f4{ey_c.y, ey_c.z, ey_c.w, sey};
+        f4 h1x, h1y, h1z;
+        {
+            f4 da, db;
+            coef4(s_ch, cur.ids, da, db);
+            h1x = da * cur.hx_o - db * ((cur.ez_py - ez_c) - (cur.ey_n - ey_c));
+            h1y = da * cur.hy_o - db * ((cur.ex_n - ex_c) - (ez_px - ez_c));
+            h1z = da * cur.hz_o - db * ((ey_px - ey_c) - (cur.ex_py - ex_c));
+        }
+        if (!live) {
+            h1x = splat(0.f);
+            h1y = splat(0.f);
+            h1z = splat(0.f);
+        }
+        s_h1[0][w][lane] = h1z;
+        s_h1[1][w][lane] = h1x;
+        lds_barrier();
+
+        // ---- step 1: E(n+1) at plane p (update_e_field), sources, probes -
+        f4 e1x = splat(0.f), e1y = splat(0.f), e1z = splat(0.f);
+        const float hz_l = __shfl_up(h1z.w, 1, 64), hy_l = __shfl_up(h1y.w, 1, 64);
+        if (w >= 1 && it >= -1) {
+            const f4 hz_my = s_h1[0][w - 1][lane], hx_my = s_h1[1][w - 1][lane];
+            const f4 hz_mx = f4{hz_l, h1z.x, h1z.y, h1z.z};
+            const f4 hy_mx = f4{hy_l, h1y.x, h1y.y, h1y.z};
+            f4 ca, cb;
+            coef4(s_ce, cur.ids, ca, cb);
+            e1x = ca * ex_c + cb * ((h1z - hz_my) - (h1y - h1y_c));
+            e1y = ca * ey_c + cb * ((h1x - h1x_c) - (h1z - hz_mx));
+            e1z = ca * ez_c + cb * ((h1y - hy_mx) - (h1x - hx_my));
+            if (live_row) {
+                const uint64_t key = (uint64_t)p * g.ny + y;
+                if (point_row_flagged(pts.src, key)) {
+                    for (int i = point_row_begin(pts.src, key); i < pts.src.n && pts.src.key[i] == key;
+                         ++i) {
+                        const PointEnt e = pts.src.ent[i];
+                        const int k = (int)e.x - x;
+                        if (live && k >= 0 && k < 4) {
+                            const float v = sv[e.col];
+                            if (e.comp == 0)
+                                quad_source(e1x, k, v, e.hard);
+                            else if (e.comp == 1)
+                                quad_source(e1y, k, v, e.hard);
+                            else
+                                quad_source(e1z, k, v, e.hard);
+                        }
+                    }
+                }
+                if (own_row && it >= 0 && it < nit && point_row_flagged(pts.prb, key)) {
+                    for (int i = point_row_begin(pts.prb, key); i < pts.prb.n && pts.prb.key[i] == key;
+                         ++i) {
+                        const PointEnt e = pts.prb.ent[i];
+                        const int k = (int)e.x - x;
+                        if (own && k >= 0 && k < 4) {
+                            const f4 &f = e.comp == 0 ? e1x
+                                        : e.comp == 1 ? e1y
+                                        : e.comp == 2 ? e1z
+                                        : e.comp == 3 ? h1x
+                                        : e.comp == 4 ? h1y
+                                                      : h1z;
+                            pts.slot[e.col] = quad_get(f, k);
+                        }
+                    }
+                }
+            }
+        }
+        s_e1[it & 1][0][w][lane] = e1z;
+        s_e1[it & 1][1][w][lane] = e1x;
+
+        // ---- step 2: H(n+3/2) at plane p-1 ------------------------------
+        f4 h2x = splat(0.f), h2y = splat(0.f), h2z = splat(0.f);
+        const float s2z = __shfl_down(e1z_c.x, 1, 64), s2y = __shfl_down(e1y_c.x, 1, 64);
+        if (it >= 0 && w >= 1 && w <= kF2 - 2) {
+            const int pb = (it - 1) & 1;
+            const f4 ez_py = s_e1[pb][0][w + 1][lane], ex_py = s_e1[pb][1][w + 1][lane];
+            const f4 ez_px2 = f4{e1z_c.y, e1z_c.z, e1z_c.w, s2z};
+            const f4 ey_px2 = f4{e1y_c.y, e1y_c.z, e1y_c.w, s2y};
+            f4 da, db;
+            coef4(s_ch, ids_c, da, db);
+            h2x = da * h1x_c - db * ((ez_py - e1z_c) - (e1y - e1y_c));
+            h2y = da * h1y_c - db * ((e1x - e1x_c) - (ez_px2 - e1z_c));
+            h2z = da * h1z_c - db * ((ey_px2 - e1y_c) - (ex_py - e1x_c));
+        }
+        s_h2[0][w][lane] = h2z;
+        s_h2[1][w][lane] = h2x;
+        lds_barrier();
+
+        // ---- step 2: E(n+2) at plane p-1, stores of the owned cells -----
+        const float h2z_l = __shfl_up(h2z.w, 1, 64), h2y_l = __shfl_up(h2y.w, 1, 64);
+        if (it >= 1 && w >= 2 && w <= kF2 - 2) {
+            const f4 hz_my = s_h2[0][w - 1][lane], hx_my = s_h2[1][w - 1][lane];
+            const f4 hz_mx = f4{h2z_l, h2z.x, h2z.y, h2z.z};
+            const f4 hy_mx = f4{h2y_l, h2y.x, h2y.y, h2y.z};
+            f4 ca, cb;
+            coef4(s_ce, ids_c, ca, cb);
+            const f4 e2x = ca * e1x_c + cb * ((h2z - hz_my) - (h2y - h2y_m));
+            const f4 e2y = ca * e1y_c + cb * ((h2x - h2x_m) - (h2z - hz_mx));
+            const f4 e2z = ca * e1z_c + cb * ((h2y - hy_mx) - (h2x - hx_my));
+            if (own) {
+                const size_t o = (size_t)(p - 1) * plane + row;
+                st4(b.f[0] + o, e2x);
+                st4(b.f[1] + o, e2y);
+                st4(b.f[2] + o, e2z);
+                st4(b.f[3] + o, h2x);
+                st4(b.f[4] + o, h2y);
+                st4(b.f[5] + o, h2z);
+            }
+        }
+
+        h2x_m = h2x;
+        h2y_m = h2y;
+        e1x_c = e1x;
+        e1y_c = e1y;
+        e1z_c = e1z;
+        h1x_c = h1x;
+        h1y_c = h1y;
+        h1z_c = h1z;
+        ids_c = cur.ids;
+        ex_c = cur.ex_n;
+        ey_c = cur.ey_n;
+        ez_c = cur.ez_n;
+        cur = nxt;
+    }
+}
+
 }  // namespace
 
 #ifndef WF_STEP_FMA
 int fused_rows_per_tile(bool pml_or_ghost) { return (pml_or_ghost ? kFYx : kFY) - 1; }
 int fused_rows_per_tile_x() { return WF_FUSED_XPML_FY - 1; }
 int split_rows_per_wg() { return kSY; }
+int twostep_rows_per_tile() { return kF2 - 3; }
 #endif
+
+hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoints &pts,
+                          hipStream_t st)
+{
+    const int n = box.gx * box.gy * box.gz;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_twostep4, dim3((unsigned)n), dim3(64, kF2), 0, st, s.g, s.cur, s.alt, box,
+                       pts);
+    return hipGetLastError();
+}
 
 hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st)
 {

@@ -156,19 +156,27 @@ class DeviceEngine:
         check(lib().wf_time_steps(self._h, nsteps, byref(ms)))
         return ms.value
 
+    def _kernel_names(self) -> tuple:
+        if self.kernel_variant() == "twostep":
+            return ("k_twostep4", "frame_step1", "frame_step2")
+        return ("k_fused4", "k_h4", "k_e4")
+
     def time_kernels(self, nsteps: int) -> dict:
-        """Mean per-launch ms of each step kernel over nsteps full steps (HIP
+        """Mean ms per launch of each step kernel over nsteps full steps (HIP
         events on the engine's stream): fused interior kernel, H and E
-        kernels."""
+        kernels; with two steps per pass (variant "twostep"), the two-step
+        kernel per launch and the frame of a pass's first and second step
+        (H + E kernels each)."""
         f, h, e = c_double(), c_double(), c_double()
         check(lib().wf_time_kernels(self._h, nsteps, byref(f), byref(h), byref(e)))
-        return {"k_fused4": f.value, "k_h4": h.value, "k_e4": e.value}
+        return dict(zip(self._kernel_names(), (f.value, h.value, e.value)))
 
     def kernel_bytes(self) -> dict:
-        """Algorithmic HBM bytes per launch of each step kernel."""
+        """Algorithmic HBM bytes per launch of each step kernel (the same keys
+        as time_kernels)."""
         f, h, e = c_uint64(), c_uint64(), c_uint64()
         check(lib().wf_kernel_bytes(self._h, byref(f), byref(h), byref(e)))
-        return {"k_fused4": f.value, "k_h4": h.value, "k_e4": e.value}
+        return dict(zip(self._kernel_names(), (f.value, h.value, e.value)))
 
     def kernel_variant(self) -> str:
         return lib().wf_kernel_variant(self._h).decode()
