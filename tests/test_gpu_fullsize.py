@@ -67,9 +67,13 @@ def oracle_run():
     return w, f0, out
 
 
-@pytest.mark.parametrize("variant", ["fused", "split"])
+@pytest.mark.parametrize("variant", ["fused", "onestep", "split"])
 def test_random_1024_full_size_bit_exact(oracle_run, variant, monkeypatch):
-    monkeypatch.setenv("WF_KERNEL", variant)
+    """fused: two steps per pass (the two-step kernel and the frame through a
+    third state); onestep: the fused interior kernel and frame kernels once
+    per step; split: the in-place H and E kernels."""
+    monkeypatch.setenv("WF_KERNEL", "split" if variant == "split" else "fused")
+    monkeypatch.setenv("WF_TWOSTEP", "0" if variant == "onestep" else "1")
     w, f0, (of, op, (oc, os_, on)) = oracle_run
     _say(f"random_1024 full size: GPU {variant}, {STEPS} steps")
     with DeviceEngine(w.dims, w.ids, w.table, w.pml()) as eng:

@@ -18,20 +18,24 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
-@pytest.fixture(autouse=True, params=["fused", "split", "single"])
+@pytest.fixture(autouse=True, params=["fused", "onestep", "split", "single"])
 def kernel_variant(request, monkeypatch):
-    """Every case runs on each step-kernel scheme: the fused interior kernel
-    with the CPML shell in separate H and E frame kernels (the default), the
-    in-place H and E kernels over the whole domain, and the fused kernel
-    over the whole domain, shell included (opt-in)."""
+    """Every case runs on each step-kernel scheme: two steps per pass (the
+    two-step kernel over the interior, the frame's H and E kernels through a
+    third state; the default), one step per pass (the fused interior kernel
+    with the CPML shell in separate H and E frame kernels; what odd step
+    counts end with), the in-place H and E kernels over the whole domain,
+    and the fused kernel over the whole domain, shell included (opt-in)."""
     monkeypatch.setenv("WF_KERNEL", "split" if request.param == "split" else "fused")
     monkeypatch.setenv("WF_PML_FUSED", "1" if request.param == "single" else "0")
+    monkeypatch.setenv("WF_TWOSTEP", "0" if request.param == "onestep" else "1")
     return request.param
 
 
 def _say(msg):
     print(f"[gpu-case] {msg} kernel={os.environ.get('WF_KERNEL')} "
-          f"single_pass={os.environ.get('WF_PML_FUSED')}", flush=True)
+          f"single_pass={os.environ.get('WF_PML_FUSED')} "
+          f"two_step={os.environ.get('WF_TWOSTEP')}", flush=True)
 
 
 def _run_workload_gpu(w, steps):
@@ -297,6 +301,81 @@ def test_sources_on_interior_tile_boundaries():
     sc, ss, n = eng.probe_sums()
     oc, os_, on = sim.probe_sums()
     assert_bitwise(sc, oc, "sum_cos")
+    eng.close()
+
+
+def test_points_on_two_step_margins():
+    """Two steps per pass: the two-step kernel's tiles recompute a margin
+    (one quad on each side in x, two rows below and one above in y, two
+    planes below and one above each z slice) whose values feed the owned
+    cells, and the first step's sources are added and its probes recorded
+    inside the kernel; the box's boundary layers are also computed by the
+    frame's first step.  A 560 x 48 x 64 domain with CPML 8: two-step box
+    x in [12, 540) (tiles of 248 cells: inner boundaries at x = 260, 508),
+    y in [10, 39) (9-row tiles: boundaries at 19, 28, 37), z in [10, 55)
+    (slices of 2-3 planes).  Soft and hard sources of every E component and
+    probes of all six components on the cells around those boundaries, on
+    the box's boundary layers and just outside them; random fields first,
+    then 3 + 17 steps (a pass and a single step, graph replays, a trailing
+    single step): fields, psi, probe sums and traces against the oracle."""
+    from wavefront_amd.source import FieldComponent as SC, Source, Waveform
+    from wavefront_amd.probe import FieldComponent as PC, Probe
+    rng = np.random.default_rng(17)
+    d = GridDims(560, 48, 64)
+    dt, dx = 1.9065752e-12, 1e-3
+    forms = (Waveform.sinusoid(3e10), Waveform.ricker_wavelet(2e10, 5e-11),
+             Waveform.gaussian_pulse(2e-11, 1e-11))
+    xs = (11, 12, 15, 16, 255, 256, 259, 260, 263, 264, 503, 504, 507, 508, 511, 512, 535, 536,
+          539, 540)
+    ys = (9, 10, 11, 17, 18, 19, 20, 27, 28, 29, 36, 37, 38, 39)
+    zs = (9, 10, 11, 12, 13, 20, 33, 53, 54, 55)
+    srcs, probes = [], []
+    for k in range(160):
+        x, y, z = (int(rng.choice(xs)), int(rng.choice(ys)), int(rng.choice(zs)))
+        srcs.append(Source(x, y, z, (SC.Ex, SC.Ey, SC.Ez)[k % 3], float(rng.uniform(-2, 2)),
+                           forms[k % 3], hard=(k % 5 == 4)))
+        if k % 4 == 0:  # the same voxel named again, later in the list
+            srcs.append(Source(x, y, z, (SC.Ex, SC.Ey, SC.Ez)[k % 3], 0.5, forms[0]))
+    comps = (PC.Ex, PC.Ey, PC.Ez, PC.Hx, PC.Hy, PC.Hz)
+    for k in range(48):
+        x, y, z = (int(rng.choice(xs)), int(rng.choice(ys)), int(rng.choice(zs)))
+        probes.append(Probe(x, y, z, comps[k % 6], [2e10, 3.5e10], 0.0))
+    cfg = PmlConfig(thickness=8)
+    prof = [PmlProfile1D.build(n, cfg, dt, dx, 8) for n in (d.nx, d.ny, d.nz)]
+    pml = PmlContext(prof, (1, 1, 1))
+    table = MaterialTable.vacuum_filled(dt, dx)
+    otable = wfo.vacuum_table(dt, dx)
+    for m in range(1, 4):
+        eps, sig = float(rng.uniform(1, 10)), float(rng.uniform(0, 0.05))
+        table.set_material(m, eps, 1.0, sig, dt, dx)
+        otable[m] = wfo.material_coeffs(eps, 1.0, sig, dt, dx)
+    ids = rng.integers(0, 4, d.voxel_count(), dtype=np.uint8)
+    f0 = rng.standard_normal((d.block_count(), 6, 512)).astype(np.float32)
+    steps = 20
+    _say("points on two-step margins 560x48x64")
+    eng = DeviceEngine(d, ids, table, pml)
+    eng.write_fields(f0)
+    eng.set_sources(srcs, dt, steps)
+    eng.set_probes(probes, dt, steps, trace=True)
+    eng.step(3)
+    eng.step(17)
+    shell, profs = oracle_pml(d, cfg, dt, dx)
+    sim = wfo.OracleSim(d.nx, d.ny, d.nz, ids=ids, table=otable, shell_blocks=shell,
+                        profiles=profs)
+    for s_ in srcs:
+        sim.add_source(s_.x, s_.y, s_.z, int(s_.component), s_.amplitude, s_.waveform.kind,
+                       s_.waveform.p0, s_.waveform.p1, hard=s_.hard)
+    for p_ in probes:
+        sim.add_probe(p_.x, p_.y, p_.z, int(p_.component), p_.frequencies_hz, 0.0)
+    sim.write_fields(f0)
+    traces = sim.run(steps, dt, threads=4, trace=True)
+    assert_bitwise(eng.read_fields(), sim.read_fields(), "fields")
+    assert_bitwise(eng.read_psi(), sim.read_psi(), "psi")
+    sc, ss, n = eng.probe_sums()
+    oc, os_, on = sim.probe_sums()
+    assert_bitwise(sc, oc, "sum_cos")
+    assert_bitwise(ss, os_, "sum_sin")
+    assert_bitwise(eng.read_trace(steps), traces, "probe traces")
     eng.close()
 
 

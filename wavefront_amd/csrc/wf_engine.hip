@@ -253,6 +253,7 @@ struct wf_ctx {
     // tables of the kernel's first step
     bool pair_env = true;   // WF_TWOSTEP=0: single steps only
     bool pair_geo = false;  // plan_boxes: the geometry allows it
+    bool pair_overlap = false;  // WF_PAIR_OVERLAP=1: first step's frame on the side stream
     Box two{};
     BoxList fr1h{}, fr1e{}, fr2{};
     Ptrs third{};
@@ -877,17 +878,29 @@ int launch_pair(wf_ctx *c, hipEvent_t *ev = nullptr)
     pts.base = c->src_base;
     pts.step = c->d_step;
     pts.slot = c->d_pslot;
-    if (ev) WF_HIP(hipEventRecord(ev[0], st));
     StepLaunch s{c->g, c->p, c->alt, c->pml, c->fzchunk, st};
-    WF_HIP(c->lt(s, c->two, pts, st));
-    if (ev) {
-        WF_HIP(hipEventRecord(ev[1], st));
-        WF_HIP(hipEventRecord(ev[2], st));
-    }
     StepLaunch s1{c->g, c->p, c->third, c->pml, c->fzchunk, st};
-    WF_HIP(c->lh(s1, c->fr1h, st));
-    WF_HIP(c->le(s1, c->fr1e, st));
-    if (ev) WF_HIP(hipEventRecord(ev[3], st));
+    if (c->pair_overlap && !ev && c->side) {
+        // the first step's frame reads state A and writes C, the two-step
+        // kernel reads A and writes B: side by side
+        WF_HIP(hipEventRecord(c->fork, st));
+        WF_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
+        WF_HIP(c->lh(s1, c->fr1h, c->side));
+        WF_HIP(c->le(s1, c->fr1e, c->side));
+        WF_HIP(hipEventRecord(c->join, c->side));
+        WF_HIP(c->lt(s, c->two, pts, st));
+        WF_HIP(hipStreamWaitEvent(st, c->join, 0));
+    } else {
+        if (ev) WF_HIP(hipEventRecord(ev[0], st));
+        WF_HIP(c->lt(s, c->two, pts, st));
+        if (ev) {
+            WF_HIP(hipEventRecord(ev[1], st));
+            WF_HIP(hipEventRecord(ev[2], st));
+        }
+        WF_HIP(c->lh(s1, c->fr1h, st));
+        WF_HIP(c->le(s1, c->fr1e, st));
+        if (ev) WF_HIP(hipEventRecord(ev[3], st));
+    }
     if (int rc = point_work(c, c->third, st, true)) return rc;
     if (ev) WF_HIP(hipEventRecord(ev[4], st));
     StepLaunch s2{c->g, c->third, c->alt, c->pml, c->fzchunk, st};
@@ -1334,6 +1347,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
             c->lt = fma::launch_twostep;
         }
     if (const char *ts = std::getenv("WF_TWOSTEP")) c->pair_env = std::atoi(ts) != 0;
+    if (const char *po = std::getenv("WF_PAIR_OVERLAP")) c->pair_overlap = std::atoi(po) != 0;
     plan_boxes(c);
     // Third state for the frame of two-step passes (its first step's values).
     if (c->fused && c->pair_env && c->pair_geo) {
@@ -2059,6 +2073,7 @@ int wf_set_fp_contract(wf_ctx *c, int on)
     c->lh = on ? fma::launch_h : launch_h;
     c->le = on ? fma::launch_e : launch_e;
     c->lf = on ? fma::launch_fused : launch_fused;
+    c->lt = on ? fma::launch_twostep : launch_twostep;
     return WF_OK;
 }
 

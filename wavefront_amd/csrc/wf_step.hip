@@ -899,6 +899,20 @@ __device__ __forceinline__ bool point_row_flagged(const PointRows &t, uint64_t k
     return t.bits && ((t.bits[key >> 5] >> (key & 31)) & 1u);
 }
 
+// a*b + c for the field updates of the two-step kernel, a the field's own
+// coefficient and b its old value.  The contracted set fuses exactly this
+// product (the coefficient is 1 in lossless cells, where the fused result
+// then equals the separately rounded one); the exact set rounds the product
+// and the sum separately, and a*b + (-(d*e)) is a*b - d*e bit for bit.
+__device__ __forceinline__ f4 madd4(f4 a, f4 b, f4 c)
+{
+#ifdef WF_STEP_FMA
+    return __builtin_elementwise_fma(a, b, c);
+#else
+    return a * b + c;
+#endif
+}
+
 // Element k (0..3) of a quad, read or replaced without dynamic indexing.
 __device__ __forceinline__ float quad_get(const f4 &v, int k)
 {
@@ -1012,9 +1026,9 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
         {
             f4 da, db;
             coef4(s_ch, cur.ids, da, db);
-            h1x = da * cur.hx_o - db * ((cur.ez_py - ez_c) - (cur.ey_n - ey_c));
-            h1y = da * cur.hy_o - db * ((cur.ex_n - ex_c) - (ez_px - ez_c));
-            h1z = da * cur.hz_o - db * ((ey_px - ey_c) - (cur.ex_py - ex_c));
+            h1x = madd4(da, cur.hx_o, -(db * ((cur.ez_py - ez_c) - (cur.ey_n - ey_c))));
+            h1y = madd4(da, cur.hy_o, -(db * ((cur.ex_n - ex_c) - (ez_px - ez_c))));
+            h1z = madd4(da, cur.hz_o, -(db * ((ey_px - ey_c) - (cur.ex_py - ex_c))));
         }
         if (!live) {
             h1x = splat(0.f);
@@ -1034,9 +1048,9 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
             const f4 hy_mx = f4{hy_l, h1y.x, h1y.y, h1y.z};
             f4 ca, cb;
             coef4(s_ce, cur.ids, ca, cb);
-            e1x = ca * ex_c + cb * ((h1z - hz_my) - (h1y - h1y_c));
-            e1y = ca * ey_c + cb * ((h1x - h1x_c) - (h1z - hz_mx));
-            e1z = ca * ez_c + cb * ((h1y - hy_mx) - (h1x - hx_my));
+            e1x = madd4(ca, ex_c, cb * ((h1z - hz_my) - (h1y - h1y_c)));
+            e1y = madd4(ca, ey_c, cb * ((h1x - h1x_c) - (h1z - hz_mx)));
+            e1z = madd4(ca, ez_c, cb * ((h1y - hy_mx) - (h1x - hx_my)));
             if (live_row) {
                 const uint64_t key = (uint64_t)p * g.ny + y;
                 if (point_row_flagged(pts.src, key)) {
@@ -1086,9 +1100,9 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
             const f4 ey_px2 = f4{e1y_c.y, e1y_c.z, e1y_c.w, s2y};
             f4 da, db;
             coef4(s_ch, ids_c, da, db);
-            h2x = da * h1x_c - db * ((ez_py - e1z_c) - (e1y - e1y_c));
-            h2y = da * h1y_c - db * ((e1x - e1x_c) - (ez_px2 - e1z_c));
-            h2z = da * h1z_c - db * ((ey_px2 - e1y_c) - (ex_py - e1x_c));
+            h2x = madd4(da, h1x_c, -(db * ((ez_py - e1z_c) - (e1y - e1y_c))));
+            h2y = madd4(da, h1y_c, -(db * ((e1x - e1x_c) - (ez_px2 - e1z_c))));
+            h2z = madd4(da, h1z_c, -(db * ((ey_px2 - e1y_c) - (ex_py - e1x_c))));
         }
         s_h2[0][w][lane] = h2z;
         s_h2[1][w][lane] = h2x;
@@ -1102,9 +1116,9 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
             const f4 hy_mx = f4{h2y_l, h2y.x, h2y.y, h2y.z};
             f4 ca, cb;
             coef4(s_ce, ids_c, ca, cb);
-            const f4 e2x = ca * e1x_c + cb * ((h2z - hz_my) - (h2y - h2y_m));
-            const f4 e2y = ca * e1y_c + cb * ((h2x - h2x_m) - (h2z - hz_mx));
-            const f4 e2z = ca * e1z_c + cb * ((h2y - hy_mx) - (h2x - hx_my));
+            const f4 e2x = madd4(ca, e1x_c, cb * ((h2z - hz_my) - (h2y - h2y_m)));
+            const f4 e2y = madd4(ca, e1y_c, cb * ((h2x - h2x_m) - (h2z - hz_mx)));
+            const f4 e2z = madd4(ca, e1z_c, cb * ((h2y - hy_mx) - (h2x - hx_my)));
             if (own) {
                 const size_t o = (size_t)(p - 1) * plane + row;
                 st4(b.f[0] + o, e2x);
