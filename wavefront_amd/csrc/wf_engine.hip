@@ -253,8 +253,10 @@ struct wf_ctx {
     // tables of the kernel's first step
     bool pair_env = true;   // WF_TWOSTEP=0: single steps only
     bool pair_geo = false;  // plan_boxes: the geometry allows it
+    bool pair_xfold = false;  // the x CPML columns are two-step tiles too (box twox)
+    bool pair_xfold_env = true;  // WF_TWOSTEP_XFOLD=0: the frame keeps the x slabs
     bool pair_overlap = false;  // WF_PAIR_OVERLAP=1: first step's frame on the side stream
-    Box two{};
+    Box two{}, twox{};
     BoxList fr1h{}, fr1e{}, fr2{};
     Ptrs third{};
     float *d_fields3 = nullptr, *d_psi3 = nullptr;
@@ -339,7 +341,8 @@ struct wf_ctx {
     hipError_t (*lh)(const StepLaunch &, const BoxList &, hipStream_t) = launch_h;
     hipError_t (*le)(const StepLaunch &, const BoxList &, hipStream_t) = launch_e;
     hipError_t (*lf)(const StepLaunch &, const Box &, bool, hipStream_t, bool) = launch_fused;
-    hipError_t (*lt)(const StepLaunch &, const Box &, const TwoStepPoints &, hipStream_t) = launch_twostep;
+    hipError_t (*lt)(const StepLaunch &, const Box &, const TwoStepPoints &, hipStream_t, bool) =
+        launch_twostep;
 };
 
 namespace {
@@ -603,6 +606,7 @@ void plan_boxes(wf_ctx *c)
         }
         if (th[0] - tl[0] >= 8 && th[1] - tl[1] >= 2 && th[2] - tl[2] >= 2) {
             Box tb{tl[0], th[0], tl[1], th[1], tl[2], th[2], 64, 0, 0, 0, 0};
+            tb.tstride = 4 * kTwoQuads;  // adjacent tile columns of 62 owned quads
             const int r2 = twostep_rows_per_tile();
             tb.gx = ((th[0] - tl[0]) / 4 + kTwoQuads - 1) / kTwoQuads;
             tb.gy = (th[1] - tl[1] + r2 - 1) / r2;
@@ -620,9 +624,30 @@ void plan_boxes(wf_ctx *c)
                 add_box(f, split_box(x1, n[0], y0, y1, z0, z1, fz));
                 return f;
             };
-            c->fr1h = frame_around(tl[0] + 4, th[0] - 4, tl[1] + 1, th[1] - 1, tl[2] + 1, th[2] - 1);
-            c->fr1e = frame_around(tl[0] + 4, th[0], tl[1] + 1, th[1], tl[2] + 1, th[2]);
-            c->fr2 = frame_around(tl[0], th[0], tl[1], th[1], tl[2], th[2]);
+            // x fold: the x CPML layers and the margin beside them, [0, T+4)
+            // and [nx-T-4, nx), become two-step tiles too (x-only CPML
+            // variant, rows of T/4 + 3 lanes), so the frame keeps only the
+            // y and z slabs and the two-step box spans x on its y, z range
+            const int W = g.T[0] + 4, lprx = g.T[0] / 4 + 3;
+            c->pair_xfold = c->pair_xfold_env && g.T[0] > 0 && lprx <= 32 && n[0] >= 2 * W + 8;
+            int fx0 = tl[0], fx1 = th[0];
+            if (c->pair_xfold) {
+                Box xb{0, n[0], tl[1], th[1], tl[2], th[2], lprx, 2, 0, 0, 0};
+                xb.tstride = n[0] - W;
+                const int rx = twostep_rows_per_tile_x(lprx);
+                xb.gy = (th[1] - tl[1] + rx - 1) / rx;
+                xb.gz = c->zchunk_env ? (th[2] - tl[2] + zc - 1) / zc
+                                      : fused_slices(2l * xb.gy, th[2] - tl[2], c->cus, 3);
+                xb.gz = std::min(xb.gz, th[2] - tl[2]);
+                c->twox = xb;
+                fx0 = 0;
+                fx1 = n[0];
+            }
+            c->fr1h = frame_around(c->pair_xfold ? 0 : tl[0] + 4, c->pair_xfold ? n[0] : th[0] - 4,
+                                   tl[1] + 1, th[1] - 1, tl[2] + 1, th[2] - 1);
+            c->fr1e = frame_around(c->pair_xfold ? 0 : tl[0] + 4, fx1, tl[1] + 1, th[1], tl[2] + 1,
+                                   th[2]);
+            c->fr2 = frame_around(fx0, fx1, tl[1], th[1], tl[2], th[2]);
             c->pair_geo = true;
         }
     }
@@ -888,11 +913,13 @@ int launch_pair(wf_ctx *c, hipEvent_t *ev = nullptr)
         WF_HIP(c->lh(s1, c->fr1h, c->side));
         WF_HIP(c->le(s1, c->fr1e, c->side));
         WF_HIP(hipEventRecord(c->join, c->side));
-        WF_HIP(c->lt(s, c->two, pts, st));
+        WF_HIP(c->lt(s, c->two, pts, st, false));
+        if (c->pair_xfold) WF_HIP(c->lt(s, c->twox, pts, st, true));
         WF_HIP(hipStreamWaitEvent(st, c->join, 0));
     } else {
         if (ev) WF_HIP(hipEventRecord(ev[0], st));
-        WF_HIP(c->lt(s, c->two, pts, st));
+        WF_HIP(c->lt(s, c->two, pts, st, false));
+        if (c->pair_xfold) WF_HIP(c->lt(s, c->twox, pts, st, true));
         if (ev) {
             WF_HIP(hipEventRecord(ev[1], st));
             WF_HIP(hipEventRecord(ev[2], st));
@@ -1347,6 +1374,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
             c->lt = fma::launch_twostep;
         }
     if (const char *ts = std::getenv("WF_TWOSTEP")) c->pair_env = std::atoi(ts) != 0;
+    if (const char *tx = std::getenv("WF_TWOSTEP_XFOLD")) c->pair_xfold_env = std::atoi(tx) != 0;
     if (const char *po = std::getenv("WF_PAIR_OVERLAP")) c->pair_overlap = std::atoi(po) != 0;
     plan_boxes(c);
     // Third state for the frame of two-step passes (its first step's values).
@@ -1557,7 +1585,9 @@ int wf_set_probes(wf_ctx *c, uint32_t n, const wf_probe_site *sites, uint64_t ns
         // first step into one slot per probe
         std::vector<std::pair<uint64_t, PointEnt>> v(n);
         std::vector<uint8_t> in(n);
+        // the two-step tiles own the box (and, x folded, every x of its rows)
         const Box &t = c->two;
+        const int ox0 = c->pair_xfold ? 0 : t.x0, ox1 = c->pair_xfold ? c->g.nx : t.x1;
         for (uint32_t p = 0; p < n; ++p) {
             const wf_probe_site &q = sites[p];
             PointEnt e{};
@@ -1565,7 +1595,7 @@ int wf_set_probes(wf_ctx *c, uint32_t n, const wf_probe_site *sites, uint64_t ns
             e.comp = (uint16_t)q.component;
             e.col = p;
             v[p] = {(uint64_t)q.z * c->g.ny + q.y, e};
-            in[p] = (int)q.x >= t.x0 && (int)q.x < t.x1 && (int)q.y >= t.y0 && (int)q.y < t.y1 &&
+            in[p] = (int)q.x >= ox0 && (int)q.x < ox1 && (int)q.y >= t.y0 && (int)q.y < t.y1 &&
                     (int)q.z >= t.z0 && (int)q.z < t.z1;
         }
         if (int rc = upload_points(c, v, c->prb_pts)) return rc;
@@ -1932,10 +1962,15 @@ int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
         // two-step kernel: 49 B per cell of its box for two steps; the
         // frame of a pass's first step (H over fr1h, E over fr1e) and of its
         // second (fr2): 37 B per cell per kernel and 8 B per psi term
+        // (x folded: the x CPML columns too, with 32 B per graded cell for
+        // its four x psi terms, read once and written once per pass)
         const Box &t = c->two;
-        *fused_bytes = 49ull * (uint64_t)(t.x1 - t.x0) * (t.y1 - t.y0) * (t.z1 - t.z0);
-        *h_bytes = 37ull * (box_cells(c->fr1h) + box_cells(c->fr1e)) + 16ull * terms;
-        *e_bytes = 37ull * 2 * box_cells(c->fr2) + 16ull * terms;
+        const uint64_t yz = (uint64_t)(t.y1 - t.y0) * (t.z1 - t.z0);
+        const uint64_t xcols = c->pair_xfold ? 2ull * (c->g.T[0] + 4) : 0;
+        const uint64_t xterms = c->pair_xfold ? 2ull * c->g.T[0] * yz * 2 : 0;  // per phase
+        *fused_bytes = 49ull * ((uint64_t)(t.x1 - t.x0) + xcols) * yz + 16ull * xterms;
+        *h_bytes = 37ull * (box_cells(c->fr1h) + box_cells(c->fr1e)) + 16ull * (terms - xterms);
+        *e_bytes = 37ull * 2 * box_cells(c->fr2) + 16ull * (terms - xterms);
         return WF_OK;
     }
     if (single_pass(c)) {

@@ -929,33 +929,60 @@ __device__ __forceinline__ void quad_source(f4 &v, int k, float val, bool hard)
     v.w = k == 3 ? nv : v.w;
 }
 
-__global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Box box,
-                                                       TwoStepPoints pts)
+// XPML: tiles over the x CPML layers (the box's y and z ranges lie outside
+// the shell): rows box.lpr lanes wide, several rows per wave, the x psi terms
+// of graded quads carried from the first step to the second in registers,
+// and lanes outside the domain (x < 0 or x >= nx) held at zero, the
+// boundary the reference reads there.
+constexpr int kF2x = 8;  // waves of the x CPML variant (its psi terms need the registers)
+template <bool XPML>
+constexpr int twostep_waves() { return XPML ? kF2x : kF2; }
+
+template <bool XPML>
+__global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, Ptrs a, Ptrs b,
+                                                                         Box box,
+                                                                         TwoStepPoints pts)
 {
+    constexpr int FY = twostep_waves<XPML>();
+    constexpr int NQ = XPML ? 4 : 1;     // x psi terms per quad (ez_dx, ey_dx, hz_dx, hy_dx)
     __shared__ float2 s_ce[256];         // (ca, cb)
     __shared__ float2 s_ch[256];         // (da, db)
-    __shared__ f4 s_h1[2][kF2][64];      // H(n+1/2) of plane p: [hz, hx][wave][lane]
-    __shared__ f4 s_e1[2][2][kF2][64];   // E(n+1): [plane parity][ez, ex][wave][lane]
-    __shared__ f4 s_h2[2][kF2][64];      // H(n+3/2) of plane p-1: [hz, hx][wave][lane]
+    __shared__ f4 s_h1[2][FY * 64];      // H(n+1/2) of plane p: [hz, hx][slot]
+    __shared__ f4 s_e1[2][2][FY * 64];   // E(n+1): [plane parity][ez, ex][slot]
+    __shared__ f4 s_h2[2][FY * 64];      // H(n+3/2) of plane p-1: [hz, hx][slot]
     const int lane = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    for (int i = w * 64 + lane; i < 256; i += 64 * kF2) {
+    for (int i = w * 64 + lane; i < 256; i += 64 * FY) {
         const float4 c = a.coef[i];
         s_ce[i] = make_float2(c.x, c.y);
         s_ch[i] = make_float2(c.z, c.w);
     }
     __syncthreads();
 
+    // lanes per row, rows per wave; row R of the tile = w * rpw + r (lanes
+    // past rpw * LPR idle); LDS slot of each lane and of its y-1 / y+1 rows
+    const int LPR = XPML ? box.lpr : 64;
+    const int rpw = XPML ? 64 / LPR : 1;
+    const int r = XPML ? lane / LPR : 0, l = XPML ? lane - r * LPR : lane;
+    const bool idle = r >= rpw;
+    const int RT = FY * rpw;
+    const int R = idle ? -8 : w * rpw + r;
+    const int me = w * 64 + lane;
+    const int below = (idle || R == 0) ? me : (r > 0 ? me - LPR : (w - 1) * 64 + (rpw - 1) * LPR + l);
+    const int above = (idle || R == RT - 1) ? me : (r < rpw - 1 ? me + LPR : (w + 1) * 64 + l);
+
     const int t = xcd_tile(blockIdx.x, box.gx * box.gy * box.gz);
     const int tx = t % box.gx, ty = (t / box.gx) % box.gy, tz = t / (box.gx * box.gy);
-    const int x = box.x0 - 4 + 4 * (kTwoQuads * tx + lane);
-    const int y = box.y0 + (kF2 - 3) * ty - 2 + w;
-    // x1 and y1 themselves hold the upper margin; beyond them lanes and
-    // rows are idle (no loads)
-    const bool live_row = y <= box.y1;
-    const bool live = live_row && x <= box.x1;
-    const bool own_row = live_row && y < box.y1 && w >= 2 && w <= kF2 - 2;
-    const bool own = own_row && live && lane >= 1 && lane <= kTwoQuads && x < box.x1;
+    const int tx0 = box.x0 + tx * box.tstride;             // the tile's owned cells: [tx0, tx1)
+    const int tx1 = min(tx0 + 4 * (LPR - 2), box.x1);
+    const int x = tx0 - 4 + 4 * l;
+    const int y = box.y0 + (RT - 3) * ty - 2 + R;
+    // tx1 and y1 themselves hold the upper margin; beyond them, outside the
+    // domain and on idle lanes nothing is loaded and every value is zero
+    const bool live_row = !idle && y <= box.y1;
+    const bool live = live_row && x >= 0 && x + 4 <= g.nx && x <= tx1;
+    const bool own_row = live_row && y < box.y1 && R >= 2 && R <= RT - 2;
+    const bool own = own_row && live && x >= tx0 && x < tx1;
     const size_t nx = g.nx, plane = nx * g.ny;
     const size_t row = (size_t)(live ? y : 0) * nx + (live ? x : 0);
     const size_t rowp = row + nx;  // E^n at y+1
@@ -968,6 +995,24 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
     const float *sv = nullptr;
     if (pts.src.bits) sv = pts.values + (*pts.step - pts.base) * (uint64_t)pts.ncol;
 
+    // x CPML of this lane's quad: in the shell (identity derivatives + 0.0f)
+    // and graded (the one-tap recursion; the x shell is the graded layer, a
+    // quad lies in one block); psi index within a plane's x slab
+    bool sh = false, gxq = false;
+    size_t gxi = 0;
+    const size_t psx = (size_t)g.ny * 2 * (size_t)g.T[0];
+    f4 pbx = splat(1.f), pax = splat(0.f), pkx = splat(1.f);
+    if (XPML && live) {
+        sh = in_shell_axis(g, 0, x);
+        gxq = graded(g, 0, x, g.nx);
+        if (gxq) {
+            gxi = (size_t)y * 2 * (size_t)g.T[0] + (size_t)slab_index(g, 0, x, g.nx);
+            pbx = ld4(a.prof[0][0] + x);
+            pax = ld4(a.prof[0][1] + x);
+            pkx = ld4(a.prof[0][2] + x);
+        }
+    }
+
     const float *__restrict__ aex = a.f[0];
     const float *__restrict__ aey = a.f[1];
     const float *__restrict__ aez = a.f[2];
@@ -975,15 +1020,17 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
     const float *__restrict__ ahy = a.f[4];
     const float *__restrict__ ahz = a.f[5];
 
-    // State A at plane p: E at p+1 and at y+1, H, the material ids; loaded
-    // a plane ahead of its use.
+    // State A at plane p: E at p+1 and at y+1, H, the material ids, the x
+    // psi terms; loaded a plane ahead of its use.
     struct Plane {
         f4 ex_n, ey_n, ez_n, ez_py, ex_py, hx_o, hy_o, hz_o;
         uint32_t ids;
+        f4 ps[NQ];
     };
     auto load = [&](Plane &P, int p) {
         P.ex_n = P.ey_n = P.ez_n = P.ez_py = P.ex_py = P.hx_o = P.hy_o = P.hz_o = splat(0.f);
         P.ids = 0;
+        for (int k = 0; k < NQ; ++k) P.ps[k] = splat(0.f);
         if (live) {
             const size_t zo = (size_t)p * plane;
             const size_t on = zo + plane + row;
@@ -993,9 +1040,11 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
             P.ez_py = ld4(aez + zo + rowp);
             P.ex_py = ld4(aex + zo + rowp);
             P.hx_o = ld4(ahx + zo + row);
-            if (w >= 1) P.hy_o = ld4(ahy + zo + row);  // row y0-2 needs H(n+1/2) hz, hx only
+            if (R >= 1) P.hy_o = ld4(ahy + zo + row);  // row y0-2 needs H(n+1/2) hz, hx only
             P.hz_o = ld4(ahz + zo + row);
             P.ids = *reinterpret_cast<const uint32_t *>(a.id + zo + row);
+            if (XPML && gxq)
+                for (int k = 0; k < NQ; ++k) P.ps[k] = ld4(a.psi[0][k] + (size_t)p * psx + gxi);
         }
     };
 
@@ -1009,6 +1058,8 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
     f4 h1x_c = splat(0.f), h1y_c = splat(0.f), h1z_c = splat(0.f);  // H(n+1/2) at p-1
     f4 e1x_c = splat(0.f), e1y_c = splat(0.f), e1z_c = splat(0.f);  // E(n+1) at p-1
     f4 h2x_m = splat(0.f), h2y_m = splat(0.f);                      // H(n+3/2) at p-2
+    f4 qc[NQ];                                                      // x psi(n+1) at p-1
+    for (int k = 0; k < NQ; ++k) qc[k] = splat(0.f);
     uint32_t ids_c = 0;                                             // ids at p-1
     Plane cur;
     load(cur, z0 - 2);
@@ -1017,8 +1068,10 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
         const int p = z0 + it;
         Plane nxt;
         if (it < nit) load(nxt, p + 1);
+        f4 q1[NQ];  // x psi(n+1) at p
+        for (int k = 0; k < NQ; ++k) q1[k] = cur.ps[k];
 
-        // ---- step 1: H(n+1/2) at plane p (update_h_field) ---------------
+        // ---- step 1: H(n+1/2) at plane p (update_h_field[_pml]) ---------
         const float sez = __shfl_down(ez_c.x, 1, 64), sey = __shfl_down(ey_c.x, 1, 64);
         const f4 ez_px = f4{ez_c.y, ez_c.z, ez_c.w, sez};
         const f4 ey_px = f4{ey_c.y, ey_c.z, ey_c.w, sey};
@@ -1026,31 +1079,64 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
         {
             f4 da, db;
             coef4(s_ch, cur.ids, da, db);
-            h1x = madd4(da, cur.hx_o, -(db * ((cur.ez_py - ez_c) - (cur.ey_n - ey_c))));
-            h1y = madd4(da, cur.hy_o, -(db * ((cur.ex_n - ex_c) - (ez_px - ez_c))));
-            h1z = madd4(da, cur.hz_o, -(db * ((ey_px - ey_c) - (cur.ex_py - ex_c))));
+            f4 d_ez_dy = cur.ez_py - ez_c, d_ey_dz = cur.ey_n - ey_c;
+            f4 d_ex_dz = cur.ex_n - ex_c, d_ez_dx = ez_px - ez_c;
+            f4 d_ey_dx = ey_px - ey_c, d_ex_dy = cur.ex_py - ex_c;
+            if (XPML && sh) {
+                d_ez_dy = d_ez_dy + 0.0f;
+                d_ey_dz = d_ey_dz + 0.0f;
+                d_ex_dz = d_ex_dz + 0.0f;
+                d_ex_dy = d_ex_dy + 0.0f;
+                if (gxq) {
+                    d_ez_dx = cpml4(d_ez_dx, pbx, pax, pkx, q1[0]);
+                    d_ey_dx = cpml4(d_ey_dx, pbx, pax, pkx, q1[XPML ? 1 : 0]);
+                } else {
+                    d_ez_dx = d_ez_dx + 0.0f;
+                    d_ey_dx = d_ey_dx + 0.0f;
+                }
+            }
+            h1x = madd4(da, cur.hx_o, -(db * (d_ez_dy - d_ey_dz)));
+            h1y = madd4(da, cur.hy_o, -(db * (d_ex_dz - d_ez_dx)));
+            h1z = madd4(da, cur.hz_o, -(db * (d_ey_dx - d_ex_dy)));
         }
         if (!live) {
             h1x = splat(0.f);
             h1y = splat(0.f);
             h1z = splat(0.f);
         }
-        s_h1[0][w][lane] = h1z;
-        s_h1[1][w][lane] = h1x;
+        s_h1[0][me] = h1z;
+        s_h1[1][me] = h1x;
         lds_barrier();
 
-        // ---- step 1: E(n+1) at plane p (update_e_field), sources, probes -
+        // ---- step 1: E(n+1) at plane p (update_e_field[_pml]), sources,
+        // probes ------------------------------------------------------------
         f4 e1x = splat(0.f), e1y = splat(0.f), e1z = splat(0.f);
         const float hz_l = __shfl_up(h1z.w, 1, 64), hy_l = __shfl_up(h1y.w, 1, 64);
-        if (w >= 1 && it >= -1) {
-            const f4 hz_my = s_h1[0][w - 1][lane], hx_my = s_h1[1][w - 1][lane];
+        if (R >= 1 && it >= -1) {
+            const f4 hz_my = s_h1[0][below], hx_my = s_h1[1][below];
             const f4 hz_mx = f4{hz_l, h1z.x, h1z.y, h1z.z};
             const f4 hy_mx = f4{hy_l, h1y.x, h1y.y, h1y.z};
             f4 ca, cb;
             coef4(s_ce, cur.ids, ca, cb);
-            e1x = madd4(ca, ex_c, cb * ((h1z - hz_my) - (h1y - h1y_c)));
-            e1y = madd4(ca, ey_c, cb * ((h1x - h1x_c) - (h1z - hz_mx)));
-            e1z = madd4(ca, ez_c, cb * ((h1y - hy_mx) - (h1x - hx_my)));
+            f4 d_hz_dy = h1z - hz_my, d_hy_dz = h1y - h1y_c;
+            f4 d_hx_dz = h1x - h1x_c, d_hz_dx = h1z - hz_mx;
+            f4 d_hy_dx = h1y - hy_mx, d_hx_dy = h1x - hx_my;
+            if (XPML && sh) {
+                d_hz_dy = d_hz_dy + 0.0f;
+                d_hy_dz = d_hy_dz + 0.0f;
+                d_hx_dz = d_hx_dz + 0.0f;
+                d_hx_dy = d_hx_dy + 0.0f;
+                if (gxq) {
+                    d_hz_dx = cpml4(d_hz_dx, pbx, pax, pkx, q1[XPML ? 2 : 0]);
+                    d_hy_dx = cpml4(d_hy_dx, pbx, pax, pkx, q1[XPML ? 3 : 0]);
+                } else {
+                    d_hz_dx = d_hz_dx + 0.0f;
+                    d_hy_dx = d_hy_dx + 0.0f;
+                }
+            }
+            e1x = madd4(ca, ex_c, cb * (d_hz_dy - d_hy_dz));
+            e1y = madd4(ca, ey_c, cb * (d_hx_dz - d_hz_dx));
+            e1z = madd4(ca, ez_c, cb * (d_hy_dx - d_hx_dy));
             if (live_row) {
                 const uint64_t key = (uint64_t)p * g.ny + y;
                 if (point_row_flagged(pts.src, key)) {
@@ -1087,38 +1173,80 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
                 }
             }
         }
-        s_e1[it & 1][0][w][lane] = e1z;
-        s_e1[it & 1][1][w][lane] = e1x;
+        if (!live) {
+            e1x = splat(0.f);
+            e1y = splat(0.f);
+            e1z = splat(0.f);
+        }
+        s_e1[it & 1][0][me] = e1z;
+        s_e1[it & 1][1][me] = e1x;
 
         // ---- step 2: H(n+3/2) at plane p-1 ------------------------------
         f4 h2x = splat(0.f), h2y = splat(0.f), h2z = splat(0.f);
         const float s2z = __shfl_down(e1z_c.x, 1, 64), s2y = __shfl_down(e1y_c.x, 1, 64);
-        if (it >= 0 && w >= 1 && w <= kF2 - 2) {
+        if (it >= 0 && R >= 1 && R <= RT - 2) {
             const int pb = (it - 1) & 1;
-            const f4 ez_py = s_e1[pb][0][w + 1][lane], ex_py = s_e1[pb][1][w + 1][lane];
+            const f4 ez_py = s_e1[pb][0][above], ex_py = s_e1[pb][1][above];
             const f4 ez_px2 = f4{e1z_c.y, e1z_c.z, e1z_c.w, s2z};
             const f4 ey_px2 = f4{e1y_c.y, e1y_c.z, e1y_c.w, s2y};
             f4 da, db;
             coef4(s_ch, ids_c, da, db);
-            h2x = madd4(da, h1x_c, -(db * ((ez_py - e1z_c) - (e1y - e1y_c))));
-            h2y = madd4(da, h1y_c, -(db * ((e1x - e1x_c) - (ez_px2 - e1z_c))));
-            h2z = madd4(da, h1z_c, -(db * ((ey_px2 - e1y_c) - (ex_py - e1x_c))));
+            f4 d_ez_dy = ez_py - e1z_c, d_ey_dz = e1y - e1y_c;
+            f4 d_ex_dz = e1x - e1x_c, d_ez_dx = ez_px2 - e1z_c;
+            f4 d_ey_dx = ey_px2 - e1y_c, d_ex_dy = ex_py - e1x_c;
+            if (XPML && sh) {
+                d_ez_dy = d_ez_dy + 0.0f;
+                d_ey_dz = d_ey_dz + 0.0f;
+                d_ex_dz = d_ex_dz + 0.0f;
+                d_ex_dy = d_ex_dy + 0.0f;
+                if (gxq) {
+                    d_ez_dx = cpml4(d_ez_dx, pbx, pax, pkx, qc[0]);
+                    d_ey_dx = cpml4(d_ey_dx, pbx, pax, pkx, qc[XPML ? 1 : 0]);
+                } else {
+                    d_ez_dx = d_ez_dx + 0.0f;
+                    d_ey_dx = d_ey_dx + 0.0f;
+                }
+            }
+            h2x = madd4(da, h1x_c, -(db * (d_ez_dy - d_ey_dz)));
+            h2y = madd4(da, h1y_c, -(db * (d_ex_dz - d_ez_dx)));
+            h2z = madd4(da, h1z_c, -(db * (d_ey_dx - d_ex_dy)));
         }
-        s_h2[0][w][lane] = h2z;
-        s_h2[1][w][lane] = h2x;
+        if (!live) {
+            h2x = splat(0.f);
+            h2y = splat(0.f);
+            h2z = splat(0.f);
+        }
+        s_h2[0][me] = h2z;
+        s_h2[1][me] = h2x;
         lds_barrier();
 
         // ---- step 2: E(n+2) at plane p-1, stores of the owned cells -----
         const float h2z_l = __shfl_up(h2z.w, 1, 64), h2y_l = __shfl_up(h2y.w, 1, 64);
-        if (it >= 1 && w >= 2 && w <= kF2 - 2) {
-            const f4 hz_my = s_h2[0][w - 1][lane], hx_my = s_h2[1][w - 1][lane];
+        if (it >= 1 && R >= 2 && R <= RT - 2) {
+            const f4 hz_my = s_h2[0][below], hx_my = s_h2[1][below];
             const f4 hz_mx = f4{h2z_l, h2z.x, h2z.y, h2z.z};
             const f4 hy_mx = f4{h2y_l, h2y.x, h2y.y, h2y.z};
             f4 ca, cb;
             coef4(s_ce, ids_c, ca, cb);
-            const f4 e2x = madd4(ca, e1x_c, cb * ((h2z - hz_my) - (h2y - h2y_m)));
-            const f4 e2y = madd4(ca, e1y_c, cb * ((h2x - h2x_m) - (h2z - hz_mx)));
-            const f4 e2z = madd4(ca, e1z_c, cb * ((h2y - hy_mx) - (h2x - hx_my)));
+            f4 d_hz_dy = h2z - hz_my, d_hy_dz = h2y - h2y_m;
+            f4 d_hx_dz = h2x - h2x_m, d_hz_dx = h2z - hz_mx;
+            f4 d_hy_dx = h2y - hy_mx, d_hx_dy = h2x - hx_my;
+            if (XPML && sh) {
+                d_hz_dy = d_hz_dy + 0.0f;
+                d_hy_dz = d_hy_dz + 0.0f;
+                d_hx_dz = d_hx_dz + 0.0f;
+                d_hx_dy = d_hx_dy + 0.0f;
+                if (gxq) {
+                    d_hz_dx = cpml4(d_hz_dx, pbx, pax, pkx, qc[XPML ? 2 : 0]);
+                    d_hy_dx = cpml4(d_hy_dx, pbx, pax, pkx, qc[XPML ? 3 : 0]);
+                } else {
+                    d_hz_dx = d_hz_dx + 0.0f;
+                    d_hy_dx = d_hy_dx + 0.0f;
+                }
+            }
+            const f4 e2x = madd4(ca, e1x_c, cb * (d_hz_dy - d_hy_dz));
+            const f4 e2y = madd4(ca, e1y_c, cb * (d_hx_dz - d_hz_dx));
+            const f4 e2z = madd4(ca, e1z_c, cb * (d_hy_dx - d_hx_dy));
             if (own) {
                 const size_t o = (size_t)(p - 1) * plane + row;
                 st4(b.f[0] + o, e2x);
@@ -1127,6 +1255,8 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
                 st4(b.f[3] + o, h2x);
                 st4(b.f[4] + o, h2y);
                 st4(b.f[5] + o, h2z);
+                if (XPML && gxq)
+                    for (int k = 0; k < NQ; ++k) st4(b.psi[0][k] + (size_t)(p - 1) * psx + gxi, qc[k]);
             }
         }
 
@@ -1138,6 +1268,7 @@ __global__ __launch_bounds__(64 * kF2) void k_twostep4(Geo g, Ptrs a, Ptrs b, Bo
         h1x_c = h1x;
         h1y_c = h1y;
         h1z_c = h1z;
+        for (int k = 0; k < NQ; ++k) qc[k] = q1[k];
         ids_c = cur.ids;
         ex_c = cur.ex_n;
         ey_c = cur.ey_n;
@@ -1153,15 +1284,22 @@ int fused_rows_per_tile(bool pml_or_ghost) { return (pml_or_ghost ? kFYx : kFY) 
 int fused_rows_per_tile_x() { return WF_FUSED_XPML_FY - 1; }
 int split_rows_per_wg() { return kSY; }
 int twostep_rows_per_tile() { return kF2 - 3; }
+int twostep_rows_per_tile_x(int lpr) { return kF2x * (64 / lpr) - 3; }
 #endif
 
 hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoints &pts,
-                          hipStream_t st)
+                          hipStream_t st, bool x_pml)
 {
     const int n = box.gx * box.gy * box.gz;
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_twostep4, dim3((unsigned)n), dim3(64, kF2), 0, st, s.g, s.cur, s.alt, box,
-                       pts);
+    if (x_pml) {
+        if (box.lpr < 3 || box.lpr > 32) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_twostep4<true>), dim3((unsigned)n), dim3(64, kF2x), 0, st, s.g, s.cur,
+                           s.alt, box, pts);
+    } else {
+        hipLaunchKernelGGL((k_twostep4<false>), dim3((unsigned)n), dim3(64, kF2), 0, st, s.g, s.cur,
+                           s.alt, box, pts);
+    }
     return hipGetLastError();
 }
 
