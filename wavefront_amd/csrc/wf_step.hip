@@ -876,7 +876,7 @@ __global__ __launch_bounds__((64 * fused_waves<PM, GHOST>())) void k_fused4(Geo 
 // update is the single-pass kernel's, so the bits are the same.
 // ===========================================================================
 #ifndef WF_TWOSTEP_FY
-#define WF_TWOSTEP_FY 12
+#define WF_TWOSTEP_FY 16
 #endif
 constexpr int kF2 = WF_TWOSTEP_FY;
 
@@ -935,6 +935,24 @@ __device__ __forceinline__ void quad_source(f4 &v, int k, float val, bool hard)
 // and lanes outside the domain (x < 0 or x >= nx) held at zero, the
 // boundary the reference reads there.
 constexpr int kF2x = 8;  // waves of the x CPML variant (its psi terms need the registers)
+// Planes loaded ahead of the one being computed (0, 1 or 2), and which of the
+// previous plane's components that also cross waves through LDS are read
+// back from there (own slot) instead of being carried in registers (0 none,
+// 1 E(n+1), H(n+1/2) and H(n+3/2), 2 E(n+1) only).  On MI355X at 1024^3 the
+// fastest is no load ahead with 16 waves per workgroup (127 VGPRs, four waves
+// per SIMD) and E(n+1) read back: 7.53-7.65 ms per step against 7.85-7.91
+// for one plane ahead with 12 waves (167 VGPRs, three per SIMD); one plane
+// ahead with everything read back gained nothing at 12 waves, and two planes
+// ahead spill.
+#ifndef WF_TWOSTEP_PF
+#define WF_TWOSTEP_PF 0
+#endif
+#ifndef WF_TWOSTEP_LDSC
+#define WF_TWOSTEP_LDSC 2
+#endif
+constexpr int kPF2 = WF_TWOSTEP_PF;
+constexpr bool kLdsCarry = WF_TWOSTEP_LDSC == 1;   // E(n+1), H(n+1/2) and H(n+3/2) components
+constexpr bool kLdsCarryE = WF_TWOSTEP_LDSC != 0;  // E(n+1) components only (2)
 template <bool XPML>
 constexpr int twostep_waves() { return XPML ? kF2x : kF2; }
 
@@ -947,9 +965,10 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
     constexpr int NQ = XPML ? 4 : 1;     // x psi terms per quad (ez_dx, ey_dx, hz_dx, hy_dx)
     __shared__ float2 s_ce[256];         // (ca, cb)
     __shared__ float2 s_ch[256];         // (da, db)
-    __shared__ f4 s_h1[2][FY * 64];      // H(n+1/2) of plane p: [hz, hx][slot]
+    constexpr int NB = kLdsCarry ? 2 : 1;  // plane-parity buffers of s_h1 / s_h2
+    __shared__ f4 s_h1[NB][2][FY * 64];  // H(n+1/2) of plane p: [parity][hz, hx][slot]
     __shared__ f4 s_e1[2][2][FY * 64];   // E(n+1): [plane parity][ez, ex][slot]
-    __shared__ f4 s_h2[2][FY * 64];      // H(n+3/2) of plane p-1: [hz, hx][slot]
+    __shared__ f4 s_h2[NB][2][FY * 64];  // H(n+3/2) of plane p-1: [parity][hz, hx][slot]
     const int lane = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
     for (int i = w * 64 + lane; i < 256; i += 64 * FY) {
@@ -1021,7 +1040,7 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
     const float *__restrict__ ahz = a.f[5];
 
     // State A at plane p: E at p+1 and at y+1, H, the material ids, the x
-    // psi terms; loaded a plane ahead of its use.
+    // psi terms; loaded kPF2 planes ahead of its use.
     struct Plane {
         f4 ex_n, ey_n, ez_n, ez_py, ex_py, hx_o, hy_o, hz_o;
         uint32_t ids;
@@ -1062,12 +1081,21 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
     for (int k = 0; k < NQ; ++k) qc[k] = splat(0.f);
     uint32_t ids_c = 0;                                             // ids at p-1
     Plane cur;
-    load(cur, z0 - 2);
+    if (kPF2 >= 1) load(cur, z0 - 2);
+    Plane nxt;
+    if (kPF2 == 2) load(nxt, z0 - 1);
 
     for (int it = -2; it <= nit; ++it) {
         const int p = z0 + it;
-        Plane nxt;
-        if (it < nit) load(nxt, p + 1);
+        const int b1 = kLdsCarry ? (it & 1) : 0, b0 = kLdsCarry ? ((it - 1) & 1) : 0;
+        if (kPF2 == 0) load(cur, p);
+        Plane far;  // the plane kPF2 ahead
+        if (kPF2 >= 1 && it + kPF2 <= nit) load(far, p + kPF2);
+        if (kLdsCarry) {
+            // the previous plane's values this wave put into LDS (own slot)
+            h1z_c = s_h1[b0][0][me];
+            h1x_c = s_h1[b0][1][me];
+        }
         f4 q1[NQ];  // x psi(n+1) at p
         for (int k = 0; k < NQ; ++k) q1[k] = cur.ps[k];
 
@@ -1104,8 +1132,8 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
             h1y = splat(0.f);
             h1z = splat(0.f);
         }
-        s_h1[0][me] = h1z;
-        s_h1[1][me] = h1x;
+        s_h1[b1][0][me] = h1z;
+        s_h1[b1][1][me] = h1x;
         lds_barrier();
 
         // ---- step 1: E(n+1) at plane p (update_e_field[_pml]), sources,
@@ -1113,7 +1141,7 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
         f4 e1x = splat(0.f), e1y = splat(0.f), e1z = splat(0.f);
         const float hz_l = __shfl_up(h1z.w, 1, 64), hy_l = __shfl_up(h1y.w, 1, 64);
         if (R >= 1 && it >= -1) {
-            const f4 hz_my = s_h1[0][below], hx_my = s_h1[1][below];
+            const f4 hz_my = s_h1[b1][0][below], hx_my = s_h1[b1][1][below];
             const f4 hz_mx = f4{hz_l, h1z.x, h1z.y, h1z.z};
             const f4 hy_mx = f4{hy_l, h1y.x, h1y.y, h1y.z};
             f4 ca, cb;
@@ -1182,6 +1210,11 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
         s_e1[it & 1][1][me] = e1x;
 
         // ---- step 2: H(n+3/2) at plane p-1 ------------------------------
+        if (kLdsCarryE) {
+            e1z_c = s_e1[(it - 1) & 1][0][me];
+            e1x_c = s_e1[(it - 1) & 1][1][me];
+        }
+        if (kLdsCarry) h2x_m = s_h2[b0][1][me];
         f4 h2x = splat(0.f), h2y = splat(0.f), h2z = splat(0.f);
         const float s2z = __shfl_down(e1z_c.x, 1, 64), s2y = __shfl_down(e1y_c.x, 1, 64);
         if (it >= 0 && R >= 1 && R <= RT - 2) {
@@ -1216,14 +1249,14 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
             h2y = splat(0.f);
             h2z = splat(0.f);
         }
-        s_h2[0][me] = h2z;
-        s_h2[1][me] = h2x;
+        s_h2[b1][0][me] = h2z;
+        s_h2[b1][1][me] = h2x;
         lds_barrier();
 
         // ---- step 2: E(n+2) at plane p-1, stores of the owned cells -----
         const float h2z_l = __shfl_up(h2z.w, 1, 64), h2y_l = __shfl_up(h2y.w, 1, 64);
         if (it >= 1 && R >= 2 && R <= RT - 2) {
-            const f4 hz_my = s_h2[0][below], hx_my = s_h2[1][below];
+            const f4 hz_my = s_h2[b1][0][below], hx_my = s_h2[b1][1][below];
             const f4 hz_mx = f4{h2z_l, h2z.x, h2z.y, h2z.z};
             const f4 hy_mx = f4{h2y_l, h2y.x, h2y.y, h2y.z};
             f4 ca, cb;
@@ -1260,20 +1293,29 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
             }
         }
 
-        h2x_m = h2x;
         h2y_m = h2y;
-        e1x_c = e1x;
         e1y_c = e1y;
-        e1z_c = e1z;
-        h1x_c = h1x;
         h1y_c = h1y;
-        h1z_c = h1z;
+        if (!kLdsCarryE) {
+            e1x_c = e1x;
+            e1z_c = e1z;
+        }
+        if (!kLdsCarry) {
+            h2x_m = h2x;
+            h1x_c = h1x;
+            h1z_c = h1z;
+        }
         for (int k = 0; k < NQ; ++k) qc[k] = q1[k];
         ids_c = cur.ids;
         ex_c = cur.ex_n;
         ey_c = cur.ey_n;
         ez_c = cur.ez_n;
-        cur = nxt;
+        if (kPF2 == 2) {
+            cur = nxt;
+            nxt = far;
+        } else if (kPF2 == 1) {
+            cur = far;
+        }
     }
 }
 
