@@ -876,7 +876,7 @@ __global__ __launch_bounds__((64 * fused_waves<PM, GHOST>())) void k_fused4(Geo 
 // update is the single-pass kernel's, so the bits are the same.
 // ===========================================================================
 #ifndef WF_TWOSTEP_FY
-#define WF_TWOSTEP_FY 16
+#define WF_TWOSTEP_FY 15
 #endif
 constexpr int kF2 = WF_TWOSTEP_FY;
 
@@ -939,17 +939,28 @@ constexpr int kF2x = 8;  // waves of the x CPML variant (its psi terms need the 
 // previous plane's components that also cross waves through LDS are read
 // back from there (own slot) instead of being carried in registers (0 none,
 // 1 E(n+1), H(n+1/2) and H(n+3/2), 2 E(n+1) only).  On MI355X at 1024^3 the
-// fastest is no load ahead with 16 waves per workgroup (127 VGPRs, four waves
-// per SIMD) and E(n+1) read back: 7.53-7.65 ms per step against 7.85-7.91
-// for one plane ahead with 12 waves (167 VGPRs, three per SIMD); one plane
-// ahead with everything read back gained nothing at 12 waves, and two planes
-// ahead spill.
+// fastest is no load ahead with 15-16 waves per workgroup (127 VGPRs, four
+// waves per SIMD) and E(n+1) read back: 7.53-7.65 ms per step against
+// 7.85-7.91 for one plane ahead with 12 waves (167 VGPRs, three per SIMD);
+// one plane ahead with everything read back gained nothing at 12 waves, and
+// two planes ahead spill.
 #ifndef WF_TWOSTEP_PF
 #define WF_TWOSTEP_PF 0
 #endif
 #ifndef WF_TWOSTEP_LDSC
 #define WF_TWOSTEP_LDSC 2
 #endif
+// E^n of the row above (y+1) that step 1's H reads: loaded from memory by
+// every row (0), or passed through LDS by the row above, which loaded it as
+// its own next plane (1: Ez, 2: Ez and Ex; the tile's top row still loads).
+// Loaded, they are fetched from HBM again (the row above loaded them one
+// plane earlier, and a plane of an XCD's 32 tiles fills its L2): at 1024^3
+// the interior kernel read 31.4 GB per launch with loads, 27.4 through LDS
+// (15 waves per workgroup, so that the LDS fits in 160 KB).
+#ifndef WF_TWOSTEP_EPY
+#define WF_TWOSTEP_EPY 2
+#endif
+constexpr int kEpy = WF_TWOSTEP_EPY;
 constexpr int kPF2 = WF_TWOSTEP_PF;
 constexpr bool kLdsCarry = WF_TWOSTEP_LDSC == 1;   // E(n+1), H(n+1/2) and H(n+3/2) components
 constexpr bool kLdsCarryE = WF_TWOSTEP_LDSC != 0;  // E(n+1) components only (2)
@@ -969,6 +980,7 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
     __shared__ f4 s_h1[NB][2][FY * 64];  // H(n+1/2) of plane p: [parity][hz, hx][slot]
     __shared__ f4 s_e1[2][2][FY * 64];   // E(n+1): [plane parity][ez, ex][slot]
     __shared__ f4 s_h2[NB][2][FY * 64];  // H(n+3/2) of plane p-1: [parity][hz, hx][slot]
+    __shared__ f4 s_ep[kEpy ? kEpy : 1][kEpy ? FY * 64 : 1];  // E^n of the next plane: [ez, ex][slot]
     const int lane = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
     for (int i = w * 64 + lane; i < 256; i += 64 * FY) {
@@ -1005,6 +1017,8 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
     const size_t nx = g.nx, plane = nx * g.ny;
     const size_t row = (size_t)(live ? y : 0) * nx + (live ? x : 0);
     const size_t rowp = row + nx;  // E^n at y+1
+    // rows whose row above is not in the tile or idle load E^n at y+1
+    const bool epy_mem = R == RT - 1 || y >= box.y1;
 
     const long nzb = box.z1 - box.z0;
     const int z0 = box.z0 + (int)(tz * nzb / box.gz), z1 = box.z0 + (int)((tz + 1) * nzb / box.gz);
@@ -1056,8 +1070,8 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
             P.ex_n = ld4(aex + on);
             P.ey_n = ld4(aey + on);
             P.ez_n = ld4(aez + on);
-            P.ez_py = ld4(aez + zo + rowp);
-            P.ex_py = ld4(aex + zo + rowp);
+            if (kEpy < 1 || epy_mem) P.ez_py = ld4(aez + zo + rowp);
+            if (kEpy < 2 || epy_mem) P.ex_py = ld4(aex + zo + rowp);
             P.hx_o = ld4(ahx + zo + row);
             if (R >= 1) P.hy_o = ld4(ahy + zo + row);  // row y0-2 needs H(n+1/2) hz, hx only
             P.hz_o = ld4(ahz + zo + row);
@@ -1073,6 +1087,11 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
         ex_c = ld4(aex + o);
         ey_c = ld4(aey + o);
         ez_c = ld4(aez + o);
+    }
+    if (kEpy) {
+        s_ep[0][me] = ez_c;
+        if (kEpy > 1) s_ep[kEpy > 1 ? 1 : 0][me] = ex_c;
+        __syncthreads();
     }
     f4 h1x_c = splat(0.f), h1y_c = splat(0.f), h1z_c = splat(0.f);  // H(n+1/2) at p-1
     f4 e1x_c = splat(0.f), e1y_c = splat(0.f), e1z_c = splat(0.f);  // E(n+1) at p-1
@@ -1107,9 +1126,12 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
         {
             f4 da, db;
             coef4(s_ch, cur.ids, da, db);
-            f4 d_ez_dy = cur.ez_py - ez_c, d_ey_dz = cur.ey_n - ey_c;
+            f4 ez_py = cur.ez_py, ex_py = cur.ex_py;
+            if (kEpy >= 1 && !epy_mem) ez_py = s_ep[0][above];
+            if (kEpy >= 2 && !epy_mem) ex_py = s_ep[kEpy > 1 ? 1 : 0][above];
+            f4 d_ez_dy = ez_py - ez_c, d_ey_dz = cur.ey_n - ey_c;
             f4 d_ex_dz = cur.ex_n - ex_c, d_ez_dx = ez_px - ez_c;
-            f4 d_ey_dx = ey_px - ey_c, d_ex_dy = cur.ex_py - ex_c;
+            f4 d_ey_dx = ey_px - ey_c, d_ex_dy = ex_py - ex_c;
             if (XPML && sh) {
                 d_ez_dy = d_ez_dy + 0.0f;
                 d_ey_dz = d_ey_dz + 0.0f;
@@ -1135,6 +1157,12 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
         s_h1[b1][0][me] = h1z;
         s_h1[b1][1][me] = h1x;
         lds_barrier();
+        if (kEpy) {
+            // this row's E^n of the next plane, for the row below (read
+            // before the next plane's first barrier, written after this one)
+            s_ep[0][me] = cur.ez_n;
+            if (kEpy > 1) s_ep[kEpy > 1 ? 1 : 0][me] = cur.ex_n;
+        }
 
         // ---- step 1: E(n+1) at plane p (update_e_field[_pml]), sources,
         // probes ------------------------------------------------------------
