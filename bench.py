@@ -441,7 +441,7 @@ def main():
             traffic = k["traffic_bytes"]
     # per step: the fused variant launches each kernel once per step, the
     # two-step variant once per pass of two steps
-    step_bytes = sum(kbytes.values()) // (2 if variant == "twostep" else 1)
+    step_bytes = sum(kbytes.values()) // (2 if variant.startswith("twostep") else 1)
     # The two-pass accounting (74 B per cell + 8 B per psi term per step) in
     # which the 70%-of-roofline target is stated: bytes a separate H pass and
     # E pass would move, not what the single-pass kernel moves, so it is

@@ -255,6 +255,11 @@ struct wf_ctx {
     bool pair_geo = false;  // plan_boxes: the geometry allows it
     bool pair_xfold = false;  // the x CPML columns are two-step tiles too (box twox)
     bool pair_xfold_env = true;  // WF_TWOSTEP_XFOLD=0: the frame keeps the x slabs
+    // the y and z slabs as two-step tiles too: the whole domain in two-step
+    // tiles, no frame and no third state (needs the x fold)
+    bool pair_full = false;
+    bool pair_full_env = true;   // WF_TWOSTEP_FULL=0: the frame keeps the y and z slabs
+    Box slab_y[2]{}, slab_z[2]{};
     bool pair_overlap = false;  // WF_PAIR_OVERLAP=1: first step's frame on the side stream
     Box two{}, twox{};
     BoxList fr1h{}, fr1e{}, fr2{};
@@ -341,7 +346,7 @@ struct wf_ctx {
     hipError_t (*lh)(const StepLaunch &, const BoxList &, hipStream_t) = launch_h;
     hipError_t (*le)(const StepLaunch &, const BoxList &, hipStream_t) = launch_e;
     hipError_t (*lf)(const StepLaunch &, const Box &, bool, hipStream_t, bool) = launch_fused;
-    hipError_t (*lt)(const StepLaunch &, const Box &, const TwoStepPoints &, hipStream_t, bool) =
+    hipError_t (*lt)(const StepLaunch &, const Box &, const TwoStepPoints &, hipStream_t, int) =
         launch_twostep;
 };
 
@@ -629,7 +634,7 @@ void plan_boxes(wf_ctx *c)
             // variant, rows of T/4 + 3 lanes), so the frame keeps only the
             // y and z slabs and the two-step box spans x on its y, z range
             const int W = g.T[0] + 4, lprx = g.T[0] / 4 + 3;
-            c->pair_xfold = c->pair_xfold_env && g.T[0] > 0 && lprx <= 32 && n[0] >= 2 * W + 8;
+            c->pair_xfold = c->pair_xfold_env && lprx <= 32 && n[0] >= 2 * W + 8;
             int fx0 = tl[0], fx1 = th[0];
             if (c->pair_xfold) {
                 Box xb{0, n[0], tl[1], th[1], tl[2], th[2], lprx, 2, 0, 0, 0};
@@ -649,6 +654,27 @@ void plan_boxes(wf_ctx *c)
                                    th[2]);
             c->fr2 = frame_around(fx0, fx1, tl[1], th[1], tl[2], th[2]);
             c->pair_geo = true;
+            // full cover: the y slabs (x and y CPML variant) over the box's z
+            // range and the z slabs (every axis) over the whole x, y extent,
+            // full 62-quad rows from x = 0 (lanes outside the domain at zero)
+            c->pair_full = c->pair_full_env && c->pair_xfold;
+            if (c->pair_full) {
+                auto slab = [&](int y0, int y1, int z0, int z1) {
+                    Box b{0, n[0], y0, y1, z0, z1, 64, 0, 0, 0, 0};
+                    b.tstride = 4 * kTwoQuads;
+                    b.gx = (n[0] / 4 + kTwoQuads - 1) / kTwoQuads;
+                    const int rows = twostep_rows_per_tile_x(64);
+                    b.gy = (y1 - y0 + rows - 1) / rows;
+                    b.gz = c->zchunk_env ? (z1 - z0 + zc - 1) / zc
+                                         : fused_slices((long)b.gx * b.gy, z1 - z0, c->cus, 3);
+                    b.gz = std::min(b.gz, z1 - z0);
+                    return b;
+                };
+                c->slab_y[0] = slab(0, tl[1], tl[2], th[2]);
+                c->slab_y[1] = slab(th[1], n[1], tl[2], th[2]);
+                c->slab_z[0] = slab(0, n[1], 0, tl[2]);
+                c->slab_z[1] = slab(0, n[1], th[2], n[2]);
+            }
         }
     }
     Box all{0, n[0], 0, n[1], 0, n[2], 64, 0, 0, 0, 0};
@@ -745,9 +771,9 @@ bool folded(const wf_ctx *c)
 // The end of a step (src/engine.rs:770-782): sources, probes, then the
 // device step counter.  slot: probes in the two-step kernel's box read the
 // values it recorded (the first step of a pass).
-int point_work(wf_ctx *c, const Ptrs &p, hipStream_t st, bool slot)
+int point_work(wf_ctx *c, const Ptrs &p, hipStream_t st, bool slot, bool sources = true)
 {
-    if (c->nsrc) {
+    if (c->nsrc && sources) {
         hipLaunchKernelGGL(k_sources, dim3((c->nsrc_grp + 255) / 256), dim3(256), 0, st, p,
                            c->d_src, (int)c->nsrc_col, c->d_src_grp, (int)c->nsrc_grp,
                            c->d_srcval, c->d_step, c->src_base);
@@ -875,9 +901,11 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
 
 // Two steps per pass apply to the fused variant without face hooks or the
 // whole-domain CPML pass, when the geometry allows the two-step box.
+bool pair_tables(const wf_ctx *c) { return c->fused && c->pair_env && c->pair_geo; }
+
 bool pair_active(const wf_ctx *c)
 {
-    if (!c->fused || !c->pair_env || !c->pair_geo || !c->d_fields3 || c->pml_fused) return false;
+    if (!pair_tables(c) || !(c->pair_full || c->d_fields3) || c->pml_fused) return false;
     for (int f = 0; f < 6; ++f)
         if (c->hook_on[f]) return false;
     return true;
@@ -904,6 +932,32 @@ int launch_pair(wf_ctx *c, hipEvent_t *ev = nullptr)
     pts.step = c->d_step;
     pts.slot = c->d_pslot;
     StepLaunch s{c->g, c->p, c->alt, c->pml, c->fzchunk, st};
+    if (c->pair_full) {
+        // the whole domain in two-step tiles: the interior, the x CPML
+        // columns, the y slabs, the z slabs; the first step's sources are
+        // all applied inside them and its probes all read from the slots
+        // (ev: event pairs around the interior (0, 1), the x columns (2, 3)
+        // and the slabs (4, 5))
+        if (ev) WF_HIP(hipEventRecord(ev[0], st));
+        WF_HIP(c->lt(s, c->two, pts, st, 0));
+        if (ev) {
+            WF_HIP(hipEventRecord(ev[1], st));
+            WF_HIP(hipEventRecord(ev[2], st));
+        }
+        WF_HIP(c->lt(s, c->twox, pts, st, 1));
+        if (ev) {
+            WF_HIP(hipEventRecord(ev[3], st));
+            WF_HIP(hipEventRecord(ev[4], st));
+        }
+        for (int k = 0; k < 2; ++k) WF_HIP(c->lt(s, c->slab_y[k], pts, st, 3));
+        for (int k = 0; k < 2; ++k) WF_HIP(c->lt(s, c->slab_z[k], pts, st, 7));
+        if (ev) WF_HIP(hipEventRecord(ev[5], st));
+        if (int rc = point_work(c, c->alt, st, true, false)) return rc;
+        if (int rc = point_work(c, c->alt, st, false)) return rc;
+        std::swap(c->p, c->alt);
+        c->step += 2;
+        return WF_OK;
+    }
     StepLaunch s1{c->g, c->p, c->third, c->pml, c->fzchunk, st};
     if (c->pair_overlap && !ev && c->side) {
         // the first step's frame reads state A and writes C, the two-step
@@ -913,13 +967,13 @@ int launch_pair(wf_ctx *c, hipEvent_t *ev = nullptr)
         WF_HIP(c->lh(s1, c->fr1h, c->side));
         WF_HIP(c->le(s1, c->fr1e, c->side));
         WF_HIP(hipEventRecord(c->join, c->side));
-        WF_HIP(c->lt(s, c->two, pts, st, false));
-        if (c->pair_xfold) WF_HIP(c->lt(s, c->twox, pts, st, true));
+        WF_HIP(c->lt(s, c->two, pts, st, 0));
+        if (c->pair_xfold) WF_HIP(c->lt(s, c->twox, pts, st, 1));
         WF_HIP(hipStreamWaitEvent(st, c->join, 0));
     } else {
         if (ev) WF_HIP(hipEventRecord(ev[0], st));
-        WF_HIP(c->lt(s, c->two, pts, st, false));
-        if (c->pair_xfold) WF_HIP(c->lt(s, c->twox, pts, st, true));
+        WF_HIP(c->lt(s, c->two, pts, st, 0));
+        if (c->pair_xfold) WF_HIP(c->lt(s, c->twox, pts, st, 1));
         if (ev) {
             WF_HIP(hipEventRecord(ev[1], st));
             WF_HIP(hipEventRecord(ev[2], st));
@@ -1375,10 +1429,12 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
         }
     if (const char *ts = std::getenv("WF_TWOSTEP")) c->pair_env = std::atoi(ts) != 0;
     if (const char *tx = std::getenv("WF_TWOSTEP_XFOLD")) c->pair_xfold_env = std::atoi(tx) != 0;
+    if (const char *tf = std::getenv("WF_TWOSTEP_FULL")) c->pair_full_env = std::atoi(tf) != 0;
     if (const char *po = std::getenv("WF_PAIR_OVERLAP")) c->pair_overlap = std::atoi(po) != 0;
     plan_boxes(c);
-    // Third state for the frame of two-step passes (its first step's values).
-    if (c->fused && c->pair_env && c->pair_geo) {
+    // Third state for the frame of two-step passes (its first step's values);
+    // none when two-step tiles cover the whole domain.
+    if (c->fused && c->pair_env && c->pair_geo && !c->pair_full) {
         if (int rc = dmalloc(&c->d_fields3, 6 * N * sizeof(float))) return bail(rc);
         if (hipMemsetAsync(c->d_fields3, 0, 6 * N * sizeof(float), c->stream) != hipSuccess)
             return bail(fail(WF_ERR_HIP, "hipMemset(fields) failed"));
@@ -1510,7 +1566,7 @@ int wf_set_source_columns(wf_ctx *c, uint32_t n, const wf_source_site *sites,
     WF_HIP(hipMemcpy(c->d_src_grp, grp.data(), grp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     c->nsrc_grp = (uint32_t)(grp.size() - 1);
     if (nsteps) WF_HIP(hipMemcpy(c->d_srcval, values, nsteps * ncol * sizeof(float), hipMemcpyHostToDevice));
-    if (c->d_fields3) {
+    if (pair_tables(c)) {
         // the two-step kernel's table: by row, list order kept per voxel
         std::vector<std::pair<uint64_t, PointEnt>> v(n);
         for (uint32_t s = 0; s < n; ++s) {
@@ -1580,14 +1636,17 @@ int wf_set_probes(wf_ctx *c, uint32_t n, const wf_probe_site *sites, uint64_t ns
     if (nsteps) WF_HIP(hipMemcpy(c->d_active, active, nsteps * n, hipMemcpyHostToDevice));
     WF_HIP(hipMemset(c->d_sums, 0, 2ull * (nf ? nf : 1) * sizeof(double)));
     WF_HIP(hipMemset(c->d_counts, 0, n * sizeof(unsigned long long)));
-    if (c->d_fields3) {
+    if (pair_tables(c)) {
         // the two-step kernel records the probes of its box at a pass's
         // first step into one slot per probe
         std::vector<std::pair<uint64_t, PointEnt>> v(n);
         std::vector<uint8_t> in(n);
-        // the two-step tiles own the box (and, x folded, every x of its rows)
+        // the two-step tiles own the box (x folded, every x of its rows;
+        // with the slabs, the whole domain)
         const Box &t = c->two;
         const int ox0 = c->pair_xfold ? 0 : t.x0, ox1 = c->pair_xfold ? c->g.nx : t.x1;
+        const int oy0 = c->pair_full ? 0 : t.y0, oy1 = c->pair_full ? c->g.ny : t.y1;
+        const int oz0 = c->pair_full ? 0 : t.z0, oz1 = c->pair_full ? c->g.nz : t.z1;
         for (uint32_t p = 0; p < n; ++p) {
             const wf_probe_site &q = sites[p];
             PointEnt e{};
@@ -1595,8 +1654,8 @@ int wf_set_probes(wf_ctx *c, uint32_t n, const wf_probe_site *sites, uint64_t ns
             e.comp = (uint16_t)q.component;
             e.col = p;
             v[p] = {(uint64_t)q.z * c->g.ny + q.y, e};
-            in[p] = (int)q.x >= ox0 && (int)q.x < ox1 && (int)q.y >= t.y0 && (int)q.y < t.y1 &&
-                    (int)q.z >= t.z0 && (int)q.z < t.z1;
+            in[p] = (int)q.x >= ox0 && (int)q.x < ox1 && (int)q.y >= oy0 && (int)q.y < oy1 &&
+                    (int)q.z >= oz0 && (int)q.z < oz1;
         }
         if (int rc = upload_points(c, v, c->prb_pts)) return rc;
         if (int rc = dmalloc(&c->d_pslot, n * sizeof(float))) return rc;
@@ -1938,7 +1997,10 @@ int wf_hbm_copy(uint64_t bytes, int device, int iters, double *ms_per_copy)
 
 const char *wf_kernel_variant(const wf_ctx *c)
 {
-    return !c ? "" : (c->fused ? (pair_active(c) ? "twostep" : "fused") : "split");
+    if (!c) return "";
+    if (!c->fused) return "split";
+    if (!pair_active(c)) return "fused";
+    return c->pair_full ? "twostep_full" : "twostep";
 }
 
 int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
@@ -1958,6 +2020,24 @@ int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
         }
         return cells;
     };
+    auto cells_of = [](const Box &b) {
+        return (uint64_t)(b.x1 - b.x0) * (b.y1 - b.y0) * (b.z1 - b.z0);
+    };
+    if (pair_active(c) && c->pair_full) {
+        // two-step tiles over the whole domain, 49 B per cell for two steps
+        // and 16 B per psi term (read once, written once per pass): the
+        // interior, the x CPML columns (x psi), the y and z slabs (the rest
+        // of the psi terms)
+        const Box &t = c->two;
+        const uint64_t yz = (uint64_t)(t.y1 - t.y0) * (t.z1 - t.z0);
+        const uint64_t xterms = 2ull * c->g.T[0] * yz * 2;  // per phase
+        *fused_bytes = 49ull * cells_of(t);
+        *h_bytes = 49ull * 2 * (c->g.T[0] + 4) * yz + 16ull * xterms;
+        uint64_t slabs = 0;
+        for (int k = 0; k < 2; ++k) slabs += cells_of(c->slab_y[k]) + cells_of(c->slab_z[k]);
+        *e_bytes = 49ull * slabs + 16ull * (terms - xterms);
+        return WF_OK;
+    }
     if (pair_active(c)) {
         // two-step kernel: 49 B per cell of its box for two steps; the
         // frame of a pass's first step (H over fr1h, E over fr1e) and of its

@@ -207,15 +207,15 @@ hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hip
 // outside the CPML shell and inside the domain, no axis periodic.  Tiles:
 // box.gx columns of kTwoQuads quads, box.gy row groups of
 // twostep_rows_per_tile() rows, box.gz even z slices.
-// x_pml: the tiles hold the x CPML layers (box.y and box.z outside the
-// shell; rows box.lpr lanes wide, box.lpr - 2 quads owned, tile tx owning
-// [x0 + tx tstride, + 4 (lpr - 2)) clipped to x1; cells outside the domain
-// read as zero).
+// pml_axes (bit 1 x, 2 y, 4 z; 0, 1, 3 or 7): the CPML axes of the cells
+// the tiles compute, margins included; the CPML variants tile rows box.lpr
+// lanes wide (box.lpr - 2 quads owned, tile tx owning [x0 + tx tstride,
+// + 4 (lpr - 2)) clipped to x1) and read cells outside the domain as zero.
 hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoints &pts,
-                          hipStream_t st, bool x_pml = false);
+                          hipStream_t st, int pml_axes = 0);
 constexpr int kTwoQuads = 62;  // owned quads per tile row (lanes 1..62)
 int twostep_rows_per_tile();
-int twostep_rows_per_tile_x(int lpr);  // owned rows per tile of the x CPML variant
+int twostep_rows_per_tile_x(int lpr);  // owned rows per tile of the CPML variants
 // Tile shapes of the two kernels (owned rows per fused tile -- of the plain
 // variant, or of the CPML / ghost variants -- and rows per split workgroup
 // of lpr = 64).
@@ -260,7 +260,7 @@ hipError_t launch_e(const StepLaunch &s, const BoxList &bl, hipStream_t st);
 hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hipStream_t st,
                         bool x_only = false);
 hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoints &pts,
-                          hipStream_t st, bool x_pml = false);
+                          hipStream_t st, int pml_axes = 0);
 hipError_t launch_batch_h(const BatchScene *scenes, const uint32_t *wg_scene, int total,
                           int zchunk, int grid, hipStream_t st);
 hipError_t launch_batch_e(const BatchScene *scenes, const uint32_t *wg_scene, int total,

@@ -929,58 +929,42 @@ __device__ __forceinline__ void quad_source(f4 &v, int k, float val, bool hard)
     v.w = k == 3 ? nv : v.w;
 }
 
-// XPML: tiles over the x CPML layers (the box's y and z ranges lie outside
-// the shell): rows box.lpr lanes wide, several rows per wave, the x psi terms
-// of graded quads carried from the first step to the second in registers,
-// and lanes outside the domain (x < 0 or x >= nx) held at zero, the
-// boundary the reference reads there.
-constexpr int kF2x = 8;  // waves of the x CPML variant (its psi terms need the registers)
-// Planes loaded ahead of the one being computed (0, 1 or 2), and which of the
-// previous plane's components that also cross waves through LDS are read
-// back from there (own slot) instead of being carried in registers (0 none,
-// 1 E(n+1), H(n+1/2) and H(n+3/2), 2 E(n+1) only).  On MI355X at 1024^3 the
-// fastest is no load ahead with 15-16 waves per workgroup (127 VGPRs, four
-// waves per SIMD) and E(n+1) read back: 7.53-7.65 ms per step against
-// 7.85-7.91 for one plane ahead with 12 waves (167 VGPRs, three per SIMD);
-// one plane ahead with everything read back gained nothing at 12 waves, and
-// two planes ahead spill.
-#ifndef WF_TWOSTEP_PF
-#define WF_TWOSTEP_PF 0
-#endif
-#ifndef WF_TWOSTEP_LDSC
-#define WF_TWOSTEP_LDSC 2
-#endif
-// E^n of the row above (y+1) that step 1's H reads: loaded from memory by
-// every row (0), or passed through LDS by the row above, which loaded it as
-// its own next plane (1: Ez, 2: Ez and Ex; the tile's top row still loads).
-// Loaded, they are fetched from HBM again (the row above loaded them one
-// plane earlier, and a plane of an XCD's 32 tiles fills its L2): at 1024^3
-// the interior kernel read 31.4 GB per launch with loads, 27.4 through LDS
-// (15 waves per workgroup, so that the LDS fits in 160 KB).
-#ifndef WF_TWOSTEP_EPY
-#define WF_TWOSTEP_EPY 2
-#endif
-constexpr int kEpy = WF_TWOSTEP_EPY;
-constexpr int kPF2 = WF_TWOSTEP_PF;
-constexpr bool kLdsCarry = WF_TWOSTEP_LDSC == 1;   // E(n+1), H(n+1/2) and H(n+3/2) components
-constexpr bool kLdsCarryE = WF_TWOSTEP_LDSC != 0;  // E(n+1) components only (2)
-template <bool XPML>
-constexpr int twostep_waves() { return XPML ? kF2x : kF2; }
+// PM: the CPML axes a variant handles, bit 1 x, 2 y, 4 z.  0: the interior
+// box (no cell it computes lies in the shell); 1: the x CPML columns beside
+// it (their y and z ranges outside the shell); 3: the y slabs (x and y);
+// 7: the z slabs (every axis).  CPML variants tile rows box.lpr lanes wide
+// (several rows per wave when narrow), carry each psi term of a graded cell
+// from the first step to the second in registers, and hold lanes, rows and
+// planes outside the domain at zero, the boundary the reference reads there.
+constexpr int kF2x = 8;  // waves of the CPML variants (their psi terms need the registers)
+// The interior variant loads each plane when it reaches it, with 15 waves
+// per workgroup (127 VGPRs: four waves per SIMD), reads the previous plane's
+// E(n+1) back from its own LDS slot, and takes E^n of the row above from the
+// row above through LDS (that row loaded it as its own next plane).  Measured
+// on MI355X at 1024^3 (A/B within one box): one plane loaded ahead with 12
+// waves (167 VGPRs, three per SIMD) ran 7.85-7.91 ms per step against
+// 7.53-7.65 for this scheme at 16 waves; reading back also H(n+1/2) and
+// H(n+3/2) gained nothing; two planes ahead spill; loading E^n at y+1 again
+// fetched it from HBM a second time (31.4 against 27.4 GB read per launch:
+// the row above loaded it a plane earlier, and a plane of an XCD's 32 tiles
+// fills its L2).  15 waves keep the LDS within 160 KB.
+template <int PM>
+constexpr int twostep_waves() { return PM ? kF2x : kF2; }
 
-template <bool XPML>
-__global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, Ptrs a, Ptrs b,
-                                                                         Box box,
-                                                                         TwoStepPoints pts)
+template <int PM>
+__global__ __launch_bounds__(64 * twostep_waves<PM>()) void k_twostep4(Geo g, Ptrs a, Ptrs b,
+                                                                       Box box,
+                                                                       TwoStepPoints pts)
 {
-    constexpr int FY = twostep_waves<XPML>();
-    constexpr int NQ = XPML ? 4 : 1;     // x psi terms per quad (ez_dx, ey_dx, hz_dx, hy_dx)
-    __shared__ float2 s_ce[256];         // (ca, cb)
-    __shared__ float2 s_ch[256];         // (da, db)
-    constexpr int NB = kLdsCarry ? 2 : 1;  // plane-parity buffers of s_h1 / s_h2
-    __shared__ f4 s_h1[NB][2][FY * 64];  // H(n+1/2) of plane p: [parity][hz, hx][slot]
-    __shared__ f4 s_e1[2][2][FY * 64];   // E(n+1): [plane parity][ez, ex][slot]
-    __shared__ f4 s_h2[NB][2][FY * 64];  // H(n+3/2) of plane p-1: [parity][hz, hx][slot]
-    __shared__ f4 s_ep[kEpy ? kEpy : 1][kEpy ? FY * 64 : 1];  // E^n of the next plane: [ez, ex][slot]
+    constexpr bool PX = (PM & 1) != 0, PY = (PM & 2) != 0, PZ = (PM & 4) != 0;
+    constexpr int FY = twostep_waves<PM>();
+    constexpr int NX = PX ? 4 : 1, NY = PY ? 4 : 1, NZ = PZ ? 4 : 1;  // psi terms per quad
+    __shared__ float2 s_ce[256];        // (ca, cb)
+    __shared__ float2 s_ch[256];        // (da, db)
+    __shared__ f4 s_h1[2][FY * 64];     // H(n+1/2) of plane p: [hz, hx][slot]
+    __shared__ f4 s_e1[2][2][FY * 64];  // E(n+1): [plane parity][ez, ex][slot]
+    __shared__ f4 s_h2[2][FY * 64];     // H(n+3/2) of plane p-1: [hz, hx][slot]
+    __shared__ f4 s_ep[2][FY * 64];     // E^n of the next plane: [ez, ex][slot]
     const int lane = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
     for (int i = w * 64 + lane; i < 256; i += 64 * FY) {
@@ -988,13 +972,12 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
         s_ce[i] = make_float2(c.x, c.y);
         s_ch[i] = make_float2(c.z, c.w);
     }
-    __syncthreads();
 
     // lanes per row, rows per wave; row R of the tile = w * rpw + r (lanes
     // past rpw * LPR idle); LDS slot of each lane and of its y-1 / y+1 rows
-    const int LPR = XPML ? box.lpr : 64;
-    const int rpw = XPML ? 64 / LPR : 1;
-    const int r = XPML ? lane / LPR : 0, l = XPML ? lane - r * LPR : lane;
+    const int LPR = PM ? box.lpr : 64;
+    const int rpw = PM ? 64 / LPR : 1;
+    const int r = PM ? lane / LPR : 0, l = PM ? lane - r * LPR : lane;
     const bool idle = r >= rpw;
     const int RT = FY * rpw;
     const int R = idle ? -8 : w * rpw + r;
@@ -1010,14 +993,16 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
     const int y = box.y0 + (RT - 3) * ty - 2 + R;
     // tx1 and y1 themselves hold the upper margin; beyond them, outside the
     // domain and on idle lanes nothing is loaded and every value is zero
-    const bool live_row = !idle && y <= box.y1;
+    const bool live_row = !idle && y >= 0 && y < g.ny && y <= box.y1;
     const bool live = live_row && x >= 0 && x + 4 <= g.nx && x <= tx1;
     const bool own_row = live_row && y < box.y1 && R >= 2 && R <= RT - 2;
     const bool own = own_row && live && x >= tx0 && x < tx1;
     const size_t nx = g.nx, plane = nx * g.ny;
     const size_t row = (size_t)(live ? y : 0) * nx + (live ? x : 0);
     const size_t rowp = row + nx;  // E^n at y+1
-    // rows whose row above is not in the tile or idle load E^n at y+1
+    // E^n at y+1: inside the domain, and loaded by the rows whose row above
+    // is not in the tile (the others take it from that row through LDS)
+    const bool has_py = y + 1 < g.ny;
     const bool epy_mem = R == RT - 1 || y >= box.y1;
 
     const long nzb = box.z1 - box.z0;
@@ -1028,15 +1013,15 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
     const float *sv = nullptr;
     if (pts.src.bits) sv = pts.values + (*pts.step - pts.base) * (uint64_t)pts.ncol;
 
-    // x CPML of this lane's quad: in the shell (identity derivatives + 0.0f)
-    // and graded (the one-tap recursion; the x shell is the graded layer, a
-    // quad lies in one block); psi index within a plane's x slab
-    bool sh = false, gxq = false;
-    size_t gxi = 0;
-    const size_t psx = (size_t)g.ny * 2 * (size_t)g.T[0];
+    // CPML of this lane's quad (x) and this row (y): in the shell (identity
+    // derivatives + 0.0f) and graded (the one-tap recursion; the shell is the
+    // graded layer, a quad lies in one block); psi offsets and profiles
+    bool shx = false, gxq = false, shy = false, gyr = false;
+    size_t gxi = 0, gyo = 0;
     f4 pbx = splat(1.f), pax = splat(0.f), pkx = splat(1.f);
-    if (XPML && live) {
-        sh = in_shell_axis(g, 0, x);
+    float pby = 1.f, pay = 0.f, pky = 1.f;
+    if (PX && live) {
+        shx = in_shell_axis(g, 0, x);
         gxq = graded(g, 0, x, g.nx);
         if (gxq) {
             gxi = (size_t)y * 2 * (size_t)g.T[0] + (size_t)slab_index(g, 0, x, g.nx);
@@ -1045,6 +1030,18 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
             pkx = ld4(a.prof[0][2] + x);
         }
     }
+    if (PY && live) {
+        shy = in_shell_axis(g, 1, y);
+        gyr = graded(g, 1, y, g.ny);
+        if (gyr) {
+            gyo = (size_t)slab_index(g, 1, y, g.ny) * nx + x;
+            pby = a.prof[1][0][y];
+            pay = a.prof[1][1][y];
+            pky = a.prof[1][2][y];
+        }
+    }
+    const size_t psx = (size_t)g.ny * 2 * (size_t)g.T[0];  // x psi per plane
+    const size_t psy = 2 * (size_t)g.T[1] * nx;             // y psi per plane
 
     const float *__restrict__ aex = a.f[0];
     const float *__restrict__ aey = a.f[1];
@@ -1053,70 +1050,93 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
     const float *__restrict__ ahy = a.f[4];
     const float *__restrict__ ahz = a.f[5];
 
-    // State A at plane p: E at p+1 and at y+1, H, the material ids, the x
-    // psi terms; loaded kPF2 planes ahead of its use.
+    // State A at plane p: E at p+1 and (rows whose row above is not in the
+    // tile) at y+1, H, the material ids, the psi terms of graded cells.
     struct Plane {
         f4 ex_n, ey_n, ez_n, ez_py, ex_py, hx_o, hy_o, hz_o;
         uint32_t ids;
-        f4 ps[NQ];
+        f4 px[NX], py[NY], pz[NZ];
     };
     auto load = [&](Plane &P, int p) {
         P.ex_n = P.ey_n = P.ez_n = P.ez_py = P.ex_py = P.hx_o = P.hy_o = P.hz_o = splat(0.f);
         P.ids = 0;
-        for (int k = 0; k < NQ; ++k) P.ps[k] = splat(0.f);
-        if (live) {
-            const size_t zo = (size_t)p * plane;
-            const size_t on = zo + plane + row;
+        for (int k = 0; k < NX; ++k) P.px[k] = splat(0.f);
+        for (int k = 0; k < NY; ++k) P.py[k] = splat(0.f);
+        for (int k = 0; k < NZ; ++k) P.pz[k] = splat(0.f);
+        // E at p+1 becomes the next plane's E^n: loaded whenever p+1 is in
+        // the domain, also for a plane p below it
+        if (live && p + 1 >= 0 && p + 1 < g.nz) {
+            const size_t on = (size_t)(p + 1) * plane + row;
             P.ex_n = ld4(aex + on);
             P.ey_n = ld4(aey + on);
             P.ez_n = ld4(aez + on);
-            if (kEpy < 1 || epy_mem) P.ez_py = ld4(aez + zo + rowp);
-            if (kEpy < 2 || epy_mem) P.ex_py = ld4(aex + zo + rowp);
+        }
+        if (live && p >= 0 && p < g.nz) {
+            const size_t zo = (size_t)p * plane;
+            if (epy_mem && has_py) {
+                P.ez_py = ld4(aez + zo + rowp);
+                P.ex_py = ld4(aex + zo + rowp);
+            }
             P.hx_o = ld4(ahx + zo + row);
             if (R >= 1) P.hy_o = ld4(ahy + zo + row);  // row y0-2 needs H(n+1/2) hz, hx only
             P.hz_o = ld4(ahz + zo + row);
             P.ids = *reinterpret_cast<const uint32_t *>(a.id + zo + row);
-            if (XPML && gxq)
-                for (int k = 0; k < NQ; ++k) P.ps[k] = ld4(a.psi[0][k] + (size_t)p * psx + gxi);
+            if (PX && gxq)
+                for (int k = 0; k < NX; ++k) P.px[k] = ld4(a.psi[0][k] + (size_t)p * psx + gxi);
+            if (PY && gyr)
+                for (int k = 0; k < NY; ++k) P.py[k] = ld4(a.psi[1][k] + (size_t)p * psy + gyo);
+            if (PZ && graded(g, 2, p, g.nz)) {
+                const size_t o = (size_t)slab_index(g, 2, p, g.nz) * plane + row;
+                for (int k = 0; k < NZ; ++k) P.pz[k] = ld4(a.psi[2][k] + o);
+            }
         }
     };
 
     f4 ex_c = splat(0.f), ey_c = splat(0.f), ez_c = splat(0.f);  // E^n at p
-    if (live) {
+    if (live && z0 - 2 >= 0) {
         const size_t o = (size_t)(z0 - 2) * plane + row;
         ex_c = ld4(aex + o);
         ey_c = ld4(aey + o);
         ez_c = ld4(aez + o);
     }
-    if (kEpy) {
-        s_ep[0][me] = ez_c;
-        if (kEpy > 1) s_ep[kEpy > 1 ? 1 : 0][me] = ex_c;
-        __syncthreads();
-    }
+    s_ep[0][me] = ez_c;
+    s_ep[1][me] = ex_c;
+    __syncthreads();
     f4 h1x_c = splat(0.f), h1y_c = splat(0.f), h1z_c = splat(0.f);  // H(n+1/2) at p-1
-    f4 e1x_c = splat(0.f), e1y_c = splat(0.f), e1z_c = splat(0.f);  // E(n+1) at p-1
+    f4 e1x_c, e1y_c = splat(0.f), e1z_c;                           // E(n+1) at p-1
     f4 h2x_m = splat(0.f), h2y_m = splat(0.f);                      // H(n+3/2) at p-2
-    f4 qc[NQ];                                                      // x psi(n+1) at p-1
-    for (int k = 0; k < NQ; ++k) qc[k] = splat(0.f);
-    uint32_t ids_c = 0;                                             // ids at p-1
+    f4 qx[NX], qy[NY], qz[NZ];  // psi(n+1) at p-1
+    for (int k = 0; k < NX; ++k) qx[k] = splat(0.f);
+    for (int k = 0; k < NY; ++k) qy[k] = splat(0.f);
+    for (int k = 0; k < NZ; ++k) qz[k] = splat(0.f);
+    uint32_t ids_c = 0;                       // ids at p-1
+    bool shz_c = false, gz_c = false;         // z CPML of plane p-1
+    float pbz_c = 1.f, paz_c = 0.f, pkz_c = 1.f;
     Plane cur;
-    if (kPF2 >= 1) load(cur, z0 - 2);
-    Plane nxt;
-    if (kPF2 == 2) load(nxt, z0 - 1);
 
     for (int it = -2; it <= nit; ++it) {
         const int p = z0 + it;
-        const int b1 = kLdsCarry ? (it & 1) : 0, b0 = kLdsCarry ? ((it - 1) & 1) : 0;
-        if (kPF2 == 0) load(cur, p);
-        Plane far;  // the plane kPF2 ahead
-        if (kPF2 >= 1 && it + kPF2 <= nit) load(far, p + kPF2);
-        if (kLdsCarry) {
-            // the previous plane's values this wave put into LDS (own slot)
-            h1z_c = s_h1[b0][0][me];
-            h1x_c = s_h1[b0][1][me];
+        const bool lp = p >= 0 && p < g.nz;      // plane p in the domain
+        const bool lq = p >= 1 && p <= g.nz;     // plane p-1 in the domain
+        load(cur, p);
+        // z CPML of plane p
+        bool shz = false, gzp = false;
+        float pbz = 1.f, paz = 0.f, pkz = 1.f;
+        if (PZ && lp) {
+            shz = in_shell_axis(g, 2, p);
+            gzp = graded(g, 2, p, g.nz);
+            if (gzp) {
+                pbz = a.prof[2][0][p];
+                paz = a.prof[2][1][p];
+                pkz = a.prof[2][2][p];
+            }
         }
-        f4 q1[NQ];  // x psi(n+1) at p
-        for (int k = 0; k < NQ; ++k) q1[k] = cur.ps[k];
+        const bool sh1 = shx || shy || shz;   // the cell's block is in the shell (plane p)
+        const bool sh2 = shx || shy || shz_c; // ... at plane p-1
+        f4 q1x[NX], q1y[NY], q1z[NZ];  // psi(n+1) at p
+        for (int k = 0; k < NX; ++k) q1x[k] = cur.px[k];
+        for (int k = 0; k < NY; ++k) q1y[k] = cur.py[k];
+        for (int k = 0; k < NZ; ++k) q1z[k] = cur.pz[k];
 
         // ---- step 1: H(n+1/2) at plane p (update_h_field[_pml]) ---------
         const float sez = __shfl_down(ez_c.x, 1, 64), sey = __shfl_down(ey_c.x, 1, 64);
@@ -1127,19 +1147,31 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
             f4 da, db;
             coef4(s_ch, cur.ids, da, db);
             f4 ez_py = cur.ez_py, ex_py = cur.ex_py;
-            if (kEpy >= 1 && !epy_mem) ez_py = s_ep[0][above];
-            if (kEpy >= 2 && !epy_mem) ex_py = s_ep[kEpy > 1 ? 1 : 0][above];
+            if (!epy_mem) {
+                ez_py = s_ep[0][above];
+                ex_py = s_ep[1][above];
+            }
             f4 d_ez_dy = ez_py - ez_c, d_ey_dz = cur.ey_n - ey_c;
             f4 d_ex_dz = cur.ex_n - ex_c, d_ez_dx = ez_px - ez_c;
             f4 d_ey_dx = ey_px - ey_c, d_ex_dy = ex_py - ex_c;
-            if (XPML && sh) {
-                d_ez_dy = d_ez_dy + 0.0f;
-                d_ey_dz = d_ey_dz + 0.0f;
-                d_ex_dz = d_ex_dz + 0.0f;
-                d_ex_dy = d_ex_dy + 0.0f;
-                if (gxq) {
-                    d_ez_dx = cpml4(d_ez_dx, pbx, pax, pkx, q1[0]);
-                    d_ey_dx = cpml4(d_ey_dx, pbx, pax, pkx, q1[XPML ? 1 : 0]);
+            if (PM && sh1) {
+                if (PY && gyr) {
+                    d_ez_dy = cpml4(d_ez_dy, splat(pby), splat(pay), splat(pky), q1y[0]);
+                    d_ex_dy = cpml4(d_ex_dy, splat(pby), splat(pay), splat(pky), q1y[PY ? 1 : 0]);
+                } else {
+                    d_ez_dy = d_ez_dy + 0.0f;
+                    d_ex_dy = d_ex_dy + 0.0f;
+                }
+                if (PZ && gzp) {
+                    d_ey_dz = cpml4(d_ey_dz, splat(pbz), splat(paz), splat(pkz), q1z[0]);
+                    d_ex_dz = cpml4(d_ex_dz, splat(pbz), splat(paz), splat(pkz), q1z[PZ ? 1 : 0]);
+                } else {
+                    d_ey_dz = d_ey_dz + 0.0f;
+                    d_ex_dz = d_ex_dz + 0.0f;
+                }
+                if (PX && gxq) {
+                    d_ez_dx = cpml4(d_ez_dx, pbx, pax, pkx, q1x[0]);
+                    d_ey_dx = cpml4(d_ey_dx, pbx, pax, pkx, q1x[PX ? 1 : 0]);
                 } else {
                     d_ez_dx = d_ez_dx + 0.0f;
                     d_ey_dx = d_ey_dx + 0.0f;
@@ -1149,27 +1181,25 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
             h1y = madd4(da, cur.hy_o, -(db * (d_ex_dz - d_ez_dx)));
             h1z = madd4(da, cur.hz_o, -(db * (d_ey_dx - d_ex_dy)));
         }
-        if (!live) {
+        if (!live || !lp) {
             h1x = splat(0.f);
             h1y = splat(0.f);
             h1z = splat(0.f);
         }
-        s_h1[b1][0][me] = h1z;
-        s_h1[b1][1][me] = h1x;
+        s_h1[0][me] = h1z;
+        s_h1[1][me] = h1x;
         lds_barrier();
-        if (kEpy) {
-            // this row's E^n of the next plane, for the row below (read
-            // before the next plane's first barrier, written after this one)
-            s_ep[0][me] = cur.ez_n;
-            if (kEpy > 1) s_ep[kEpy > 1 ? 1 : 0][me] = cur.ex_n;
-        }
+        // this row's E^n of the next plane, for the row below (read before
+        // the next plane's first barrier, written after this one)
+        s_ep[0][me] = cur.ez_n;
+        s_ep[1][me] = cur.ex_n;
 
         // ---- step 1: E(n+1) at plane p (update_e_field[_pml]), sources,
         // probes ------------------------------------------------------------
         f4 e1x = splat(0.f), e1y = splat(0.f), e1z = splat(0.f);
         const float hz_l = __shfl_up(h1z.w, 1, 64), hy_l = __shfl_up(h1y.w, 1, 64);
         if (R >= 1 && it >= -1) {
-            const f4 hz_my = s_h1[b1][0][below], hx_my = s_h1[b1][1][below];
+            const f4 hz_my = s_h1[0][below], hx_my = s_h1[1][below];
             const f4 hz_mx = f4{hz_l, h1z.x, h1z.y, h1z.z};
             const f4 hy_mx = f4{hy_l, h1y.x, h1y.y, h1y.z};
             f4 ca, cb;
@@ -1177,14 +1207,24 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
             f4 d_hz_dy = h1z - hz_my, d_hy_dz = h1y - h1y_c;
             f4 d_hx_dz = h1x - h1x_c, d_hz_dx = h1z - hz_mx;
             f4 d_hy_dx = h1y - hy_mx, d_hx_dy = h1x - hx_my;
-            if (XPML && sh) {
-                d_hz_dy = d_hz_dy + 0.0f;
-                d_hy_dz = d_hy_dz + 0.0f;
-                d_hx_dz = d_hx_dz + 0.0f;
-                d_hx_dy = d_hx_dy + 0.0f;
-                if (gxq) {
-                    d_hz_dx = cpml4(d_hz_dx, pbx, pax, pkx, q1[XPML ? 2 : 0]);
-                    d_hy_dx = cpml4(d_hy_dx, pbx, pax, pkx, q1[XPML ? 3 : 0]);
+            if (PM && sh1) {
+                if (PY && gyr) {
+                    d_hz_dy = cpml4(d_hz_dy, splat(pby), splat(pay), splat(pky), q1y[PY ? 2 : 0]);
+                    d_hx_dy = cpml4(d_hx_dy, splat(pby), splat(pay), splat(pky), q1y[PY ? 3 : 0]);
+                } else {
+                    d_hz_dy = d_hz_dy + 0.0f;
+                    d_hx_dy = d_hx_dy + 0.0f;
+                }
+                if (PZ && gzp) {
+                    d_hy_dz = cpml4(d_hy_dz, splat(pbz), splat(paz), splat(pkz), q1z[PZ ? 2 : 0]);
+                    d_hx_dz = cpml4(d_hx_dz, splat(pbz), splat(paz), splat(pkz), q1z[PZ ? 3 : 0]);
+                } else {
+                    d_hy_dz = d_hy_dz + 0.0f;
+                    d_hx_dz = d_hx_dz + 0.0f;
+                }
+                if (PX && gxq) {
+                    d_hz_dx = cpml4(d_hz_dx, pbx, pax, pkx, q1x[PX ? 2 : 0]);
+                    d_hy_dx = cpml4(d_hy_dx, pbx, pax, pkx, q1x[PX ? 3 : 0]);
                 } else {
                     d_hz_dx = d_hz_dx + 0.0f;
                     d_hy_dx = d_hy_dx + 0.0f;
@@ -1193,7 +1233,7 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
             e1x = madd4(ca, ex_c, cb * (d_hz_dy - d_hy_dz));
             e1y = madd4(ca, ey_c, cb * (d_hx_dz - d_hz_dx));
             e1z = madd4(ca, ez_c, cb * (d_hy_dx - d_hx_dy));
-            if (live_row) {
+            if (live_row && lp) {
                 const uint64_t key = (uint64_t)p * g.ny + y;
                 if (point_row_flagged(pts.src, key)) {
                     for (int i = point_row_begin(pts.src, key); i < pts.src.n && pts.src.key[i] == key;
@@ -1229,7 +1269,7 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
                 }
             }
         }
-        if (!live) {
+        if (!live || !lp) {
             e1x = splat(0.f);
             e1y = splat(0.f);
             e1z = splat(0.f);
@@ -1238,11 +1278,9 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
         s_e1[it & 1][1][me] = e1x;
 
         // ---- step 2: H(n+3/2) at plane p-1 ------------------------------
-        if (kLdsCarryE) {
-            e1z_c = s_e1[(it - 1) & 1][0][me];
-            e1x_c = s_e1[(it - 1) & 1][1][me];
-        }
-        if (kLdsCarry) h2x_m = s_h2[b0][1][me];
+        // E(n+1) of plane p-1, as this row put it into LDS
+        e1z_c = s_e1[(it - 1) & 1][0][me];
+        e1x_c = s_e1[(it - 1) & 1][1][me];
         f4 h2x = splat(0.f), h2y = splat(0.f), h2z = splat(0.f);
         const float s2z = __shfl_down(e1z_c.x, 1, 64), s2y = __shfl_down(e1y_c.x, 1, 64);
         if (it >= 0 && R >= 1 && R <= RT - 2) {
@@ -1255,14 +1293,24 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
             f4 d_ez_dy = ez_py - e1z_c, d_ey_dz = e1y - e1y_c;
             f4 d_ex_dz = e1x - e1x_c, d_ez_dx = ez_px2 - e1z_c;
             f4 d_ey_dx = ey_px2 - e1y_c, d_ex_dy = ex_py - e1x_c;
-            if (XPML && sh) {
-                d_ez_dy = d_ez_dy + 0.0f;
-                d_ey_dz = d_ey_dz + 0.0f;
-                d_ex_dz = d_ex_dz + 0.0f;
-                d_ex_dy = d_ex_dy + 0.0f;
-                if (gxq) {
-                    d_ez_dx = cpml4(d_ez_dx, pbx, pax, pkx, qc[0]);
-                    d_ey_dx = cpml4(d_ey_dx, pbx, pax, pkx, qc[XPML ? 1 : 0]);
+            if (PM && sh2) {
+                if (PY && gyr) {
+                    d_ez_dy = cpml4(d_ez_dy, splat(pby), splat(pay), splat(pky), qy[0]);
+                    d_ex_dy = cpml4(d_ex_dy, splat(pby), splat(pay), splat(pky), qy[PY ? 1 : 0]);
+                } else {
+                    d_ez_dy = d_ez_dy + 0.0f;
+                    d_ex_dy = d_ex_dy + 0.0f;
+                }
+                if (PZ && gz_c) {
+                    d_ey_dz = cpml4(d_ey_dz, splat(pbz_c), splat(paz_c), splat(pkz_c), qz[0]);
+                    d_ex_dz = cpml4(d_ex_dz, splat(pbz_c), splat(paz_c), splat(pkz_c), qz[PZ ? 1 : 0]);
+                } else {
+                    d_ey_dz = d_ey_dz + 0.0f;
+                    d_ex_dz = d_ex_dz + 0.0f;
+                }
+                if (PX && gxq) {
+                    d_ez_dx = cpml4(d_ez_dx, pbx, pax, pkx, qx[0]);
+                    d_ey_dx = cpml4(d_ey_dx, pbx, pax, pkx, qx[PX ? 1 : 0]);
                 } else {
                     d_ez_dx = d_ez_dx + 0.0f;
                     d_ey_dx = d_ey_dx + 0.0f;
@@ -1272,19 +1320,19 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
             h2y = madd4(da, h1y_c, -(db * (d_ex_dz - d_ez_dx)));
             h2z = madd4(da, h1z_c, -(db * (d_ey_dx - d_ex_dy)));
         }
-        if (!live) {
+        if (!live || !lq) {
             h2x = splat(0.f);
             h2y = splat(0.f);
             h2z = splat(0.f);
         }
-        s_h2[b1][0][me] = h2z;
-        s_h2[b1][1][me] = h2x;
+        s_h2[0][me] = h2z;
+        s_h2[1][me] = h2x;
         lds_barrier();
 
         // ---- step 2: E(n+2) at plane p-1, stores of the owned cells -----
         const float h2z_l = __shfl_up(h2z.w, 1, 64), h2y_l = __shfl_up(h2y.w, 1, 64);
         if (it >= 1 && R >= 2 && R <= RT - 2) {
-            const f4 hz_my = s_h2[b1][0][below], hx_my = s_h2[b1][1][below];
+            const f4 hz_my = s_h2[0][below], hx_my = s_h2[1][below];
             const f4 hz_mx = f4{h2z_l, h2z.x, h2z.y, h2z.z};
             const f4 hy_mx = f4{h2y_l, h2y.x, h2y.y, h2y.z};
             f4 ca, cb;
@@ -1292,14 +1340,24 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
             f4 d_hz_dy = h2z - hz_my, d_hy_dz = h2y - h2y_m;
             f4 d_hx_dz = h2x - h2x_m, d_hz_dx = h2z - hz_mx;
             f4 d_hy_dx = h2y - hy_mx, d_hx_dy = h2x - hx_my;
-            if (XPML && sh) {
-                d_hz_dy = d_hz_dy + 0.0f;
-                d_hy_dz = d_hy_dz + 0.0f;
-                d_hx_dz = d_hx_dz + 0.0f;
-                d_hx_dy = d_hx_dy + 0.0f;
-                if (gxq) {
-                    d_hz_dx = cpml4(d_hz_dx, pbx, pax, pkx, qc[XPML ? 2 : 0]);
-                    d_hy_dx = cpml4(d_hy_dx, pbx, pax, pkx, qc[XPML ? 3 : 0]);
+            if (PM && sh2) {
+                if (PY && gyr) {
+                    d_hz_dy = cpml4(d_hz_dy, splat(pby), splat(pay), splat(pky), qy[PY ? 2 : 0]);
+                    d_hx_dy = cpml4(d_hx_dy, splat(pby), splat(pay), splat(pky), qy[PY ? 3 : 0]);
+                } else {
+                    d_hz_dy = d_hz_dy + 0.0f;
+                    d_hx_dy = d_hx_dy + 0.0f;
+                }
+                if (PZ && gz_c) {
+                    d_hy_dz = cpml4(d_hy_dz, splat(pbz_c), splat(paz_c), splat(pkz_c), qz[PZ ? 2 : 0]);
+                    d_hx_dz = cpml4(d_hx_dz, splat(pbz_c), splat(paz_c), splat(pkz_c), qz[PZ ? 3 : 0]);
+                } else {
+                    d_hy_dz = d_hy_dz + 0.0f;
+                    d_hx_dz = d_hx_dz + 0.0f;
+                }
+                if (PX && gxq) {
+                    d_hz_dx = cpml4(d_hz_dx, pbx, pax, pkx, qx[PX ? 2 : 0]);
+                    d_hy_dx = cpml4(d_hy_dx, pbx, pax, pkx, qx[PX ? 3 : 0]);
                 } else {
                     d_hz_dx = d_hz_dx + 0.0f;
                     d_hy_dx = d_hy_dx + 0.0f;
@@ -1316,34 +1374,35 @@ __global__ __launch_bounds__(64 * twostep_waves<XPML>()) void k_twostep4(Geo g, 
                 st4(b.f[3] + o, h2x);
                 st4(b.f[4] + o, h2y);
                 st4(b.f[5] + o, h2z);
-                if (XPML && gxq)
-                    for (int k = 0; k < NQ; ++k) st4(b.psi[0][k] + (size_t)(p - 1) * psx + gxi, qc[k]);
+                if (PX && gxq)
+                    for (int k = 0; k < NX; ++k) st4(b.psi[0][k] + (size_t)(p - 1) * psx + gxi, qx[k]);
+                if (PY && gyr)
+                    for (int k = 0; k < NY; ++k) st4(b.psi[1][k] + (size_t)(p - 1) * psy + gyo, qy[k]);
+                if (PZ && gz_c) {
+                    const size_t oz = (size_t)slab_index(g, 2, p - 1, g.nz) * plane + row;
+                    for (int k = 0; k < NZ; ++k) st4(b.psi[2][k] + oz, qz[k]);
+                }
             }
         }
 
+        h2x_m = h2x;
         h2y_m = h2y;
         e1y_c = e1y;
+        h1x_c = h1x;
         h1y_c = h1y;
-        if (!kLdsCarryE) {
-            e1x_c = e1x;
-            e1z_c = e1z;
-        }
-        if (!kLdsCarry) {
-            h2x_m = h2x;
-            h1x_c = h1x;
-            h1z_c = h1z;
-        }
-        for (int k = 0; k < NQ; ++k) qc[k] = q1[k];
+        h1z_c = h1z;
+        for (int k = 0; k < NX; ++k) qx[k] = q1x[k];
+        for (int k = 0; k < NY; ++k) qy[k] = q1y[k];
+        for (int k = 0; k < NZ; ++k) qz[k] = q1z[k];
+        shz_c = shz;
+        gz_c = gzp;
+        pbz_c = pbz;
+        paz_c = paz;
+        pkz_c = pkz;
         ids_c = cur.ids;
         ex_c = cur.ex_n;
         ey_c = cur.ey_n;
         ez_c = cur.ez_n;
-        if (kPF2 == 2) {
-            cur = nxt;
-            nxt = far;
-        } else if (kPF2 == 1) {
-            cur = far;
-        }
     }
 }
 
@@ -1358,17 +1417,30 @@ int twostep_rows_per_tile_x(int lpr) { return kF2x * (64 / lpr) - 3; }
 #endif
 
 hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoints &pts,
-                          hipStream_t st, bool x_pml)
+                          hipStream_t st, int pml_axes)
 {
     const int n = box.gx * box.gy * box.gz;
     if (n <= 0) return hipSuccess;
-    if (x_pml) {
-        if (box.lpr < 3 || box.lpr > 32) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_twostep4<true>), dim3((unsigned)n), dim3(64, kF2x), 0, st, s.g, s.cur,
+    if (pml_axes != 0 && (box.lpr < 3 || box.lpr > 64)) return hipErrorInvalidValue;
+    switch (pml_axes) {
+    case 0:
+        hipLaunchKernelGGL((k_twostep4<0>), dim3((unsigned)n), dim3(64, kF2), 0, st, s.g, s.cur,
                            s.alt, box, pts);
-    } else {
-        hipLaunchKernelGGL((k_twostep4<false>), dim3((unsigned)n), dim3(64, kF2), 0, st, s.g, s.cur,
+        break;
+    case 1:
+        hipLaunchKernelGGL((k_twostep4<1>), dim3((unsigned)n), dim3(64, kF2x), 0, st, s.g, s.cur,
                            s.alt, box, pts);
+        break;
+    case 3:
+        hipLaunchKernelGGL((k_twostep4<3>), dim3((unsigned)n), dim3(64, kF2x), 0, st, s.g, s.cur,
+                           s.alt, box, pts);
+        break;
+    case 7:
+        hipLaunchKernelGGL((k_twostep4<7>), dim3((unsigned)n), dim3(64, kF2x), 0, st, s.g, s.cur,
+                           s.alt, box, pts);
+        break;
+    default:
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
