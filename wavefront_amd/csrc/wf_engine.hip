@@ -259,7 +259,16 @@ struct wf_ctx {
     // tiles, no frame and no third state (needs the x fold)
     bool pair_full = false;
     bool pair_full_env = true;   // WF_TWOSTEP_FULL=0: the frame keeps the y and z slabs
-    Box slab_y[2]{}, slab_z[2]{};
+    int pair_lpr = 64;           // WF_TWOSTEP_LPR: lanes per row of the interior box (64 or 32)
+    // the regions of the full cover (box and its CPML axes, bit 1 x, 2 y,
+    // 4 z): per y and z range (shell below, interior, shell above) the x
+    // interior range and the two x shell ranges (one launch)
+    struct Region {
+        Box box;
+        int axes;
+    };
+    Region regions[18]{};
+    int nregions = 0;
     bool pair_overlap = false;  // WF_PAIR_OVERLAP=1: first step's frame on the side stream
     Box two{}, twox{};
     BoxList fr1h{}, fr1e{}, fr2{};
@@ -612,7 +621,7 @@ void plan_boxes(wf_ctx *c)
         if (th[0] - tl[0] >= 8 && th[1] - tl[1] >= 2 && th[2] - tl[2] >= 2) {
             Box tb{tl[0], th[0], tl[1], th[1], tl[2], th[2], 64, 0, 0, 0, 0};
             tb.tstride = 4 * kTwoQuads;  // adjacent tile columns of 62 owned quads
-            const int r2 = twostep_rows_per_tile();
+            const int r2 = twostep_rows_per_tile(0, 64);
             tb.gx = ((th[0] - tl[0]) / 4 + kTwoQuads - 1) / kTwoQuads;
             tb.gy = (th[1] - tl[1] + r2 - 1) / r2;
             tb.gz = c->zchunk_env ? (th[2] - tl[2] + zc - 1) / zc
@@ -639,7 +648,7 @@ void plan_boxes(wf_ctx *c)
             if (c->pair_xfold) {
                 Box xb{0, n[0], tl[1], th[1], tl[2], th[2], lprx, 2, 0, 0, 0};
                 xb.tstride = n[0] - W;
-                const int rx = twostep_rows_per_tile_x(lprx);
+                const int rx = twostep_rows_per_tile(1, lprx);
                 xb.gy = (th[1] - tl[1] + rx - 1) / rx;
                 xb.gz = c->zchunk_env ? (th[2] - tl[2] + zc - 1) / zc
                                       : fused_slices(2l * xb.gy, th[2] - tl[2], c->cus, 3);
@@ -658,22 +667,34 @@ void plan_boxes(wf_ctx *c)
             // range and the z slabs (every axis) over the whole x, y extent,
             // full 62-quad rows from x = 0 (lanes outside the domain at zero)
             c->pair_full = c->pair_full_env && c->pair_xfold;
+            c->nregions = 0;
             if (c->pair_full) {
-                auto slab = [&](int y0, int y1, int z0, int z1) {
-                    Box b{0, n[0], y0, y1, z0, z1, 64, 0, 0, 0, 0};
-                    b.tstride = 4 * kTwoQuads;
-                    b.gx = (n[0] / 4 + kTwoQuads - 1) / kTwoQuads;
-                    const int rows = twostep_rows_per_tile_x(64);
-                    b.gy = (y1 - y0 + rows - 1) / rows;
-                    b.gz = c->zchunk_env ? (z1 - z0 + zc - 1) / zc
-                                         : fused_slices((long)b.gx * b.gy, z1 - z0, c->cus, 3);
-                    b.gz = std::min(b.gz, z1 - z0);
-                    return b;
+                const int yr[3][2] = {{0, tl[1]}, {tl[1], th[1]}, {th[1], n[1]}};
+                const int zr[3][2] = {{0, tl[2]}, {tl[2], th[2]}, {th[2], n[2]}};
+                auto tile = [&](Box b, int axes) {
+                    const int rows = twostep_rows_per_tile(axes, b.lpr);
+                    b.gy = (b.y1 - b.y0 + rows - 1) / rows;
+                    b.gz = c->zchunk_env ? (b.z1 - b.z0 + zc - 1) / zc
+                                         : fused_slices((long)b.gx * b.gy, b.z1 - b.z0, c->cus, 3);
+                    b.gz = std::min(b.gz, b.z1 - b.z0);
+                    c->regions[c->nregions++] = wf_ctx::Region{b, axes};
                 };
-                c->slab_y[0] = slab(0, tl[1], tl[2], th[2]);
-                c->slab_y[1] = slab(th[1], n[1], tl[2], th[2]);
-                c->slab_z[0] = slab(0, n[1], 0, tl[2]);
-                c->slab_z[1] = slab(0, n[1], th[2], n[2]);
+                for (int zi = 0; zi < 3; ++zi)
+                    for (int yi = 0; yi < 3; ++yi) {
+                        const int y0 = yr[yi][0], y1 = yr[yi][1], z0 = zr[zi][0], z1 = zr[zi][1];
+                        if (y1 <= y0 || z1 <= z0) continue;
+                        const int axes = (yi != 1 ? 2 : 0) | (zi != 1 ? 4 : 0);
+                        // the x interior range: rows of 64 lanes (interior) or 32
+                        const int lpr = axes ? 32 : c->pair_lpr;
+                        Box m{tl[0], th[0], y0, y1, z0, z1, lpr, 0, 0, 0, 0};
+                        m.tstride = 4 * (lpr - 2);
+                        m.gx = ((th[0] - tl[0]) / 4 + lpr - 3) / (lpr - 2);
+                        tile(m, axes);
+                        // the two x shell ranges, tile columns nx - W apart
+                        Box e{0, n[0], y0, y1, z0, z1, lprx, 2, 0, 0, 0};
+                        e.tstride = n[0] - W;
+                        tile(e, axes | 1);
+                    }
             }
         }
     }
@@ -936,22 +957,17 @@ int launch_pair(wf_ctx *c, hipEvent_t *ev = nullptr)
         // the whole domain in two-step tiles: the interior, the x CPML
         // columns, the y slabs, the z slabs; the first step's sources are
         // all applied inside them and its probes all read from the slots
-        // (ev: event pairs around the interior (0, 1), the x columns (2, 3)
-        // and the slabs (4, 5))
-        if (ev) WF_HIP(hipEventRecord(ev[0], st));
-        WF_HIP(c->lt(s, c->two, pts, st, 0));
-        if (ev) {
-            WF_HIP(hipEventRecord(ev[1], st));
-            WF_HIP(hipEventRecord(ev[2], st));
+        // (ev: event pairs around the interior (0, 1), the x shell regions
+        // (2, 3) and the y and z shell regions (4, 5))
+        for (int grp = 0; grp < 3; ++grp) {
+            if (ev) WF_HIP(hipEventRecord(ev[2 * grp], st));
+            for (int k = 0; k < c->nregions; ++k) {
+                const int ax = c->regions[k].axes;
+                if ((grp == 0 && ax == 0) || (grp == 1 && (ax & 1)) || (grp == 2 && ax && !(ax & 1)))
+                    WF_HIP(c->lt(s, c->regions[k].box, pts, st, ax));
+            }
+            if (ev) WF_HIP(hipEventRecord(ev[2 * grp + 1], st));
         }
-        WF_HIP(c->lt(s, c->twox, pts, st, 1));
-        if (ev) {
-            WF_HIP(hipEventRecord(ev[3], st));
-            WF_HIP(hipEventRecord(ev[4], st));
-        }
-        for (int k = 0; k < 2; ++k) WF_HIP(c->lt(s, c->slab_y[k], pts, st, 3));
-        for (int k = 0; k < 2; ++k) WF_HIP(c->lt(s, c->slab_z[k], pts, st, 7));
-        if (ev) WF_HIP(hipEventRecord(ev[5], st));
         if (int rc = point_work(c, c->alt, st, true, false)) return rc;
         if (int rc = point_work(c, c->alt, st, false)) return rc;
         std::swap(c->p, c->alt);
@@ -1430,6 +1446,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     if (const char *ts = std::getenv("WF_TWOSTEP")) c->pair_env = std::atoi(ts) != 0;
     if (const char *tx = std::getenv("WF_TWOSTEP_XFOLD")) c->pair_xfold_env = std::atoi(tx) != 0;
     if (const char *tf = std::getenv("WF_TWOSTEP_FULL")) c->pair_full_env = std::atoi(tf) != 0;
+    if (const char *tl = std::getenv("WF_TWOSTEP_LPR")) c->pair_lpr = std::atoi(tl) == 32 ? 32 : 64;
     if (const char *po = std::getenv("WF_PAIR_OVERLAP")) c->pair_overlap = std::atoi(po) != 0;
     plan_boxes(c);
     // Third state for the frame of two-step passes (its first step's values);
@@ -2020,22 +2037,33 @@ int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
         }
         return cells;
     };
-    auto cells_of = [](const Box &b) {
-        return (uint64_t)(b.x1 - b.x0) * (b.y1 - b.y0) * (b.z1 - b.z0);
-    };
     if (pair_active(c) && c->pair_full) {
-        // two-step tiles over the whole domain, 49 B per cell for two steps
-        // and 16 B per psi term (read once, written once per pass): the
-        // interior, the x CPML columns (x psi), the y and z slabs (the rest
-        // of the psi terms)
-        const Box &t = c->two;
-        const uint64_t yz = (uint64_t)(t.y1 - t.y0) * (t.z1 - t.z0);
-        const uint64_t xterms = 2ull * c->g.T[0] * yz * 2;  // per phase
-        *fused_bytes = 49ull * cells_of(t);
-        *h_bytes = 49ull * 2 * (c->g.T[0] + 4) * yz + 16ull * xterms;
-        uint64_t slabs = 0;
-        for (int k = 0; k < 2; ++k) slabs += cells_of(c->slab_y[k]) + cells_of(c->slab_z[k]);
-        *e_bytes = 49ull * slabs + 16ull * (terms - xterms);
+        // two-step tiles over the whole domain: 49 B per cell for two steps
+        // and 16 B per psi term (2 per phase of each graded axis of a cell,
+        // read once and written once per pass); the interior, the x shell
+        // regions, the y and z shell regions
+        const Geo &g = c->g;
+        uint64_t grp[3] = {0, 0, 0};
+        for (int k = 0; k < c->nregions; ++k) {
+            const Box &b = c->regions[k].box;
+            const int ax = c->regions[k].axes;
+            // the x shell regions: two columns of nx - tstride cells
+            const uint64_t wx = (ax & 1) ? 2ull * (uint64_t)(b.x1 - b.tstride) : (uint64_t)(b.x1 - b.x0);
+            const uint64_t wy = b.y1 - b.y0, wz = b.z1 - b.z0;
+            auto gr = [&](int a, int lo, int hi, int n) {
+                uint64_t k2 = 0;
+                for (int q = lo; q < hi; ++q) k2 += (g.T[a] > 0 && (q < g.T[a] || q >= n - g.T[a])) ? 1 : 0;
+                return k2;
+            };
+            const uint64_t gxc = (ax & 1) ? 2ull * g.T[0] : 0;
+            const uint64_t psi = 2 * (gxc * wy * wz + gr(1, b.y0, b.y1, g.ny) * wx * wz +
+                                      gr(2, b.z0, b.z1, g.nz) * wx * wy);
+            const int gi = ax == 0 ? 0 : ((ax & 1) ? 1 : 2);
+            grp[gi] += 49ull * wx * wy * wz + 16ull * psi;
+        }
+        *fused_bytes = grp[0];
+        *h_bytes = grp[1];
+        *e_bytes = grp[2];
         return WF_OK;
     }
     if (pair_active(c)) {

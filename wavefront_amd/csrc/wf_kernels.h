@@ -214,8 +214,7 @@ hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hip
 hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoints &pts,
                           hipStream_t st, int pml_axes = 0);
 constexpr int kTwoQuads = 62;  // owned quads per tile row (lanes 1..62)
-int twostep_rows_per_tile();
-int twostep_rows_per_tile_x(int lpr);  // owned rows per tile of the CPML variants
+int twostep_rows_per_tile(int pml_axes, int lpr);  // owned rows per tile
 // Tile shapes of the two kernels (owned rows per fused tile -- of the plain
 // variant, or of the CPML / ghost variants -- and rows per split workgroup
 // of lpr = 64).

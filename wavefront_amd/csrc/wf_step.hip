@@ -929,12 +929,12 @@ __device__ __forceinline__ void quad_source(f4 &v, int k, float val, bool hard)
     v.w = k == 3 ? nv : v.w;
 }
 
-// PM: the CPML axes a variant handles, bit 1 x, 2 y, 4 z.  0: the interior
-// box (no cell it computes lies in the shell); 1: the x CPML columns beside
-// it (their y and z ranges outside the shell); 3: the y slabs (x and y);
-// 7: the z slabs (every axis).  CPML variants tile rows box.lpr lanes wide
-// (several rows per wave when narrow), carry each psi term of a graded cell
-// from the first step to the second in registers, and hold lanes, rows and
+// PM: the CPML axes of the cells a variant computes, bit 1 x, 2 y, 4 z (the
+// domain is cut into regions, shell or not on each axis, and each region
+// runs the variant of its axes); LPRT: lanes per row, fixed (64, 32) or box.lpr
+// (0: the x CPML regions, whose rows are narrow), several rows per wave when
+// narrow.  CPML variants carry each psi term of a graded cell from the first
+// step to the second in registers; every variant holds lanes, rows and
 // planes outside the domain at zero, the boundary the reference reads there.
 constexpr int kF2x = 8;  // waves of the CPML variants (their psi terms need the registers)
 // The interior variant loads each plane when it reaches it, with 15 waves
@@ -949,9 +949,9 @@ constexpr int kF2x = 8;  // waves of the CPML variants (their psi terms need the
 // the row above loaded it a plane earlier, and a plane of an XCD's 32 tiles
 // fills its L2).  15 waves keep the LDS within 160 KB.
 template <int PM>
-constexpr int twostep_waves() { return PM ? kF2x : kF2; }
+constexpr int twostep_waves() { return PM == 0 ? kF2 : kF2x; }
 
-template <int PM>
+template <int PM, int LPRT>
 __global__ __launch_bounds__(64 * twostep_waves<PM>()) void k_twostep4(Geo g, Ptrs a, Ptrs b,
                                                                        Box box,
                                                                        TwoStepPoints pts)
@@ -975,9 +975,9 @@ __global__ __launch_bounds__(64 * twostep_waves<PM>()) void k_twostep4(Geo g, Pt
 
     // lanes per row, rows per wave; row R of the tile = w * rpw + r (lanes
     // past rpw * LPR idle); LDS slot of each lane and of its y-1 / y+1 rows
-    const int LPR = PM ? box.lpr : 64;
-    const int rpw = PM ? 64 / LPR : 1;
-    const int r = PM ? lane / LPR : 0, l = PM ? lane - r * LPR : lane;
+    const int LPR = LPRT ? LPRT : box.lpr;
+    const int rpw = 64 / LPR;
+    const int r = LPRT == 64 ? 0 : lane / LPR, l = lane - r * LPR;
     const bool idle = r >= rpw;
     const int RT = FY * rpw;
     const int R = idle ? -8 : w * rpw + r;
@@ -1412,8 +1412,18 @@ __global__ __launch_bounds__(64 * twostep_waves<PM>()) void k_twostep4(Geo g, Pt
 int fused_rows_per_tile(bool pml_or_ghost) { return (pml_or_ghost ? kFYx : kFY) - 1; }
 int fused_rows_per_tile_x() { return WF_FUSED_XPML_FY - 1; }
 int split_rows_per_wg() { return kSY; }
-int twostep_rows_per_tile() { return kF2 - 3; }
-int twostep_rows_per_tile_x(int lpr) { return kF2x * (64 / lpr) - 3; }
+int twostep_rows_per_tile(int pml_axes, int lpr)
+{
+    int waves = twostep_waves<1>();
+    switch (pml_axes) {
+    case 0: waves = twostep_waves<0>(); break;
+    case 2: waves = twostep_waves<2>(); break;
+    case 4: waves = twostep_waves<4>(); break;
+    case 6: waves = twostep_waves<6>(); break;
+    default: break;
+    }
+    return waves * (64 / lpr) - 3;
+}
 #endif
 
 hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoints &pts,
@@ -1421,27 +1431,29 @@ hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoin
 {
     const int n = box.gx * box.gy * box.gz;
     if (n <= 0) return hipSuccess;
-    if (pml_axes != 0 && (box.lpr < 3 || box.lpr > 64)) return hipErrorInvalidValue;
-    switch (pml_axes) {
-    case 0:
-        hipLaunchKernelGGL((k_twostep4<0>), dim3((unsigned)n), dim3(64, kF2), 0, st, s.g, s.cur,
-                           s.alt, box, pts);
-        break;
-    case 1:
-        hipLaunchKernelGGL((k_twostep4<1>), dim3((unsigned)n), dim3(64, kF2x), 0, st, s.g, s.cur,
-                           s.alt, box, pts);
-        break;
-    case 3:
-        hipLaunchKernelGGL((k_twostep4<3>), dim3((unsigned)n), dim3(64, kF2x), 0, st, s.g, s.cur,
-                           s.alt, box, pts);
-        break;
-    case 7:
-        hipLaunchKernelGGL((k_twostep4<7>), dim3((unsigned)n), dim3(64, kF2x), 0, st, s.g, s.cur,
-                           s.alt, box, pts);
-        break;
-    default:
-        return hipErrorInvalidValue;
+    const dim3 grid((unsigned)n);
+#define WF_TWOSTEP_LAUNCH(PM, L)                                                                  \
+    hipLaunchKernelGGL((k_twostep4<PM, L>), grid, dim3(64, twostep_waves<PM>()), 0, st, s.g, s.cur, \
+                       s.alt, box, pts)
+    if (pml_axes & 1) {  // x CPML regions: narrow rows, box.lpr lanes
+        if (box.lpr < 3 || box.lpr > 32) return hipErrorInvalidValue;
+        if (pml_axes == 1) WF_TWOSTEP_LAUNCH(1, 0);
+        else if (pml_axes == 3) WF_TWOSTEP_LAUNCH(3, 0);
+        else if (pml_axes == 5) WF_TWOSTEP_LAUNCH(5, 0);
+        else if (pml_axes == 7) WF_TWOSTEP_LAUNCH(7, 0);
+        else return hipErrorInvalidValue;
+    } else if (pml_axes == 0) {
+        if (box.lpr == 64) WF_TWOSTEP_LAUNCH(0, 64);
+        else if (box.lpr == 32) WF_TWOSTEP_LAUNCH(0, 32);
+        else return hipErrorInvalidValue;
+    } else {  // y and/or z CPML regions: rows of 32 lanes
+        if (box.lpr != 32) return hipErrorInvalidValue;
+        if (pml_axes == 2) WF_TWOSTEP_LAUNCH(2, 32);
+        else if (pml_axes == 4) WF_TWOSTEP_LAUNCH(4, 32);
+        else if (pml_axes == 6) WF_TWOSTEP_LAUNCH(6, 32);
+        else return hipErrorInvalidValue;
     }
+#undef WF_TWOSTEP_LAUNCH
     return hipGetLastError();
 }
 
