@@ -84,18 +84,17 @@ __global__ void k_sources(Ptrs p, const DevSource *__restrict__ src, int ncol,
 // ---------------------------------------------------------------------------
 // Probes (src/probe.rs:126-138): one lane per probe, frequencies in order.
 // ---------------------------------------------------------------------------
-// slot / inslot (two-step passes, first step): probes whose cell the
-// two-step kernel owns read the value it recorded.
+// slot (two-step passes, first step): the values the two-step kernels
+// recorded, one per probe.
 __device__ __forceinline__ void accumulate_probe(const Ptrs &p, const DevProbe &q, int i, int n,
                                                  int nf, const double *__restrict__ cos_tab,
                                                  const double *__restrict__ sin_tab,
                                                  const uint8_t *__restrict__ active,
                                                  uint64_t row, double *sums,
                                                  unsigned long long *counts, float *trace,
-                                                 const float *slot = nullptr,
-                                                 const uint8_t *inslot = nullptr)
+                                                 const float *slot = nullptr)
 {
-    const float v32 = (slot && inslot[i]) ? slot[i] : p.f[q.comp][q.off];
+    const float v32 = slot ? slot[i] : p.f[q.comp][q.off];
     if (trace) trace[row * (uint64_t)n + i] = v32;
     if (!active[row * (uint64_t)n + i]) return;
     const double v = (double)v32;
@@ -114,12 +113,12 @@ __global__ void k_probes(Ptrs p, const DevProbe *__restrict__ pr, int n, int nf,
                          const double *__restrict__ cos_tab, const double *__restrict__ sin_tab,
                          const uint8_t *__restrict__ active, const uint64_t *__restrict__ step,
                          uint64_t base, double *sums, unsigned long long *counts, float *trace,
-                         const float *slot, const uint8_t *inslot)
+                         const float *slot)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     accumulate_probe(p, pr[i], i, n, nf, cos_tab, sin_tab, active, *step - base, sums, counts,
-                     trace, slot, inslot);
+                     trace, slot);
 }
 
 // Batched point work, one workgroup per scene: sources, then (next launch)
@@ -246,34 +245,19 @@ struct wf_ctx {
     BoxList lofix{};    // learned faces without a CPML: the low faces' boundary cells
     bool pml_fused = false;  // WF_PML_FUSED=1: single pass over the shell too (slower, see launch_step)
     BoxList whole{};    // split variant: the whole domain
-    // two steps per pass (launch_pair): the two-step kernel's box; the frame
-    // of the pass's first step (H over the rest and all six boundary layers
-    // of the box, E over the rest and its three low layers), computed into a
-    // third state C, and of its second step (the rest, C -> B); the point
-    // tables of the kernel's first step
+    // two steps per pass (launch_pair): the domain cut into regions, shell
+    // below / interior / shell above on each axis, each swept by the
+    // two-step variant of its CPML axes (bit 1 x, 2 y, 4 z): per y and z range
+    // the x interior range and the two x shell ranges (one launch, tile
+    // columns nx - W apart); the point tables of the kernels' first step
     bool pair_env = true;   // WF_TWOSTEP=0: single steps only
     bool pair_geo = false;  // plan_boxes: the geometry allows it
-    bool pair_xfold = false;  // the x CPML columns are two-step tiles too (box twox)
-    bool pair_xfold_env = true;  // WF_TWOSTEP_XFOLD=0: the frame keeps the x slabs
-    // the y and z slabs as two-step tiles too: the whole domain in two-step
-    // tiles, no frame and no third state (needs the x fold)
-    bool pair_full = false;
-    bool pair_full_env = true;   // WF_TWOSTEP_FULL=0: the frame keeps the y and z slabs
-    int pair_lpr = 64;           // WF_TWOSTEP_LPR: lanes per row of the interior box (64 or 32)
-    // the regions of the full cover (box and its CPML axes, bit 1 x, 2 y,
-    // 4 z): per y and z range (shell below, interior, shell above) the x
-    // interior range and the two x shell ranges (one launch)
     struct Region {
         Box box;
         int axes;
     };
     Region regions[18]{};
     int nregions = 0;
-    bool pair_overlap = false;  // WF_PAIR_OVERLAP=1: first step's frame on the side stream
-    Box two{}, twox{};
-    BoxList fr1h{}, fr1e{}, fr2{};
-    Ptrs third{};
-    float *d_fields3 = nullptr, *d_psi3 = nullptr;
     struct PointTable {
         uint32_t *bits = nullptr;
         uint64_t *key = nullptr;
@@ -281,7 +265,6 @@ struct wf_ctx {
         int n = 0;
     } src_pts, prb_pts;
     float *d_pslot = nullptr;   // probe values of a pass's first step, per probe
-    uint8_t *d_pin = nullptr;   // 1: the probe's cell belongs to the two-step box
     hipStream_t side = nullptr;            // frame H kernel, overlapped
     hipEvent_t fork = nullptr, join = nullptr;
     bool overlap = false;                  // WF_OVERLAP=1
@@ -452,7 +435,6 @@ void free_probes(wf_ctx *c)
     dfree(c->d_trace);
     free_points(c->prb_pts);
     dfree(c->d_pslot);
-    dfree(c->d_pin);
     c->nprobe = 0;
     c->nfreq = 0;
     c->probe_steps = 0;
@@ -603,100 +585,56 @@ void plan_boxes(wf_ctx *c)
         c->framex = fx;
         c->xfold = true;
     }
-    // Two steps per pass: the two-step kernel's box keeps two cells (x: one
-    // quad) below it and one above it on every axis outside the shell, and
-    // on y and z one more above it inside the domain (its upper margin reads
-    // E^n there); no periodic axis.  Its first step's frame covers the rest
-    // of the domain plus the box's boundary layers (H: all six, so that E
-    // just above the box finds H(n+1/2) below it; E: the three low ones,
-    // which H(n+3/2) just below the box reads), its second step's frame the
-    // rest alone.
+    // Two steps per pass: every region's tiles compute a margin two cells
+    // below and one above their box on each axis (x: a quad), which must be
+    // outside the shell on the axes the region's variant treats as plain and
+    // inside the domain or at its zero boundary.  Per axis the interior range
+    // [lo, hi) keeps that margin outside the shell: x in [Wx, nx - Wx) with
+    // Wx = T + 4 (the x shell ranges [0, Wx) and [nx - Wx, nx) run narrow rows
+    // of Wx/4 + 2 lanes), y and z in [T + 2, n - max(T, 1) - 1) (the shell
+    // ranges below and above take the rest).  No periodic axis.
     c->pair_geo = false;
-    if (!g.periodic) {
-        int tl[3], th[3];
-        for (int a = 0; a < 3; ++a) {
-            tl[a] = g.T[a] + (a == 0 ? 4 : 2);
-            th[a] = a == 0 ? n[a] - g.T[a] - 4 : n[a] - std::max(g.T[a], 1) - 1;
-        }
-        if (th[0] - tl[0] >= 8 && th[1] - tl[1] >= 2 && th[2] - tl[2] >= 2) {
-            Box tb{tl[0], th[0], tl[1], th[1], tl[2], th[2], 64, 0, 0, 0, 0};
-            tb.tstride = 4 * kTwoQuads;  // adjacent tile columns of 62 owned quads
-            const int r2 = twostep_rows_per_tile(0, 64);
-            tb.gx = ((th[0] - tl[0]) / 4 + kTwoQuads - 1) / kTwoQuads;
-            tb.gy = (th[1] - tl[1] + r2 - 1) / r2;
-            tb.gz = c->zchunk_env ? (th[2] - tl[2] + zc - 1) / zc
-                                  : fused_slices((long)tb.gx * tb.gy, th[2] - tl[2], c->cus, 3);
-            tb.gz = std::min(tb.gz, th[2] - tl[2]);
-            c->two = tb;
-            auto frame_around = [&](int x0, int x1, int y0, int y1, int z0, int z1) {
-                BoxList f{};
-                add_box(f, split_box(0, n[0], 0, n[1], 0, z0, fz));
-                add_box(f, split_box(0, n[0], 0, n[1], z1, n[2], fz));
-                add_box(f, split_box(0, n[0], 0, y0, z0, z1, fz));
-                add_box(f, split_box(0, n[0], y1, n[1], z0, z1, fz));
-                add_box(f, split_box(0, x0, y0, y1, z0, z1, fz));
-                add_box(f, split_box(x1, n[0], y0, y1, z0, z1, fz));
-                return f;
-            };
-            // x fold: the x CPML layers and the margin beside them, [0, T+4)
-            // and [nx-T-4, nx), become two-step tiles too (x-only CPML
-            // variant, rows of T/4 + 3 lanes), so the frame keeps only the
-            // y and z slabs and the two-step box spans x on its y, z range
-            const int W = g.T[0] + 4, lprx = g.T[0] / 4 + 3;
-            c->pair_xfold = c->pair_xfold_env && lprx <= 32 && n[0] >= 2 * W + 8;
-            int fx0 = tl[0], fx1 = th[0];
-            if (c->pair_xfold) {
-                Box xb{0, n[0], tl[1], th[1], tl[2], th[2], lprx, 2, 0, 0, 0};
-                xb.tstride = n[0] - W;
-                const int rx = twostep_rows_per_tile(1, lprx);
-                xb.gy = (th[1] - tl[1] + rx - 1) / rx;
-                xb.gz = c->zchunk_env ? (th[2] - tl[2] + zc - 1) / zc
-                                      : fused_slices(2l * xb.gy, th[2] - tl[2], c->cus, 3);
-                xb.gz = std::min(xb.gz, th[2] - tl[2]);
-                c->twox = xb;
-                fx0 = 0;
-                fx1 = n[0];
+    c->nregions = 0;
+    const int Wx = g.T[0] + 4, lprx = Wx / 4 + 2;
+    int tl[3], th[3];
+    tl[0] = Wx;
+    th[0] = n[0] - Wx;
+    for (int a = 1; a < 3; ++a) {
+        tl[a] = g.T[a] + 2;
+        th[a] = n[a] - std::max(g.T[a], 1) - 1;
+    }
+    if (!g.periodic && lprx <= 32 && th[0] - tl[0] >= 8 && th[1] >= tl[1] && th[2] >= tl[2]) {
+        const int yr[3][2] = {{0, tl[1]}, {tl[1], th[1]}, {th[1], n[1]}};
+        const int zr[3][2] = {{0, tl[2]}, {tl[2], th[2]}, {th[2], n[2]}};
+        auto tile = [&](Box b, int axes) {
+            const int rows = twostep_rows_per_tile(axes, b.lpr);
+            b.gy = (b.y1 - b.y0 + rows - 1) / rows;
+            b.gz = c->zchunk_env ? (b.z1 - b.z0 + zc - 1) / zc
+                                 : fused_slices((long)b.gx * b.gy, b.z1 - b.z0, c->cus, 3);
+            b.gz = std::min(b.gz, b.z1 - b.z0);
+            c->regions[c->nregions++] = wf_ctx::Region{b, axes};
+        };
+        for (int zi = 0; zi < 3; ++zi)
+            for (int yi = 0; yi < 3; ++yi) {
+                const int y0 = yr[yi][0], y1 = yr[yi][1], z0 = zr[zi][0], z1 = zr[zi][1];
+                if (y1 <= y0 || z1 <= z0) continue;
+                const int axes = (yi != 1 ? 2 : 0) | (zi != 1 ? 4 : 0);
+                // the x interior range: rows of 64 lanes in the interior, 32
+                // in the y and z shell ranges (taller tiles; measured at
+                // 1024^3: interior 32 lanes 8.5-8.9 ms per step against
+                // 6.7, shell ranges 64 lanes 2.40 ms per pass against 2.30)
+                const int lpr = axes ? 32 : 64;
+                Box m{tl[0], th[0], y0, y1, z0, z1, lpr, 0, 0, 0, 0};
+                m.tstride = 4 * (lpr - 2);
+                m.gx = ((th[0] - tl[0]) / 4 + lpr - 3) / (lpr - 2);
+                tile(m, axes);
+                // the two x shell ranges (56 cells wide at 1024^3 with 16-lane
+                // rows: 7.15 ms per step against 6.74)
+                Box e{0, n[0], y0, y1, z0, z1, lprx, 2, 0, 0, 0};
+                e.tstride = n[0] - Wx;
+                tile(e, axes | 1);
             }
-            c->fr1h = frame_around(c->pair_xfold ? 0 : tl[0] + 4, c->pair_xfold ? n[0] : th[0] - 4,
-                                   tl[1] + 1, th[1] - 1, tl[2] + 1, th[2] - 1);
-            c->fr1e = frame_around(c->pair_xfold ? 0 : tl[0] + 4, fx1, tl[1] + 1, th[1], tl[2] + 1,
-                                   th[2]);
-            c->fr2 = frame_around(fx0, fx1, tl[1], th[1], tl[2], th[2]);
-            c->pair_geo = true;
-            // full cover: the y slabs (x and y CPML variant) over the box's z
-            // range and the z slabs (every axis) over the whole x, y extent,
-            // full 62-quad rows from x = 0 (lanes outside the domain at zero)
-            c->pair_full = c->pair_full_env && c->pair_xfold;
-            c->nregions = 0;
-            if (c->pair_full) {
-                const int yr[3][2] = {{0, tl[1]}, {tl[1], th[1]}, {th[1], n[1]}};
-                const int zr[3][2] = {{0, tl[2]}, {tl[2], th[2]}, {th[2], n[2]}};
-                auto tile = [&](Box b, int axes) {
-                    const int rows = twostep_rows_per_tile(axes, b.lpr);
-                    b.gy = (b.y1 - b.y0 + rows - 1) / rows;
-                    b.gz = c->zchunk_env ? (b.z1 - b.z0 + zc - 1) / zc
-                                         : fused_slices((long)b.gx * b.gy, b.z1 - b.z0, c->cus, 3);
-                    b.gz = std::min(b.gz, b.z1 - b.z0);
-                    c->regions[c->nregions++] = wf_ctx::Region{b, axes};
-                };
-                for (int zi = 0; zi < 3; ++zi)
-                    for (int yi = 0; yi < 3; ++yi) {
-                        const int y0 = yr[yi][0], y1 = yr[yi][1], z0 = zr[zi][0], z1 = zr[zi][1];
-                        if (y1 <= y0 || z1 <= z0) continue;
-                        const int axes = (yi != 1 ? 2 : 0) | (zi != 1 ? 4 : 0);
-                        // the x interior range: rows of 64 lanes (interior) or 32
-                        const int lpr = axes ? 32 : c->pair_lpr;
-                        Box m{tl[0], th[0], y0, y1, z0, z1, lpr, 0, 0, 0, 0};
-                        m.tstride = 4 * (lpr - 2);
-                        m.gx = ((th[0] - tl[0]) / 4 + lpr - 3) / (lpr - 2);
-                        tile(m, axes);
-                        // the two x shell ranges, tile columns nx - W apart
-                        Box e{0, n[0], y0, y1, z0, z1, lprx, 2, 0, 0, 0};
-                        e.tstride = n[0] - W;
-                        tile(e, axes | 1);
-                    }
-            }
-        }
+        c->pair_geo = true;
     }
     Box all{0, n[0], 0, n[1], 0, n[2], 64, 0, 0, 0, 0};
     all.gx = (n[0] + 255) / 256;
@@ -803,7 +741,7 @@ int point_work(wf_ctx *c, const Ptrs &p, hipStream_t st, bool slot, bool sources
         hipLaunchKernelGGL(k_probes, dim3((c->nprobe + 63) / 64), dim3(64), 0, st, p,
                            c->d_probe, (int)c->nprobe, (int)c->nfreq, c->d_cos, c->d_sin,
                            c->d_active, c->d_step, c->probe_base, c->d_sums, c->d_counts,
-                           c->d_trace, slot ? c->d_pslot : nullptr, slot ? c->d_pin : nullptr);
+                           c->d_trace, slot ? c->d_pslot : nullptr);
     }
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, c->d_step);
     WF_HIP(hipGetLastError());
@@ -926,21 +864,19 @@ bool pair_tables(const wf_ctx *c) { return c->fused && c->pair_env && c->pair_ge
 
 bool pair_active(const wf_ctx *c)
 {
-    if (!pair_tables(c) || !(c->pair_full || c->d_fields3) || c->pml_fused) return false;
+    if (!pair_tables(c) || c->pml_fused) return false;
     for (int f = 0; f < 6; ++f)
         if (c->hook_on[f]) return false;
     return true;
 }
 
 // Two loop iterations in one pass (src/engine.rs:747-782 twice): the
-// two-step kernel over its box, state A -> B, adds the first step's sources
-// to E(n+1) on the way and records its probes in the box; the frame's first
-// step A -> C over the rest and the box's boundary layers, then the first
-// step's sources and probes (probes in the box from the kernel's slots);
-// the frame's second step C -> B (its E reads the kernel's H(n+3/2) just
-// inside the box), then the second step's sources and probes on B.  With
-// ev (6 timing events): event pairs around the two-step kernel (0, 1), the
-// first step's frame (2, 3) and the second step's frame (4, 5).
+// two-step kernels over every region, state A -> B, add the first step's
+// sources to E(n+1) on the way and record its probes; then the first step's
+// probes (from the slots) and step counter, and the second step's sources,
+// probes and counter on B.  With ev (6 timing events): event pairs around
+// the interior (0, 1), the x shell regions (2, 3) and the y and z shell
+// regions (4, 5).
 int launch_pair(wf_ctx *c, hipEvent_t *ev = nullptr)
 {
     hipStream_t st = c->stream;
@@ -953,57 +889,16 @@ int launch_pair(wf_ctx *c, hipEvent_t *ev = nullptr)
     pts.step = c->d_step;
     pts.slot = c->d_pslot;
     StepLaunch s{c->g, c->p, c->alt, c->pml, c->fzchunk, st};
-    if (c->pair_full) {
-        // the whole domain in two-step tiles: the interior, the x CPML
-        // columns, the y slabs, the z slabs; the first step's sources are
-        // all applied inside them and its probes all read from the slots
-        // (ev: event pairs around the interior (0, 1), the x shell regions
-        // (2, 3) and the y and z shell regions (4, 5))
-        for (int grp = 0; grp < 3; ++grp) {
-            if (ev) WF_HIP(hipEventRecord(ev[2 * grp], st));
-            for (int k = 0; k < c->nregions; ++k) {
-                const int ax = c->regions[k].axes;
-                if ((grp == 0 && ax == 0) || (grp == 1 && (ax & 1)) || (grp == 2 && ax && !(ax & 1)))
-                    WF_HIP(c->lt(s, c->regions[k].box, pts, st, ax));
-            }
-            if (ev) WF_HIP(hipEventRecord(ev[2 * grp + 1], st));
+    for (int grp = 0; grp < 3; ++grp) {
+        if (ev) WF_HIP(hipEventRecord(ev[2 * grp], st));
+        for (int k = 0; k < c->nregions; ++k) {
+            const int ax = c->regions[k].axes;
+            if ((grp == 0 && ax == 0) || (grp == 1 && (ax & 1)) || (grp == 2 && ax && !(ax & 1)))
+                WF_HIP(c->lt(s, c->regions[k].box, pts, st, ax));
         }
-        if (int rc = point_work(c, c->alt, st, true, false)) return rc;
-        if (int rc = point_work(c, c->alt, st, false)) return rc;
-        std::swap(c->p, c->alt);
-        c->step += 2;
-        return WF_OK;
+        if (ev) WF_HIP(hipEventRecord(ev[2 * grp + 1], st));
     }
-    StepLaunch s1{c->g, c->p, c->third, c->pml, c->fzchunk, st};
-    if (c->pair_overlap && !ev && c->side) {
-        // the first step's frame reads state A and writes C, the two-step
-        // kernel reads A and writes B: side by side
-        WF_HIP(hipEventRecord(c->fork, st));
-        WF_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
-        WF_HIP(c->lh(s1, c->fr1h, c->side));
-        WF_HIP(c->le(s1, c->fr1e, c->side));
-        WF_HIP(hipEventRecord(c->join, c->side));
-        WF_HIP(c->lt(s, c->two, pts, st, 0));
-        if (c->pair_xfold) WF_HIP(c->lt(s, c->twox, pts, st, 1));
-        WF_HIP(hipStreamWaitEvent(st, c->join, 0));
-    } else {
-        if (ev) WF_HIP(hipEventRecord(ev[0], st));
-        WF_HIP(c->lt(s, c->two, pts, st, 0));
-        if (c->pair_xfold) WF_HIP(c->lt(s, c->twox, pts, st, 1));
-        if (ev) {
-            WF_HIP(hipEventRecord(ev[1], st));
-            WF_HIP(hipEventRecord(ev[2], st));
-        }
-        WF_HIP(c->lh(s1, c->fr1h, st));
-        WF_HIP(c->le(s1, c->fr1e, st));
-        if (ev) WF_HIP(hipEventRecord(ev[3], st));
-    }
-    if (int rc = point_work(c, c->third, st, true)) return rc;
-    if (ev) WF_HIP(hipEventRecord(ev[4], st));
-    StepLaunch s2{c->g, c->third, c->alt, c->pml, c->fzchunk, st};
-    WF_HIP(c->lh(s2, c->fr2, st));
-    WF_HIP(c->le(s2, c->fr2, st));
-    if (ev) WF_HIP(hipEventRecord(ev[5], st));
+    if (int rc = point_work(c, c->alt, st, true, false)) return rc;
     if (int rc = point_work(c, c->alt, st, false)) return rc;
     std::swap(c->p, c->alt);
     c->step += 2;
@@ -1444,33 +1339,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
             c->lt = fma::launch_twostep;
         }
     if (const char *ts = std::getenv("WF_TWOSTEP")) c->pair_env = std::atoi(ts) != 0;
-    if (const char *tx = std::getenv("WF_TWOSTEP_XFOLD")) c->pair_xfold_env = std::atoi(tx) != 0;
-    if (const char *tf = std::getenv("WF_TWOSTEP_FULL")) c->pair_full_env = std::atoi(tf) != 0;
-    if (const char *tl = std::getenv("WF_TWOSTEP_LPR")) c->pair_lpr = std::atoi(tl) == 32 ? 32 : 64;
-    if (const char *po = std::getenv("WF_PAIR_OVERLAP")) c->pair_overlap = std::atoi(po) != 0;
     plan_boxes(c);
-    // Third state for the frame of two-step passes (its first step's values);
-    // none when two-step tiles cover the whole domain.
-    if (c->fused && c->pair_env && c->pair_geo && !c->pair_full) {
-        if (int rc = dmalloc(&c->d_fields3, 6 * N * sizeof(float))) return bail(rc);
-        if (hipMemsetAsync(c->d_fields3, 0, 6 * N * sizeof(float), c->stream) != hipSuccess)
-            return bail(fail(WF_ERR_HIP, "hipMemset(fields) failed"));
-        if (psi_total) {
-            if (int rc = dmalloc(&c->d_psi3, psi_total * sizeof(float))) return bail(rc);
-            if (hipMemsetAsync(c->d_psi3, 0, psi_total * sizeof(float), c->stream) != hipSuccess)
-                return bail(fail(WF_ERR_HIP, "hipMemset(psi) failed"));
-        }
-        c->third = c->p;
-        for (int i = 0; i < 6; ++i) c->third.f[i] = c->d_fields3 + (uint64_t)i * N;
-        float *pp = c->d_psi3;
-        for (int a = 0; a < 3; ++a)
-            for (int t = 0; t < 4; ++t) {
-                c->third.psi[a][t] = slab[a] ? pp : nullptr;
-                pp += slab[a];
-            }
-        if (hipStreamSynchronize(c->stream) != hipSuccess)
-            return bail(fail(WF_ERR_HIP, "initialisation failed"));
-    }
     // Field placement (see choose_placement): by default eight extra state
     // buffers tried, one held at a time beside the current ones, when one
     // state's fields take at least 8 GiB (4 steps per arrangement timed, 17
@@ -1515,8 +1384,6 @@ void wf_destroy(wf_ctx *c)
     dfree(c->d_fields);
     dfree(c->d_fields2);
     dfree(c->d_psi2);
-    dfree(c->d_fields3);
-    dfree(c->d_psi3);
     dfree(c->d_step);
     dfree(c->d_id);
     dfree(c->d_coef);
@@ -1654,16 +1521,9 @@ int wf_set_probes(wf_ctx *c, uint32_t n, const wf_probe_site *sites, uint64_t ns
     WF_HIP(hipMemset(c->d_sums, 0, 2ull * (nf ? nf : 1) * sizeof(double)));
     WF_HIP(hipMemset(c->d_counts, 0, n * sizeof(unsigned long long)));
     if (pair_tables(c)) {
-        // the two-step kernel records the probes of its box at a pass's
-        // first step into one slot per probe
+        // the two-step regions own every cell: each probe's value of a
+        // pass's first step is recorded into its slot by the tile owning it
         std::vector<std::pair<uint64_t, PointEnt>> v(n);
-        std::vector<uint8_t> in(n);
-        // the two-step tiles own the box (x folded, every x of its rows;
-        // with the slabs, the whole domain)
-        const Box &t = c->two;
-        const int ox0 = c->pair_xfold ? 0 : t.x0, ox1 = c->pair_xfold ? c->g.nx : t.x1;
-        const int oy0 = c->pair_full ? 0 : t.y0, oy1 = c->pair_full ? c->g.ny : t.y1;
-        const int oz0 = c->pair_full ? 0 : t.z0, oz1 = c->pair_full ? c->g.nz : t.z1;
         for (uint32_t p = 0; p < n; ++p) {
             const wf_probe_site &q = sites[p];
             PointEnt e{};
@@ -1671,14 +1531,10 @@ int wf_set_probes(wf_ctx *c, uint32_t n, const wf_probe_site *sites, uint64_t ns
             e.comp = (uint16_t)q.component;
             e.col = p;
             v[p] = {(uint64_t)q.z * c->g.ny + q.y, e};
-            in[p] = (int)q.x >= ox0 && (int)q.x < ox1 && (int)q.y >= oy0 && (int)q.y < oy1 &&
-                    (int)q.z >= oz0 && (int)q.z < oz1;
         }
         if (int rc = upload_points(c, v, c->prb_pts)) return rc;
         if (int rc = dmalloc(&c->d_pslot, n * sizeof(float))) return rc;
-        if (int rc = dmalloc(&c->d_pin, n)) return rc;
         WF_HIP(hipMemset(c->d_pslot, 0, n * sizeof(float)));
-        WF_HIP(hipMemcpy(c->d_pin, in.data(), n, hipMemcpyHostToDevice));
     }
     c->nprobe = n;
     c->nfreq = nf;
@@ -2016,8 +1872,7 @@ const char *wf_kernel_variant(const wf_ctx *c)
 {
     if (!c) return "";
     if (!c->fused) return "split";
-    if (!pair_active(c)) return "fused";
-    return c->pair_full ? "twostep_full" : "twostep";
+    return pair_active(c) ? "twostep" : "fused";
 }
 
 int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
@@ -2029,15 +1884,7 @@ int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
     // non-identity psi terms per phase: 2 per graded axis per cell
     const uint64_t terms = 2ull * (2ull * c->g.T[0] * ny * nz + 2ull * c->g.T[1] * nx * nz +
                                    2ull * c->g.T[2] * nx * ny);
-    auto box_cells = [](const BoxList &bl) {
-        uint64_t cells = 0;
-        for (int k = 0; k < bl.n; ++k) {
-            const Box &b = bl.b[k];
-            cells += (uint64_t)(b.x1 - b.x0) * (b.y1 - b.y0) * (b.z1 - b.z0);
-        }
-        return cells;
-    };
-    if (pair_active(c) && c->pair_full) {
+    if (pair_active(c)) {
         // two-step tiles over the whole domain: 49 B per cell for two steps
         // and 16 B per psi term (2 per phase of each graded axis of a cell,
         // read once and written once per pass); the interior, the x shell
@@ -2064,21 +1911,6 @@ int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
         *fused_bytes = grp[0];
         *h_bytes = grp[1];
         *e_bytes = grp[2];
-        return WF_OK;
-    }
-    if (pair_active(c)) {
-        // two-step kernel: 49 B per cell of its box for two steps; the
-        // frame of a pass's first step (H over fr1h, E over fr1e) and of its
-        // second (fr2): 37 B per cell per kernel and 8 B per psi term
-        // (x folded: the x CPML columns too, with 32 B per graded cell for
-        // its four x psi terms, read once and written once per pass)
-        const Box &t = c->two;
-        const uint64_t yz = (uint64_t)(t.y1 - t.y0) * (t.z1 - t.z0);
-        const uint64_t xcols = c->pair_xfold ? 2ull * (c->g.T[0] + 4) : 0;
-        const uint64_t xterms = c->pair_xfold ? 2ull * c->g.T[0] * yz * 2 : 0;  // per phase
-        *fused_bytes = 49ull * ((uint64_t)(t.x1 - t.x0) + xcols) * yz + 16ull * xterms;
-        *h_bytes = 37ull * (box_cells(c->fr1h) + box_cells(c->fr1e)) + 16ull * (terms - xterms);
-        *e_bytes = 37ull * 2 * box_cells(c->fr2) + 16ull * (terms - xterms);
         return WF_OK;
     }
     if (single_pass(c)) {

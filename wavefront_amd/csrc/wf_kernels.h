@@ -202,18 +202,16 @@ hipError_t launch_fused(const StepLaunch &s, const Box &box, bool pml_cells, hip
                         bool x_only = false);
 // Two-step kernel over a box: E(n+2), H(n+3/2) of the box's cells into
 // s.alt from s.cur (E^n, H^{n-1/2}), the sources of step n+1 applied and
-// its probes recorded on the way.  Every cell within two cells below and
-// one above the box on each axis (x: one quad below and above) must lie
-// outside the CPML shell and inside the domain, no axis periodic.  Tiles:
-// box.gx columns of kTwoQuads quads, box.gy row groups of
-// twostep_rows_per_tile() rows, box.gz even z slices.
-// pml_axes (bit 1 x, 2 y, 4 z; 0, 1, 3 or 7): the CPML axes of the cells
-// the tiles compute, margins included; the CPML variants tile rows box.lpr
-// lanes wide (box.lpr - 2 quads owned, tile tx owning [x0 + tx tstride,
-// + 4 (lpr - 2)) clipped to x1) and read cells outside the domain as zero.
+// its probes recorded on the way.  pml_axes (bit 1 x, 2 y, 4 z): the CPML
+// axes of the cells the tiles compute, margins included (two cells below
+// and one above the box on each axis, x: a quad); on the other axes those
+// cells must lie outside the shell; cells outside the domain read as zero;
+// no axis periodic.  Rows are box.lpr lanes wide (64 or 32 without an x
+// CPML, 3..32 with one), box.lpr - 2 quads owned: tile column tx owns
+// [x0 + tx tstride, + 4 (lpr - 2)) clipped to x1; box.gy row groups of
+// twostep_rows_per_tile(pml_axes, lpr) rows; box.gz even z slices.
 hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoints &pts,
                           hipStream_t st, int pml_axes = 0);
-constexpr int kTwoQuads = 62;  // owned quads per tile row (lanes 1..62)
 int twostep_rows_per_tile(int pml_axes, int lpr);  // owned rows per tile
 // Tile shapes of the two kernels (owned rows per fused tile -- of the plain
 // variant, or of the CPML / ghost variants -- and rows per split workgroup

@@ -851,25 +851,26 @@ __global__ __launch_bounds__((64 * fused_waves<PM, GHOST>())) void k_fused4(Geo 
 
 // ===========================================================================
 // Two-step kernel: two whole timesteps of a box in one sweep, state A (E^n,
-// H^{n-1/2}) -> state B (E^{n+2}, H^{n+3/2}), no CPML (every cell it
-// computes lies outside the PML shell).  Per owned cell 24 B read + 24 B
-// written + 1 B id for the two steps: half the single-pass kernel's bytes
-// per step.
+// H^{n-1/2}) -> state B (E^{n+2}, H^{n+3/2}).  Per owned cell 24 B read +
+// 24 B written + 1 B id (+ 8 B per psi term) for the two steps: half the
+// single-pass kernel's bytes per step.
 //
-// Workgroup = kF2 waves x 64 lanes; lane l holds the quad
-// x = x0 - 4 + 4 (kTwoQuads tx + l), wave w the row y = y0 + (kF2-3) ty - 2 + w.
-// Two steps reach two cells down and one up on each axis, so a tile also
-// computes a margin that it never stores: lanes 0 and 63 (one quad on each
-// side), waves 0 and 1 (two rows below), wave kF2-1 (one row above), two
-// planes below its z slice and one above.  The outermost margin values are
-// wrong (their neighbours lie outside the tile), and each step reads them
-// only for margin cells one cell further out:
-//   step 1 at plane p: H(n+1/2) on every wave, E(n+1) on waves >= 1;
+// Rows of LPR lanes (64 / LPR rows per wave); lane l of a row holds the quad
+// x = tx0 - 4 + 4l, row R of the tile (R = wave * rows per wave + row) the
+// row y = y0 + (rows - 3) ty - 2 + R.  Two steps reach two cells down and one
+// up on each axis, so a tile also computes a margin that it never stores:
+// the first and last lane of each row (a quad on each side), rows 0 and 1
+// (two below), the last row (one above), two planes below its z slice and
+// one above.  The outermost margin values are wrong (their neighbours lie
+// outside the tile), and each step reads them only for margin cells one
+// cell further out:
+//   step 1 at plane p: H(n+1/2) on every row, E(n+1) on rows >= 1;
 //   step 2 at plane p-1 (H(n+3/2) at p-1 reads E(n+1) at p): H(n+3/2) on
-//   waves 1..kF2-2, E(n+2) on waves 2..kF2-2, the rows the tile owns.
-// y-neighbours cross waves through LDS (H(n+1/2) at y-1, E(n+1) at y+1 one
-// plane back, H(n+3/2) at y-1; two LDS-only barriers per plane),
-// x-neighbours come from the adjacent lane, z-neighbours ride in registers.
+//   rows 1..last-1, E(n+2) on rows 2..last-1, the rows the tile owns.
+// y-neighbours cross rows through LDS (H(n+1/2) at y-1, E(n+1) at y+1 one
+// plane back, H(n+3/2) at y-1, E^n of the next plane at y+1; two LDS-only
+// barriers per plane), x-neighbours come from the adjacent lane,
+// z-neighbours ride in registers.
 // The sources of step n+1 are added to E(n+1) wherever it is computed (rows
 // flagged in a bitmap, entries in list order), and the probes of step n+1
 // at owned cells are recorded into their slots.  The arithmetic of every
@@ -1446,12 +1447,18 @@ hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoin
         if (box.lpr == 64) WF_TWOSTEP_LAUNCH(0, 64);
         else if (box.lpr == 32) WF_TWOSTEP_LAUNCH(0, 32);
         else return hipErrorInvalidValue;
-    } else {  // y and/or z CPML regions: rows of 32 lanes
-        if (box.lpr != 32) return hipErrorInvalidValue;
+    } else if (box.lpr == 32) {  // y and/or z CPML regions: rows of 32 or 64 lanes
         if (pml_axes == 2) WF_TWOSTEP_LAUNCH(2, 32);
         else if (pml_axes == 4) WF_TWOSTEP_LAUNCH(4, 32);
         else if (pml_axes == 6) WF_TWOSTEP_LAUNCH(6, 32);
         else return hipErrorInvalidValue;
+    } else if (box.lpr == 64) {
+        if (pml_axes == 2) WF_TWOSTEP_LAUNCH(2, 64);
+        else if (pml_axes == 4) WF_TWOSTEP_LAUNCH(4, 64);
+        else if (pml_axes == 6) WF_TWOSTEP_LAUNCH(6, 64);
+        else return hipErrorInvalidValue;
+    } else {
+        return hipErrorInvalidValue;
     }
 #undef WF_TWOSTEP_LAUNCH
     return hipGetLastError();

@@ -157,21 +157,16 @@ class DeviceEngine:
         return ms.value
 
     def _kernel_names(self) -> tuple:
-        v = self.kernel_variant()
-        if v == "twostep_full":
-            return ("k_twostep4", "k_twostep4_xcols", "k_twostep4_slabs")
-        if v == "twostep":
-            return ("k_twostep4", "frame_step1", "frame_step2")
+        if self.kernel_variant() == "twostep":
+            return ("k_twostep4", "k_twostep4_xshell", "k_twostep4_yzshell")
         return ("k_fused4", "k_h4", "k_e4")
 
     def time_kernels(self, nsteps: int) -> dict:
         """Mean ms per launch of each step kernel over nsteps full steps (HIP
         events on the engine's stream): fused interior kernel, H and E
         kernels; with two steps per pass (variant "twostep"), the two-step
-        kernel per launch and the frame of a pass's first and second step
-        (H + E kernels each); with the whole domain in two-step tiles
-        ("twostep_full"), the interior, the x CPML columns and the y and z
-        slabs per pass."""
+        kernels per pass over the interior region, the x shell regions and
+        the y and z shell regions."""
         f, h, e = c_double(), c_double(), c_double()
         check(lib().wf_time_kernels(self._h, nsteps, byref(f), byref(h), byref(e)))
         return dict(zip(self._kernel_names(), (f.value, h.value, e.value)))
