@@ -142,10 +142,12 @@ def run_spheres_512(args):
 
 def run_batch_96(args):
     """96 independent scenes (edges 32-192, 1-4 spheres, CPML 8 or 16, one
-    Gaussian source each) for 500 steps: batched (one launch per phase over
-    all scenes, tiling autotuned on the first steps) against running the
-    scenes one after another on their own engines.  value = sum of cells x
-    steps / time of the batched run, tuning steps included."""
+    Gaussian source each) for 500 steps: batched (two steps per pass, one
+    two-step launch per CPML variant over every scene's regions; or, with
+    WF_BATCH_TWOSTEP=0, one launch per phase over all scenes with the tiling
+    autotuned on the first steps) against running the scenes one after
+    another on their own engines.  value = sum of cells x steps / time of
+    the batched run, tuning steps included."""
     from wavefront_amd import workloads
     from wavefront_amd.engine import DeviceBatch, DeviceEngine
 
@@ -165,6 +167,7 @@ def run_batch_96(args):
 
     engs = engines()
     with DeviceBatch(engs) as b:
+        two_step, pass_bytes = b.two_step()
         t0 = time.perf_counter()
         z, lpr, grid, taken = b.tune(4)
         ms_rest = b.time_steps(steps - taken)
@@ -172,37 +175,55 @@ def run_batch_96(args):
         wall = time.perf_counter() - t0
     for e in engs:
         e.close()
-    log(f"batched: tuned zchunk={z} lanes/row={lpr} grid={grid or 'per tile'} on {taken} steps; "
-        f"{wall:.3f}s total, "
-        f"{ms_rest / (steps - taken):.3f} ms/step after tuning")
+    if two_step:
+        log(f"batched: two steps per pass, {wall:.3f}s total, "
+            f"{ms_rest / steps:.3f} ms/step")
+    else:
+        log(f"batched: tuned zchunk={z} lanes/row={lpr} grid={grid or 'per tile'} on {taken} "
+            f"steps; {wall:.3f}s total, "
+            f"{ms_rest / (steps - taken):.3f} ms/step after tuning")
     # the same scenes one at a time on their own engines (graph-replayed steps)
-    engs = engines()
-    solo_ms = 0.0
-    for e in engs:
-        solo_ms += e.time_steps(steps)
-        e.close()
-    log(f"solo, one scene after another: {solo_ms / 1e3:.3f}s")
+    solo_ms = float("nan")
+    if not args.no_solo:
+        engs = engines()
+        solo_ms = 0.0
+        for e in engs:
+            solo_ms += e.time_steps(steps)
+            e.close()
+        log(f"solo, one scene after another: {solo_ms / 1e3:.3f}s")
     mcells = cells * steps / wall / 1e6
-    # algorithmic bytes per step: the two-pass 74 B per cell plus 8 B (read +
-    # write) per non-identity psi term, 4 terms per graded cell per axis
-    from wavefront_amd.layout import effective_pml_thickness_blocks
-    step_bytes = 0
-    for w in ws:
-        n = (w.dims.nx, w.dims.ny, w.dims.nz)
-        t = 8 * effective_pml_thickness_blocks(w.dims, w.pml_config.thickness) if w.pml_config else 0
-        graded = sum(2 * t * w.cells() // n[a] for a in range(3))
-        step_bytes += 74 * w.cells() + 8 * 4 * graded
+    if two_step:
+        # algorithmic bytes of a pass (two steps): 49 B per cell and 32 B per
+        # graded axis of a cell (its four psi terms read and written once)
+        step_bytes = pass_bytes // 2
+        kernel = "k_twostep4_batch (8 CPML-variant launches per pass of two steps), per step"
+        tkey = "pass"
+    else:
+        # algorithmic bytes per step: the two-pass 74 B per cell plus 8 B
+        # (read + write) per non-identity psi term, 4 terms per graded cell
+        # per axis
+        from wavefront_amd.layout import effective_pml_thickness_blocks
+        step_bytes = 0
+        for w in ws:
+            n = (w.dims.nx, w.dims.ny, w.dims.nz)
+            t = (8 * effective_pml_thickness_blocks(w.dims, w.pml_config.thickness)
+                 if w.pml_config else 0)
+            graded = sum(2 * t * w.cells() // n[a] for a in range(3))
+            step_bytes += 74 * w.cells() + 8 * 4 * graded
+        kernel = "k_h4_batch + k_e4_batch (one step after tuning)"
+        tkey = "step"
     ms_step = ms_rest / (steps - taken)
     achieved = step_bytes / (ms_step / 1e3) / 1e9
-    # HBM bytes per step of the two batched kernels, measured with rocprofv3
+    # HBM bytes per step of the batched kernels, measured with rocprofv3
     # --pmc (FETCH_SIZE, WRITE_SIZE, scaled by a plain copy kernel)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "batch_96_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            t = json.load(f).get("step", {})
-        if t.get("algorithmic_bytes") == step_bytes:
-            traffic = t["traffic_bytes"]
+            t = json.load(f).get(tkey, {})
+        per = 2 if two_step else 1  # the two-step figures are per pass
+        if t.get("algorithmic_bytes") == step_bytes * per:
+            traffic = t["traffic_bytes"] // per
     out = {
         "metric": "Mcell-updates/s, 96 batched scenes, FP32, 1 MI355X",
         "value": round(mcells, 1),
@@ -220,16 +241,18 @@ def run_batch_96(args):
             "workload": "batch_96: 96 scenes, edges U{32..192 step 8}, 1-4 spheres, CPML 8 or "
                         "16, Gaussian Ez source + 1 probe each, seed 4",
             "cells": cells,
+            "schedule": "two steps per pass" if two_step else "split H/E steps, tuned tiling",
             "tuned_zchunk": z,
             "tuned_lanes_per_row": lpr,
             "tuned_grid": grid,
             "tuning_steps": taken,
             "batched_ms_per_step_after_tuning": round(ms_rest / (steps - taken), 4),
-            "solo_sequential_seconds": round(solo_ms / 1e3, 3),
-            "solo_sequential_mcells": round(cells * steps / (solo_ms / 1e3) / 1e6, 1),
+            "solo_sequential_seconds": None if args.no_solo else round(solo_ms / 1e3, 3),
+            "solo_sequential_mcells": (None if args.no_solo
+                                       else round(cells * steps / (solo_ms / 1e3) / 1e6, 1)),
             "bytes_per_step_algorithmic": step_bytes,
         },
-        "roofline": {"bound": "hbm", "kernel": "k_h4_batch + k_e4_batch (one step after tuning)",
+        "roofline": {"bound": "hbm", "kernel": kernel,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
@@ -368,6 +391,8 @@ def main():
                          "batch_96 = 96 small scenes batched; learned = the learned thin "
                          "boundary's reflection and throughput")
     ap.add_argument("--c4-steps", type=int, default=500)
+    ap.add_argument("--no-solo", action="store_true",
+                    help="batch_96: skip running the scenes one after another")
     ap.add_argument("--c2-steps", type=int, default=2000)
     ap.add_argument("--c2-every", type=int, default=100)
     ap.add_argument("--output", default="", help="snapshot file of the spheres_512 run")

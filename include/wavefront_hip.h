@@ -260,6 +260,14 @@ int wf_batch_time_steps(wf_batch *batch, uint64_t nsteps, double *total_ms);
 int wf_batch_tune(wf_batch *batch, uint32_t steps_per_candidate, int *zchunk,
                   int *lanes_per_row, int *grid, uint64_t *steps_taken);
 int wf_batch_set_tiling(wf_batch *batch, int zchunk, int lanes_per_row, int grid);
+/* Whether the batch advances two steps per pass (*two_step = 1: every scene
+ * fused, no periodic axis, WF_BATCH_TWOSTEP not 0; each scene's regions --
+ * the single-scene split into CPML shell and interior ranges, or the whole
+ * scene in one box when that split does not fit -- swept by one launch per
+ * CPML variant over all scenes, an odd last step on the H/E kernels above)
+ * and the algorithmic HBM bytes of one pass (49 B per cell, 32 B per graded
+ * axis of a cell). */
+int wf_batch_two_step(const wf_batch *batch, int *two_step, uint64_t *pass_bytes);
 /* Sum of the scenes' cells. */
 uint64_t wf_batch_cells(const wf_batch *batch);
 

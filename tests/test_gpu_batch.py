@@ -18,22 +18,38 @@ def _engine(w, steps):
     return eng
 
 
-def test_batch_matches_oracle_with_tuning():
+@pytest.mark.parametrize("schedule", ["twostep", "split"])
+def test_batch_matches_oracle_with_tuning(schedule, monkeypatch):
     """The first six scenes of the 96-scene set plus three hand-picked shapes
     (rows of 10 quads and of 66 quads, split in two fitted segments of 33),
-    48 steps: 13 tuner candidates x 3 real steps each, then 9 more."""
+    48 steps.  Split steps: 13 tuner candidates x 3 real steps each, then 9
+    more.  Two steps per pass (nothing to tune): 5 steps (two passes and a
+    split step), then 43 from the other state (21 passes, a split step).
+    Scenes 6, 8 (x all shell) and 7 (y shells meeting) take the
+    whole-scene every-axis CPML box, the others the region split."""
+    monkeypatch.setenv("WF_BATCH_TWOSTEP", "1" if schedule == "twostep" else "0")
     ws = workloads.batch_scenes(nscenes=6, steps=48)
     ws += workloads.batch_scenes(nscenes=3, steps=48, seed=11,
                                  edges=[(32, 48, 40), (200, 32, 64), (40, 24, 32)], max_edge=264)
     ws[-1] = workloads.batch_scenes(nscenes=1, steps=48, seed=12, edges=[(264, 16, 24)],
                                     max_edge=264)[0]
-    print("[gpu-case] batch of 9 vs oracle, 48 steps: "
+    print(f"[gpu-case] batch of 9 vs oracle, 48 steps, {schedule}: "
           + ", ".join(w.name for w in ws), flush=True)
     engs = [_engine(w, 48) for w in ws]
     with DeviceBatch(engs) as b:
+        two, pass_bytes = b.two_step()
+        assert two == (schedule == "twostep")
         z, lpr, grid, taken = b.tune(3)
-        assert taken == 39 and z >= 1 and lpr in (0, 16, 64) and grid in (0, 2048, 8192)
-        b.step(48 - taken)
+        if two:
+            assert taken == 0 and pass_bytes > 49 * sum(w.cells() for w in ws)
+            b.step(5)
+            b.sync()
+            assert [e.step_count() for e in engs] == [5] * len(engs)
+            b.step(43)
+        else:
+            assert pass_bytes == 0
+            assert taken == 39 and z >= 1 and lpr in (0, 16, 64) and grid in (0, 2048, 8192)
+            b.step(48 - taken)
         b.sync()
     for w, eng in zip(ws, engs):
         assert eng.step_count() == 48

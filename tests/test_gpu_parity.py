@@ -18,24 +18,28 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
-@pytest.fixture(autouse=True, params=["fused", "onestep", "split", "single"])
+@pytest.fixture(autouse=True, params=["fused", "rows", "onestep", "split", "single"])
 def kernel_variant(request, monkeypatch):
-    """Every case runs on each step-kernel scheme: two steps per pass (the
-    two-step kernel over the interior, the frame's H and E kernels through a
-    third state; the default), one step per pass (the fused interior kernel
-    with the CPML shell in separate H and E frame kernels; what odd step
-    counts end with), the in-place H and E kernels over the whole domain,
-    and the fused kernel over the whole domain, shell included (opt-in)."""
+    """Every case runs on each step-kernel scheme: two steps per pass on the
+    regions (the two-step kernel variant of each region's CPML axes; the
+    schedule of large grids), two steps per pass in full rows through the
+    batched two-step kernel (one launch per CPML variant), one step per
+    pass (the fused interior kernel with the CPML shell in separate H and E
+    frame kernels; what odd step counts end with), the in-place H and E
+    kernels over the whole domain, and the fused kernel over the whole
+    domain, shell included (opt-in).  The schedule is set, not timed."""
     monkeypatch.setenv("WF_KERNEL", "split" if request.param == "split" else "fused")
     monkeypatch.setenv("WF_PML_FUSED", "1" if request.param == "single" else "0")
     monkeypatch.setenv("WF_TWOSTEP", "0" if request.param == "onestep" else "1")
+    monkeypatch.setenv("WF_SCHEDULE", {"rows": "1", "onestep": "2"}.get(request.param, "0"))
     return request.param
 
 
 def _say(msg):
     print(f"[gpu-case] {msg} kernel={os.environ.get('WF_KERNEL')} "
           f"single_pass={os.environ.get('WF_PML_FUSED')} "
-          f"two_step={os.environ.get('WF_TWOSTEP')}", flush=True)
+          f"two_step={os.environ.get('WF_TWOSTEP')} "
+          f"schedule={os.environ.get('WF_SCHEDULE')}", flush=True)
 
 
 def _run_workload_gpu(w, steps):

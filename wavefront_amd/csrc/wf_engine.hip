@@ -141,6 +141,27 @@ __global__ void k_probes_batch(const BatchScene *__restrict__ sc)
     if (threadIdx.x == 0) *d.step = step + 1ull;
 }
 
+// Point work of a batched two-step pass, one workgroup per scene (d.p: the
+// state the pass wrote): the first step's probes from the values the
+// two-step kernels recorded, the second step's sources (list order) and
+// probes, the step counter advanced by two.
+__global__ void k_pair_points_batch(const BatchScene *__restrict__ sc)
+{
+    const BatchScene &d = sc[blockIdx.x];
+    const uint64_t step = *d.step;
+    for (int i = threadIdx.x; i < d.nprobe; i += blockDim.x)
+        accumulate_probe(d.p, d.probe[i], i, d.nprobe, d.nfreq, d.cos_tab, d.sin_tab, d.active,
+                         step - d.probe_base, d.sums, d.counts, d.trace, d.slot);
+    if (threadIdx.x == 0 && d.nsrc)
+        apply_sources(d.p, d.src, d.nsrc, d.ncol, d.srcval, step + 1 - d.src_base);
+    __syncthreads();  // the second step's sources are in place
+    for (int i = threadIdx.x; i < d.nprobe; i += blockDim.x)
+        accumulate_probe(d.p, d.probe[i], i, d.nprobe, d.nfreq, d.cos_tab, d.sin_tab, d.active,
+                         step + 1 - d.probe_base, d.sums, d.counts, d.trace, nullptr);
+    __syncthreads();  // every lane has read the counter
+    if (threadIdx.x == 0) *d.step = step + 2ull;
+}
+
 // Advances the device step counter at the end of every step.
 __global__ void k_tick(uint64_t *step)
 {
@@ -219,6 +240,15 @@ __global__ void k_pack_psi(Ptrs p, Geo g, const uint64_t *__restrict__ blocks, u
 // ===========================================================================
 // Context
 // ===========================================================================
+// One box of a two-step pass run through the batched kernel: its scene,
+// the box (box.first: its first tile in its variant's launch) and the
+// variant's CPML axes.
+struct PassItem {
+    int scene;
+    wf::Box box;
+    int axes;
+};
+
 struct wf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -265,6 +295,20 @@ struct wf_ctx {
         int n = 0;
     } src_pts, prb_pts;
     float *d_pslot = nullptr;   // probe values of a pass's first step, per probe
+    // schedule: 0 the regions on the two-step kernels (18 launches per pass
+    // at most), 1 rows through the batched two-step kernel (full x rows per
+    // y and z range, one launch per CPML variant), 2 single steps; chosen at
+    // wf_create by timing for grids under 2^25 cells, else 0 (WF_SCHEDULE
+    // sets it)
+    int sched = 0;
+    double sched_ms[3] = {-1.0, -1.0, -1.0};
+    std::vector<PassItem> rows_plan;
+    int rows_first[8] = {}, rows_total[8] = {};
+    uint64_t rows_bytes = 0;
+    TwoStepItem *d_ritems = nullptr;  // [2][rows_plan.size()]: the state in c->p at
+    uint32_t *d_rwg = nullptr;        // writing (rows_key) read, then the other
+    const float *rows_key = nullptr;
+    bool rows_dirty = true;
     hipStream_t side = nullptr;            // frame H kernel, overlapped
     hipEvent_t fork = nullptr, join = nullptr;
     bool overlap = false;                  // WF_OVERLAP=1
@@ -435,6 +479,7 @@ void free_probes(wf_ctx *c)
     dfree(c->d_trace);
     free_points(c->prb_pts);
     dfree(c->d_pslot);
+    c->rows_dirty = true;
     c->nprobe = 0;
     c->nfreq = 0;
     c->probe_steps = 0;
@@ -501,6 +546,163 @@ int fused_slices(long tiles, int planes, int cus, int extra = 1)
         }
     }
     return best;
+}
+
+// Algorithmic bytes of one two-step pass over a box's owned cells (tile
+// column tx owns [x0 + tx tstride, + 4 (lpr - 2)) clipped to x1): 49 B per
+// cell for the two steps, and 16 B per pair of non-identity psi terms (2 per
+// phase of each graded axis of a cell, read once and written once).
+uint64_t twostep_box_bytes(const Geo &g, const Box &b)
+{
+    auto graded_at = [&](int a, int c, int n) {
+        return (uint64_t)(g.T[a] > 0 && (c < g.T[a] || c >= n - g.T[a]));
+    };
+    uint64_t wx = 0, gx = 0, gy = 0, gz = 0;
+    for (int tx = 0; tx < b.gx; ++tx) {
+        const int x0 = b.x0 + tx * b.tstride, x1 = std::min(x0 + 4 * (b.lpr - 2), b.x1);
+        for (int x = x0; x < x1; ++x) {
+            ++wx;
+            gx += graded_at(0, x, g.nx);
+        }
+    }
+    for (int y = b.y0; y < b.y1; ++y) gy += graded_at(1, y, g.ny);
+    for (int z = b.z0; z < b.z1; ++z) gz += graded_at(2, z, g.nz);
+    const uint64_t wy = b.y1 - b.y0, wz = b.z1 - b.z0;
+    return 49ull * wx * wy * wz + 32ull * (gx * wy * wz + gy * wx * wz + gz * wx * wy);
+}
+
+// Tile columns of a box whose x range holds q quads, for the two-step
+// variant `axes`, rows of at most `cap` lanes: the column count (from the
+// fewest possible up to three more) whose columns x row groups are fewest,
+// each column as narrow as the count allows (ties: fewer columns).  Short
+// rows leave lanes idle (64 / lanes-per-row rows per wave).
+void fit_columns(Box &b, int cap, int axes)
+{
+    const int q = (b.x1 - b.x0) / 4, h = b.y1 - b.y0;
+    const int k0 = (q + cap - 3) / (cap - 2);
+    long best = -1;
+    for (int k = k0; k <= std::min(q, k0 + 3); ++k) {
+        const int per = (q + k - 1) / k;
+        const int rows = twostep_batch_rows_per_tile(axes, per + 2);
+        const long tiles = (long)k * ((h + rows - 1) / rows);
+        if (best < 0 || tiles < best) {
+            best = tiles;
+            b.lpr = per + 2;
+            b.tstride = 4 * per;
+            b.gx = k;
+        }
+    }
+}
+
+// Two-step items of one scene (mode -1: full rows unless the x interior
+// range is at least 496 cells wide, then the region split; 0: the region
+// split; 1: full rows; 2: the whole scene in one box):
+//  * the region split, as the solo plan (plan_boxes): per y and z range
+//    (shell below / interior / shell above) the x interior range, rows
+//    fitted to its width (64 lanes at most in the interior, 32 in the y and
+//    z shell ranges), and the two x shell ranges as narrow columns;
+//  * full rows: per y and z range one box over every x, on the variant
+//    with the x CPML (rows of up to 32 lanes).  The narrow x shell columns
+//    waste lanes and lines; on the 96-scene set (edges 32-192) full rows
+//    took 1.77 ms per step against 2.29 for the split, measured on MI355X;
+//  * the whole scene as one box of the every-axis CPML variant: where the
+//    split does not fit (x interior under 8 cells, or y or z shells that
+//    meet), and as mode 2.
+void pass_items(const Geo &g, int mode, int scene, std::vector<PassItem> &items)
+{
+    const int n[3] = {g.nx, g.ny, g.nz};
+    const int Wx = g.T[0] + 4, lprx = Wx / 4 + 2;
+    int tl[3], th[3];
+    tl[0] = Wx;
+    th[0] = n[0] - Wx;
+    for (int a = 1; a < 3; ++a) {
+        tl[a] = g.T[a] + 2;
+        th[a] = n[a] - std::max(g.T[a], 1) - 1;
+    }
+    const bool fits = lprx <= 32 && th[0] - tl[0] >= 8 && th[1] >= tl[1] && th[2] >= tl[2];
+    if (mode < 0) mode = th[0] - tl[0] >= 496 ? 0 : 1;
+    if (!fits || mode == 2) {
+        Box w{0, n[0], 0, n[1], 0, n[2], 32, 0, 0, 0, 0};
+        fit_columns(w, 32, 7);
+        items.push_back({scene, w, 7});
+        return;
+    }
+    const int yr[3][2] = {{0, tl[1]}, {tl[1], th[1]}, {th[1], n[1]}};
+    const int zr[3][2] = {{0, tl[2]}, {tl[2], th[2]}, {th[2], n[2]}};
+    for (int zi = 0; zi < 3; ++zi)
+        for (int yi = 0; yi < 3; ++yi) {
+            const int y0 = yr[yi][0], y1 = yr[yi][1], z0 = zr[zi][0], z1 = zr[zi][1];
+            if (y1 <= y0 || z1 <= z0) continue;
+            const int axes = (yi != 1 ? 2 : 0) | (zi != 1 ? 4 : 0);
+            if (mode == 1) {
+                Box m{0, n[0], y0, y1, z0, z1, 32, 0, 0, 0, 0};
+                fit_columns(m, 32, axes | 1);
+                items.push_back({scene, m, axes | 1});
+            } else {
+                Box m{tl[0], th[0], y0, y1, z0, z1, 64, 0, 0, 0, 0};
+                fit_columns(m, axes ? 32 : 64, axes);
+                items.push_back({scene, m, axes});
+                Box e{0, n[0], y0, y1, z0, z1, lprx, 2, 0, 0, 0};
+                e.tstride = n[0] - Wx;
+                items.push_back({scene, e, axes | 1});
+            }
+        }
+}
+
+// Groups items by variant (one launch each) into `plan`, with row groups and
+// z slices: per launch the slice length (`slice` planes, or 0: chosen from
+// candidates) minimises the launch-round estimate sum of the tiles' sweeps /
+// CUs + the longest sweep, a sweep being the slice's planes + 3.
+// var_first / var_total: each variant's offset in the workgroup table and
+// tile count; returns the algorithmic bytes of one pass.
+uint64_t group_items(const std::vector<PassItem> &items, const std::vector<const Geo *> &geo,
+                     int slice, int cus, std::vector<PassItem> &plan, int var_first[8],
+                     int var_total[8])
+{
+    plan.clear();
+    uint64_t first = 0, bytes = 0;
+    for (int v = 0; v < 8; ++v) {
+        std::vector<PassItem> vi;
+        for (const auto &it : items)
+            if (it.axes == v) vi.push_back(it);
+        int depth = 0;  // the deepest box of the variant
+        for (auto &it : vi) {
+            const int rows = twostep_batch_rows_per_tile(v, it.box.lpr);
+            it.box.gy = (it.box.y1 - it.box.y0 + rows - 1) / rows;
+            depth = std::max(depth, it.box.z1 - it.box.z0);
+        }
+        static const int cand[] = {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 1 << 30};
+        int pick = slice;
+        if (pick <= 0) {
+            double best = -1.0;
+            for (int S : cand) {
+                double sum = 0.0;
+                for (const auto &it : vi) {
+                    const int nz = it.box.z1 - it.box.z0, gz = (nz + S - 1) / S;
+                    sum += (double)it.box.gx * it.box.gy * (nz + 3.0 * gz);
+                }
+                const double cost = sum / cus + (double)std::min(S, depth) + 3.0;
+                if (best < 0.0 || cost < best) {
+                    best = cost;
+                    pick = S;
+                }
+                if (S >= depth) break;
+            }
+        }
+        var_first[v] = (int)first;
+        uint64_t tiles = 0;
+        for (auto &it : vi) {
+            const int nz = it.box.z1 - it.box.z0;
+            it.box.gz = std::min(nz, (nz + pick - 1) / std::max(pick, 1));
+            it.box.first = (int)tiles;
+            tiles += (uint64_t)it.box.gx * it.box.gy * it.box.gz;
+            bytes += twostep_box_bytes(*geo[it.scene], it.box);
+            plan.push_back(it);
+        }
+        var_total[v] = (int)tiles;
+        first += tiles;
+    }
+    return bytes;
 }
 
 void plan_boxes(wf_ctx *c)
@@ -635,6 +837,13 @@ void plan_boxes(wf_ctx *c)
                 tile(e, axes | 1);
             }
         c->pair_geo = true;
+    }
+    // the rows schedule's items (pass_items, full rows), grouped by variant
+    {
+        std::vector<PassItem> items;
+        pass_items(g, 1, 0, items);
+        std::vector<const Geo *> geo{&g};
+        c->rows_bytes = group_items(items, geo, 0, c->cus, c->rows_plan, c->rows_first, c->rows_total);
     }
     Box all{0, n[0], 0, n[1], 0, n[2], 64, 0, 0, 0, 0};
     all.gx = (n[0] + 255) / 256;
@@ -859,15 +1068,65 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
 }
 
 // Two steps per pass apply to the fused variant without face hooks or the
-// whole-domain CPML pass, when the geometry allows the two-step box.
-bool pair_tables(const wf_ctx *c) { return c->fused && c->pair_env && c->pair_geo; }
+// whole-domain CPML pass: on the regions when the geometry allows them, or
+// in rows through the batched kernel (any geometry, not the contracted
+// set); a batch runs them for any fused context without a periodic axis, so
+// the point tables of the two-step kernels are built for all of those.
+bool pair_tables(const wf_ctx *c) { return c->fused && c->pair_env && !c->g.periodic; }
 
-bool pair_active(const wf_ctx *c)
+// 0: single steps; 1: two-step passes on the regions; 2: in rows.
+int pair_kind(const wf_ctx *c)
 {
-    if (!pair_tables(c) || c->pml_fused) return false;
+    if (!pair_tables(c) || c->pml_fused || c->sched == 2) return 0;
     for (int f = 0; f < 6; ++f)
-        if (c->hook_on[f]) return false;
-    return true;
+        if (c->hook_on[f]) return 0;
+    if (c->sched == 1 && !c->contract) return 2;
+    return c->pair_geo ? 1 : 0;
+}
+
+bool pair_active(const wf_ctx *c) { return pair_kind(c) != 0; }
+
+// The rows schedule's items (once) and their tables for both state
+// parities, rewritten after the point tables or the buffers change.
+int rows_tables(wf_ctx *c)
+{
+    if (pair_kind(c) != 2 || !c->rows_dirty) return WF_OK;
+    if (!c->d_ritems) {
+        uint64_t tiles = 0;
+        for (int v = 0; v < 8; ++v) tiles += (uint64_t)c->rows_total[v];
+        std::vector<uint32_t> wg(tiles);
+        for (size_t k = 0; k < c->rows_plan.size(); ++k) {
+            const PassItem &it = c->rows_plan[k];
+            const uint64_t cnt = (uint64_t)it.box.gx * it.box.gy * it.box.gz;
+            for (uint64_t j = 0; j < cnt; ++j) wg[c->rows_first[it.axes] + it.box.first + j] = (uint32_t)k;
+        }
+        if (int rc = dmalloc(&c->d_rwg, std::max<uint64_t>(tiles, 1) * sizeof(uint32_t))) return rc;
+        if (int rc = dmalloc(&c->d_ritems, 2 * c->rows_plan.size() * sizeof(TwoStepItem))) return rc;
+        if (tiles) WF_HIP(hipMemcpy(c->d_rwg, wg.data(), tiles * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    const size_t m = c->rows_plan.size();
+    std::vector<TwoStepItem> it(2 * m);
+    for (size_t k = 0; k < 2 * m; ++k) {
+        TwoStepItem &d = it[k];
+        std::memset(&d, 0, sizeof d);
+        d.g = c->g;
+        d.a = k < m ? c->p : c->alt;
+        d.b = k < m ? c->alt : c->p;
+        d.box = c->rows_plan[k % m].box;
+        d.pts.src = PointRows{c->src_pts.bits, c->src_pts.key, c->src_pts.ent, c->src_pts.n};
+        d.pts.prb = PointRows{c->prb_pts.bits, c->prb_pts.key, c->prb_pts.ent, c->prb_pts.n};
+        d.pts.values = c->d_srcval;
+        d.pts.ncol = (int)c->nsrc_col;
+        d.pts.base = c->src_base;
+        d.pts.step = c->d_step;
+        d.pts.slot = c->d_pslot;
+    }
+    // earlier passes may still read the tables
+    WF_HIP(hipStreamSynchronize(c->stream));
+    WF_HIP(hipMemcpy(c->d_ritems, it.data(), 2 * m * sizeof(TwoStepItem), hipMemcpyHostToDevice));
+    c->rows_key = c->p.f[0];
+    c->rows_dirty = false;
+    return WF_OK;
 }
 
 // Two loop iterations in one pass (src/engine.rs:747-782 twice): the
@@ -880,6 +1139,22 @@ bool pair_active(const wf_ctx *c)
 int launch_pair(wf_ctx *c, hipEvent_t *ev = nullptr)
 {
     hipStream_t st = c->stream;
+    if (pair_kind(c) == 2) {
+        // rows: one batched launch per variant (event pair 0, 1 around all)
+        if (int rc = rows_tables(c)) return rc;
+        const size_t m = c->rows_plan.size();
+        const TwoStepItem *items = c->d_ritems + (c->p.f[0] == c->rows_key ? 0 : m);
+        if (ev) WF_HIP(hipEventRecord(ev[0], st));
+        for (int v = 0; v < 8; ++v)
+            WF_HIP(launch_batch_twostep(items, c->d_rwg + c->rows_first[v], c->rows_total[v], v, st));
+        if (ev)
+            for (int k = 1; k < 6; ++k) WF_HIP(hipEventRecord(ev[k], st));
+        if (int rc = point_work(c, c->alt, st, true, false)) return rc;
+        if (int rc = point_work(c, c->alt, st, false)) return rc;
+        std::swap(c->p, c->alt);
+        c->step += 2;
+        return WF_OK;
+    }
     TwoStepPoints pts;
     pts.src = PointRows{c->src_pts.bits, c->src_pts.key, c->src_pts.ent, c->src_pts.n};
     pts.prb = PointRows{c->prb_pts.bits, c->prb_pts.key, c->prb_pts.ent, c->prb_pts.n};
@@ -973,6 +1248,20 @@ void point_fields(wf_ctx *c, uint64_t N)
 // arrangement.  A buffer that cannot be allocated, or a search that has run
 // for budget_ms, ends it (the current arrangement stays); HIP errors are
 // returned.
+// Back to the initial state after a probe: zero fields and psi, step 0.
+int reset_state(wf_ctx *c)
+{
+    const uint64_t bytes = 6ull * c->g.nx * c->g.ny * c->g.nz * sizeof(float);
+    WF_HIP(hipMemsetAsync(c->d_fields, 0, bytes, c->stream));
+    if (c->d_fields2) WF_HIP(hipMemsetAsync(c->d_fields2, 0, bytes, c->stream));
+    if (c->d_psi) WF_HIP(hipMemsetAsync(c->d_psi, 0, c->psi_bytes, c->stream));
+    if (c->d_psi2) WF_HIP(hipMemsetAsync(c->d_psi2, 0, c->psi_bytes, c->stream));
+    WF_HIP(hipMemsetAsync(c->d_step, 0, sizeof(uint64_t), c->stream));
+    WF_HIP(hipStreamSynchronize(c->stream));
+    c->step = 0;
+    return WF_OK;
+}
+
 int choose_placement(wf_ctx *c, int extra, uint64_t N, double budget_ms)
 {
     const auto t_start = std::chrono::steady_clock::now();
@@ -1023,15 +1312,31 @@ int choose_placement(wf_ctx *c, int extra, uint64_t N, double budget_ms)
         }
         if (rc) return rc;
     }
-    // Back to the initial state: zero fields and psi, step 0.
-    WF_HIP(hipMemsetAsync(c->d_fields, 0, bytes, c->stream));
-    if (c->d_fields2) WF_HIP(hipMemsetAsync(c->d_fields2, 0, bytes, c->stream));
-    if (c->d_psi) WF_HIP(hipMemsetAsync(c->d_psi, 0, c->psi_bytes, c->stream));
-    if (c->d_psi2) WF_HIP(hipMemsetAsync(c->d_psi2, 0, c->psi_bytes, c->stream));
-    WF_HIP(hipMemsetAsync(c->d_step, 0, sizeof(uint64_t), c->stream));
-    WF_HIP(hipStreamSynchronize(c->stream));
-    c->step = 0;
-    return WF_OK;
+    c->rows_dirty = true;
+    return reset_state(c);
+}
+
+// Schedule of a small grid: the two-step regions (where the geometry allows
+// them), the rows and single steps, each timed over 4 steps after 2 on the
+// zeroed state; the fastest is kept.  Small scenes are launch- and
+// latency-bound: the regions' 18 launches per pass cost most there.
+int choose_schedule(wf_ctx *c)
+{
+    double best = -1.0;
+    int pick = 2;
+    for (int k = 0; k < 3; ++k) {
+        c->sched = k;
+        if (k == 0 && pair_kind(c) != 1) continue;
+        double ms = 0.0;
+        if (int rc = probe_step_ms(c, 2, 4, &ms)) return rc;
+        c->sched_ms[k] = ms;
+        if (best < 0.0 || ms < best) {
+            best = ms;
+            pick = k;
+        }
+    }
+    c->sched = pick;
+    return reset_state(c);
 }
 
 constexpr int kGraphSteps = 8;
@@ -1043,6 +1348,7 @@ constexpr int kGraphSteps = 8;
 int capture_graph(wf_ctx *c)
 {
     drop_graph_exec(c);
+    if (int rc = rows_tables(c)) return rc;  // no synchronous copy under capture
     const uint64_t step0 = c->step;
     const Ptrs p0 = c->p, a0 = c->alt;
     hipGraph_t graph = nullptr;
@@ -1355,6 +1661,13 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
         if (tries > 0 && 6 * N * sizeof(float) >= min_bytes)
             if (int rc = choose_placement(c, std::min(tries, kPlaceMax), N, 4000.0)) return bail(rc);
     }
+    // Schedule (choose_schedule) for grids under 2^25 cells, else the
+    // regions; WF_SCHEDULE=0 (regions), 1 (rows) or 2 (single steps) sets it.
+    if (const char *sc = std::getenv("WF_SCHEDULE")) {
+        c->sched = std::min(2, std::max(0, std::atoi(sc)));
+    } else if (pair_tables(c) && !c->pml_fused && !c->contract && N < (1ull << 25)) {
+        if (int rc = choose_schedule(c)) return bail(rc);
+    }
     *out = c;
     return WF_OK;
 }
@@ -1384,6 +1697,8 @@ void wf_destroy(wf_ctx *c)
     dfree(c->d_fields);
     dfree(c->d_fields2);
     dfree(c->d_psi2);
+    dfree(c->d_ritems);
+    dfree(c->d_rwg);
     dfree(c->d_step);
     dfree(c->d_id);
     dfree(c->d_coef);
@@ -1872,7 +2187,8 @@ const char *wf_kernel_variant(const wf_ctx *c)
 {
     if (!c) return "";
     if (!c->fused) return "split";
-    return pair_active(c) ? "twostep" : "fused";
+    const int k = pair_kind(c);
+    return k == 1 ? "twostep" : (k == 2 ? "twostep-rows" : "fused");
 }
 
 int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
@@ -1884,6 +2200,12 @@ int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,
     // non-identity psi terms per phase: 2 per graded axis per cell
     const uint64_t terms = 2ull * (2ull * c->g.T[0] * ny * nz + 2ull * c->g.T[1] * nx * nz +
                                    2ull * c->g.T[2] * nx * ny);
+    if (pair_kind(c) == 2) {
+        // rows: one pass over every cell (the batched kernels, one event pair)
+        *fused_bytes = c->rows_bytes;
+        *h_bytes = *e_bytes = 0;
+        return WF_OK;
+    }
     if (pair_active(c)) {
         // two-step tiles over the whole domain: 49 B per cell for two steps
         // and 16 B per psi term (2 per phase of each graded axis of a cell,
@@ -2382,9 +2704,37 @@ struct wf_batch {
     int grid = 0;  // 0: one workgroup per tile; else persistent workgroups
     int tiled_zchunk = -1, tiled_lpr = -1;  // tiling the wg table was built for
     std::vector<Box> boxes;
+    // two steps per pass (every scene fused, no periodic axis; WF_BATCH_TWOSTEP=0
+    // disables): each scene's regions as items, grouped by CPML variant, one
+    // launch per variant; item tables for both state parities (a = the
+    // context's current state, then swapped), as the scene descriptors
+    bool pairs = false;
+    std::vector<PassItem> plan;
+    int var_first[8] = {}, var_total[8] = {};  // workgroup table offset, tiles per variant
+    TwoStepItem *d_items = nullptr;            // [2][plan.size()]
+    uint32_t *d_wgi = nullptr;                 // workgroup -> item
+    uint64_t pass_bytes = 0;                   // algorithmic bytes of one pass
+    int slice = 0;                             // WF_BATCH_SLICE: planes per z slice (0: modelled)
+    int plan_mode = -1;  // WF_BATCH_PLAN: -1 by width, 0 regions, 1 full rows, 2 whole scenes
+    int nstreams = 4;                          // WF_BATCH_STREAMS: variant launches spread over
+    hipStream_t vs[4] = {};                    // the extra streams
+    hipEvent_t fork = nullptr, join[4] = {};
 };
 
 namespace {
+
+// The batch's items: every scene's, grouped by variant.
+void batch_plan(wf_batch *b)
+{
+    std::vector<PassItem> items;
+    std::vector<const Geo *> geo;
+    for (size_t i = 0; i < b->scenes.size(); ++i) {
+        pass_items(b->scenes[i]->g, b->plan_mode, (int)i, items);
+        geo.push_back(&b->scenes[i]->g);
+    }
+    b->pass_bytes = group_items(items, geo, b->slice, b->scenes[0]->cus, b->plan, b->var_first,
+                                b->var_total);
+}
 
 // Scene descriptors from the contexts' current state and tables; the
 // workgroup -> scene table is rebuilt when the tiling changes.
@@ -2420,14 +2770,17 @@ int batch_upload(wf_batch *b)
         b->tiled_zchunk = b->zchunk;
         b->tiled_lpr = b->lpr_cap;
     }
-    std::vector<BatchScene> h(n);
-    for (size_t i = 0; i < n; ++i) {
-        const wf_ctx *c = b->scenes[i];
+    // descriptors of both parities: [0] on the current state, [1] on the
+    // other one (the current state after an odd number of two-step passes)
+    std::vector<BatchScene> h(2 * n);
+    for (size_t i = 0; i < 2 * n; ++i) {
+        const wf_ctx *c = b->scenes[i % n];
         BatchScene &d = h[i];
         std::memset(&d, 0, sizeof d);
         d.g = c->g;
-        d.p = c->p;
-        d.box = b->boxes[i];
+        d.p = i < n ? c->p : c->alt;
+        d.slot = c->d_pslot;
+        d.box = b->boxes[i % n];
         d.src = c->d_src;
         d.srcval = c->d_srcval;
         d.src_base = c->src_base;
@@ -2445,24 +2798,78 @@ int batch_upload(wf_batch *b)
         d.nfreq = (int)c->nfreq;
         d.step = c->d_step;
     }
-    WF_HIP(hipMemcpy(b->d_scenes, h.data(), n * sizeof(BatchScene), hipMemcpyHostToDevice));
+    WF_HIP(hipMemcpy(b->d_scenes, h.data(), 2 * n * sizeof(BatchScene), hipMemcpyHostToDevice));
+    if (b->pairs) {
+        // the regions' items of both parities: [0] reads the current state
+        // and writes the other, [1] the reverse
+        const size_t m = b->plan.size();
+        std::vector<TwoStepItem> it(2 * m);
+        for (size_t k = 0; k < 2 * m; ++k) {
+            const PassItem &pi = b->plan[k % m];
+            const wf_ctx *c = b->scenes[pi.scene];
+            TwoStepItem &d = it[k];
+            std::memset(&d, 0, sizeof d);
+            d.g = c->g;
+            d.a = k < m ? c->p : c->alt;
+            d.b = k < m ? c->alt : c->p;
+            d.box = pi.box;
+            d.pts.src = PointRows{c->src_pts.bits, c->src_pts.key, c->src_pts.ent, c->src_pts.n};
+            d.pts.prb = PointRows{c->prb_pts.bits, c->prb_pts.key, c->prb_pts.ent, c->prb_pts.n};
+            d.pts.values = c->d_srcval;
+            d.pts.ncol = (int)c->nsrc_col;
+            d.pts.base = c->src_base;
+            d.pts.step = c->d_step;
+            d.pts.slot = c->d_pslot;
+        }
+        WF_HIP(hipMemcpy(b->d_items, it.data(), 2 * m * sizeof(TwoStepItem), hipMemcpyHostToDevice));
+    }
     return WF_OK;
 }
 
 int batch_launch_steps(wf_batch *b, uint64_t nsteps)
 {
     const unsigned n = (unsigned)b->scenes.size();
-    for (uint64_t s = 0; s < nsteps; ++s) {
-        WF_HIP(launch_batch_h(b->d_scenes, b->d_wg, b->total, b->zchunk, b->grid, b->stream));
-        WF_HIP(launch_batch_e(b->d_scenes, b->d_wg, b->total, b->zchunk, b->grid, b->stream));
-        hipLaunchKernelGGL(k_sources_batch, dim3(n), dim3(64), 0, b->stream, b->d_scenes);
-        hipLaunchKernelGGL(k_probes_batch, dim3(n), dim3(64), 0, b->stream, b->d_scenes);
+    uint64_t s = 0, passes = 0;
+    if (b->pairs) {
+        // two loop iterations per pass: the two-step kernels of every
+        // variant (state A -> B, the first step's sources and probe values
+        // in-kernel), then every scene's remaining point work on B
+        const size_t m = b->plan.size();
+        for (; nsteps - s >= 2; s += 2, ++passes) {
+            const TwoStepItem *items = b->d_items + (passes & 1) * m;
+            const int ns = b->nstreams;
+            if (ns > 1) {
+                WF_HIP(hipEventRecord(b->fork, b->stream));
+                for (int k = 1; k < ns; ++k) WF_HIP(hipStreamWaitEvent(b->vs[k], b->fork, 0));
+            }
+            for (int v = 0, j = 0; v < 8; ++v) {
+                if (!b->var_total[v]) continue;
+                hipStream_t st = (j % ns) ? b->vs[j % ns] : b->stream;
+                ++j;
+                WF_HIP(launch_batch_twostep(items, b->d_wgi + b->var_first[v], b->var_total[v], v, st));
+            }
+            for (int k = 1; k < ns; ++k) {
+                WF_HIP(hipEventRecord(b->join[k], b->vs[k]));
+                WF_HIP(hipStreamWaitEvent(b->stream, b->join[k], 0));
+            }
+            hipLaunchKernelGGL(k_pair_points_batch, dim3(n), dim3(64), 0, b->stream,
+                               b->d_scenes + ((passes + 1) & 1) * n);
+            WF_HIP(hipGetLastError());
+        }
+    }
+    const BatchScene *sc = b->d_scenes + (passes & 1) * n;
+    for (; s < nsteps; ++s) {
+        WF_HIP(launch_batch_h(sc, b->d_wg, b->total, b->zchunk, b->grid, b->stream));
+        WF_HIP(launch_batch_e(sc, b->d_wg, b->total, b->zchunk, b->grid, b->stream));
+        hipLaunchKernelGGL(k_sources_batch, dim3(n), dim3(64), 0, b->stream, sc);
+        hipLaunchKernelGGL(k_probes_batch, dim3(n), dim3(64), 0, b->stream, sc);
         WF_HIP(hipGetLastError());
     }
     // later work queued on a scene's own stream runs after these steps
     WF_HIP(hipEventRecord(b->done, b->stream));
     for (wf_ctx *c : b->scenes) {
         WF_HIP(hipStreamWaitEvent(c->stream, b->done, 0));
+        if (passes & 1) std::swap(c->p, c->alt);
         c->step += nsteps;
     }
     return WF_OK;
@@ -2483,6 +2890,8 @@ int batch_prepare(wf_batch *b, uint64_t nsteps)
         if (c->nprobe && c->step + nsteps > c->probe_base + c->probe_steps)
             return fail(WF_ERR_STATE, "batch scene %zu: probe table covers %llu steps", i,
                         (unsigned long long)c->probe_steps);
+        if (b->pairs && ((c->nsrc && !c->src_pts.bits) || (c->nprobe && (!c->prb_pts.bits || !c->d_pslot))))
+            return fail(WF_ERR_STATE, "batch scene %zu: no two-step point tables", i);
     }
     WF_HIP(hipSetDevice(b->device));
     for (wf_ctx *c : b->scenes) WF_HIP(hipStreamSynchronize(c->stream));
@@ -2515,12 +2924,44 @@ int wf_batch_create(uint32_t n, wf_ctx *const *scenes, wf_batch **out)
     b->device = scenes[0]->device;
     b->scenes.assign(scenes, scenes + n);
     if (const char *z = std::getenv("WF_BATCH_ZCHUNK")) b->zchunk = std::max(1, std::atoi(z));
+    if (const char *z = std::getenv("WF_BATCH_SLICE")) b->slice = std::max(0, std::atoi(z));
+    if (const char *z = std::getenv("WF_BATCH_PLAN")) b->plan_mode = std::min(2, std::max(-1, std::atoi(z)));
+    if (const char *z = std::getenv("WF_BATCH_STREAMS")) b->nstreams = std::min(4, std::max(1, std::atoi(z)));
+    const char *tw = std::getenv("WF_BATCH_TWOSTEP");
+    b->pairs = !(tw && tw[0] == '0');
+    for (const wf_ctx *c : b->scenes)
+        if (!pair_tables(c) || !c->d_fields2) b->pairs = false;
     int rc = WF_OK;
     if (hipSetDevice(b->device) != hipSuccess ||
         hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess)
         rc = fail(WF_ERR_HIP, "wf_batch_create: stream creation failed");
-    if (!rc) rc = dmalloc(&b->d_scenes, n * sizeof(BatchScene));
+    if (!rc) rc = dmalloc(&b->d_scenes, 2ull * n * sizeof(BatchScene));
+    if (!rc && b->nstreams > 1) {
+        if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+            rc = fail(WF_ERR_HIP, "wf_batch_create: event creation failed");
+        for (int k = 1; k < b->nstreams && !rc; ++k)
+            if (hipStreamCreateWithFlags(&b->vs[k], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) != hipSuccess)
+                rc = fail(WF_ERR_HIP, "wf_batch_create: stream creation failed");
+    }
+    if (!rc && b->pairs) {
+        batch_plan(b);
+        uint64_t tiles = 0;
+        for (int v = 0; v < 8; ++v) tiles += (uint64_t)b->var_total[v];
+        if (tiles > 0x7fffffffull) rc = fail(WF_ERR_INVALID, "batch: too many workgroups");
+        std::vector<uint32_t> wg(tiles);
+        for (size_t k = 0; k < b->plan.size(); ++k) {
+            const PassItem &it = b->plan[k];
+            const uint64_t cnt = (uint64_t)it.box.gx * it.box.gy * it.box.gz;
+            for (uint64_t j = 0; j < cnt; ++j) wg[b->var_first[it.axes] + it.box.first + j] = (uint32_t)k;
+        }
+        if (!rc) rc = dmalloc(&b->d_items, 2 * b->plan.size() * sizeof(TwoStepItem));
+        if (!rc) rc = dmalloc(&b->d_wgi, std::max<uint64_t>(tiles, 1) * sizeof(uint32_t));
+        if (!rc && tiles &&
+            hipMemcpy(b->d_wgi, wg.data(), tiles * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(WF_ERR_HIP, "wf_batch_create: table upload failed");
+    }
     if (rc) {
         wf_batch_destroy(b);
         return rc;
@@ -2536,6 +2977,16 @@ void wf_batch_destroy(wf_batch *b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     dfree(b->d_scenes);
     dfree(b->d_wg);
+    dfree(b->d_items);
+    dfree(b->d_wgi);
+    for (int k = 1; k < 4; ++k) {
+        if (b->vs[k]) {
+            (void)hipStreamSynchronize(b->vs[k]);
+            (void)hipStreamDestroy(b->vs[k]);
+        }
+        if (b->join[k]) (void)hipEventDestroy(b->join[k]);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -2595,6 +3046,15 @@ int wf_batch_tune(wf_batch *b, uint32_t steps_per_candidate, int *zchunk, int *l
                                   {2, 0, 0},     {4, 0, 0},     {8, 0, 0},
                                   {4, 0, 2048}};
     const int ncand = (int)(sizeof cand / sizeof cand[0]);
+    if (b->pairs) {
+        // two-step passes: the regions follow each scene's geometry and the
+        // tiling above applies only to a trailing single step; no steps taken
+        if (zchunk) *zchunk = b->zchunk;
+        if (lanes_per_row) *lanes_per_row = b->lpr_cap;
+        if (grid) *grid = b->grid;
+        if (steps_taken) *steps_taken = 0;
+        return WF_OK;
+    }
     if (int rc = batch_prepare(b, (uint64_t)ncand * steps_per_candidate)) return rc;
     hipEvent_t e0, e1;
     WF_HIP(hipEventCreate(&e0));
@@ -2646,6 +3106,14 @@ int wf_batch_set_tiling(wf_batch *b, int zchunk, int lanes_per_row, int grid)
     b->zchunk = zchunk;
     b->lpr_cap = lanes_per_row;
     b->grid = grid;
+    return WF_OK;
+}
+
+int wf_batch_two_step(const wf_batch *b, int *two_step, uint64_t *pass_bytes)
+{
+    if (!b || !two_step || !pass_bytes) return fail(WF_ERR_INVALID, "wf_batch_two_step: null pointer");
+    *two_step = b->pairs ? 1 : 0;
+    *pass_bytes = b->pairs ? b->pass_bytes : 0;
     return WF_OK;
 }
 

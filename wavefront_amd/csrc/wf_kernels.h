@@ -238,9 +238,29 @@ struct BatchScene {
     float *trace;
     uint64_t probe_base;
     uint64_t *step;  // the scene's device step counter
+    const float *slot;  // two-step passes: probe values of the first step
     int nsrc, nprobe, nfreq;
     int ncol;  // source value columns: values[row * ncol + src.col]
 };
+
+// One region of one scene in a batched two-step pass: the scene's geometry,
+// the state read (a) and the state written (b), the region's box (rows of
+// box.lpr lanes, 3..64; box.first = its first tile in its variant's launch)
+// and the point work of the pass's first step.
+struct TwoStepItem {
+    Geo g;
+    Ptrs a, b;
+    Box box;
+    TwoStepPoints pts;
+};
+
+// Two-step kernel of the CPML variant pml_axes over the items' tiles in one
+// launch: tile t (after XCD renumbering) belongs to item wg_item[t].  Every
+// item's box obeys launch_twostep's rules for that variant, with any row
+// width.
+hipError_t launch_batch_twostep(const TwoStepItem *items, const uint32_t *wg_item, int total,
+                                int pml_axes, hipStream_t st);
+int twostep_batch_rows_per_tile(int pml_axes, int lpr);  // owned rows per batched tile
 
 // H / E kernels over every scene of a batch in one launch: tile t (after
 // XCD renumbering) belongs to scene wg_scene[t].  grid = 0: one workgroup

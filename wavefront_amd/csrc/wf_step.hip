@@ -952,13 +952,13 @@ constexpr int kF2x = 8;  // waves of the CPML variants (their psi terms need the
 template <int PM>
 constexpr int twostep_waves() { return PM == 0 ? kF2 : kF2x; }
 
-template <int PM, int LPRT>
-__global__ __launch_bounds__(64 * twostep_waves<PM>()) void k_twostep4(Geo g, Ptrs a, Ptrs b,
-                                                                       Box box,
-                                                                       TwoStepPoints pts)
+// One tile of the two-step sweep with FY waves: tile t of `box` (x fastest,
+// then y, then the z slice), state a -> b.
+template <int PM, int LPRT, int FY>
+__device__ __forceinline__ void twostep_tile(const Geo &g, const Ptrs &a, const Ptrs &b,
+                                             const Box &box, const TwoStepPoints &pts, int t)
 {
     constexpr bool PX = (PM & 1) != 0, PY = (PM & 2) != 0, PZ = (PM & 4) != 0;
-    constexpr int FY = twostep_waves<PM>();
     constexpr int NX = PX ? 4 : 1, NY = PY ? 4 : 1, NZ = PZ ? 4 : 1;  // psi terms per quad
     __shared__ float2 s_ce[256];        // (ca, cb)
     __shared__ float2 s_ch[256];        // (da, db)
@@ -986,7 +986,6 @@ __global__ __launch_bounds__(64 * twostep_waves<PM>()) void k_twostep4(Geo g, Pt
     const int below = (idle || R == 0) ? me : (r > 0 ? me - LPR : (w - 1) * 64 + (rpw - 1) * LPR + l);
     const int above = (idle || R == RT - 1) ? me : (r < rpw - 1 ? me + LPR : (w + 1) * 64 + l);
 
-    const int t = xcd_tile(blockIdx.x, box.gx * box.gy * box.gz);
     const int tx = t % box.gx, ty = (t / box.gx) % box.gy, tz = t / (box.gx * box.gy);
     const int tx0 = box.x0 + tx * box.tstride;             // the tile's owned cells: [tx0, tx1)
     const int tx1 = min(tx0 + 4 * (LPR - 2), box.x1);
@@ -1407,6 +1406,40 @@ __global__ __launch_bounds__(64 * twostep_waves<PM>()) void k_twostep4(Geo g, Pt
     }
 }
 
+template <int PM, int LPRT>
+__global__ __launch_bounds__(64 * twostep_waves<PM>()) void k_twostep4(Geo g, Ptrs a, Ptrs b,
+                                                                       Box box,
+                                                                       TwoStepPoints pts)
+{
+    twostep_tile<PM, LPRT, twostep_waves<PM>()>(g, a, b, box, pts,
+                                                xcd_tile(blockIdx.x, box.gx * box.gy * box.gz));
+}
+
+#ifndef WF_STEP_FMA
+// Batched two-step pass: the tiles of the regions of one variant over every
+// scene of a batch in one launch (rows of box.lpr lanes); workgroup i
+// computes tile t - box.first of item wg_item[t], t = i after XCD
+// renumbering.
+#ifndef WF_BATCH_FY1
+#define WF_BATCH_FY1 8
+#endif
+template <int PM>
+constexpr int twostep_batch_waves()
+{
+    return PM == 0 ? kF2 : ((PM == 1 || PM == 2 || PM == 4) ? WF_BATCH_FY1 : kF2x);
+}
+
+template <int PM>
+__global__ __launch_bounds__(64 * twostep_batch_waves<PM>()) void k_twostep4_batch(
+    const TwoStepItem *__restrict__ items, const uint32_t *__restrict__ wg_item, int total)
+{
+    const int t = xcd_tile(blockIdx.x, total);
+    const TwoStepItem &it = items[wg_item[t]];
+    twostep_tile<PM, 0, twostep_batch_waves<PM>()>(it.g, it.a, it.b, it.box, it.pts,
+                                                   t - it.box.first);
+}
+#endif
+
 }  // namespace
 
 #ifndef WF_STEP_FMA
@@ -1421,6 +1454,18 @@ int twostep_rows_per_tile(int pml_axes, int lpr)
     case 2: waves = twostep_waves<2>(); break;
     case 4: waves = twostep_waves<4>(); break;
     case 6: waves = twostep_waves<6>(); break;
+    default: break;
+    }
+    return waves * (64 / lpr) - 3;
+}
+int twostep_batch_rows_per_tile(int pml_axes, int lpr)
+{
+    int waves = twostep_batch_waves<7>();
+    switch (pml_axes) {
+    case 0: waves = twostep_batch_waves<0>(); break;
+    case 1: waves = twostep_batch_waves<1>(); break;
+    case 2: waves = twostep_batch_waves<2>(); break;
+    case 4: waves = twostep_batch_waves<4>(); break;
     default: break;
     }
     return waves * (64 / lpr) - 3;
@@ -1463,6 +1508,31 @@ hipError_t launch_twostep(const StepLaunch &s, const Box &box, const TwoStepPoin
 #undef WF_TWOSTEP_LAUNCH
     return hipGetLastError();
 }
+
+#ifndef WF_STEP_FMA
+hipError_t launch_batch_twostep(const TwoStepItem *items, const uint32_t *wg_item, int total,
+                                int pml_axes, hipStream_t st)
+{
+    if (total <= 0) return hipSuccess;
+    const dim3 grid((unsigned)total);
+#define WF_TWOSTEP_BATCH(PM)                                                                  \
+    hipLaunchKernelGGL((k_twostep4_batch<PM>), grid, dim3(64, twostep_batch_waves<PM>()), 0, st, items, \
+                       wg_item, total)
+    switch (pml_axes) {
+    case 0: WF_TWOSTEP_BATCH(0); break;
+    case 1: WF_TWOSTEP_BATCH(1); break;
+    case 2: WF_TWOSTEP_BATCH(2); break;
+    case 3: WF_TWOSTEP_BATCH(3); break;
+    case 4: WF_TWOSTEP_BATCH(4); break;
+    case 5: WF_TWOSTEP_BATCH(5); break;
+    case 6: WF_TWOSTEP_BATCH(6); break;
+    case 7: WF_TWOSTEP_BATCH(7); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef WF_TWOSTEP_BATCH
+    return hipGetLastError();
+}
+#endif
 
 hipError_t launch_h(const StepLaunch &s, const BoxList &bl, hipStream_t st)
 {

@@ -157,8 +157,11 @@ class DeviceEngine:
         return ms.value
 
     def _kernel_names(self) -> tuple:
-        if self.kernel_variant() == "twostep":
+        v = self.kernel_variant()
+        if v == "twostep":
             return ("k_twostep4", "k_twostep4_xshell", "k_twostep4_yzshell")
+        if v == "twostep-rows":
+            return ("k_twostep4_batch", "unused_h", "unused_e")
         return ("k_fused4", "k_h4", "k_e4")
 
     def time_kernels(self, nsteps: int) -> dict:
@@ -166,7 +169,8 @@ class DeviceEngine:
         events on the engine's stream): fused interior kernel, H and E
         kernels; with two steps per pass (variant "twostep"), the two-step
         kernels per pass over the interior region, the x shell regions and
-        the y and z shell regions."""
+        the y and z shell regions; in rows (variant "twostep-rows"), the
+        batched two-step launches of a pass together."""
         f, h, e = c_double(), c_double(), c_double()
         check(lib().wf_time_kernels(self._h, nsteps, byref(f), byref(h), byref(e)))
         return dict(zip(self._kernel_names(), (f.value, h.value, e.value)))
@@ -249,8 +253,10 @@ class DeviceEngine:
 
 class DeviceBatch:
     """Many independent scenes (DeviceEngines on one GPU) advanced together:
-    one H and one E launch per step over every scene, then the sources and
-    probes of all scenes.  Each scene's results equal stepping it alone."""
+    two steps per pass, one two-step launch per CPML variant over every
+    scene's regions, then the point work of all scenes (or, with
+    WF_BATCH_TWOSTEP=0, a periodic axis or a split-variant engine, one H and
+    one E launch per step).  Each scene's results equal stepping it alone."""
 
     def __init__(self, engines: list[DeviceEngine]):
         self.engines = list(engines)
@@ -297,6 +303,13 @@ class DeviceBatch:
 
     def set_tiling(self, zchunk: int, lanes_per_row: int, grid: int = 0) -> None:
         check(lib().wf_batch_set_tiling(self._h, zchunk, lanes_per_row, grid))
+
+    def two_step(self):
+        """(True, algorithmic bytes of one pass) when the batch runs two
+        steps per pass, else (False, 0)."""
+        t, nb = c_int(), c_uint64()
+        check(lib().wf_batch_two_step(self._h, byref(t), byref(nb)))
+        return bool(t.value), int(nb.value)
 
     def cells(self) -> int:
         return int(lib().wf_batch_cells(self._h))
