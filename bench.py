@@ -149,8 +149,12 @@ def run_batch_96(args):
     another on their own engines.  value = sum of cells x steps / time of
     the batched run, tuning steps included."""
     from wavefront_amd import workloads
-    from wavefront_amd.engine import DeviceBatch, DeviceEngine
+    from wavefront_amd.engine import DeviceBatch, DeviceEngine, hbm_copy_gbs
 
+    copy_gbs = None
+    if args.copy_gib > 0:
+        copy_gbs = hbm_copy_gbs(int(args.copy_gib * (1 << 30)))
+        log(f"plain copy kernel: {copy_gbs:.0f} GB/s (read + write)")
     steps = args.c4_steps
     ws = workloads.batch_scenes(steps=steps)
     cells = sum(w.cells() for w in ws)
@@ -254,7 +258,8 @@ def run_batch_96(args):
         },
         "roofline": {"bound": "hbm", "kernel": kernel,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "copy_kernel_gbs": None if copy_gbs is None else round(copy_gbs, 1)},
     }
     print(json.dumps(out), flush=True)
 

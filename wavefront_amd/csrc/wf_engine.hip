@@ -604,7 +604,8 @@ void fit_columns(Box &b, int cap, int axes)
 //  * full rows: per y and z range one box over every x, on the variant
 //    with the x CPML (rows of up to 32 lanes).  The narrow x shell columns
 //    waste lanes and lines; on the 96-scene set (edges 32-192) full rows
-//    took 1.77 ms per step against 2.29 for the split, measured on MI355X;
+//    took 1.89 ms per step against 2.54 for the split and 2.31 for whole-
+//    scene boxes, measured on MI355X;
 //  * the whole scene as one box of the every-axis CPML variant: where the
 //    split does not fit (x interior under 8 cells, or y or z shells that
 //    meet), and as mode 2.
