@@ -236,13 +236,16 @@ int wf_run(wf_ctx *ctx, uint64_t num_steps, uint64_t snapshot_every, const char 
 /* Replaces calling the loop body of engine::run (src/engine.rs:747-782) for
  * each of many small scenes in turn: the scenes (contexts created with
  * wf_create on one device, sources and probes set as usual) advance together,
- * one H launch and one E launch per step covering every scene, then one
- * launch each for all sources and all probes.  Each scene's results are
- * bit-identical to stepping it alone.  The batch updates each context's
- * current state in place and advances its step count; contexts stay owned by
+ * two steps per pass with one two-step launch per CPML variant covering
+ * every scene (see wf_batch_two_step), or one H launch and one E launch per
+ * step, then one launch each for all sources and all probes.  Each scene's
+ * results are bit-identical to stepping it alone.  The batch advances each
+ * context's current state (the two-step passes write the context's second
+ * state, which becomes the current one) and its step count; contexts stay owned by
  * the caller and must outlive the batch.  Work queued on a context's own
  * stream is waited for at each wf_batch_step; wf_batch_sync before reading a
- * scene back. */
+ * scene back.  A batch runs the exact (uncontracted) kernels whatever a
+ * scene's wf_set_fp_contract setting. */
 typedef struct wf_batch wf_batch;
 int wf_batch_create(uint32_t n, wf_ctx *const *scenes, wf_batch **out);
 void wf_batch_destroy(wf_batch *batch);

@@ -316,6 +316,7 @@ struct wf_ctx {
 
     float *d_fields = nullptr;
     float *d_fields2 = nullptr;
+    uint64_t fstride = 0;  // floats from one field of a state to the next (>= cells)
     float *d_psi2 = nullptr;
     uint8_t *d_id = nullptr;
     float4 *d_coef = nullptr;
@@ -1165,6 +1166,9 @@ int launch_pair(wf_ctx *c, hipEvent_t *ev = nullptr)
     pts.step = c->d_step;
     pts.slot = c->d_pslot;
     StepLaunch s{c->g, c->p, c->alt, c->pml, c->fzchunk, st};
+    // one stream: the shell regions' launches spread over three more streams
+    // beside the interior's measured 7.02-7.28 ms per step against 6.65-6.68
+    // at 1024^3 (four runs each, one box)
     for (int grp = 0; grp < 3; ++grp) {
         if (ev) WF_HIP(hipEventRecord(ev[2 * grp], st));
         for (int k = 0; k < c->nregions; ++k) {
@@ -1237,11 +1241,11 @@ int probe_step_ms(wf_ctx *c, int warm, int timed, double *ms)
     return rc;
 }
 
-void point_fields(wf_ctx *c, uint64_t N)
+void point_fields(wf_ctx *c, uint64_t)
 {
     for (int i = 0; i < 6; ++i) {
-        c->p.f[i] = c->d_fields + (uint64_t)i * N;
-        c->alt.f[i] = c->d_fields2 ? c->d_fields2 + (uint64_t)i * N : nullptr;
+        c->p.f[i] = c->d_fields + (uint64_t)i * c->fstride;
+        c->alt.f[i] = c->d_fields2 ? c->d_fields2 + (uint64_t)i * c->fstride : nullptr;
     }
 }
 
@@ -1252,7 +1256,7 @@ void point_fields(wf_ctx *c, uint64_t N)
 // Back to the initial state after a probe: zero fields and psi, step 0.
 int reset_state(wf_ctx *c)
 {
-    const uint64_t bytes = 6ull * c->g.nx * c->g.ny * c->g.nz * sizeof(float);
+    const uint64_t bytes = 6ull * c->fstride * sizeof(float);
     WF_HIP(hipMemsetAsync(c->d_fields, 0, bytes, c->stream));
     if (c->d_fields2) WF_HIP(hipMemsetAsync(c->d_fields2, 0, bytes, c->stream));
     if (c->d_psi) WF_HIP(hipMemsetAsync(c->d_psi, 0, c->psi_bytes, c->stream));
@@ -1269,7 +1273,7 @@ int choose_placement(wf_ctx *c, int extra, uint64_t N, double budget_ms)
     auto elapsed_ms = [&] {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     };
-    const size_t bytes = 6 * N * sizeof(float);
+    const size_t bytes = 6 * c->fstride * sizeof(float);
     float **slot[2] = {&c->d_fields, &c->d_fields2};
     const int nslot = c->d_fields2 ? 2 : 1;
     c->place_n = 0;
@@ -1498,11 +1502,19 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
         else return bail(fail(WF_ERR_INVALID, "WF_KERNEL must be 'fused' or 'split' (got '%s')", kv));
     }
     const uint64_t N = (uint64_t)nx * ny * nz;
-    if (int rc = dmalloc(&c->d_fields, 6 * N * sizeof(float))) return bail(rc);
+    // the six fields of a state lie N + 1024 floats apart (WF_FIELD_PAD bytes,
+    // whole 256 B, instead of 4096): with N a power of two the fields would
+    // start at power-of-two distances; measured at 1024^3 the two-step
+    // interior kernel takes 9.23-9.25 ms per launch against 9.34-9.51
+    // unpadded (four runs each, one box), the step 0.1-0.3% less
+    uint64_t pad = 4096;
+    if (const char *fp = std::getenv("WF_FIELD_PAD")) pad = (uint64_t)std::max(0L, std::atol(fp));
+    c->fstride = N + pad / 256 * 64;
+    const uint64_t sbytes = 6 * c->fstride * sizeof(float);
+    if (int rc = dmalloc(&c->d_fields, sbytes)) return bail(rc);
     if (c->fused) {
-        if (int rc = dmalloc(&c->d_fields2, 6 * N * sizeof(float))) return bail(rc);
-        for (int i = 0; i < 6; ++i) c->alt.f[i] = c->d_fields2 + (uint64_t)i * N;
-        if (hipMemsetAsync(c->d_fields2, 0, 6 * N * sizeof(float), c->stream) != hipSuccess)
+        if (int rc = dmalloc(&c->d_fields2, sbytes)) return bail(rc);
+        if (hipMemsetAsync(c->d_fields2, 0, sbytes, c->stream) != hipSuccess)
             return bail(fail(WF_ERR_HIP, "hipMemset(fields) failed"));
     }
     if (int rc = dmalloc(&c->d_id, N)) return bail(rc);
@@ -1510,11 +1522,11 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     if (int rc = dmalloc(&c->d_step, sizeof(uint64_t))) return bail(rc);
     if (hipMemsetAsync(c->d_step, 0, sizeof(uint64_t), c->stream) != hipSuccess)
         return bail(fail(WF_ERR_HIP, "hipMemset(step) failed"));
-    for (int i = 0; i < 6; ++i) c->p.f[i] = c->d_fields + (uint64_t)i * N;
+    point_fields(c, N);
     c->p.id = c->alt.id = c->d_id;
     c->p.coef = c->alt.coef = c->d_coef;
 
-    if (hipMemsetAsync(c->d_fields, 0, 6 * N * sizeof(float), c->stream) != hipSuccess)
+    if (hipMemsetAsync(c->d_fields, 0, sbytes, c->stream) != hipSuccess)
         return bail(fail(WF_ERR_HIP, "hipMemset(fields) failed"));
     if (material_ids) {
         if (hipMemcpyAsync(c->d_id, material_ids, N, hipMemcpyHostToDevice, c->stream) != hipSuccess)

@@ -161,7 +161,7 @@ class DeviceEngine:
         if v == "twostep":
             return ("k_twostep4", "k_twostep4_xshell", "k_twostep4_yzshell")
         if v == "twostep-rows":
-            return ("k_twostep4_batch", "unused_h", "unused_e")
+            return ("k_twostep4_batch",)
         return ("k_fused4", "k_h4", "k_e4")
 
     def time_kernels(self, nsteps: int) -> dict:
