@@ -840,10 +840,12 @@ void plan_boxes(wf_ctx *c)
             }
         c->pair_geo = true;
     }
-    // the rows schedule's items (pass_items, full rows), grouped by variant
+    // the rows schedule's items (pass_items, by width), grouped by variant
     {
         std::vector<PassItem> items;
-        pass_items(g, 1, 0, items);
+        int mode = -1;
+        if (const char *rp = std::getenv("WF_ROWS_PLAN")) mode = std::min(2, std::max(-1, std::atoi(rp)));
+        pass_items(g, mode, 0, items);
         std::vector<const Geo *> geo{&g};
         c->rows_bytes = group_items(items, geo, 0, c->cus, c->rows_plan, c->rows_first, c->rows_total);
     }
