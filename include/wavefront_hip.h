@@ -434,24 +434,40 @@ int wf_read_trace(wf_ctx *ctx, float *trace, uint64_t max_steps);
  * writes the elapsed milliseconds. */
 int wf_time_steps(wf_ctx *ctx, uint64_t nsteps, double *total_ms);
 /* Runs nsteps full steps with a HIP event pair around every step-kernel
- * launch (same stream); writes the mean per-launch milliseconds of the fused
- * single-pass kernel (0 in the split variant), the H-phase kernel and the
- * E-phase kernel (in the fused variant those two cover only the frame of
- * cells around the fused kernel's interior box, which holds the CPML). */
+ * launch (same stream); writes the mean per-launch milliseconds of three
+ * kernel groups, by wf_kernel_variant: "fused" -- the single-pass kernel
+ * over the interior box, the H-phase and the E-phase kernels over the frame
+ * of cells around it (which holds the CPML); "split" -- 0, the H and the E
+ * kernel over the whole domain; "twostep" -- per pass of two steps, the
+ * two-step kernels over the interior region, over the x shell regions and
+ * over the y and z shell regions; "twostep-rows" -- per pass, the batched
+ * two-step launches together in fused_ms, 0, 0. */
 int wf_time_kernels(wf_ctx *ctx, uint64_t nsteps, double *fused_ms, double *h_ms,
                     double *e_ms);
-/* Algorithmic HBM bytes of one launch of each step kernel.  Fused kernel:
+/* Algorithmic HBM bytes of one launch of each group above.  Fused kernel:
  * 24 B read + 24 B written + 1 B material id per interior cell.  H or E
  * kernel: per cell it covers 12 B read of the curl field, 12 B read + 12 B
  * write of the updated field and 1 B id, plus 8 B (read + write) per
- * non-identity psi term of that phase. */
+ * non-identity psi term of that phase.  Two-step kernels: 49 B per cell for
+ * the two steps and 32 B per graded axis of a cell. */
 int wf_kernel_bytes(const wf_ctx *ctx, uint64_t *fused_bytes, uint64_t *h_bytes,
                     uint64_t *e_bytes);
-/* "fused" (default: single-pass kernel over the interior, split H/E kernels
- * over the CPML frame, ping-pong state) or "split" (H kernel then E kernel
- * over the whole domain, in place); chosen at wf_create from the WF_KERNEL
- * environment variable (WF_ZCHUNK sets the z planes per workgroup). */
+/* The step scheme in use: "twostep" (two steps per pass on the two-step
+ * kernels over the CPML regions and the interior, state A -> B), "twostep-
+ * rows" (two steps per pass through the batched two-step kernel over full
+ * x rows), "fused" (one step per pass: the single-pass kernel over the
+ * interior, split H/E kernels over the CPML frame, ping-pong state) or
+ * "split" (H kernel then E kernel over the whole domain, in place; chosen
+ * at wf_create by WF_KERNEL=split). */
 const char *wf_kernel_variant(const wf_ctx *ctx);
+/* Schedule chosen at wf_create: 0 two-step regions, 1 two-step rows, 2
+ * single steps (WF_SCHEDULE sets it; otherwise grids under 2^25 cells time
+ * the three on the zeroed state and keep the fastest, larger grids take 0).
+ * probe_ms (may be null): the three timings in ms per step, -1 where not
+ * timed.  wf_kernel_variant reports what runs (face hooks, periodic axes or
+ * the contracted set fall back to other schemes).  No counterpart in the
+ * reference. */
+int wf_schedule(const wf_ctx *ctx, int *schedule, double *probe_ms);
 /* Field placement chosen by wf_create: the step time (ms) of each
  * arrangement it timed, in order (up to `cap` written; the initial buffers
  * first, then each extra state buffer in place of state A and, in the fused

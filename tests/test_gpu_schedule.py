@@ -49,6 +49,7 @@ def test_forced_schedule(shape, schedule, monkeypatch):
     w = _workload(shape, steps)
     print(f"[gpu-case] schedule {schedule} {w.name}", flush=True)
     eng = _run(w, steps)
+    assert eng.schedule() == (int(schedule), [-1.0, -1.0, -1.0])
     v = eng.kernel_variant()
     if schedule == "1":
         assert v == "twostep-rows"
@@ -78,7 +79,12 @@ def test_timed_schedule(shape, monkeypatch):
     w = _workload(shape, steps)
     print(f"[gpu-case] timed schedule {w.name}", flush=True)
     eng = _run(w, steps)
-    assert eng.kernel_variant() in ("twostep", "twostep-rows", "fused")
+    k, ms = eng.schedule()
+    timed = [m for m in ms if m >= 0.0]
+    # the regions are timed only where the geometry allows them
+    assert (ms[0] >= 0.0) == (shape == (96, 72, 64)) and ms[1] > 0.0 and ms[2] > 0.0
+    assert ms[k] == min(timed)
+    assert eng.kernel_variant() == ("twostep", "twostep-rows", "fused")[k]
     assert eng.step_count() == 0
     eng.step(steps)
     eng.sync()

@@ -99,6 +99,7 @@ _SIGNATURES = {
                               POINTER(c_int), POINTER(c_uint64)]),
     "wf_batch_set_tiling": (c_int, [c_void_p, c_int, c_int, c_int]),
     "wf_batch_two_step": (c_int, [c_void_p, POINTER(c_int), POINTER(c_uint64)]),
+    "wf_schedule": (c_int, [c_void_p, POINTER(c_int), POINTER(c_double)]),
     "wf_batch_cells": (c_uint64, [c_void_p]),
     "wf_mlp_param_count": (c_uint32, []),
     "wf_mlp_trainer_create": (c_int, [c_int, c_uint64, POINTER(c_float), POINTER(c_float),

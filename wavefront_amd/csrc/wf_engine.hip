@@ -551,8 +551,8 @@ int fused_slices(long tiles, int planes, int cus, int extra = 1)
 
 // Algorithmic bytes of one two-step pass over a box's owned cells (tile
 // column tx owns [x0 + tx tstride, + 4 (lpr - 2)) clipped to x1): 49 B per
-// cell for the two steps, and 16 B per pair of non-identity psi terms (2 per
-// phase of each graded axis of a cell, read once and written once).
+// cell for the two steps, and 32 B per graded axis of a cell (its four
+// non-identity psi terms, each read once and written once).
 uint64_t twostep_box_bytes(const Geo &g, const Box &b)
 {
     auto graded_at = [&](int a, int c, int n) {
@@ -579,7 +579,7 @@ uint64_t twostep_box_bytes(const Geo &g, const Box &b)
 // rows leave lanes idle (64 / lanes-per-row rows per wave).
 void fit_columns(Box &b, int cap, int axes)
 {
-    const int q = (b.x1 - b.x0) / 4, h = b.y1 - b.y0;
+    const int q = std::max((b.x1 - b.x0) / 4, 1), h = b.y1 - b.y0;
     const int k0 = (q + cap - 3) / (cap - 2);
     long best = -1;
     for (int k = k0; k <= std::min(q, k0 + 3); ++k) {
@@ -1095,7 +1095,9 @@ bool pair_active(const wf_ctx *c) { return pair_kind(c) != 0; }
 int rows_tables(wf_ctx *c)
 {
     if (pair_kind(c) != 2 || !c->rows_dirty) return WF_OK;
-    if (!c->d_ritems) {
+    if (!c->d_ritems || !c->d_rwg) {
+        dfree(c->d_ritems);
+        dfree(c->d_rwg);
         uint64_t tiles = 0;
         for (int v = 0; v < 8; ++v) tiles += (uint64_t)c->rows_total[v];
         std::vector<uint32_t> wg(tiles);
@@ -1168,9 +1170,9 @@ int launch_pair(wf_ctx *c, hipEvent_t *ev = nullptr)
     pts.step = c->d_step;
     pts.slot = c->d_pslot;
     StepLaunch s{c->g, c->p, c->alt, c->pml, c->fzchunk, st};
-    // one stream: the shell regions' launches spread over three more streams
-    // beside the interior's measured 7.02-7.28 ms per step against 6.65-6.68
-    // at 1024^3 (four runs each, one box)
+    // every region on one stream: spreading the shell regions' launches over
+    // three more streams beside the interior's measured 7.02-7.28 ms per step
+    // against 6.65-6.68 at 1024^3 (four runs each, one box)
     for (int grp = 0; grp < 3; ++grp) {
         if (ev) WF_HIP(hipEventRecord(ev[2 * grp], st));
         for (int k = 0; k < c->nregions; ++k) {
@@ -2204,6 +2206,15 @@ const char *wf_kernel_variant(const wf_ctx *c)
     if (!c->fused) return "split";
     const int k = pair_kind(c);
     return k == 1 ? "twostep" : (k == 2 ? "twostep-rows" : "fused");
+}
+
+int wf_schedule(const wf_ctx *c, int *schedule, double *probe_ms)
+{
+    if (!c || !schedule) return fail(WF_ERR_INVALID, "wf_schedule: null pointer");
+    *schedule = c->sched;
+    if (probe_ms)
+        for (int k = 0; k < 3; ++k) probe_ms[k] = c->sched_ms[k];
+    return WF_OK;
 }
 
 int wf_kernel_bytes(const wf_ctx *c, uint64_t *fused_bytes, uint64_t *h_bytes,

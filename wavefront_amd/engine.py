@@ -185,6 +185,14 @@ class DeviceEngine:
     def kernel_variant(self) -> str:
         return lib().wf_kernel_variant(self._h).decode()
 
+    def schedule(self):
+        """(schedule chosen at creation: 0 regions, 1 rows, 2 single steps;
+        the creation probe's ms per step of each, -1 where not timed)."""
+        k = c_int()
+        ms = (c_double * 3)()
+        check(lib().wf_schedule(self._h, byref(k), ms))
+        return k.value, [ms[0], ms[1], ms[2]]
+
     def placement(self) -> dict:
         """Field placement chosen at creation: ms per step of each arrangement
         timed (initial buffers, then each extra state buffer as state A and as
