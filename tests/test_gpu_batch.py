@@ -161,3 +161,55 @@ def test_batch_then_solo_steps_continue():
         sim.run(k, w.dt, threads=8)
         assert_bitwise(eng.read_fields(), sim.read_fields(), f"{w.name} fields")
         eng.close()
+
+
+@pytest.mark.parametrize("schedule", ["twostep", "split"])
+def test_batch_many_sources_list_order(schedule, monkeypatch):
+    """A batched scene with 3000 sources (an Ez sheet, scattered points and
+    voxels named several times, soft and hard mixed, the three E
+    components), applied in parallel per (component, voxel) group by the
+    batched point kernels and, on two-step passes, inside the two-step
+    kernels: 11 steps (five passes and a split step, or split steps only)
+    equal the oracle's list-order loop; a CPML scene beside it."""
+    from oracle import wfo
+    from wavefront_amd.layout import GridDims
+    from wavefront_amd.source import FieldComponent as SC, Source, Waveform
+    monkeypatch.setenv("WF_BATCH_TWOSTEP", "1" if schedule == "twostep" else "0")
+    steps = 11
+    rng = np.random.default_rng(21)
+    d = GridDims(32, 24, 40)
+    dt, dx = 1.5e-12, 1e-3
+    comps = (SC.Ex, SC.Ey, SC.Ez)
+    forms = (Waveform.sinusoid(3e10), Waveform.ricker_wavelet(2e10, 5e-11),
+             Waveform.gaussian_pulse(2e-11, 1e-11))
+    srcs = [Source(x, y, 9, SC.Ez, 0.5, forms[0]) for y in range(d.ny) for x in range(d.nx)]
+    pts = [(int(rng.integers(d.nx)), int(rng.integers(d.ny)), int(rng.integers(d.nz)))
+           for _ in range(300)]
+    while len(srcs) < 3000:
+        x, y, z = pts[int(rng.integers(len(pts)))] if rng.random() < 0.7 else (
+            int(rng.integers(d.nx)), 5, int(rng.integers(d.nz)))
+        srcs.append(Source(x, y, z, comps[int(rng.integers(3))], float(rng.uniform(-2, 2)),
+                           forms[int(rng.integers(3))], hard=bool(rng.random() < 0.2)))
+    srcs = [srcs[i] for i in rng.permutation(len(srcs))]
+    print(f"[gpu-case] batch many sources n={len(srcs)} {schedule}", flush=True)
+    sheet = DeviceEngine(d, None, None, None, dt=dt, dx=dx)
+    sheet.set_sources(srcs, dt, steps)
+    w = workloads.batch_scenes(nscenes=1, steps=steps, seed=7, edges=[(48, 40, 56)])[0]
+    other = _engine(w, steps)
+    with DeviceBatch([sheet, other]) as b:
+        assert b.two_step()[0] == (schedule == "twostep")
+        b.step(steps)
+        b.sync()
+    sim = wfo.OracleSim(d.nx, d.ny, d.nz, table=wfo.vacuum_table(dt, dx))
+    for s in srcs:
+        sim.add_source(s.x, s.y, s.z, int(s.component), s.amplitude, s.waveform.kind,
+                       s.waveform.p0, s.waveform.p1, hard=s.hard)
+    sim.run(steps, dt)
+    assert_bitwise(sheet.read_fields(), sim.read_fields(), "sheet scene fields")
+    sim.close()
+    sim = oracle_sim_for(w)
+    sim.run(steps, w.dt, threads=4)
+    assert_bitwise(other.read_fields(), sim.read_fields(), f"{w.name} fields")
+    sim.close()
+    sheet.close()
+    other.close()

@@ -60,12 +60,16 @@ __device__ __forceinline__ void apply_source(const Ptrs &p, const DevSource &q,
         *f = *f + v[q.col];
 }
 
-__device__ __forceinline__ void apply_sources(const Ptrs &p, const DevSource *__restrict__ src,
-                                              int n, int ncol, const float *__restrict__ values,
-                                              uint64_t row)
+// One workgroup's lanes over the groups [grp[g], grp[g+1]) of a scene's
+// grouped table, each lane a group in list order.
+__device__ __forceinline__ void apply_source_groups(const Ptrs &p, const DevSource *__restrict__ src,
+                                                   const uint32_t *__restrict__ grp, int ngrp,
+                                                   int ncol, const float *__restrict__ values,
+                                                   uint64_t row)
 {
     const float *v = values + row * (uint64_t)ncol;
-    for (int s = 0; s < n; ++s) apply_source(p, src[s], v);
+    for (int g = threadIdx.x; g < ngrp; g += blockDim.x)
+        for (uint32_t s = grp[g]; s < grp[g + 1]; ++s) apply_source(p, src[s], v);
 }
 
 // One lane per group [grp[i], grp[i+1]) of the grouped table; a step's
@@ -126,8 +130,8 @@ __global__ void k_probes(Ptrs p, const DevProbe *__restrict__ pr, int n, int nf,
 __global__ void k_sources_batch(const BatchScene *__restrict__ sc)
 {
     const BatchScene &d = sc[blockIdx.x];
-    if (threadIdx.x != 0 || d.nsrc == 0) return;
-    apply_sources(d.p, d.src, d.nsrc, d.ncol, d.srcval, *d.step - d.src_base);
+    if (d.nsrc == 0) return;
+    apply_source_groups(d.p, d.src, d.src_grp, d.nsrc_grp, d.ncol, d.srcval, *d.step - d.src_base);
 }
 
 __global__ void k_probes_batch(const BatchScene *__restrict__ sc)
@@ -152,8 +156,9 @@ __global__ void k_pair_points_batch(const BatchScene *__restrict__ sc)
     for (int i = threadIdx.x; i < d.nprobe; i += blockDim.x)
         accumulate_probe(d.p, d.probe[i], i, d.nprobe, d.nfreq, d.cos_tab, d.sin_tab, d.active,
                          step - d.probe_base, d.sums, d.counts, d.trace, d.slot);
-    if (threadIdx.x == 0 && d.nsrc)
-        apply_sources(d.p, d.src, d.nsrc, d.ncol, d.srcval, step + 1 - d.src_base);
+    if (d.nsrc)
+        apply_source_groups(d.p, d.src, d.src_grp, d.nsrc_grp, d.ncol, d.srcval,
+                            step + 1 - d.src_base);
     __syncthreads();  // the second step's sources are in place
     for (int i = threadIdx.x; i < d.nprobe; i += blockDim.x)
         accumulate_probe(d.p, d.probe[i], i, d.nprobe, d.nfreq, d.cos_tab, d.sin_tab, d.active,
@@ -2808,6 +2813,8 @@ int batch_upload(wf_batch *b)
         d.slot = c->d_pslot;
         d.box = b->boxes[i % n];
         d.src = c->d_src;
+        d.src_grp = c->d_src_grp;
+        d.nsrc_grp = (int)c->nsrc_grp;
         d.srcval = c->d_srcval;
         d.src_base = c->src_base;
         d.nsrc = (int)c->nsrc;

@@ -227,7 +227,9 @@ struct BatchScene {
     Geo g;
     Ptrs p;
     Box box;  // box.first = the scene's first workgroup in the batched launch
-    const DevSource *src;
+    const DevSource *src;     // grouped by (component, voxel), list order within a group
+    const uint32_t *src_grp;  // group starts, nsrc_grp + 1 entries
+    int nsrc_grp;
     const float *srcval;
     uint64_t src_base;
     const DevProbe *probe;
