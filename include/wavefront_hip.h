@@ -253,7 +253,9 @@ int wf_batch_step(wf_batch *batch, uint64_t nsteps);
 int wf_batch_sync(wf_batch *batch);
 /* nsteps steps between two events on the batch stream; *total_ms = their time. */
 int wf_batch_time_steps(wf_batch *batch, uint64_t nsteps, double *total_ms);
-/* Autotuner: runs steps_per_candidate real steps (the first untimed) under
+/* Autotuner of the split steps (a batch on two-step passes takes no steps
+ * and reports its current tiling with *steps_taken = 0): runs
+ * steps_per_candidate real steps (the first untimed) under
  * each candidate tiling and schedule (z-chunk, lanes-per-row cap -- a power
  * of two up to 64, or 0 for rows fitted to each scene's width --, grid: 0 =
  * one workgroup per tile, n = n persistent workgroups striding over the

@@ -302,8 +302,11 @@ class DeviceBatch:
         return ms.value
 
     def tune(self, steps_per_candidate: int = 4):
-        """Autotunes tiling and schedule on real steps; returns (zchunk,
-        lanes_per_row, grid, steps_taken), grid 0 = one workgroup per tile."""
+        """Autotunes the split steps' tiling and schedule on real steps;
+        returns (zchunk, lanes_per_row, grid, steps_taken), grid 0 = one
+        workgroup per tile.  A batch on two-step passes has nothing to tune
+        (its regions follow each scene's geometry): the current tiling and 0
+        steps."""
         z, l, g, n = c_int(), c_int(), c_int(), c_uint64()
         check(lib().wf_batch_tune(self._h, steps_per_candidate, byref(z), byref(l), byref(g),
                                   byref(n)))
