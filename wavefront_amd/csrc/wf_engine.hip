@@ -277,6 +277,7 @@ struct wf_ctx {
     Box xmid{}, xedges{};  // xedges: two tile columns, x from 0 and from nx - 256
     BoxList framex{};
     Box all{};          // fused variant with a CPML: the whole domain, one pass
+    Box allg{};         // the same swept by the ghost variant (learned faces)
     BoxList lofix{};    // learned faces without a CPML: the low faces' boundary cells
     bool pml_fused = false;  // WF_PML_FUSED=1: single pass over the shell too (slower, see launch_step)
     BoxList whole{};    // split variant: the whole domain
@@ -859,6 +860,13 @@ void plan_boxes(wf_ctx *c)
     all.gy = (n[1] + frx - 1) / frx;  // swept by the CPML or ghost variant
     all.gz = slices((long)all.gx * all.gy, n[2]);
     c->all = all;
+    Box allg = all;
+    {
+        const int fg = fused_rows_per_tile_ghost();
+        allg.gy = (n[1] + fg - 1) / fg;
+        allg.gz = slices((long)allg.gx * allg.gy, n[2]);
+    }
+    c->allg = allg;
     BoxList w{};
     add_box(w, split_box(0, n[0], 0, n[1], 0, n[2], zc));
     c->whole = w;
@@ -974,7 +982,7 @@ int launch_step(wf_ctx *c, hipEvent_t *ev = nullptr)
         s.ghost = true;
         if (int rc = run_hooks(c, true, c->p, c->p, st)) return rc;
         if (ev) WF_HIP(hipEventRecord(ev[0], st));
-        WF_HIP(c->lf(s, c->all, false, st, false));
+        WF_HIP(c->lf(s, c->allg, false, st, false));
         if (ev) {
             WF_HIP(hipEventRecord(ev[1], st));
             WF_HIP(hipEventRecord(ev[2], st));

@@ -218,6 +218,7 @@ int twostep_rows_per_tile(int pml_axes, int lpr);  // owned rows per tile
 // of lpr = 64).
 int fused_rows_per_tile(bool pml_or_ghost);
 int fused_rows_per_tile_x();  // the x-only CPML variant
+int fused_rows_per_tile_ghost();  // the ghost variant (learned faces)
 int split_rows_per_wg();
 
 // One scene of a batch: its geometry and state (updated in place by the
