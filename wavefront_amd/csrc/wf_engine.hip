@@ -1296,7 +1296,12 @@ int choose_placement(wf_ctx *c, int extra, uint64_t N, double budget_ms)
     c->place_n = 0;
     c->place_kept = 0;
     double best = 0.0;
-    if (int rc = probe_step_ms(c, 2, 2, &best)) return rc;
+    // three timed passes per arrangement (WF_PLACEMENT_PASSES): with one, the
+    // kept arrangement's bench time came 0.02-0.35 ms per step above its
+    // probe time at 1024^3, with three a steady 0.10-0.11 (four runs each)
+    int timed = 6;
+    if (const char *pp = std::getenv("WF_PLACEMENT_PASSES")) timed = 2 * std::max(1, std::atoi(pp));
+    if (int rc = probe_step_ms(c, 2, timed, &best)) return rc;
     c->place_ms[c->place_n++] = best;
     for (int t = 0; t < extra && t < kPlaceMax && c->place_n + nslot <= kPlaceProbes && elapsed_ms() < budget_ms;
          ++t) {
@@ -1313,7 +1318,7 @@ int choose_placement(wf_ctx *c, int extra, uint64_t N, double budget_ms)
             *slot[k] = x;
             point_fields(c, N);
             double ms = 0.0;
-            rc = probe_step_ms(c, 2, 2, &ms);
+            rc = probe_step_ms(c, 2, timed, &ms);
             *slot[k] = held;
             point_fields(c, N);
             if (rc) break;
@@ -1678,7 +1683,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     plan_boxes(c);
     // Field placement (see choose_placement): by default eight extra state
     // buffers tried, one held at a time beside the current ones, when one
-    // state's fields take at least 8 GiB (4 steps per arrangement timed, 17
+    // state's fields take at least 8 GiB (2 + 6 steps per arrangement, 17
     // arrangements: about 1 s at 1024^3), no new buffer after 4 s.
     // WF_PLACEMENT=n tries n (0 = no probe, at most 8); WF_PLACEMENT_MIN_MIB
     // moves the size threshold.
