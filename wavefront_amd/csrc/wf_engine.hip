@@ -469,6 +469,7 @@ void free_sources(wf_ctx *c)
     c->nsrc_grp = 0;
     dfree(c->d_srcval);
     free_points(c->src_pts);
+    c->rows_dirty = true;  // the batched kernel's item tables hold the point tables
     c->nsrc = 0;
     c->nsrc_col = 0;
     c->src_steps = 0;
@@ -1260,6 +1261,9 @@ int probe_step_ms(wf_ctx *c, int warm, int timed, double *ms)
 
 void point_fields(wf_ctx *c, uint64_t)
 {
+    // the batched kernel's item tables hold the state pointers: rewritten
+    // before their next use
+    c->rows_dirty = true;
     for (int i = 0; i < 6; ++i) {
         c->p.f[i] = c->d_fields + (uint64_t)i * c->fstride;
         c->alt.f[i] = c->d_fields2 ? c->d_fields2 + (uint64_t)i * c->fstride : nullptr;
