@@ -961,8 +961,11 @@ template <int PM>
 constexpr int twostep_waves() { return PM == 0 ? kF2 : kF2x; }
 
 // One tile of the two-step sweep with FY waves: tile t of `box` (x fastest,
-// then y, then the z slice), state a -> b.
-template <int PM, int LPRT, int FY>
+// then y, then the z slice), state a -> b.  QA (a CPML axis of the variant,
+// bit 1 x, 2 y, 4 z; 0 none): that axis's psi(n+1), carried to the second
+// step, waits in the lane's own LDS slot, each half swapped in where the
+// second step uses it, instead of in registers.
+template <int PM, int LPRT, int FY, int QA = 0>
 __device__ __forceinline__ void twostep_tile(const Geo &g, const Ptrs &a, const Ptrs &b,
                                              const Box &box, const TwoStepPoints &pts, int t)
 {
@@ -974,6 +977,8 @@ __device__ __forceinline__ void twostep_tile(const Geo &g, const Ptrs &a, const 
     __shared__ f4 s_e1[2][2][FY * 64];  // E(n+1): [plane parity][ez, ex][slot]
     __shared__ f4 s_h2[2][FY * 64];     // H(n+3/2) of plane p-1: [hz, hx][slot]
     __shared__ f4 s_ep[2][FY * 64];     // E^n of the next plane: [ez, ex][slot]
+    constexpr bool QL = QA == 1 && PX, QY = QA == 2 && PY, QZ = QA == 4 && PZ;
+    __shared__ f4 s_q[QA ? 4 : 1][QA ? FY * 64 : 1];  // psi(n+1) of axis QA at p-1, own slot
     const int lane = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
     for (int i = w * 64 + lane; i < 256; i += 64 * FY) {
@@ -1115,6 +1120,8 @@ __device__ __forceinline__ void twostep_tile(const Geo &g, const Ptrs &a, const 
     f4 h2x_m = splat(0.f), h2y_m = splat(0.f);                      // H(n+3/2) at p-2
     f4 qx[NX], qy[NY], qz[NZ];  // psi(n+1) at p-1
     for (int k = 0; k < NX; ++k) qx[k] = splat(0.f);
+    if (QA)
+        for (int k = 0; k < 4; ++k) s_q[k][me] = splat(0.f);
     for (int k = 0; k < NY; ++k) qy[k] = splat(0.f);
     for (int k = 0; k < NZ; ++k) qz[k] = splat(0.f);
     uint32_t ids_c = 0;                       // ids at p-1
@@ -1291,6 +1298,16 @@ __device__ __forceinline__ void twostep_tile(const Geo &g, const Ptrs &a, const 
         e1x_c = s_e1[(it - 1) & 1][1][me];
         f4 h2x = splat(0.f), h2y = splat(0.f), h2z = splat(0.f);
         const float s2z = __shfl_down(e1z_c.x, 1, 64), s2y = __shfl_down(e1y_c.x, 1, 64);
+        // the H-phase halves of the carried psi: previous plane's in, this one's out
+        auto swap_in = [&](f4 *q, f4 *q1, int k0) {
+            q[k0] = s_q[k0][me];
+            q[k0 + 1] = s_q[k0 + 1][me];
+            s_q[k0][me] = q1[k0];
+            s_q[k0 + 1][me] = q1[k0 + 1];
+        };
+        if (QL) swap_in(qx, q1x, 0);
+        if (QY) swap_in(qy, q1y, 0);
+        if (QZ) swap_in(qz, q1z, 0);
         if (it >= 0 && R >= 1 && R <= RT - 2) {
             const int pb = (it - 1) & 1;
             const f4 ez_py = s_e1[pb][0][above], ex_py = s_e1[pb][1][above];
@@ -1327,6 +1344,23 @@ __device__ __forceinline__ void twostep_tile(const Geo &g, const Ptrs &a, const 
             h2x = madd4(da, h1x_c, -(db * (d_ez_dy - d_ey_dz)));
             h2y = madd4(da, h1y_c, -(db * (d_ex_dz - d_ez_dx)));
             h2z = madd4(da, h1z_c, -(db * (d_ey_dx - d_ex_dy)));
+            if (own && it >= 1) {  // the carried axis's H-phase psi(n+2)
+                if (QL && gxq) {
+                    const size_t o = (size_t)(p - 1) * psx + gxi;
+                    st4(b.psi[0][0] + o, qx[0]);
+                    st4(b.psi[0][1] + o, qx[1]);
+                }
+                if (QY && gyr) {
+                    const size_t o = (size_t)(p - 1) * psy + gyo;
+                    st4(b.psi[1][0] + o, qy[0]);
+                    st4(b.psi[1][1] + o, qy[1]);
+                }
+                if (QZ && gz_c) {
+                    const size_t o = (size_t)slab_index(g, 2, p - 1, g.nz) * plane + row;
+                    st4(b.psi[2][0] + o, qz[0]);
+                    st4(b.psi[2][1] + o, qz[1]);
+                }
+            }
         }
         if (!live || !lq) {
             h2x = splat(0.f);
@@ -1339,6 +1373,9 @@ __device__ __forceinline__ void twostep_tile(const Geo &g, const Ptrs &a, const 
 
         // ---- step 2: E(n+2) at plane p-1, stores of the owned cells -----
         const float h2z_l = __shfl_up(h2z.w, 1, 64), h2y_l = __shfl_up(h2y.w, 1, 64);
+        if (QL) swap_in(qx, q1x, 2);
+        if (QY) swap_in(qy, q1y, 2);
+        if (QZ) swap_in(qz, q1z, 2);
         if (it >= 1 && R >= 2 && R <= RT - 2) {
             const f4 hz_my = s_h2[0][below], hx_my = s_h2[1][below];
             const f4 hz_mx = f4{h2z_l, h2z.x, h2z.y, h2z.z};
@@ -1382,13 +1419,16 @@ __device__ __forceinline__ void twostep_tile(const Geo &g, const Ptrs &a, const 
                 st4(b.f[3] + o, h2x);
                 st4(b.f[4] + o, h2y);
                 st4(b.f[5] + o, h2z);
-                if (PX && gxq)
-                    for (int k = 0; k < NX; ++k) st4(b.psi[0][k] + (size_t)(p - 1) * psx + gxi, qx[k]);
+                if (PX && gxq) {
+                    for (int k = QL ? 2 : 0; k < NX; ++k)
+                        st4(b.psi[0][k] + (size_t)(p - 1) * psx + gxi, qx[k]);
+                }
                 if (PY && gyr)
-                    for (int k = 0; k < NY; ++k) st4(b.psi[1][k] + (size_t)(p - 1) * psy + gyo, qy[k]);
+                    for (int k = QY ? 2 : 0; k < NY; ++k)
+                        st4(b.psi[1][k] + (size_t)(p - 1) * psy + gyo, qy[k]);
                 if (PZ && gz_c) {
                     const size_t oz = (size_t)slab_index(g, 2, p - 1, g.nz) * plane + row;
-                    for (int k = 0; k < NZ; ++k) st4(b.psi[2][k] + oz, qz[k]);
+                    for (int k = QZ ? 2 : 0; k < NZ; ++k) st4(b.psi[2][k] + oz, qz[k]);
                 }
             }
         }
@@ -1399,9 +1439,12 @@ __device__ __forceinline__ void twostep_tile(const Geo &g, const Ptrs &a, const 
         h1x_c = h1x;
         h1y_c = h1y;
         h1z_c = h1z;
-        for (int k = 0; k < NX; ++k) qx[k] = q1x[k];
-        for (int k = 0; k < NY; ++k) qy[k] = q1y[k];
-        for (int k = 0; k < NZ; ++k) qz[k] = q1z[k];
+        if (!QL)
+            for (int k = 0; k < NX; ++k) qx[k] = q1x[k];
+        if (!QY)
+            for (int k = 0; k < NY; ++k) qy[k] = q1y[k];
+        if (!QZ)
+            for (int k = 0; k < NZ; ++k) qz[k] = q1z[k];
         shz_c = shz;
         gz_c = gzp;
         pbz_c = pbz;
@@ -1428,13 +1471,33 @@ __global__ __launch_bounds__(64 * twostep_waves<PM>()) void k_twostep4(Geo g, Pt
 // scene of a batch in one launch (rows of box.lpr lanes); workgroup i
 // computes tile t - box.first of item wg_item[t], t = i after XCD
 // renumbering.
+// The batch's x CPML variant <1> carries its x psi(n+1) in LDS (168 VGPRs)
+// and runs 10 waves, three per SIMD: on the 96-scene set 1.823 against
+// 1.878 ms per step with registers at 8 waves (LDS carry at 8 waves: 1.891).
+#ifndef WF_BATCH_FYX
+#define WF_BATCH_FYX 10
+#endif
+// The y and z variants <2>, <4> can carry theirs the same way
+// (WF_BATCH_CARRY_YZ=1: 164-168 VGPRs at 10 waves); on the 96-scene set that
+// measured 1.832 ms per step, the same as registers at 8 waves (182 VGPRs),
+// which stay the default.  Registers at 10 waves spill.
+#ifndef WF_BATCH_CARRY_YZ
+#define WF_BATCH_CARRY_YZ 0
+#endif
 #ifndef WF_BATCH_FY1
-#define WF_BATCH_FY1 8
+#define WF_BATCH_FY1 (WF_BATCH_CARRY_YZ ? 10 : 8)
 #endif
 template <int PM>
 constexpr int twostep_batch_waves()
 {
-    return PM == 0 ? kF2 : ((PM == 1 || PM == 2 || PM == 4) ? WF_BATCH_FY1 : kF2x);
+    return PM == 0 ? kF2 : (PM == 1 ? WF_BATCH_FYX : ((PM == 2 || PM == 4) ? WF_BATCH_FY1 : kF2x));
+}
+// The axis whose psi(n+1) a batched variant carries in LDS (single-axis
+// variants only: one axis's slots fit beside 10 waves' exchanges).
+template <int PM>
+constexpr int twostep_batch_carry()
+{
+    return PM == 1 ? 1 : ((WF_BATCH_CARRY_YZ && (PM == 2 || PM == 4)) ? PM : 0);
 }
 
 template <int PM>
@@ -1443,8 +1506,8 @@ __global__ __launch_bounds__(64 * twostep_batch_waves<PM>()) void k_twostep4_bat
 {
     const int t = xcd_tile(blockIdx.x, total);
     const TwoStepItem &it = items[wg_item[t]];
-    twostep_tile<PM, 0, twostep_batch_waves<PM>()>(it.g, it.a, it.b, it.box, it.pts,
-                                                   t - it.box.first);
+    twostep_tile<PM, 0, twostep_batch_waves<PM>(), twostep_batch_carry<PM>()>(
+        it.g, it.a, it.b, it.box, it.pts, t - it.box.first);
 }
 #endif
 
