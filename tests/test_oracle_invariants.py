@@ -278,6 +278,51 @@ def test_waveforms():
     assert abs(lib.wfo_waveform(2, 1.0, 2.0, 2.0) - 1.0) < 1e-6
 
 
+def _ulps(a, b):
+    """Distance between two finite f32 values in units in the last place."""
+    ia, ib = (np.array([v], f32).view(np.int32)[0].astype(np.int64) for v in (a, b))
+    ia = ia if ia >= 0 else -(ia & 0x7FFFFFFF)
+    ib = ib if ib >= 0 else -(ib & 0x7FFFFFFF)
+    return abs(int(ia) - int(ib))
+
+
+def test_waveforms_against_f64_closed_forms():
+    """src/source.rs:45-60 evaluated in f32 in the reference's operation order
+    (Rust evaluates left to right: (-0.5 * arg) * arg, ((2 PI) f) t,
+    (PI f) (t - t0), (1 - 2 sq) * exp(-sq)), with exp and sin taken in f64
+    from the rounded f32 argument and rounded once.  The oracle's value is
+    within one ulp of that everywhere and equal to it for at least 98% of
+    5000 random (parameters, t) per waveform: a libm exp/sin more than an ulp
+    off, a changed operation order (the sinusoid's argument reaches 157 rad,
+    where one ulp of it moves sin by hundreds of ulps), swapped parameters or
+    a dropped factor fail."""
+    lib = W.lib()
+    rng = np.random.default_rng(3)
+    pi = f32(math.pi)
+    for kind in range(3):
+        exact = 0
+        n = 5000
+        for _ in range(n):
+            if kind == 0:
+                p0, p1, t = f32(rng.uniform(1e-11, 1e-10)), f32(rng.uniform(5e-12, 3e-11)), \
+                    f32(rng.uniform(0, 3e-10))
+                arg = f32(f32(t - p0) / p1)
+                exp = f32(math.exp(float(f32(f32(f32(-0.5) * arg) * arg))))
+            elif kind == 1:
+                p0, p1, t = f32(rng.uniform(1e9, 5e10)), f32(0), f32(rng.uniform(0, 5e-10))
+                exp = f32(math.sin(float(f32(f32(f32(f32(2.0) * pi) * p0) * t))))
+            else:
+                p0, p1, t = f32(rng.uniform(5e9, 5e10)), f32(rng.uniform(1e-11, 1e-10)), \
+                    f32(rng.uniform(0, 3e-10))
+                arg = f32(f32(pi * p0) * f32(t - p1))
+                sq = f32(arg * arg)
+                exp = f32(f32(f32(1.0) - f32(f32(2.0) * sq)) * f32(math.exp(-float(sq))))
+            got = f32(lib.wfo_waveform(kind, float(p0), float(p1), float(t)))
+            assert _ulps(got, exp) <= 1, (kind, p0, p1, t, got, exp)
+            exact += got == exp
+        assert exact >= 0.98 * n, (kind, exact)
+
+
 def test_soft_source_adds():
     """src/source.rs:131-156."""
     sim = W.OracleSim(16, 16, 16, table=W.vacuum_table(1e-12, 1e-3))
