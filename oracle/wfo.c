@@ -14,6 +14,7 @@
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <strings.h>
 
 #define BD WFO_BLOCK_DIM
 #define VPB WFO_VOX_PER_BLOCK
@@ -861,6 +862,28 @@ typedef struct {
     int nv;
 } prim_t;
 
+/* src/scene.rs:96 parses each number with Rust's f32 FromStr, whose token
+ * grammar is [+-]? (inf | infinity | nan | decimal), letters in any case, the
+ * decimal D+ | D+ '.' D* | D* '.' D+ with an optional exponent [eE][+-]?D+.
+ * strtof takes more (hex floats, nan(...)); the reference rejects those. */
+static int rust_f32_token(const char *s)
+{
+    if (*s == '+' || *s == '-') s++;
+    if (!strcasecmp(s, "inf") || !strcasecmp(s, "infinity") || !strcasecmp(s, "nan")) return 1;
+    int nd = 0;
+    while (*s >= '0' && *s <= '9') s++, nd++;
+    if (*s == '.')
+        for (s++; *s >= '0' && *s <= '9'; s++) nd++;
+    if (!nd) return 0;
+    if (*s == 'e' || *s == 'E') {
+        s++;
+        if (*s == '+' || *s == '-') s++;
+        if (!(*s >= '0' && *s <= '9')) return 0;
+        while (*s >= '0' && *s <= '9') s++;
+    }
+    return *s == 0;
+}
+
 static int parse_line(const char *line, size_t len, prim_t *out, int *has, char *msg,
                       size_t msg_len, size_t line_no)
 {
@@ -883,7 +906,7 @@ static int parse_line(const char *line, size_t len, prim_t *out, int *has, char 
     while ((tok = strtok_r(NULL, " \t\r\v\f", &save)) != NULL) {
         char *end = NULL;
         float f = strtof(tok, &end);
-        if (end == tok || *end != 0) {
+        if (!rust_f32_token(tok) || end == tok || *end != 0) {
             snprintf(msg, msg_len, "scene line %zu: invalid number '%s'", line_no, tok);
             return -1;
         }

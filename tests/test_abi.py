@@ -239,6 +239,87 @@ def test_sphere_rasterizer_equals_full_scan():
     assert np.flatnonzero(row).tolist() == list(range(96, 128))
 
 
+# Tokens against the documented grammar of Rust's f32 FromStr, which the
+# reference parses scene numbers with (src/scene.rs:96): [+-]? then inf,
+# infinity, nan (any case) or D+ | D+ . D* | D* . D+ with [eE][+-]?D+.
+_NUMBER_TOKENS = {
+    "1": True, "+1": True, "-1": True, "1.": True, ".5": True, "-.5e-2": True,
+    "1e3": True, "1E+3": True, "0000.0000": True, "1e400": True, "1e-50": True,
+    "inf": True, "-INF": True, "Infinity": True, "+infinity": True, "nan": True,
+    "NaN": True, "-nan": True,
+    "0x10": False, "0x1p3": False, "nan(1)": False, ".": False, "e5": False,
+    "1e": False, "1e+": False, "--1": False, "1.2.3": False, "1,5": False,
+    "infin": False, "1f": False, "+": False, "1_0": False, "inf1": False,
+}
+
+
+def test_scene_numbers_follow_the_reference_grammar():
+    """Product and oracle accept exactly the tokens Rust's f32 parser takes:
+    strtof's hex floats and nan(...) are errors, as in the reference."""
+    for tok, ok in _NUMBER_TOKENS.items():
+        text = f"sphere 2 1 0 {tok} 8 8 3\n"
+        grid = MaterialGrid.create(GridDims(16, 16, 16))
+        table = MaterialTable.vacuum_filled(1.9e-12, 1e-3)
+        if ok:
+            Scene.parse(text).voxelize(grid, table, 1.9e-12, 1e-3)
+            W.scene_voxelize(text, 16, 16, 16, 1.9e-12, 1e-3)
+        else:
+            with pytest.raises(_lib.WavefrontError, match="invalid number"):
+                Scene.parse(text).voxelize(grid, table, 1.9e-12, 1e-3)
+            with pytest.raises(ValueError, match="invalid number"):
+                W.scene_voxelize(text, 16, 16, 16, 1.9e-12, 1e-3)
+
+
+def _nearest_f32(q):
+    """The f32 nearest to the rational q, ties to even (exact arithmetic)."""
+    from fractions import Fraction
+    a = np.float32(float(q))
+    best = None
+    for c in (np.nextafter(a, np.float32(-np.inf)), a, np.nextafter(a, np.float32(np.inf))):
+        d = abs(Fraction(float(c)) - q)
+        key = (d, int(np.array([c], np.float32).view(np.uint32)[0]) & 1)
+        if best is None or key < best[0]:
+            best = (key, c)
+    return best[1]
+
+
+def test_strtof_rounds_decimals_correctly():
+    """Both parsers convert with the C library's strtof; the reference's
+    Rust parser rounds correctly to nearest, ties to even.  strtof does the
+    same on decimal strings exactly halfway between two f32 values and a
+    hair either side of them (where rounding through f64 first can go
+    wrong), checked against exact rational arithmetic."""
+    import ctypes
+    from fractions import Fraction
+    libc = ctypes.CDLL(None)
+    libc.strtof.restype = ctypes.c_float
+    libc.strtof.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+    rng = np.random.default_rng(11)
+    for _ in range(300):
+        x = np.float32(rng.uniform(-1e6, 1e6) * 10.0 ** rng.integers(-20, 20))
+        y = np.nextafter(x, np.float32(np.inf))
+        mid = (Fraction(float(x)) + Fraction(float(y))) / 2
+        # a finite binary fraction has a finite decimal expansion
+        k = 0
+        while (mid * 10 ** k).denominator != 1:
+            k += 1
+        digits = str(abs(mid * 10 ** k).numerator)
+        sign = "-" if mid < 0 else ""
+        for extra, q in (("", mid), ("1", None), ("-1", None)):
+            if extra == "":
+                text = f"{sign}{digits}e-{k}"
+            else:
+                # a 1 appended 40 decimals further down moves |q| away from the midpoint;
+                # borrowing one there moves it towards zero
+                step = Fraction(1, 10 ** (k + 40))
+                q = mid + step if (extra == "1") == (mid > 0) else mid - step
+                n = abs(q * 10 ** (k + 40)).numerator
+                text = f"{sign}{n}e-{k + 40}"
+            q = mid if q is None else q
+            got = libc.strtof(text.encode(), None)
+            assert np.float32(got) == _nearest_f32(q), text
+
+
 def test_source_columns_share_equal_sources():
     """Sources with the same amplitude and waveform share one value column,
     whose values equal each source's own amplitude * waveform(t)."""

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <strings.h>
 #include <unordered_map>
 #include <vector>
 
@@ -268,6 +269,31 @@ void sphere_range(float c, float r2, uint32_t n, uint32_t *lo, uint32_t *hi)
     *hi = a == n ? 0 : b;
 }
 
+// The token grammar of Rust's f32 FromStr, which the reference parses scene
+// numbers with (src/scene.rs:96): [+-]? then inf | infinity | nan | a decimal
+// (D+ | D+ '.' D* | D* '.' D+, exponent [eE][+-]?D+ optional), letters in any
+// case.  strtof reads more (hex floats, nan(...)); those tokens are errors.
+bool decimal_token(const char *s)
+{
+    if (*s == '+' || *s == '-') ++s;
+    if (strcasecmp(s, "inf") == 0 || strcasecmp(s, "infinity") == 0 || strcasecmp(s, "nan") == 0)
+        return true;
+    int digits = 0;
+    while (std::isdigit(static_cast<unsigned char>(*s))) ++s, ++digits;
+    if (*s == '.') {
+        ++s;
+        while (std::isdigit(static_cast<unsigned char>(*s))) ++s, ++digits;
+    }
+    if (digits == 0) return false;
+    if (*s == 'e' || *s == 'E') {
+        ++s;
+        if (*s == '+' || *s == '-') ++s;
+        if (!std::isdigit(static_cast<unsigned char>(*s))) return false;
+        while (std::isdigit(static_cast<unsigned char>(*s))) ++s;
+    }
+    return *s == 0;
+}
+
 int parse_scene(const char *text, std::vector<Primitive> &out)
 {
     size_t line_no = 0;
@@ -300,7 +326,7 @@ int parse_scene(const char *text, std::vector<Primitive> &out)
                 const char *s = tok[k + 1].c_str();
                 char *end = nullptr;
                 const float f = std::strtof(s, &end);
-                if (end == s || *end != 0)
+                if (!decimal_token(s) || end == s || *end != 0)
                     return fail(WF_ERR_INVALID, "scene line %zu: invalid number '%s'",
                                 line_no, s);
                 pr.v[k] = f;
