@@ -957,8 +957,19 @@ constexpr int kF2x = 8;  // waves of the CPML variants (their psi terms need the
 // fetched it from HBM a second time (31.4 against 27.4 GB read per launch:
 // the row above loaded it a plane earlier, and a plane of an XCD's 32 tiles
 // fills its L2).  15 waves keep the LDS within 160 KB.
+// The x CPML variant <1> (the x shell columns) can carry its x psi(n+1) in
+// LDS instead of registers, which lets it run more waves: WF_TWOSTEP_FYX
+// waves, the carry on when that is above kF2x.  At 10 waves it holds 168
+// VGPRs with 10 spilled, and at 1024^3 its launches took 1.81-1.83 ms against
+// 1.71-1.73 for registers at 8 waves (one box, alternating runs), which stay
+// the default.
+#ifndef WF_TWOSTEP_FYX
+#define WF_TWOSTEP_FYX 8
+#endif
 template <int PM>
-constexpr int twostep_waves() { return PM == 0 ? kF2 : kF2x; }
+constexpr int twostep_waves() { return PM == 0 ? kF2 : (PM == 1 ? WF_TWOSTEP_FYX : kF2x); }
+template <int PM>
+constexpr int twostep_carry() { return PM == 1 && WF_TWOSTEP_FYX > kF2x ? 1 : 0; }
 
 // One tile of the two-step sweep with FY waves: tile t of `box` (x fastest,
 // then y, then the z slice), state a -> b.  QA (a CPML axis of the variant,
@@ -1462,8 +1473,8 @@ __global__ __launch_bounds__(64 * twostep_waves<PM>()) void k_twostep4(Geo g, Pt
                                                                        Box box,
                                                                        TwoStepPoints pts)
 {
-    twostep_tile<PM, LPRT, twostep_waves<PM>()>(g, a, b, box, pts,
-                                                xcd_tile(blockIdx.x, box.gx * box.gy * box.gz));
+    twostep_tile<PM, LPRT, twostep_waves<PM>(), twostep_carry<PM>()>(
+        g, a, b, box, pts, xcd_tile(blockIdx.x, box.gx * box.gy * box.gz));
 }
 
 #ifndef WF_STEP_FMA
@@ -1520,11 +1531,14 @@ int fused_rows_per_tile_ghost() { return fused_waves<0, true>() - 1; }
 int split_rows_per_wg() { return kSY; }
 int twostep_rows_per_tile(int pml_axes, int lpr)
 {
-    int waves = twostep_waves<1>();
+    int waves = twostep_waves<7>();
     switch (pml_axes) {
     case 0: waves = twostep_waves<0>(); break;
+    case 1: waves = twostep_waves<1>(); break;
     case 2: waves = twostep_waves<2>(); break;
+    case 3: waves = twostep_waves<3>(); break;
     case 4: waves = twostep_waves<4>(); break;
+    case 5: waves = twostep_waves<5>(); break;
     case 6: waves = twostep_waves<6>(); break;
     default: break;
     }
