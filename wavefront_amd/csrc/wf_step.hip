@@ -458,12 +458,12 @@ constexpr int kFYx = 8;
 #ifndef WF_FUSED_XPML_FY
 #define WF_FUSED_XPML_FY 8  // 198 VGPRs: 12 waves would spill
 #endif
-template <int PM, bool GHOST>
 // The ghost variant (learned faces) at 8 waves, 182 VGPRs: at 12 waves it
 // spills 14 and the learned 256^3 step took 0.281 against 0.267 ms.
 #ifndef WF_FUSED_GHOST_FY
 #define WF_FUSED_GHOST_FY 8
 #endif
+template <int PM, bool GHOST>
 constexpr int fused_waves()
 {
     return PM == 1 ? kFYx : (GHOST ? WF_FUSED_GHOST_FY : (PM == 2 ? WF_FUSED_XPML_FY : kFY));
