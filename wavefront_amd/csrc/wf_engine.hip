@@ -1186,7 +1186,7 @@ int launch_pair(wf_ctx *c, hipEvent_t *ev = nullptr)
     StepLaunch s{c->g, c->p, c->alt, c->pml, c->fzchunk, st};
     // every region on one stream: spreading the shell regions' launches over
     // three more streams beside the interior's measured 7.02-7.28 ms per step
-    // against 6.65-6.68 at 1024^3 (four runs each, one box)
+    // against 6.65-6.68 at 1024^3 (four runs each, one GPU)
     for (int grp = 0; grp < 3; ++grp) {
         if (ev) WF_HIP(hipEventRecord(ev[2 * grp], st));
         for (int k = 0; k < c->nregions; ++k) {
@@ -1532,7 +1532,7 @@ int wf_create(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t *material_ids
     // whole 256 B, instead of 4096): with N a power of two the fields would
     // start at power-of-two distances; measured at 1024^3 the two-step
     // interior kernel takes 9.23-9.25 ms per launch against 9.34-9.51
-    // unpadded (four runs each, one box), the step 0.1-0.3% less
+    // unpadded (four runs each, one GPU), the step 0.1-0.3% less
     uint64_t pad = 4096;
     if (const char *fp = std::getenv("WF_FIELD_PAD")) pad = (uint64_t)std::max(0L, std::atol(fp));
     c->fstride = N + pad / 256 * 64;

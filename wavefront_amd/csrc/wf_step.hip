@@ -950,7 +950,7 @@ constexpr int kF2x = 8;  // waves of the CPML variants (their psi terms need the
 // per workgroup (127 VGPRs: four waves per SIMD), reads the previous plane's
 // E(n+1) back from its own LDS slot, and takes E^n of the row above from the
 // row above through LDS (that row loaded it as its own next plane).  Measured
-// on MI355X at 1024^3 (A/B within one box): one plane loaded ahead with 12
+// on MI355X at 1024^3 (A/B on one GPU): one plane loaded ahead with 12
 // waves (167 VGPRs, three per SIMD) ran 7.85-7.91 ms per step against
 // 7.53-7.65 for this scheme at 16 waves; reading back also H(n+1/2) and
 // H(n+3/2) gained nothing; two planes ahead spill; loading E^n at y+1 again
@@ -961,7 +961,7 @@ constexpr int kF2x = 8;  // waves of the CPML variants (their psi terms need the
 // LDS instead of registers, which lets it run more waves: WF_TWOSTEP_FYX
 // waves, the carry on when that is above kF2x.  At 10 waves it holds 168
 // VGPRs with 10 spilled, and at 1024^3 its launches took 1.81-1.83 ms against
-// 1.71-1.73 for registers at 8 waves (one box, alternating runs), which stay
+// 1.71-1.73 for registers at 8 waves (one GPU, alternating runs), which stay
 // the default.
 #ifndef WF_TWOSTEP_FYX
 #define WF_TWOSTEP_FYX 8
