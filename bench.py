@@ -264,6 +264,37 @@ def run_batch_96(args):
     print(json.dumps(out), flush=True)
 
 
+def run_learned_throughput(args, L, dx, dt):
+    """Interior throughput of a 248^3 vacuum cube with a source (ms per step
+    over 100 steps after 16) for the 4-cell learned layer, the zero boundary
+    and 32-cell CPML, as the full learned workload ends with.  The layer is
+    the Higdon head with the network's output held at its start value: the
+    trained layer runs the same kernels on the same sizes, only the weights
+    differ."""
+    from wavefront_amd.engine import hbm_copy_gbs
+
+    if args.copy_gib > 0:
+        copy_gbs = hbm_copy_gbs(int(args.copy_gib * (1 << 30)))
+        log(f"plain copy kernel: {copy_gbs:.0f} GB/s (read + write)")
+    base = L.FaceNets(L.higdon_base(dt, dx), dt, dx, L.HEAD_HIGDON, False)
+    tp = {}
+    for kind in ("learned", "zero", "cpml32"):
+        mc, ms, cells = L.throughput(kind, base, interior=248, steps=100)
+        tp[kind] = {"interior_mcells": round(mc, 1), "ms_per_step": round(ms, 4), "cells": cells}
+        log(f"throughput {kind}: {mc:.0f} interior Mcell/s, {ms:.3f} ms/step")
+    print(json.dumps({
+        "metric": "interior Mcell-updates/s of a 248^3 vacuum cube, learned 4-cell layer "
+                  "(network output held) against 32-cell CPML",
+        "value": tp["learned"]["interior_mcells"], "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": 100, "warmup": 16, "ms_per_step": tp["learned"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 fields, bf16 network on MFMA", "data": "synthetic",
+        "config": {"workload": "learned throughput, 248^3 interior", "throughput_interior_248": tp,
+                   "throughput_ratio_learned_vs_cpml32": round(
+                       tp["learned"]["interior_mcells"] / tp["cpml32"]["interior_mcells"], 3)},
+    }), flush=True)
+
+
 def run_learned(args):
     """The learned thin boundary at its 256^3 workload.  The shipped layer:
     per face the second-order Higdon update (the Higdon head: a fixed
@@ -289,6 +320,9 @@ def run_learned(args):
     from wavefront_amd.layout import GridDims
 
     dx, dt, _ = L.study_grid()
+    if args.learned_throughput:
+        run_learned_throughput(args, L, dx, dt)
+        return
     t0 = time.perf_counter()
     S1 = L.collect_3d(GridDims(96, 96, 96), runs=0, steps=0, every=1, per_face=200000, seed=11,
                       dt=dt, dx=dx, plane_runs=32)
@@ -395,6 +429,11 @@ def main():
                     help="spheres_512 = the 512^3 two-sphere run with snapshot egress; "
                          "batch_96 = 96 small scenes batched; learned = the learned thin "
                          "boundary's reflection and throughput")
+    ap.add_argument("--learned-throughput", action="store_true",
+                    help="learned: only the interior throughput of the 4-cell layer (the "
+                         "Higdon head with the network output held: the trained layer's "
+                         "kernels and work per step), the zero boundary and 32-cell CPML; "
+                         "no data, training or reflection runs (for counter passes)")
     ap.add_argument("--c4-steps", type=int, default=500)
     ap.add_argument("--no-solo", action="store_true",
                     help="batch_96: skip running the scenes one after another")
